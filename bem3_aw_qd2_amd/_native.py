@@ -1,0 +1,81 @@
+"""ctypes bindings of libbqd2_hip.so / libbem3_aw_qd2.so.
+
+Loading fails loudly when a library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+
+_hip = None
+_host = None
+
+
+class BQD2Error(RuntimeError):
+    pass
+
+
+def build(jobs=8):
+    """Compile both libraries for gfx950 (hipcc cross-compiles without a GPU)."""
+    subprocess.run(["make", "-C", CSRC, "-j%d" % jobs], check=True)
+
+
+def _load(name):
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise BQD2Error("%s is not built (run bem3_aw_qd2_amd.build() or make -C %s)" % (path, CSRC))
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+class QPD2(C.Structure):
+    """Layout of the reference's QPD2 (qgf2_src/d3_qpgf_d2.h:32-36)."""
+    _fields_ = [("k", C.c_double), ("vk", C.c_double * 3),
+                ("vd1", C.c_double * 2), ("vd2", C.c_double * 2)]
+
+
+class GfParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("k", "vkx", "vky", "d1x", "d1y", "d2x", "d2y", "idd", "aidd",
+                 "g1x", "g1y", "g2x", "g2y", "d2n", "g2n", "g12", "E0", "bN", "bNp", "bNm")] + \
+               [("nc", C.c_int), ("ewald_only", C.c_int)]
+
+
+def hip_lib():
+    global _hip
+    if _hip is None:
+        lib = _load("libbqd2_hip.so")
+        lib.bqd2_last_error.restype = C.c_char_p
+        lib.bqd2_gf_params.argtypes = [C.POINTER(C.c_double), C.POINTER(GfParams)]
+        lib.bqd2_gf_batch.argtypes = [C.POINTER(GfParams), C.c_int, C.c_int, C.c_void_p, C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _hip = lib
+    return _hip
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        hip_lib()
+        _host = _load("libbem3_aw_qd2.so")
+    return _host
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = hip_lib().bqd2_last_error().decode(errors="replace")
+        raise BQD2Error("%s failed (%d): %s" % (what, rc, msg))
+
+
+def gf_params(qd):
+    """GfParams from a QPD2 (or an 8-sequence k, vk[3], vd1[2], vd2[2])."""
+    if isinstance(qd, QPD2):
+        vals = [qd.k, qd.vk[0], qd.vk[1], qd.vk[2], qd.vd1[0], qd.vd1[1], qd.vd2[0], qd.vd2[1]]
+    else:
+        vals = list(qd)
+    arr = (C.c_double * 8)(*vals)
+    p = GfParams()
+    check(hip_lib().bqd2_gf_params(arr, C.byref(p)), "bqd2_gf_params")
+    return p

@@ -1,0 +1,501 @@
+// gf_dev.h -- device quasi-periodic Helmholtz Green's function (2-D lattice in
+// the xy-plane) and the Faddeeva function it needs.
+//
+// Restates qgf2_src/d3_qpgf_d2.c of the reference: method choice between the
+// Ewald split and the pure spectral sum (:124-176), unit-cell reduction with the
+// Bloch phase (:409-438), Ewald parameter (:1680-1693), spatial image sum
+// (:1714-1838) and spectral sum (:1840-2009) with the same row-by-row adaptive
+// stopping rules (:471-482), and the Hessian variants (:2011-2398).  The
+// reference evaluates one point per call; here one lane evaluates one point and
+// the instance constants (lattice, reciprocal lattice, Ewald parameter, method
+// probe) are computed once on the host into GfParams.
+//
+// Faddeeva w(z): Weideman's rational approximation, N = 40 (SIAM J. Numer.
+// Anal. 31 (1994) 1497-1518), branch-free on Im z >= 0, reflection below.
+#pragma once
+#include "cplx.h"
+#include "../../../include/weideman_coef.h"
+#include "../../../include/bqd2.h"
+
+// Products and sums are rounded one at a time, as the reference's C does on
+// x86-64 (no contraction into FMA): the Ewald spectral sum at z == 0 decides
+// where to stop from the rounding residue of exp(bn^2 E^2) * exp(-(bn E)^2) - 1,
+// so the inputs to that residue have to be rounded identically.  Explicit fma()
+// calls (complex products, the Weideman recurrence) are unaffected.
+#pragma clang fp contract(off)
+
+#define BQD2_EFS 0.1
+#define BQD2_EW_LIMIT 100
+#define BQD2_FD_LIMIT 100
+#define EW_EXPM_D 3.0
+
+enum { GF_QG = 0, GF_DQG = 1, GF_V1 = 2, GF_V2 = 3 };
+
+// ---------------------------------------------------------------- Faddeeva w
+__device__ __forceinline__ cd weideman_upper(double x, double y) {
+  // w(z) ~ 2 p(Z)/(L - iz)^2 + (1/sqrt(pi))/(L - iz), Z = (L + iz)/(L - iz)
+  const double L = WEID_L;
+  const double a[WEID_N] = {WEID_COEFS};
+  cd iden = crecip(cd{L + y, -x});  // 1/(L - i z)
+  cd Z = cd{L - y, x} * iden;        // (L + i z)/(L - i z)
+  cd p = cd{a[WEID_N - 1], 0.0};
+#pragma unroll
+  for (int n = WEID_N - 2; n >= 0; n--) p = p * Z + cd{a[n], 0.0};
+  const double isp = 0.56418958354775628695;  // 1/sqrt(pi)
+  return 2.0 * (p * iden * iden) + isp * iden;
+}
+
+// w(x + i y) with the reference's conventions: real axis returns Re w = exp(-x^2)
+// exactly (Faddeeva.cc:684-686), lower half plane by reflection.
+__device__ __forceinline__ cd faddeeva_w(double x, double y) {
+  if (y > 0.0) return weideman_upper(x, y);
+  if (y == 0.0) {
+    cd w = weideman_upper(x, 0.0);
+    return cd{exp(-__dmul_rn(x, x)), w.y};
+  }
+  // w(z) = 2 exp(-z^2) - w(-z)
+  cd wm = weideman_upper(-x, -y);
+  double e = 2.0 * exp(y * y - x * x);
+  cd ph = cis(-2.0 * x * y);
+  return cd{e * ph.x - wm.x, e * ph.y - wm.y};
+}
+
+// ---------------------------------------------------------------- sums
+struct Sum10 {
+  cd g, dg[3], d2g[6];
+};
+
+__device__ __forceinline__ bool conv_same(cd a, cd b, double eps) {
+  return 2.0 * fabs(a.x - b.x) <= eps * (fabs(a.x) + fabs(b.x)) &&
+         2.0 * fabs(a.y - b.y) <= eps * (fabs(a.y) + fabs(b.y));
+}
+__device__ __forceinline__ bool conv_small(cd t, cd s, double eps) {
+  return fabs(t.x) <= fabs(s.x) * BQD2_EFS * eps && fabs(t.y) <= fabs(s.y) * BQD2_EFS * eps;
+}
+
+template <int M>
+__device__ __forceinline__ void sum_add(Sum10& s, const Sum10& t) {
+  if (M != GF_DQG) s.g += t.g;
+  if (M != GF_QG) {
+    s.dg[0] += t.dg[0];
+    s.dg[1] += t.dg[1];
+    s.dg[2] += t.dg[2];
+  }
+  if (M == GF_V2) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) s.d2g[i] += t.d2g[i];
+  }
+}
+
+// stopping quantities of each mode (d3_qpgf_d2.c, per chain)
+template <int M>
+__device__ __forceinline__ bool rows_conv_spatial(const Sum10& s, const Sum10& o, double eps) {
+  if (M == GF_QG || M == GF_V1) return conv_same(s.g, o.g, eps);
+  if (M == GF_DQG) return conv_same(s.dg[0], o.dg[0], eps) && conv_same(s.dg[1], o.dg[1], eps);
+  return conv_same(s.d2g[0], o.d2g[0], eps) && conv_same(s.d2g[1], o.d2g[1], eps);
+}
+template <int M>
+__device__ __forceinline__ bool term_small_spatial(const Sum10& t, const Sum10& s, double eps) {
+  if (M == GF_QG || M == GF_V1) return conv_small(t.g, s.g, eps);
+  if (M == GF_DQG) return conv_small(t.dg[0], s.dg[0], eps) && conv_small(t.dg[1], s.dg[1], eps);
+  return conv_small(t.d2g[0], s.d2g[0], eps) && conv_small(t.d2g[1], s.d2g[1], eps);
+}
+template <int M>
+__device__ __forceinline__ bool rows_conv_spectral(const Sum10& s, const Sum10& o, double eps) {
+  if (M == GF_QG) return conv_same(s.g, o.g, eps);
+  if (M == GF_DQG || M == GF_V1) return conv_same(s.dg[2], o.dg[2], eps);
+  return conv_same(s.d2g[0], o.d2g[0], eps) && conv_same(s.d2g[1], o.d2g[1], eps);
+}
+template <int M>
+__device__ __forceinline__ bool term_small_spectral(const Sum10& t, const Sum10& s, double eps) {
+  if (M == GF_QG) return conv_small(t.g, s.g, eps);
+  if (M == GF_DQG || M == GF_V1) return conv_small(t.dg[2], s.dg[2], eps);
+  return conv_small(t.d2g[0], s.d2g[0], eps) && conv_small(t.d2g[1], s.d2g[1], eps);
+}
+template <int M>
+__device__ __forceinline__ bool rows_conv_fd(const Sum10& s, const Sum10& o, double eps) {
+  if (M == GF_QG) return conv_same(s.g, o.g, eps);
+  if (M == GF_DQG || M == GF_V1) return conv_same(s.dg[2], o.dg[2], eps);
+  return conv_same(s.d2g[2], o.d2g[2], eps);
+}
+template <int M>
+__device__ __forceinline__ bool term_small_fd(const Sum10& t, const Sum10& s, double eps) {
+  if (M == GF_QG) return conv_small(t.g, s.g, eps);
+  if (M == GF_DQG || M == GF_V1) return conv_small(t.dg[2], s.dg[2], eps);
+  return conv_small(t.d2g[2], s.d2g[2], eps);
+}
+
+// ---------------------------------------------------------------- terms
+// spatial image (l1, l2): d3_qpgf_d2.c:1814-1838 (v1) and :2131-2170 (v2)
+template <int M>
+__device__ __forceinline__ void spatial_term(const GfParams& P, double rx, double ry, double rz, double E,
+                                             int l1, int l2, Sum10& t) {
+  double Rl0 = l1 * P.d1x + l2 * P.d2x;
+  double Rl1 = l1 * P.d1y + l2 * P.d2y;
+  double ra0 = rx + Rl0, ra1 = ry + Rl1;
+  double rl = sqrt(ra0 * ra0 + ra1 * ra1 + rz * rz);
+  double irl = 1.0 / rl;
+  double iE = 1.0 / E;
+  double ke = P.k * E, y = rl * 0.5 * iE;
+  cd ce = cis(P.vkx * Rl0 + P.vky * Rl1);
+  double ef = exp(ke * ke - y * y);
+  cd w = weideman_upper(ke, y);
+  const double isp = 0.56418958354775628695;
+  cd cer = ce * (ef * w.x);
+  t.g = cer * irl;
+  if (M != GF_QG) {
+    cd cf1 = (irl * irl) * (cer * irl - ce * (P.k * ef * w.y) + ce * (ef * iE * isp));
+    t.dg[0] = ra0 * cf1;
+    t.dg[1] = ra1 * cf1;
+    t.dg[2] = cf1;
+    if (M == GF_V2) {
+      cd cf2 = (irl * irl) * (cer * (P.k * P.k * irl * 2.0) - ce * (ef * iE * iE * iE * isp));
+      cd cf3 = cf1 * (-6.0 * irl * irl) + cf2;
+      t.d2g[0] = 2.0 * cf1 + (ra0 * ra0) * cf3;
+      t.d2g[1] = 2.0 * cf1 + (ra1 * ra1) * cf3;
+      t.d2g[2] = 2.0 * cf1 + (rz * rz) * cf3;
+      t.d2g[3] = (ra0 * ra1) * cf3;
+      t.d2g[4] = ra1 * cf3;
+      t.d2g[5] = ra0 * cf3;
+    }
+  }
+}
+
+// spectral order (n1, n2): d3_qpgf_d2.c:1965-2009 (v1) and :2349-2398 (v2).
+// returns 0 propagating, 1 evanescent, -3 Wood anomaly.
+template <int M>
+__device__ __forceinline__ int spectral_term(const GfParams& P, double rx, double ry, double rz, double E,
+                                             int n1, int n2, Sum10& t) {
+  double an0 = -P.vkx + n1 * P.g1x + n2 * P.g2x;
+  double an1 = -P.vky + n1 * P.g1y + n2 * P.g2y;
+  double bn2 = P.k * P.k - an0 * an0 - an1 * an1;
+  cd ce = cis(an0 * rx + an1 * ry);
+  double azve = rz / (2.0 * E);
+  const double isp = 0.56418958354775628695;
+  if (bn2 > 0.0) {
+    double bn = sqrt(bn2);
+    cd ff = cis(bn * rz);
+    // ef and w kept apart as the reference does: at z == 0, ef*Re(w) - ff is
+    // pure rounding and the row stopping rule reads it (products not fused)
+    double ef = exp(__dmul_rn(__dmul_rn(bn2, E), E) - __dmul_rn(azve, azve));
+    cd w = faddeeva_w(bn * E, azve);
+    cd F = cd{2.0 * ff.x, 2.0 * (__dmul_rn(ef, w.y) + ff.y)};
+    double ib = 1.0 / bn;
+    t.g = ce * F * ib;
+    if (M != GF_QG) {
+      cd Gz = cd{2.0 * (__dmul_rn(ef, w.x) - ff.x), -2.0 * ff.y};
+      t.dg[0] = t.g * an0;
+      t.dg[1] = t.g * an1;
+      t.dg[2] = ce * Gz;
+      if (M == GF_V2) {
+        t.d2g[0] = t.dg[0] * an0;
+        t.d2g[1] = t.dg[1] * an1;
+        t.d2g[2] = ce * (mul_i(F) * bn + cd{2.0 * isp / E * ef, 0.0});
+        t.d2g[3] = t.dg[0] * an1;
+        t.d2g[4] = an1 * t.dg[2];
+        t.d2g[5] = an0 * t.dg[2];
+      }
+    }
+    return 0;
+  } else if (bn2 < 0.0) {
+    double bn = sqrt(-bn2);
+    double ebz = exp(bn * rz);
+    double f1 = ebz * erfc(bn * E + azve);
+    double f2 = 1.0 / ebz * erfc(bn * E - azve);
+    double ib = 1.0 / bn;
+    t.g = mul_i(ce) * (-(f1 + f2) * ib);
+    if (M != GF_QG) {
+      t.dg[0] = t.g * an0;
+      t.dg[1] = t.g * an1;
+      t.dg[2] = ce * (f1 - f2);
+      if (M == GF_V2) {
+        double ef = exp(bn2 * E * E - azve * azve);
+        t.d2g[0] = t.dg[0] * an0;
+        t.d2g[1] = t.dg[1] * an1;
+        t.d2g[2] = ce * (-bn * (f1 + f2) + 2.0 * isp / E * ef);
+        t.d2g[3] = t.dg[0] * an1;
+        t.d2g[4] = an1 * t.dg[2];
+        t.d2g[5] = an0 * t.dg[2];
+      }
+    }
+    return 1;
+  }
+  return -3;
+}
+
+// pure spectral order: d3_qpgf_d2.c:581-604, :1013-1043
+template <int M>
+__device__ __forceinline__ int fd_term(const GfParams& P, double rx, double ry, double rz, int n1, int n2,
+                                       Sum10& t) {
+  double an0 = -P.vkx + n1 * P.g1x + n2 * P.g2x;
+  double an1 = -P.vky + n1 * P.g1y + n2 * P.g2y;
+  double bn2 = P.k * P.k - an0 * an0 - an1 * an1;
+  int kind;
+  cd e;
+  double bn;
+  if (bn2 > 0.0) {
+    bn = sqrt(bn2);
+    e = cis(an0 * rx + an1 * ry + bn * fabs(rz));
+    t.g = e / bn;
+    kind = 0;
+  } else if (bn2 < 0.0) {
+    bn = sqrt(-bn2);
+    e = cis(an0 * rx + an1 * ry) * exp(-bn * fabs(rz));
+    t.g = mul_i(e) * (-1.0 / bn);
+    kind = 1;
+  } else {
+    return -2;
+  }
+  t.dg[2] = e;
+  t.dg[0] = t.g * an0;
+  t.dg[1] = t.g * an1;
+  if (M == GF_V2) {
+    t.d2g[0] = an0 * t.dg[0];
+    t.d2g[1] = an1 * t.dg[1];
+    t.d2g[2] = bn2 * t.g;
+    t.d2g[3] = an0 * t.dg[1];
+    t.d2g[4] = an1 * t.dg[2];
+    t.d2g[5] = an0 * t.dg[2];
+  }
+  return kind;
+}
+
+// ---------------------------------------------------------------- rows
+template <int M>
+__device__ int spatial_row(const GfParams& P, double rx, double ry, double rz, double E, double eps, int l1,
+                           Sum10& S) {
+  int lc = -(int)round(((rx + l1 * P.d1x) * P.d2x + (ry + l1 * P.d1y) * P.d2y) / P.d2n);
+  Sum10 t;
+  spatial_term<M>(P, rx, ry, rz, E, l1, lc, t);
+  sum_add<M>(S, t);
+  int cc = 1, l2;
+  for (l2 = lc + 1; l2 < lc + BQD2_EW_LIMIT; l2++) {
+    cc++;
+    spatial_term<M>(P, rx, ry, rz, E, l1, l2, t);
+    sum_add<M>(S, t);
+    if (term_small_spatial<M>(t, S, eps)) break;
+  }
+  if (l2 == lc + BQD2_EW_LIMIT) return -1;
+  for (l2 = lc - 1; l2 > lc - BQD2_EW_LIMIT; l2--) {
+    cc++;
+    spatial_term<M>(P, rx, ry, rz, E, l1, l2, t);
+    sum_add<M>(S, t);
+    if (term_small_spatial<M>(t, S, eps)) break;
+  }
+  if (l2 == lc - BQD2_EW_LIMIT) return -1;
+  return cc;
+}
+
+// the row flag is 1 when no order other than the centre one was propagating:
+// the reference counts propagating orders only in its outward loops
+template <int M>
+__device__ int spectral_row(const GfParams& P, double rx, double ry, double rz, double E, double eps, int n1,
+                            Sum10& S, int& flag) {
+  int nc = (int)round(-((-P.vkx + n1 * P.g1x) * P.g2x + (-P.vky + n1 * P.g1y) * P.g2y) / P.g2n);
+  Sum10 t;
+  int k = spectral_term<M>(P, rx, ry, rz, E, n1, nc, t);
+  if (k < 0) return k;
+  sum_add<M>(S, t);
+  int cc = 1, n2, prop = 0;
+  for (n2 = nc + 1; n2 < nc + BQD2_EW_LIMIT; n2++) {
+    cc++;
+    k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, t);
+    if (k < 0) return k;
+    sum_add<M>(S, t);
+    if (k > 0) {
+      if (term_small_spectral<M>(t, S, eps)) break;
+    } else {
+      prop++;
+    }
+  }
+  if (n2 == nc + BQD2_EW_LIMIT) return -2;
+  for (n2 = nc - 1; n2 > nc - BQD2_EW_LIMIT; n2--) {
+    cc++;
+    k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, t);
+    if (k < 0) return k;
+    sum_add<M>(S, t);
+    if (k > 0) {
+      if (term_small_spectral<M>(t, S, eps)) break;
+    } else {
+      prop++;
+    }
+  }
+  if (n2 == nc - BQD2_EW_LIMIT) return -2;
+  flag = (prop == 0);
+  return cc;
+}
+
+template <int M>
+__device__ int fd_row(const GfParams& P, double rx, double ry, double rz, double eps, int n1, Sum10& S) {
+  int nc = (int)round((P.vkx * P.g2x + P.vky * P.g2y - n1 * P.g12) / P.g2n);
+  Sum10 t;
+  int k = fd_term<M>(P, rx, ry, rz, n1, nc, t);
+  if (k < 0) return k;
+  sum_add<M>(S, t);
+  int cc = 1, n2;
+  for (n2 = nc + 1; n2 < nc + BQD2_FD_LIMIT; n2++) {
+    k = fd_term<M>(P, rx, ry, rz, n1, n2, t);
+    if (k < 0) return k;
+    sum_add<M>(S, t);
+    cc++;
+    if (k > 0 && term_small_fd<M>(t, S, eps)) break;
+  }
+  if (n2 == nc + BQD2_FD_LIMIT) return -1;
+  for (n2 = nc - 1; n2 > nc - BQD2_FD_LIMIT; n2--) {
+    k = fd_term<M>(P, rx, ry, rz, n1, n2, t);
+    if (k < 0) return k;
+    sum_add<M>(S, t);
+    cc++;
+    if (k > 0 && term_small_fd<M>(t, S, eps)) break;
+  }
+  if (n2 == nc - BQD2_FD_LIMIT) return -1;
+  return cc;
+}
+
+__device__ __forceinline__ void sum_zero(Sum10& s) {
+  s.g = cd{0, 0};
+#pragma unroll
+  for (int i = 0; i < 3; i++) s.dg[i] = cd{0, 0};
+#pragma unroll
+  for (int i = 0; i < 6; i++) s.d2g[i] = cd{0, 0};
+}
+
+// Ewald evaluation in the unit cell (d3_qpgf_d2.c:1671-1712, :2011-2052)
+template <int M>
+__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+  double E = P.E0;
+  double r2 = rx * rx + ry * ry + rz * rz;
+  double k2 = P.k * P.k;
+  if (k2 * E * E - r2 / (4.0 * E * E) > EW_EXPM_D) E = sqrt(EW_EXPM_D + sqrt(k2 * r2 + EW_EXPM_D * EW_EXPM_D)) / (sqrt(2.0) * P.k);
+
+  Sum10 S1, S2, old;
+  sum_zero(S1);
+  sum_zero(S2);
+  int c1 = spatial_row<M>(P, rx, ry, rz, E, eps, 0, S1);
+  if (c1 < 0) return -1;
+  int l1;
+  for (l1 = 1; l1 < BQD2_EW_LIMIT; l1++) {
+    old = S1;
+    int c = spatial_row<M>(P, rx, ry, rz, E, eps, l1, S1);
+    if (c < 0) return -1;
+    c1 += c;
+    if (rows_conv_spatial<M>(S1, old, eps)) break;
+  }
+  if (l1 == BQD2_EW_LIMIT) return -1;
+  for (l1 = -1; l1 > -BQD2_EW_LIMIT; l1--) {
+    old = S1;
+    int c = spatial_row<M>(P, rx, ry, rz, E, eps, l1, S1);
+    if (c < 0) return -1;
+    c1 += c;
+    if (rows_conv_spatial<M>(S1, old, eps)) break;
+  }
+  if (l1 == -BQD2_EW_LIMIT) return -1;
+
+  int flag = 0;
+  int c2 = spectral_row<M>(P, rx, ry, rz, E, eps, P.nc, S2, flag);
+  if (c2 < 0) return c2;
+  int n1;
+  for (n1 = P.nc + 1; n1 < P.nc + BQD2_EW_LIMIT; n1++) {
+    old = S2;
+    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag);
+    if (c < 0) return c;
+    c2 += c;
+    if (flag && rows_conv_spectral<M>(S2, old, eps)) break;
+  }
+  if (n1 == P.nc + BQD2_EW_LIMIT) return -2;
+  for (n1 = P.nc - 1; n1 > P.nc - BQD2_EW_LIMIT; n1--) {
+    old = S2;
+    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag);
+    if (c < 0) return c;
+    c2 += c;
+    if (flag && rows_conv_spectral<M>(S2, old, eps)) break;
+  }
+  if (n1 == P.nc - BQD2_EW_LIMIT) return -2;
+
+  const double ifp = 0.079577471545947667884;  // 1/(4 pi)
+  const double iep = 0.039788735772973833942;  // 1/(8 pi)
+  double cdd = 0.25 * P.aidd;
+  if (M != GF_DQG) out.g = S1.g * ifp + mul_i(S2.g) * cdd;
+  if (M != GF_QG) {
+    out.dg[0] = S1.dg[0] * (-ifp) - S2.dg[0] * cdd;
+    out.dg[1] = S1.dg[1] * (-ifp) - S2.dg[1] * cdd;
+    out.dg[2] = S1.dg[2] * (-rz * ifp) + S2.dg[2] * cdd;
+  }
+  if (M == GF_V2) {
+    out.d2g[0] = S1.d2g[0] * (-iep) - mul_i(S2.d2g[0]) * cdd;
+    out.d2g[1] = S1.d2g[1] * (-iep) - mul_i(S2.d2g[1]) * cdd;
+    out.d2g[2] = S1.d2g[2] * (-iep) - S2.d2g[2] * cdd;
+    out.d2g[3] = S1.d2g[3] * (-iep) - mul_i(S2.d2g[3]) * cdd;
+    out.d2g[4] = S1.d2g[4] * (-rz * iep) + mul_i(S2.d2g[4]) * cdd;
+    out.d2g[5] = S1.d2g[5] * (-rz * iep) + mul_i(S2.d2g[5]) * cdd;
+  }
+  return c1 + c2;
+}
+
+// pure spectral evaluation in the unit cell (d3_qpgf_d2.c:484-533, :887-950)
+template <int M>
+__device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+  if (rz == 0.0) return -4;
+  Sum10 S, old;
+  sum_zero(S);
+  int cc = fd_row<M>(P, rx, ry, rz, eps, P.nc, S);
+  if (cc < 0) return cc;
+  int n1;
+  for (n1 = P.nc + 1; n1 < P.nc + BQD2_FD_LIMIT; n1++) {
+    old = S;
+    int c = fd_row<M>(P, rx, ry, rz, eps, n1, S);
+    if (c < 0) return c;
+    cc += c;
+    if (rows_conv_fd<M>(S, old, eps)) break;
+  }
+  if (n1 == P.nc + BQD2_FD_LIMIT) return -1;
+  for (n1 = P.nc - 1; n1 > P.nc - BQD2_FD_LIMIT; n1--) {
+    old = S;
+    int c = fd_row<M>(P, rx, ry, rz, eps, n1, S);
+    if (c < 0) return c;
+    cc += c;
+    if (rows_conv_fd<M>(S, old, eps)) break;
+  }
+  if (n1 == P.nc - BQD2_FD_LIMIT) return -1;
+  double a = P.aidd, sz = rz / fabs(rz);
+  out.g = mul_i(S.g) * (0.5 * a);
+  out.dg[0] = S.dg[0] * (-0.5 * a);
+  out.dg[1] = S.dg[1] * (-0.5 * a);
+  out.dg[2] = S.dg[2] * (-0.5 * a * sz);
+  if (M == GF_V2) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) out.d2g[i] = mul_i(S.d2g[i]) * (-0.5 * a);
+    out.d2g[4] = mul_i(S.d2g[4]) * (-0.5 * a * sz);
+    out.d2g[5] = mul_i(S.d2g[5]) * (-0.5 * a * sz);
+  }
+  return cc;
+}
+
+// full evaluation at offset r (source minus target); returns the term count or
+// the reference's negative code (-1x pure spectral, -2x Ewald)
+template <int M>
+__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+  bool use_fd = false;
+  if (!P.ewald_only) {
+    use_fd = (P.bN / P.bNp * exp(-P.bNp * fabs(rz)) < eps) && (P.bN / P.bNm * exp(-P.bNm * fabs(rz)) < eps);
+  }
+  int l1 = (int)floor((rx * P.d2y - ry * P.d2x) * P.idd + 0.5);
+  int l2 = (int)floor(-(rx * P.d1y - ry * P.d1x) * P.idd + 0.5);
+  double tx = rx - l1 * P.d1x - l2 * P.d2x;
+  double ty = ry - l1 * P.d1y - l2 * P.d2y;
+  sum_zero(out);
+  int err = use_fd ? fd_cell<M>(P, tx, ty, rz, eps, out) : ewald_cell<M>(P, tx, ty, rz, eps, out);
+  if (l1 != 0 || l2 != 0) {
+    double arg = -(P.vkx * (l1 * P.d1x + l2 * P.d2x) + P.vky * (l1 * P.d1y + l2 * P.d2y));
+    cd cf = cis(arg);
+    out.g *= cf;
+#pragma unroll
+    for (int i = 0; i < 3; i++) out.dg[i] *= cf;
+    if (M == GF_V2) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) out.d2g[i] *= cf;
+    }
+  }
+  if (err < 0) err -= use_fd ? 10 : 20;
+  return err;
+}
