@@ -1,0 +1,14 @@
+/* host_internal.h -- helpers shared by the host C sources of libbem3_aw_qd2.so. */
+#ifndef BQD2_HOST_INTERNAL_H_
+#define BQD2_HOST_INTERNAL_H_
+#include "bem3_aw_qd2.h"
+
+/* element constants cr, cw and quadrature tables (model.c) */
+void aqd2_element_constants(BOUD *bd);
+void aqd2_quadrature_tables(BOUD *bd);
+/* position and unnormalised normal at (z, e) of element s > 0 */
+void aqd2_rw(const BOUD *bd, int s, double z, double e, double *r, double *w);
+/* release the GPU-side state attached to qd (device.c) */
+void aqd2_release_device(AQD2 *qd);
+
+#endif
