@@ -43,11 +43,24 @@ class GfParams(C.Structure):
                [("nc", C.c_int), ("ewald_only", C.c_int)]
 
 
+class AssemblyStats(C.Structure):
+    _fields_ = [("pairs_gl", C.c_longlong), ("pairs_gh", C.c_longlong), ("self_rows", C.c_longlong),
+                ("gf_evals", C.c_longlong), ("gf_terms", C.c_longlong),
+                ("ms_decide", C.c_double), ("ms_pairs", C.c_double), ("ms_self", C.c_double)]
+
+
 def hip_lib():
     global _hip
     if _hip is None:
         lib = _load("libbqd2_hip.so")
         lib.bqd2_last_error.restype = C.c_char_p
+        lib.bqd2_assemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(AssemblyStats)]
+        lib.bqd2_download_domain.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
+        lib.bqd2_solve_surface.argtypes = [C.c_void_p, C.c_void_p]
+        lib.bqd2_solve_velocity.argtypes = [C.c_void_p]
+        lib.bqd2_download_solution.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        lib.bqd2_zgetrf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.bqd2_zgetrs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.bqd2_gf_params.argtypes = [C.POINTER(C.c_double), C.POINTER(GfParams)]
         lib.bqd2_gf_batch.argtypes = [C.POINTER(GfParams), C.c_int, C.c_int, C.c_void_p, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -59,7 +72,10 @@ def host_lib():
     global _host
     if _host is None:
         hip_lib()
-        _host = _load("libbem3_aw_qd2.so")
+        lib = _load("libbem3_aw_qd2.so")
+        lib.aqd2_problem.restype = C.c_void_p
+        lib.aqd2_problem.argtypes = [C.c_void_p, C.c_void_p]
+        _host = lib
     return _host
 
 

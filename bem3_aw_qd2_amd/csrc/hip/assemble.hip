@@ -1,0 +1,585 @@
+// assemble.hip -- coefficient matrices of one domain (bilinear elements).
+//
+// The reference fills one row (target node t,tn) at a time and, inside it, one
+// source element s at a time (bem3_aw_qd2_solve_bieq.c:157-199), calling the
+// Green's function once per quadrature point and shape function.  Here:
+//   k_decide      one lane per (row, s): the GL12 dF/dv test that picks GL12 or
+//                 GH64 (d3qd2_elem.c:536-537, :851-852), bit-exact;
+//   k_q0_pairs    one wavefront per (row, s) pair of the periodic domain: the
+//                 chosen Gauss-Legendre rule spread over 64 lanes, one
+//                 Green's function evaluation per point shared by the four
+//                 shape functions, cross-lane reduction
+//                 (q0_bil_dcoef_v2_cc_GL/GH, q0_bil_dcoef_v2_cc.c:129-281);
+//   k_fs_pairs    the same with the free-space kernel and tangential
+//                 derivatives (bil_dcoef_v2_GL/GH, bil_dcoef_v2.c:141-325);
+//   k_q0_self,    one wavefront per row for the singular self element: polar
+//   k_fs_self     rules about the node (q0_bil_coef.c:246-338, bil_coef.c:244-342)
+//                 and the principal-value derivatives (bil_dcoef.c:369-740);
+//   k_rows        per-row sum of F in source order and the 1+F / F diagonal.
+#include <cstring>
+#include <vector>
+#include "elem_dev.h"
+
+#define WAVE 64
+
+// row -> (t, tn); returns false for the unused fourth slot of a triangle
+__device__ __forceinline__ bool row_target(const DevMesh &M, const DevDomain &D, int row, int &t, int &tn, int &td) {
+  t = row / 4 + 1;
+  tn = row % 4;
+  td = D.sid[t];
+  return !(M.tri[abs(td)] && tn == 3);
+}
+
+__global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)D.N * D.Ne) return;
+  int row = (int)(gid / D.Ne), s = (int)(gid % D.Ne) + 1;
+  int t, tn, td;
+  unsigned char rule = 0;
+  if (row_target(M, D, row, t, tn, td) && s != t) {
+    const Tab &T = *tab;
+    int at = abs(td), as = abs(D.sid[s]);
+    double rt[3] = {M.ren[12 * at + 3 * tn], M.ren[12 * at + 3 * tn + 1], M.ren[12 * at + 3 * tn + 2]};
+    double vtz[3], vte[3];
+    node_tangents(M, T, at, tn, vtz, vte);
+    ElemC E;
+    load_elem(M, as, E);
+    rule = needs_gh(E, T, rt, vtz, vte) ? 1 : 0;
+  }
+  D.rule[gid] = rule;
+}
+
+// ------------------------------------------------------------------ periodic pairs
+__global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                   long long *__restrict__ counters) {
+  const int s = blockIdx.x + 1, row = blockIdx.y, lane = threadIdx.x;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td) || s == t) return;
+  const Tab &T = *tab;
+  const int at = abs(td), as = D.sid[s];  // periodic domain: source ids are positive
+  const bool gh = D.rule[(long long)row * D.Ne + (s - 1)] != 0;
+  const int n = gh ? 64 : 12;
+  const double *x = gh ? T.xhi : T.xli;
+  const double *w = gh ? T.whi : T.wli;
+  double rt[3] = {M.ren[12 * at + 3 * tn], M.ren[12 * at + 3 * tn + 1], M.ren[12 * at + 3 * tn + 2]};
+  double vtz[3], vte[3];
+  node_tangents(M, T, at, tn, vtz, vte);
+  ElemC E;
+  load_elem(M, as, E);
+
+  cd g[4] = {}, h[4] = {};
+  double F = 0.0, F0 = 0.0, F1 = 0.0;
+  long long terms = 0;
+  int err = 0;
+  const int npts = n * n;
+  for (int p = lane; p < npts; p += WAVE) {
+    const int i = p / n, j = p - i * n;
+    const double z = x[i], e = x[j], wt = w[i] * w[j];
+    PtGeo q;
+    point_geo(E, z, e, rt, vtz, vte, q);
+    Sum10 gf;
+    int c = qpgf_eval<GF_V1>(P, q.vR[0], q.vR[1], q.vR[2], 1.0e-15, gf);
+    if (c < 0) err = c;
+    else terms += c;
+    cd hn = gf.dg[0] * q.vW[0] + gf.dg[1] * q.vW[1] + gf.dg[2] * q.vW[2];
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      double Np = bil_N(ep, z, e);
+      g[ep] += gf.g * (wt * Np * q.aW);
+      h[ep] += hn * (wt * Np);
+    }
+    F += wt * q.rdW * q.iR * q.iR;
+    F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
+    F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
+  }
+#pragma unroll
+  for (int ep = 0; ep < 4; ep++) {
+    g[ep].x = wave_sum(g[ep].x); g[ep].y = wave_sum(g[ep].y);
+    h[ep].x = wave_sum(h[ep].x); h[ep].y = wave_sum(h[ep].y);
+  }
+  F = wave_sum(F); F0 = wave_sum(F0); F1 = wave_sum(F1);
+  terms = wave_sum_ll(terms);
+  int anyerr = __any(err != 0);
+  if (lane == 0) {
+    const size_t base = ((size_t)row * D.N + (size_t)(s - 1) * 4) * 2;
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      D.G[base + 2 * ep] = g[ep].x; D.G[base + 2 * ep + 1] = g[ep].y;
+      D.H[base + 2 * ep] = h[ep].x; D.H[base + 2 * ep + 1] = h[ep].y;
+    }
+    double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
+    fp[0] = F * BQD2_IFP; fp[1] = F0 * BQD2_IFP; fp[2] = F1 * BQD2_IFP;
+    atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
+    atomicAdd((unsigned long long *)&counters[1], (unsigned long long)npts);
+    if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+  }
+}
+
+// ------------------------------------------------------------------ free-space pairs
+__global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  const int s = blockIdx.x + 1, row = blockIdx.y, lane = threadIdx.x;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td) || s == t) return;
+  const Tab &T = *tab;
+  const int at = abs(td), sd = D.sid[s], as = abs(sd);
+  const bool gh = D.rule[(long long)row * D.Ne + (s - 1)] != 0;
+  const int n = gh ? 64 : 12;
+  const double *x = gh ? T.xhi : T.xli;
+  const double *w = gh ? T.whi : T.wli;
+  const double k = D.k;
+  double rt[3] = {M.ren[12 * at + 3 * tn], M.ren[12 * at + 3 * tn + 1], M.ren[12 * at + 3 * tn + 2]};
+  double vtz[3], vte[3];
+  node_tangents(M, T, at, tn, vtz, vte);
+  ElemC E;
+  load_elem(M, as, E);
+
+  cd g[4] = {}, h[4] = {}, dg0[4] = {}, dg1[4] = {}, dh0[4] = {}, dh1[4] = {};
+  double F = 0.0, F0 = 0.0, F1 = 0.0;
+  const int npts = n * n;
+  for (int p = lane; p < npts; p += WAVE) {
+    const int i = p / n, j = p - i * n;
+    const double z = x[i], e = x[j], wt = w[i] * w[j];
+    PtGeo q;
+    point_geo(E, z, e, rt, vtz, vte, q);
+    cd ce = cis(k * q.aR);
+    cd cb = cd{-1.0, k * q.aR};            // i k R - 1
+    double tD = q.rdW * q.iR * q.iR;
+    double kk = k * k * q.aR * q.aR;
+    cd cg = ce * (q.iR * q.aW);                          // e^{ikR}/R |W|
+    cd chh = cb * ce * tD;                                // (ikR-1) e^{ikR} R.W/R^3
+    cd cdg = cb * ce * (q.iR * q.iR * q.aW);              // (ikR-1) e^{ikR}/R^2 |W|
+    cd c3 = cd{kk + 3.0 * cb.x, 3.0 * cb.y};               // k^2R^2 + 3(ikR-1)
+    cd cdh0 = ce * (q.iR * q.iR * q.iR) * (c3 * (q.rdv0 * q.rdW) - cb * q.Wdv0);
+    cd cdh1 = ce * (q.iR * q.iR * q.iR) * (c3 * (q.rdv1 * q.rdW) - cb * q.Wdv1);
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      double Nw = wt * bil_N(ep, z, e);
+      g[ep] += cg * Nw;
+      h[ep] += chh * Nw;
+      dg0[ep] += cdg * (Nw * q.rdv0);
+      dg1[ep] += cdg * (Nw * q.rdv1);
+      dh0[ep] += cdh0 * Nw;
+      dh1[ep] += cdh1 * Nw;
+    }
+    F += wt * tD;
+    F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
+    F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
+  }
+  // scaling (bil_dcoef_v2.c:207-218) and the sign of the sub side (:220-224)
+  const double sg = sd < 0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int ep = 0; ep < 4; ep++) {
+    g[ep] = cd{wave_sum(g[ep].x), wave_sum(g[ep].y)} * BQD2_IFP;
+    h[ep] = cd{wave_sum(h[ep].x), wave_sum(h[ep].y)} * (BQD2_IFP * sg);
+    dg0[ep] = cd{wave_sum(dg0[ep].x), wave_sum(dg0[ep].y)} * (-BQD2_IFP);
+    dg1[ep] = cd{wave_sum(dg1[ep].x), wave_sum(dg1[ep].y)} * (-BQD2_IFP);
+    dh0[ep] = cd{wave_sum(dh0[ep].x), wave_sum(dh0[ep].y)} * (BQD2_IFP * sg);
+    dh1[ep] = cd{wave_sum(dh1[ep].x), wave_sum(dh1[ep].y)} * (BQD2_IFP * sg);
+  }
+  F = wave_sum(F) * BQD2_IFP * sg;
+  F0 = wave_sum(F0) * BQD2_IFP * sg;
+  F1 = wave_sum(F1) * BQD2_IFP * sg;
+  if (lane == 0) {
+    const size_t N = D.N, col = (size_t)(s - 1) * 4;
+    const size_t base = ((size_t)row * N + col) * 2;
+    const size_t b0 = ((size_t)(2 * row) * N + col) * 2, b1 = ((size_t)(2 * row + 1) * N + col) * 2;
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      D.G[base + 2 * ep] = g[ep].x; D.G[base + 2 * ep + 1] = g[ep].y;
+      D.H[base + 2 * ep] = h[ep].x; D.H[base + 2 * ep + 1] = h[ep].y;
+      D.dG[b0 + 2 * ep] = dg0[ep].x; D.dG[b0 + 2 * ep + 1] = dg0[ep].y;
+      D.dG[b1 + 2 * ep] = dg1[ep].x; D.dG[b1 + 2 * ep + 1] = dg1[ep].y;
+      D.dH[b0 + 2 * ep] = dh0[ep].x; D.dH[b0 + 2 * ep + 1] = dh0[ep].y;
+      D.dH[b1 + 2 * ep] = dh1[ep].x; D.dH[b1 + 2 * ep + 1] = dh1[ep].y;
+    }
+    double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
+    fp[0] = F; fp[1] = F0; fp[2] = F1;
+  }
+}
+
+// ------------------------------------------------------------------ self elements
+// static F of the self element at (zt, et): polar rule, wave-parallel over the
+// 4 x 128 x 12 points (bil_coef.c:525-536 integrand, q0_bil_coef.c:315-337 rule)
+__device__ double self_F_wave(const ElemC &E, const Tab &T, double zt, double et, int lane) {
+  if (bil_is_plane(E)) return 0.0;
+  double r[5], th[5];
+  bil_corners(zt, et, r, th);
+  const double bdC = E.cr[0][1] * E.cw[0][2] + E.cr[1][1] * E.cw[1][2] + E.cr[2][1] * E.cw[2][2];
+  double acc = 0.0;
+  // point index p = ((j*64 + m)*2 + h)*12 + i
+  for (int p = lane; p < 4 * 64 * 2 * 12; p += WAVE) {
+    int i = p % 12, q = p / 12, h = q % 2, m = (q / 2) % 64, j = q / 128;
+    double t = theta_node(th, j, h, T.xhi[m]);
+    double rm = polar_rm(r, th, j, t);
+    double c = cos(t), sn = sin(t);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0), K[3];
+    polar_K(E, c, sn, rr, zt, et, K);
+    double aK = abs3(K);
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i];
+    acc += wgt * (1.0 / (aK * aK * aK) * c * sn);
+  }
+  return wave_sum(acc) * BQD2_IFP * bdC;
+}
+
+__global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                  long long *__restrict__ counters) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td)) return;
+  const Tab &T = *tab;
+  const int s = abs(td);
+  ElemC E;
+  load_elem(M, s, E);
+  const double zt = T.zt_44[tn], et = T.et_44[tn];
+  double r[5], th[5];
+  bil_corners(zt, et, r, th);
+  cd g[4] = {}, h[4] = {};
+  long long terms = 0;
+  int err = 0;
+  const int npts = 4 * 64 * 2 * 12;
+  for (int p = lane; p < npts; p += WAVE) {
+    int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0);
+    double z = rr * c + zt, e = rr * sn + et, vR[3], vW[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      vR[l] = rr * (E.cr[l][1] * c + E.cr[l][2] * sn + E.cr[l][3] * (rr * c * sn + zt * sn + et * c));
+      vW[l] = E.cw[l][0] + E.cw[l][1] * z + E.cw[l][2] * e;
+    }
+    double aW = abs3(vW);
+    Sum10 gf;
+    int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+    if (cnt < 0) err = cnt;
+    else terms += cnt;
+    cd hn = gf.dg[0] * vW[0] + gf.dg[1] * vW[1] + gf.dg[2] * vW[2];
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i] * rr;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      double Nn = bil_N(n, z, e) * wgt;
+      g[n] += gf.g * (Nn * aW);
+      h[n] += hn * Nn;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    g[n] = cd{wave_sum(g[n].x), wave_sum(g[n].y)};
+    h[n] = cd{wave_sum(h[n].x), wave_sum(h[n].y)};
+  }
+  terms = wave_sum_ll(terms);
+  int anyerr = __any(err != 0);
+  const double F = self_F_wave(E, T, zt, et, lane);
+  // central difference of F along the element parameters (q0_bil_dcoef_v2_cc.c:336-390)
+  double Tz[3], Te[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    Tz[i] = E.cr[i][1] + E.cr[i][3] * et;
+    Te[i] = E.cr[i][2] + E.cr[i][3] * zt;
+  }
+  const double h_ = 1.0e-3;
+  double Fzp = self_F_wave(E, T, zt + h_, et, lane), Fzm = self_F_wave(E, T, zt - h_, et, lane);
+  double Fep = self_F_wave(E, T, zt, et + h_, lane), Fem = self_F_wave(E, T, zt, et - h_, lane);
+  if (lane == 0) {
+    const size_t base = ((size_t)row * D.N + (size_t)(t - 1) * 4) * 2;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
+      D.H[base + 2 * n] = h[n].x; D.H[base + 2 * n + 1] = h[n].y;
+    }
+    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    fp[0] = F;
+    fp[1] = 1.0 / abs3(Tz) * (Fzp - Fzm) / (2.0 * h_);
+    fp[2] = 1.0 / abs3(Te) * (Fep - Fem) / (2.0 * h_);
+    atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
+    atomicAdd((unsigned long long *)&counters[1], (unsigned long long)npts);
+    if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+  }
+}
+
+// free-space self element: G, H, F (bil_coef_bd) and the principal-value
+// tangential derivatives dG, dH, dF along t_zeta and t_eta (bil_dcoef.c:369-740)
+__global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td)) return;
+  const Tab &T = *tab;
+  const int s = abs(td);
+  const double sg = td < 0 ? -1.0 : 1.0;
+  const double k = D.k;
+  ElemC E;
+  load_elem(M, s, E);
+  const double zt = T.zt_44[tn], et = T.et_44[tn];
+  double r[5], th[5];
+  bil_corners(zt, et, r, th);
+  const bool plane = bil_is_plane(E);
+  const double bdC = E.cr[0][1] * E.cw[0][2] + E.cr[1][1] * E.cw[1][2] + E.cr[2][1] * E.cw[2][2];
+  // unnormalised tangents T_zeta, T_eta at the node
+  double Tv[2][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    Tv[0][i] = E.cr[i][1] + E.cr[i][3] * et;
+    Tv[1][i] = E.cr[i][2] + E.cr[i][3] * zt;
+  }
+
+  // ---- G, H on the polar rule (12 points in r)
+  cd g[4] = {}, h[4] = {};
+  for (int p = lane; p < 4 * 64 * 2 * 12; p += WAVE) {
+    int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0);
+    double z = rr * c + zt, e = rr * sn + et, K[3], W[3];
+    polar_K(E, c, sn, rr, zt, et, K);
+#pragma unroll
+    for (int l = 0; l < 3; l++) W[l] = E.cw[l][0] + E.cw[l][1] * z + E.cw[l][2] * e;
+    double aK = abs3(K), aW = abs3(W);
+    cd ce = cis(k * aK * rr);
+    cd cb = cd{-1.0, k * rr * aK};
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i];
+    cd cg = ce * (aW / aK * wgt);
+    cd chh = cb * ce * (1.0 / (aK * aK * aK) * c * sn * wgt);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      double Nn = bil_N(n, z, e);
+      g[n] += cg * Nn;
+      h[n] += chh * Nn;
+    }
+  }
+
+  // ---- principal-value derivatives, both tangent directions at once
+  cd dG[2][4] = {}, dH[2][4] = {};
+  double dFv[2] = {0.0, 0.0};
+  // D_bar(eps): 4 sub-triangles x 128 theta x 64 r on [DEPS, rm]
+  for (int p = lane; p < 4 * 64 * 2 * 64; p += WAVE) {
+    int i = p % 64, q = p / 64, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * (rm - 0.1) * T.xhi[i] + 0.5 * (rm + 0.1);
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * (rm - 0.1) * T.whi[i];
+    double z = rr * c + zt, e = rr * sn + et, K[3], W[3];
+    polar_K(E, c, sn, rr, zt, et, K);
+#pragma unroll
+    for (int l = 0; l < 3; l++) W[l] = E.cw[l][0] + E.cw[l][1] * z + E.cw[l][2] * e;
+    double aK = abs3(K), iK = 1.0 / aK, aW = abs3(W);
+    double kR = k * rr * aK;
+    cd ce = cis(kR), cb = cd{-1.0, kR};
+    cd c3 = cd{kR * kR + 3.0 * cb.x, 3.0 * cb.y};
+    double aK2 = aK * aK, iK5 = 1.0 / (aK2 * aK2 * aK);
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+      double KdT = dot3(K, Tv[dir]);
+      double sel = dir == 0 ? c : sn;    // inner factor: cos for zeta, sin for eta
+      double outer = dir == 0 ? sn : c;  // outer factor of the H and F kernels
+      cd cg = cb * ce * (iK * iK / rr * (KdT * iK) * aW * wgt);
+      cd chh = ce * (iK5 / rr) * (c3 * (KdT * sel) - cb * aK2) * (outer * wgt);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        double Nn = bil_N(n, z, e);
+        dG[dir][n] += cg * Nn;
+        dH[dir][n] += chh * Nn;
+      }
+      dFv[dir] += outer * wgt * (iK5 / rr * (3.0 * KdT * sel - aK2));
+    }
+  }
+  // D_eps: theta in four quarters of (-pi, pi], r in [0, DEPS] (GL12)
+  for (int p = lane; p < 4 * 64 * 12; p += WAVE) {
+    int i = p % 12, q = p / 12, qq = q % 4, m = q / 4;
+    const double off = -0.75 + 0.5 * qq;
+    double tt = 0.25 * M_PI * T.xhi[m] + off * M_PI;
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * 0.1 * (T.xli[i] + 1.0);
+    double wgt = 0.25 * M_PI * T.whi[m] * 0.5 * 0.1 * T.wli[i];
+    double z = rr * c + zt, e = rr * sn + et, K[3], W[3];
+    polar_K(E, c, sn, rr, zt, et, K);
+#pragma unroll
+    for (int l = 0; l < 3; l++) W[l] = E.cw[l][0] + E.cw[l][1] * z + E.cw[l][2] * e;
+    double aK = abs3(K), iK = 1.0 / aK, aW = abs3(W);
+    double kR = k * aK * rr;
+    double cc, cs;
+    sincos(kR, &cs, &cc);
+    cd ce = cd{cc, cs};
+    cd cb = cd{-1.0, kR};
+    double aK2 = aK * aK;
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+      double KdT = dot3(K, Tv[dir]);
+      double sel = dir == 0 ? c : sn, outer = dir == 0 ? sn : c;
+      // i * [ (cb sin kR + kR cos kR) / (K^2 r) (K.T/|K|) |W| ]
+      cd cg0 = cd{cb.x * cs + kR * cc, cb.y * cs} * (iK * iK / rr * (KdT * iK) * aW * wgt);
+      cd cg = mul_i(cg0);
+      // De0 - i De1
+      cd cbk = cd{0.0, k};
+      cd h0 = ce * (1.0 / (aK2 * aK2)) * (cd{k * kR, 0.0} * (KdT * sel) + cbk * (3.0 * KdT * sel) - cbk * aK2);
+      double h1 = cs / (aK2 * aK2 * aK * rr) * (3.0 * KdT * sel - aK2);
+      cd chh = (h0 - mul_i(cd{h1, 0.0})) * (outer * wgt);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        double Nn = bil_N(n, z, e);
+        dG[dir][n] += cg * Nn;
+        dH[dir][n] += chh * Nn;
+      }
+    }
+  }
+  // D_eps principal-value pairs (+r, -r): theta in (0, pi], r in [0, DEPS]
+  for (int p = lane; p < 2 * 64 * 12; p += WAVE) {
+    int i = p % 12, q = p / 12, qq = q % 2, m = q / 2;
+    const double off = qq == 0 ? 0.25 : 0.75;
+    double tt = 0.25 * M_PI * T.xhi[m] + off * M_PI;
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * 0.1 * (T.xli[i] + 1.0);
+    double wgt = 0.25 * M_PI * T.whi[m] * 0.5 * 0.1 * T.wli[i];
+    double zp = rr * c + zt, ep = rr * sn + et, zm = -rr * c + zt, em = -rr * sn + et;
+    double Kp[3], Km[3], Wp[3], Wm[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      Kp[l] = E.cr[l][1] * c + E.cr[l][2] * sn + E.cr[l][3] * (rr * c * sn + zt * sn + et * c);
+      Km[l] = E.cr[l][1] * c + E.cr[l][2] * sn + E.cr[l][3] * (-rr * c * sn + zt * sn + et * c);
+      Wp[l] = E.cw[l][0] + E.cw[l][1] * zp + E.cw[l][2] * ep;
+      Wm[l] = E.cw[l][0] + E.cw[l][1] * zm + E.cw[l][2] * em;
+    }
+    double aKp = abs3(Kp), aKm = abs3(Km), iKp = 1.0 / aKp, iKm = 1.0 / aKm, aWp = abs3(Wp), aWm = abs3(Wm);
+    double ccp = cos(k * aKp * rr), ccm = cos(-k * aKm * rr);
+    double iKp5 = 1.0 / (aKp * aKp * aKp * aKp * aKp), iKm5 = 1.0 / (aKm * aKm * aKm * aKm * aKm);
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+      double KpT = dot3(Kp, Tv[dir]), KmT = dot3(Km, Tv[dir]);
+      double sel = dir == 0 ? c : sn, outer = dir == 0 ? sn : c;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        double Np = bil_N(n, zp, ep), Nm = bil_N(n, zm, em);
+        double vg = 1.0 / rr * (ccp * iKp * iKp * (KpT * iKp) * Np * aWp - ccm * iKm * iKm * (KmT * iKm) * Nm * aWm);
+        double vh = 1.0 / rr * (ccp * iKp5 * (3.0 * KpT * sel - aKp * aKp) * Np - ccm * iKm5 * (3.0 * KmT * sel - aKm * aKm) * Nm);
+        dG[dir][n] -= cd{vg * wgt, 0.0};
+        dH[dir][n] -= cd{vh * outer * wgt, 0.0};
+      }
+      dFv[dir] += outer * wgt * (1.0 / rr * (iKp5 * (3.0 * KpT * sel - aKp * aKp) - iKm5 * (3.0 * KmT * sel - aKm * aKm)));
+    }
+  }
+
+  const double F = self_F_wave(E, T, zt, et, lane);
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    g[n] = cd{wave_sum(g[n].x), wave_sum(g[n].y)} * BQD2_IFP;
+    h[n] = plane ? cd{0.0, 0.0} : cd{wave_sum(h[n].x), wave_sum(h[n].y)} * (BQD2_IFP * bdC * sg);
+  }
+  double iT[2] = {1.0 / abs3(Tv[0]), 1.0 / abs3(Tv[1])};
+#pragma unroll
+  for (int dir = 0; dir < 2; dir++) {
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      dG[dir][n] = cd{wave_sum(dG[dir][n].x), wave_sum(dG[dir][n].y)} * (-BQD2_IFP * iT[dir]);
+      dH[dir][n] = plane ? cd{0.0, 0.0}
+                         : cd{wave_sum(dH[dir][n].x), wave_sum(dH[dir][n].y)} * (bdC * BQD2_IFP * iT[dir] * sg);
+    }
+    dFv[dir] = plane ? 0.0 : wave_sum(dFv[dir]) * (bdC * BQD2_IFP * iT[dir] * sg);
+  }
+  if (lane == 0) {
+    const size_t N = D.N, col = (size_t)(t - 1) * 4;
+    const size_t base = ((size_t)row * N + col) * 2;
+    const size_t b0 = ((size_t)(2 * row) * N + col) * 2, b1 = ((size_t)(2 * row + 1) * N + col) * 2;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
+      D.H[base + 2 * n] = h[n].x; D.H[base + 2 * n + 1] = h[n].y;
+      D.dG[b0 + 2 * n] = dG[0][n].x; D.dG[b0 + 2 * n + 1] = dG[0][n].y;
+      D.dG[b1 + 2 * n] = dG[1][n].x; D.dG[b1 + 2 * n + 1] = dG[1][n].y;
+      D.dH[b0 + 2 * n] = dH[0][n].x; D.dH[b0 + 2 * n + 1] = dH[0][n].y;
+      D.dH[b1 + 2 * n] = dH[1][n].x; D.dH[b1 + 2 * n + 1] = dH[1][n].y;
+    }
+    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    fp[0] = F * sg; fp[1] = dFv[0]; fp[2] = dFv[1];
+  }
+}
+
+// per-row F sums in source order and the diagonal term (bem3_aw_qd2_solve_bieq.c:185-198)
+__global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= D.N) return;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td)) {
+    D.dF[3 * row] = D.dF[3 * row + 1] = D.dF[3 * row + 2] = 0.0;
+    return;
+  }
+  double F = 0.0, Fz = 0.0, Fe = 0.0;
+  const double *fp = D.Fpair + (size_t)row * D.Ne * 3;
+  for (int s = 0; s < D.Ne; s++) {
+    F += fp[3 * s];
+    Fz += fp[3 * s + 1];
+    Fe += fp[3 * s + 2];
+  }
+  double diag = D.did == 0 ? 1.0 + F : F;
+  D.H[((size_t)row * D.N + row) * 2] += diag;
+  D.dF[3 * row] = diag;
+  D.dF[3 * row + 1] = Fz;
+  D.dF[3 * row + 2] = Fe;
+}
+
+// ------------------------------------------------------------------ entry point
+extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_assemble: bad domain %d", did);
+  DevDomain &D = p->dom[did];
+  if (D.Ne == 0) return BQD2_OK;
+  const bool periodic = (did == 0 && p->m.ps == 1);
+  hipStream_t st_ = p->stream;
+  hipEvent_t e0, e1, e2, e3;
+  BQD2_HIP(hipEventCreate(&e0));
+  BQD2_HIP(hipEventCreate(&e1));
+  BQD2_HIP(hipEventCreate(&e2));
+  BQD2_HIP(hipEventCreate(&e3));
+  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st_));
+  BQD2_HIP(hipMemsetAsync(D.G, 0, (size_t)D.N * D.N * 16, st_));
+  BQD2_HIP(hipMemsetAsync(D.H, 0, (size_t)D.N * D.N * 16, st_));
+  if (D.dG) {
+    BQD2_HIP(hipMemsetAsync(D.dG, 0, (size_t)2 * D.N * D.N * 16, st_));
+    BQD2_HIP(hipMemsetAsync(D.dH, 0, (size_t)2 * D.N * D.N * 16, st_));
+  }
+  BQD2_HIP(hipMemsetAsync(D.Fpair, 0, (size_t)D.N * D.Ne * 3 * 8, st_));
+  BQD2_HIP(hipEventRecord(e0, st_));
+  long long npairs = (long long)D.N * D.Ne;
+  hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, st_, p->m, p->tab, D);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipEventRecord(e1, st_));
+  if (periodic) {
+    hipLaunchKernelGGL(k_q0_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
+  } else {
+    hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+  }
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipEventRecord(e2, st_));
+  if (periodic) hipLaunchKernelGGL(k_q0_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
+  else hipLaunchKernelGGL(k_fs_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+  BQD2_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_rows, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st_, p->m, D);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipEventRecord(e3, st_));
+  BQD2_HIP(hipEventSynchronize(e3));
+  long long cnt[4];
+  BQD2_HIP(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  std::vector<unsigned char> rule((size_t)npairs);
+  BQD2_HIP(hipMemcpy(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost));
+  long long ngh = 0;
+  for (unsigned char r : rule) ngh += r;
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    float a, b, c;
+    BQD2_HIP(hipEventElapsedTime(&a, e0, e1));
+    BQD2_HIP(hipEventElapsedTime(&b, e1, e2));
+    BQD2_HIP(hipEventElapsedTime(&c, e2, e3));
+    st->ms_decide = a; st->ms_pairs = b; st->ms_self = c;
+    st->gf_terms = cnt[0];
+    st->gf_evals = cnt[1];
+    st->pairs_gh = ngh;
+    long long valid_rows = 0;
+    for (int row = 0; row < D.N; row++) valid_rows += 1;  // quads only: every slot is a node
+    st->self_rows = valid_rows;
+    st->pairs_gl = valid_rows * (D.Ne - 1) - ngh;
+  }
+  hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
+  BQD2_CHECK(cnt[2] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)", cnt[2], did);
+  D.assembled = true;
+  return BQD2_OK;
+}

@@ -1,0 +1,163 @@
+// elem_dev.h -- device element geometry, interpolation functions and the
+// element-rule switch for bilinear (quadrilateral) elements.
+//
+// Element conventions of the reference (d3qd2_elem.c:903-985): r(z,e) =
+// cr0 + cr1 z + cr2 e + cr3 z e, unnormalised normal w(z,e) = cw0 + cw1 z + cw2 e,
+// interpolation functions N_n on the collocation nodes at (+-1/sqrt3, +-1/sqrt3).
+#pragma once
+#include "problem.h"
+
+#pragma clang fp contract(off)
+
+#define BQD2_IFP 7.957747154594766788444e-2  // 1/(4 pi)
+#define BQD2_P44 0.5773502691896257645091    // 1/sqrt(3)
+
+struct ElemC {
+  double cr[3][4], cw[3][3];
+};
+
+__device__ __forceinline__ void load_elem(const DevMesh &M, int s, ElemC &E) {
+  const double *c = M.cr + 12 * s;
+  const double *w = M.cw + 9 * s;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) E.cr[d][j] = c[4 * d + j];
+#pragma unroll
+    for (int j = 0; j < 3; j++) E.cw[d][j] = w[3 * d + j];
+  }
+}
+
+// d3qd2_elem.c:903-915
+__device__ __forceinline__ double bil_N(int n, double z, double e) {
+  const double ia = 1.0 / BQD2_P44;
+  double a = (n == 1 || n == 2) ? 1.0 + z * ia : 1.0 - z * ia;
+  double b = (n >= 2) ? 1.0 + e * ia : 1.0 - e * ia;
+  return 0.25 * a * b;
+}
+
+__device__ __forceinline__ void bil_rw(const ElemC &E, double z, double e, double *r, double *w) {
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    r[i] = E.cr[i][0] + E.cr[i][1] * z + E.cr[i][2] * e + E.cr[i][3] * z * e;
+    w[i] = E.cw[i][0] + E.cw[i][1] * z + E.cw[i][2] * e;
+  }
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ double abs3(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+// unit tangents at a collocation node (d3qd2_elem.c:122-143)
+__device__ __forceinline__ void node_tangents(const DevMesh &M, const Tab &T, int s, int nid, double *vtz, double *vte) {
+  const double *c = M.cr + 12 * s;
+  if (!M.tri[s]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      vtz[i] = c[4 * i + 1] + c[4 * i + 3] * T.et_44[nid];
+      vte[i] = c[4 * i + 2] + c[4 * i + 3] * T.zt_44[nid];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      vtz[i] = c[4 * i + 1];
+      vte[i] = c[4 * i + 2];
+    }
+  }
+  double a = abs3(vtz);
+  if (a != 0.0) { vtz[0] /= a; vtz[1] /= a; vtz[2] /= a; }
+  a = abs3(vte);
+  if (a != 0.0) { vte[0] /= a; vte[1] /= a; vte[2] /= a; }
+}
+
+// geometry of one quadrature point shared by both kernels
+struct PtGeo {
+  double vR[3], vW[3], aR, iR, aW, rdW, rdv0, rdv1, Wdv0, Wdv1;
+};
+
+__device__ __forceinline__ void point_geo(const ElemC &E, double z, double e, const double *rt, const double *v0,
+                                          const double *v1, PtGeo &g) {
+  bil_rw(E, z, e, g.vR, g.vW);
+  g.vR[0] -= rt[0];
+  g.vR[1] -= rt[1];
+  g.vR[2] -= rt[2];
+  g.aR = abs3(g.vR);
+  g.iR = 1.0 / g.aR;
+  g.aW = abs3(g.vW);
+  g.rdW = dot3(g.vR, g.vW) * g.iR;
+  g.rdv0 = dot3(g.vR, v0) * g.iR;
+  g.rdv1 = dot3(g.vR, v1) * g.iR;
+  g.Wdv0 = dot3(g.vW, v0);
+  g.Wdv1 = dot3(g.vW, v1);
+}
+
+// GL12 dF/dv0, dF/dv1 with the reference's exact operation order and summation
+// order (inner j sum, times w_i, then 1/(4 pi)), the quantities that
+// d3qd2_elem.c:536-537 / :851-852 compare with CBD_SDF = 0.5.  Products and sums
+// are not contracted in this file, so the values match the host reference bit
+// for bit and the GL12 / GH64 choice is the same pair by pair.
+__device__ __forceinline__ bool needs_gh(const ElemC &E, const Tab &T, const double *rt, const double *v0,
+                                         const double *v1) {
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = 0; i < 12; i++) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int j = 0; j < 12; j++) {
+      PtGeo g;
+      point_geo(E, T.xli[i], T.xli[j], rt, v0, v1, g);
+      t0 += T.wli[j] * g.iR * g.iR * g.iR * (3.0 * g.rdv0 * g.rdW - g.Wdv0);
+      t1 += T.wli[j] * g.iR * g.iR * g.iR * (3.0 * g.rdv1 * g.rdW - g.Wdv1);
+    }
+    a0 += T.wli[i] * t0;
+    a1 += T.wli[i] * t1;
+  }
+  double dF0 = a0 * BQD2_IFP, dF1 = a1 * BQD2_IFP;
+  return fabs(dF0) > 0.5 || fabs(dF1) > 0.5;
+}
+
+// wavefront-wide sum (64 lanes) of a double
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// corner distances / angles seen from (zt, et) (d3qd2_elem.c:1017-1034)
+__device__ __forceinline__ void bil_corners(double zt, double et, double *r, double *th) {
+  double zp = 1.0 - zt, zm = -1.0 - zt, ep = 1.0 - et, em = -1.0 - et;
+  r[0] = sqrt(zm * zm + em * em);
+  r[1] = sqrt(zp * zp + em * em);
+  r[2] = sqrt(zp * zp + ep * ep);
+  r[3] = sqrt(zm * zm + ep * ep);
+  r[4] = r[0];
+  th[0] = atan2(em, zm);
+  th[1] = atan2(em, zp);
+  th[2] = atan2(ep, zp);
+  th[3] = atan2(ep, zm);
+  th[4] = th[0] + 2.0 * M_PI;
+}
+
+__device__ __forceinline__ bool bil_is_plane(const ElemC &E) {
+  double bdc = E.cr[0][1] * E.cw[0][2] + E.cr[1][1] * E.cw[1][2] + E.cr[2][1] * E.cw[2][2];
+  double ab = sqrt(E.cr[0][1] * E.cr[0][1] + E.cr[1][1] * E.cr[1][1] + E.cr[2][1] * E.cr[2][1]);
+  double aC = sqrt(E.cw[0][2] * E.cw[0][2] + E.cw[1][2] * E.cw[1][2] + E.cw[2][2] * E.cw[2][2]);
+  return fabs(bdc / (ab * aC)) < 1.0e-15;
+}
+
+__device__ __forceinline__ void polar_K(const ElemC &E, double c, double s, double r, double zt, double et, double *K) {
+#pragma unroll
+  for (int i = 0; i < 3; i++) K[i] = E.cr[i][1] * c + E.cr[i][2] * s + E.cr[i][3] * (r * c * s + zt * s + et * c);
+}
+
+// theta node of sub-triangle j, half h, Legendre abscissa x (q0_bil_coef.c:286-300)
+__device__ __forceinline__ double theta_node(const double *th, int j, int h, double x) {
+  return h == 0 ? 0.25 * (th[j + 1] - th[j]) * x + 0.25 * (th[j + 1] + 3.0 * th[j])
+                : 0.25 * (th[j + 1] - th[j]) * x + 0.25 * (3.0 * th[j + 1] + th[j]);
+}
+
+__device__ __forceinline__ double polar_rm(const double *r, const double *th, int j, double t) {
+  return r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(t - th[j]) - r[j + 1] * sin(t - th[j + 1]));
+}

@@ -1,0 +1,63 @@
+// problem.h -- device-resident state of one boundary-value problem.
+#pragma once
+#include <vector>
+#include "common.h"
+#include "gf_dev.h"
+
+// quadrature and collocation tables, one device copy per problem
+struct Tab {
+  double xli[12], wli[12], xhi[64], whi[64];
+  double zt_44[4], et_44[4], wt_44[4], zt_34[4], et_34[4], wt_34[4];
+  double zt_49[9], et_49[9], wt_49[9], zt_37[7], et_37[7], wt_37[7];
+};
+
+// element data on the device, indexed by element id (row 0 unused)
+struct DevMesh {
+  int Ne, NN, MN, ps;
+  const double *cr;   // (Ne+1) x 12
+  const double *cw;   // (Ne+1) x 9
+  const double *ren;  // (Ne+1) x 12
+  const double *wen;  // (Ne+1) x 12
+  const int *tri;     // (Ne+1): 1 for a linear triangle
+  const int *eni;     // (Ne+1) x 4
+  const int *md, *sd; // (Ne+1)
+  const double *Pi, *dPi;  // (Ne+1) x 4 complex
+};
+
+// per-domain device buffers
+struct DevDomain {
+  int did, Ne, N;      // N = 4 Ne
+  int nn;              // collocation nodes of the domain (3 or 4 per element)
+  int *sid;            // Ne+1 signed element ids
+  int *rowmap;         // N: compressed node index of row (t-1)*4+tn, -1 for unused triangle slot
+  double k;            // real wave number of the domain
+  double *G, *H;       // N x N complex
+  double *dG, *dH;     // 2N x N complex (interior / isolated domains)
+  double *dF;          // N x 3
+  double *Fpair;       // N x Ne x 3 : per pair F, dF/dtz, dF/dte
+  unsigned char *rule; // N x Ne : 1 = GH64 rule
+  double *P, *dP;      // (Ne+1) x 4 complex
+  double *pv, *dpv;    // (Ne+1) x 4 x 3 complex
+  bool assembled;
+};
+
+struct bqd2_problem {
+  DevMesh m;
+  Tab *tab;            // device
+  Tab htab;            // host copy
+  GfParams gf;
+  double k2[16][2];    // host copies of the media constants (MN+1 <= 16)
+  double k0[16], rho0[16];
+  double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
+  std::vector<DevDomain> dom;
+  std::vector<void *> owned;  // device allocations released by bqd2_destroy
+  hipStream_t stream;
+  long long *d_counters;      // device counters (GF terms)
+  // system
+  double *A, *b;
+  int *ipiv;
+  int na;
+  bool solved;
+};
+
+int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);

@@ -1,0 +1,180 @@
+// problem.hip -- creation, transfer and release of a device-resident problem.
+#include <cstring>
+#include "problem.h"
+
+int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    bqd2_set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    return BQD2_E_NOMEM;
+  }
+  p->owned.push_back(*ptr);
+  return BQD2_OK;
+}
+
+template <class T>
+static int upload(bqd2_problem *p, T **dst, const T *src, size_t n) {
+  int rc = bqd2_alloc(p, (void **)dst, n * sizeof(T));
+  if (rc) return rc;
+  BQD2_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return BQD2_OK;
+}
+
+#define TRY(x)                \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_) {                \
+      bqd2_destroy(p);        \
+      return rc_;             \
+    }                         \
+  } while (0)
+
+extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
+  BQD2_CHECK(m && out, BQD2_E_ARG, "bqd2_create: null argument");
+  BQD2_CHECK(m->Ne > 0 && m->MN >= 0 && m->MN < 16, BQD2_E_ARG, "bqd2_create: bad sizes Ne=%d MN=%d", m->Ne, m->MN);
+  for (int s = 1; s <= m->Ne; s++)
+    BQD2_CHECK(m->ed[4 * s + 3] != 0, BQD2_E_ARG,
+               "bqd2_create: element %d is a linear triangle; this build assembles bilinear elements only", s);
+  bqd2_problem *p = new bqd2_problem();
+  *out = nullptr;
+  p->m.Ne = m->Ne;
+  p->m.NN = m->NN;
+  p->m.MN = m->MN;
+  p->m.ps = m->ps;
+  const size_t ne1 = (size_t)m->Ne + 1;
+  std::vector<int> tri(ne1, 0);
+  for (size_t s = 1; s < ne1; s++) tri[s] = (m->ed[4 * s + 3] == 0);
+  TRY(upload(p, (double **)&p->m.cr, m->cr, ne1 * 12));
+  TRY(upload(p, (double **)&p->m.cw, m->cw, ne1 * 9));
+  TRY(upload(p, (double **)&p->m.ren, m->ren, ne1 * 12));
+  TRY(upload(p, (double **)&p->m.wen, m->wen, ne1 * 12));
+  TRY(upload(p, (int **)&p->m.tri, tri.data(), ne1));
+  TRY(upload(p, (int **)&p->m.eni, m->eni, ne1 * 4));
+  TRY(upload(p, (int **)&p->m.md, m->md, ne1));
+  TRY(upload(p, (int **)&p->m.sd, m->sd, ne1));
+  TRY(upload(p, (double **)&p->m.Pi, m->Pi, ne1 * 8));
+  TRY(upload(p, (double **)&p->m.dPi, m->dPi, ne1 * 8));
+
+  Tab &T = p->htab;
+  memcpy(T.xli, m->xli, sizeof(T.xli)); memcpy(T.wli, m->wli, sizeof(T.wli));
+  memcpy(T.xhi, m->xhi, sizeof(T.xhi)); memcpy(T.whi, m->whi, sizeof(T.whi));
+  memcpy(T.zt_44, m->zt_44, sizeof(T.zt_44)); memcpy(T.et_44, m->et_44, sizeof(T.et_44));
+  memcpy(T.wt_44, m->wt_44, sizeof(T.wt_44)); memcpy(T.zt_34, m->zt_34, sizeof(T.zt_34));
+  memcpy(T.et_34, m->et_34, sizeof(T.et_34)); memcpy(T.wt_34, m->wt_34, sizeof(T.wt_34));
+  memcpy(T.zt_49, m->zt_49, sizeof(T.zt_49)); memcpy(T.et_49, m->et_49, sizeof(T.et_49));
+  memcpy(T.wt_49, m->wt_49, sizeof(T.wt_49)); memcpy(T.zt_37, m->zt_37, sizeof(T.zt_37));
+  memcpy(T.et_37, m->et_37, sizeof(T.et_37)); memcpy(T.wt_37, m->wt_37, sizeof(T.wt_37));
+  TRY(upload(p, &p->tab, &T, 1));
+
+  for (int d = 0; d <= m->MN; d++) {
+    p->k0[d] = m->k0[d];
+    p->rho0[d] = m->rho0[d];
+    p->k2[d][0] = m->k2[2 * d];
+    p->k2[d][1] = m->k2[2 * d + 1];
+  }
+  memcpy(p->pw_p, m->pw_p, sizeof(p->pw_p));
+  p->pw_rho0 = m->pw_rho0; p->pw_c0 = m->pw_c0; p->pw_k0 = m->pw_k0;
+  memcpy(p->pw_tv, m->pw_tv, sizeof(p->pw_tv));
+  memcpy(p->pw_R, m->pw_R, sizeof(p->pw_R));
+  if (m->ps == 1) TRY(bqd2_gf_params(m->qd, &p->gf));
+  else memset(&p->gf, 0, sizeof(p->gf));
+
+  TRY(bqd2_alloc(p, (void **)&p->d_counters, 8 * sizeof(long long)));
+  BQD2_HIP(hipStreamCreate(&p->stream));
+
+  size_t off = 0;
+  p->dom.resize(m->MN + 1);
+  for (int d = 0; d <= m->MN; d++) {
+    DevDomain &D = p->dom[d];
+    memset(&D, 0, sizeof(D));
+    D.did = d;
+    D.Ne = m->dom_ne[d];
+    D.N = 4 * D.Ne;
+    D.k = m->k0[d];
+    const int *sid = m->dom_sid + off;
+    off += D.Ne + 1;
+    int nn = 0;
+    std::vector<int> rowmap(D.N > 0 ? D.N : 1, -1);
+    for (int t = 1; t <= D.Ne; t++) {
+      int nl = tri[abs(sid[t])] ? 3 : 4;
+      for (int tn = 0; tn < nl; tn++) rowmap[(t - 1) * 4 + tn] = nn++;
+    }
+    D.nn = nn;
+    TRY(upload(p, &D.sid, sid, (size_t)D.Ne + 1));
+    TRY(upload(p, &D.rowmap, rowmap.data(), rowmap.size()));
+    const size_t N = D.N;
+    const bool periodic0 = (d == 0 && m->ps == 1);
+    TRY(bqd2_alloc(p, (void **)&D.G, N * N * 16));
+    TRY(bqd2_alloc(p, (void **)&D.H, N * N * 16));
+    if (!periodic0) {
+      TRY(bqd2_alloc(p, (void **)&D.dG, 2 * N * N * 16));
+      TRY(bqd2_alloc(p, (void **)&D.dH, 2 * N * N * 16));
+    }
+    TRY(bqd2_alloc(p, (void **)&D.dF, N * 3 * 8));
+    TRY(bqd2_alloc(p, (void **)&D.Fpair, N * D.Ne * 3 * 8));
+    TRY(bqd2_alloc(p, (void **)&D.rule, N * D.Ne));
+    TRY(bqd2_alloc(p, (void **)&D.P, ((size_t)D.Ne + 1) * 4 * 16));
+    TRY(bqd2_alloc(p, (void **)&D.dP, ((size_t)D.Ne + 1) * 4 * 16));
+    TRY(bqd2_alloc(p, (void **)&D.pv, ((size_t)D.Ne + 1) * 12 * 16));
+    TRY(bqd2_alloc(p, (void **)&D.dpv, ((size_t)D.Ne + 1) * 12 * 16));
+    BQD2_HIP(hipMemset(D.P, 0, ((size_t)D.Ne + 1) * 4 * 16));
+    BQD2_HIP(hipMemset(D.dP, 0, ((size_t)D.Ne + 1) * 4 * 16));
+    BQD2_HIP(hipMemset(D.pv, 0, ((size_t)D.Ne + 1) * 12 * 16));
+    BQD2_HIP(hipMemset(D.dpv, 0, ((size_t)D.Ne + 1) * 12 * 16));
+  }
+  p->na = 2 * m->NN;
+  p->A = p->b = nullptr;
+  p->ipiv = nullptr;
+  p->solved = false;
+  *out = p;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_destroy(bqd2_problem *p) {
+  if (!p) return BQD2_OK;
+  if (p->stream) hipStreamSynchronize(p->stream);
+  for (void *ptr : p->owned) hipFree(ptr);
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_download_domain(bqd2_problem *p, int did, double *G, double *H, double *dG, double *dH, double *dF) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_download_domain: bad domain");
+  DevDomain &D = p->dom[did];
+  BQD2_CHECK(D.assembled, BQD2_E_STATE, "bqd2_download_domain: domain %d not assembled", did);
+  const size_t N = D.N;
+  BQD2_HIP(hipStreamSynchronize(p->stream));
+  if (G) BQD2_HIP(hipMemcpy(G, D.G, N * N * 16, hipMemcpyDeviceToHost));
+  if (H) BQD2_HIP(hipMemcpy(H, D.H, N * N * 16, hipMemcpyDeviceToHost));
+  if (dG && D.dG) BQD2_HIP(hipMemcpy(dG, D.dG, 2 * N * N * 16, hipMemcpyDeviceToHost));
+  if (dH && D.dH) BQD2_HIP(hipMemcpy(dH, D.dH, 2 * N * N * 16, hipMemcpyDeviceToHost));
+  if (dF) BQD2_HIP(hipMemcpy(dF, D.dF, N * 3 * 8, hipMemcpyDeviceToHost));
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_download_solution(bqd2_problem *p, int did, double *P, double *dP, double *pv, double *dpv) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_download_solution: bad domain");
+  DevDomain &D = p->dom[did];
+  const size_t n1 = (size_t)D.Ne + 1;
+  BQD2_HIP(hipStreamSynchronize(p->stream));
+  if (P) BQD2_HIP(hipMemcpy(P, D.P, n1 * 4 * 16, hipMemcpyDeviceToHost));
+  if (dP) BQD2_HIP(hipMemcpy(dP, D.dP, n1 * 4 * 16, hipMemcpyDeviceToHost));
+  if (pv) BQD2_HIP(hipMemcpy(pv, D.pv, n1 * 12 * 16, hipMemcpyDeviceToHost));
+  if (dpv) BQD2_HIP(hipMemcpy(dpv, D.dpv, n1 * 12 * 16, hipMemcpyDeviceToHost));
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double *dP, const double *pv,
+                                    const double *dpv) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_upload_solution: bad domain");
+  DevDomain &D = p->dom[did];
+  const size_t n1 = (size_t)D.Ne + 1;
+  BQD2_HIP(hipStreamSynchronize(p->stream));
+  if (P) BQD2_HIP(hipMemcpy(D.P, P, n1 * 4 * 16, hipMemcpyHostToDevice));
+  if (dP) BQD2_HIP(hipMemcpy(D.dP, dP, n1 * 4 * 16, hipMemcpyHostToDevice));
+  if (pv) BQD2_HIP(hipMemcpy(D.pv, pv, n1 * 12 * 16, hipMemcpyHostToDevice));
+  if (dpv) BQD2_HIP(hipMemcpy(D.dpv, dpv, n1 * 12 * 16, hipMemcpyHostToDevice));
+  return BQD2_OK;
+}

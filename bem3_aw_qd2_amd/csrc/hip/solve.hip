@@ -1,0 +1,380 @@
+// solve.hip -- boundary-value system, its solution and the velocity recovery.
+//
+//   k_system_A / k_system_b   create_matrix_csr_dac (bem3_aw_qd2_solve_bieq.c:244-345),
+//                             written densely and column-major for the LU
+//   k_store_solution          tmatrix_bd_store (:370-427)
+//   k_velocity_rows           dpdt_fcm + inv_matrix_33 per node (:493-530, :564-607, :689-729)
+//   k_pv_open_domain          the open-domain velocity from the interface
+//                             conditions of q_solve_pv_bv (:711-748)
+//   k_dpv_system              G dpv = H pv of solve_dpv_bv (:533-653)
+#include <cstring>
+#include "elem_dev.h"
+
+struct PwDev {  // incident plane wave (aw_pw.c:128-199)
+  double p[2], rho0, c0, k0, tv[3], R[9];
+};
+
+__device__ __forceinline__ void pw_pv(const PwDev &W, const double *r, cd &p, cd *v) {
+  const double *R = W.R;
+  double rc0 = r[0] - W.tv[0], rc1 = r[1] - W.tv[1], rc2 = r[2] - W.tv[2];
+  double z = -R[6] * rc0 - R[7] * rc1 + R[8] * rc2;
+  cd ce = cis(W.k0 * z);
+  p = cd{W.p[0], W.p[1]} * ce;
+  cd vz = p * (1.0 / (W.rho0 * W.c0));
+  for (int i = 0; i < 3; i++) v[i] = vz * R[3 * i + 2];
+}
+
+__device__ __forceinline__ void inv33(double *ma, const double *vn, const double *vtz, const double *vte) {
+  double idet = 1.0 / (vn[0] * vtz[1] * vte[2] + vn[1] * vtz[2] * vte[0] + vn[2] * vtz[0] * vte[1] -
+                       (vn[2] * vtz[1] * vte[0] + vn[1] * vtz[0] * vte[2] + vn[0] * vtz[2] * vte[1]));
+  ma[0] = vtz[1] * vte[2] - vtz[2] * vte[1];
+  ma[1] = -vn[1] * vte[2] + vn[2] * vte[1];
+  ma[2] = vn[1] * vtz[2] - vn[2] * vtz[1];
+  ma[3] = -vtz[0] * vte[2] + vtz[2] * vte[0];
+  ma[4] = vn[0] * vte[2] - vn[2] * vte[0];
+  ma[5] = -vn[0] * vtz[2] + vn[2] * vtz[0];
+  ma[6] = vtz[0] * vte[1] - vtz[1] * vte[0];
+  ma[7] = -vn[0] * vte[1] + vn[1] * vte[0];
+  ma[8] = vn[0] * vtz[1] - vn[1] * vtz[0];
+  for (int i = 0; i < 9; i++) ma[i] *= idet;
+}
+
+// A entries of domain D's rows: one thread per (row, s).  Main side: P column
+// H, dP column -G; sub side: dP column G (its P column hk2*H is written by
+// k_system_b together with the right-hand side).
+__global__ void __launch_bounds__(256) k_system_A(DevMesh M, DevDomain D, int rowoff, double *__restrict__ A, int na) {
+  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)D.N * D.Ne) return;
+  const int row = (int)(gid / D.Ne), s = (int)(gid % D.Ne) + 1;
+  const int arow_local = D.rowmap[row];
+  if (arow_local < 0) return;
+  const size_t arow = (size_t)rowoff + arow_local;
+  const int asd = abs(D.sid[s]);
+  const int nl = M.tri[asd] ? 3 : 4;
+  const bool main_side = (D.did == M.md[asd]);
+  const int nn = M.NN;
+  const double *Gr = D.G + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
+  const double *Hr = D.H + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
+  for (int l = 0; l < nl; l++) {
+    const int col = M.eni[4 * asd + l];
+    double *b = A + 2 * ((size_t)(nn + col) * na + arow);
+    if (main_side) {
+      double *a = A + 2 * ((size_t)col * na + arow);
+      a[0] = Hr[2 * l];
+      a[1] = Hr[2 * l + 1];
+      b[0] = -Gr[2 * l];
+      b[1] = -Gr[2 * l + 1];
+    } else {
+      b[0] = Gr[2 * l];
+      b[1] = Gr[2 * l + 1];
+    }
+  }
+}
+
+// sub-side P columns carry hk2 = k2[md]/k2[sd]; right-hand side per row
+__global__ void __launch_bounds__(64) k_system_b(DevMesh M, DevDomain D, int rowoff,
+                                                              double *__restrict__ A, int na,
+                                                              const double *__restrict__ hk2tab,
+                                                              double *__restrict__ bvec) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  const int arow_local = D.rowmap[row];
+  if (arow_local < 0) return;
+  const size_t arow = (size_t)rowoff + arow_local;
+  cd acc = cd{0.0, 0.0};
+  for (int s = 1 + lane; s <= D.Ne; s += 64) {
+    const int sd = D.sid[s], asd = abs(sd);
+    const int mdid = M.md[asd], sdid = M.sd[asd];
+    if (D.did == mdid) continue;
+    const int nl = M.tri[asd] ? 3 : 4;
+    cd hk2 = cd{hk2tab[2 * (mdid * 16 + sdid)], hk2tab[2 * (mdid * 16 + sdid) + 1]};
+    const double *Gr = D.G + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
+    const double *Hr = D.H + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
+    for (int l = 0; l < nl; l++) {
+      const int col = M.eni[4 * asd + l];
+      cd h = hk2 * cd{Hr[2 * l], Hr[2 * l + 1]};
+      double *a = A + 2 * ((size_t)col * na + arow);
+      a[0] = h.x; a[1] = h.y;
+      if (mdid == 0) {
+        cd pi = cd{M.Pi[2 * (4 * asd + l)], M.Pi[2 * (4 * asd + l) + 1]};
+        cd dpi = cd{M.dPi[2 * (4 * asd + l)], M.dPi[2 * (4 * asd + l) + 1]};
+        acc -= h * pi + cd{Gr[2 * l], Gr[2 * l + 1]} * dpi;
+      }
+    }
+  }
+  acc.x = wave_sum(acc.x);
+  acc.y = wave_sum(acc.y);
+  if (lane == 0) {
+    bvec[2 * arow] = acc.x;
+    bvec[2 * arow + 1] = acc.y;
+  }
+}
+
+// scatter the system solution to P, dP of every element side of domain D
+__global__ void __launch_bounds__(256) k_store_solution(DevMesh M, DevDomain D, const double *__restrict__ x,
+                                                        const double *__restrict__ hk2tab) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (s > D.Ne) return;
+  const int sd = D.sid[s], asd = abs(sd);
+  const int mdid = M.md[asd], sdid = M.sd[asd];
+  const int nl = M.tri[asd] ? 3 : 4, nn = M.NN;
+  cd hk2 = cd{hk2tab[2 * (mdid * 16 + sdid)], hk2tab[2 * (mdid * 16 + sdid) + 1]};
+  for (int l = 0; l < nl; l++) {
+    int e = M.eni[4 * asd + l];
+    cd xp = cd{x[2 * e], x[2 * e + 1]}, xd = cd{x[2 * (nn + e)], x[2 * (nn + e) + 1]};
+    cd P, dP;
+    if (D.did == mdid) {
+      P = xp;
+      dP = xd;
+    } else {
+      P = hk2 * xp;
+      dP = -xd;
+      if (mdid == 0) {
+        P += hk2 * cd{M.Pi[2 * (4 * asd + l)], M.Pi[2 * (4 * asd + l) + 1]};
+        dP -= cd{M.dPi[2 * (4 * asd + l)], M.dPi[2 * (4 * asd + l) + 1]};
+      }
+    }
+    D.P[2 * (4 * s + l)] = P.x; D.P[2 * (4 * s + l) + 1] = P.y;
+    D.dP[2 * (4 * s + l)] = dP.x; D.dP[2 * (4 * s + l) + 1] = dP.y;
+  }
+}
+
+// per node (t, tn) of domain D: tangential derivatives of P from the dG, dH rows
+// and the node velocity pv = -[n; tz; te]^-1 (dP/dn, dP/dtz, dP/dte); with
+// open_out != NULL also the open-domain velocity from the interface conditions
+__global__ void __launch_bounds__(64) k_velocity_rows(DevMesh M, DevDomain D, const Tab *__restrict__ tab, PwDev W,
+                                                      double rho_d, double rho_0, double *__restrict__ open_out) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  int t = row / 4 + 1, tn = row % 4;
+  const int at = D.sid[t], atd = abs(at);
+  if (M.tri[atd] && tn == 3) return;
+  const size_t N = D.N;
+  const double *g0 = D.dG + 2 * (size_t)(2 * row) * N, *g1 = D.dG + 2 * (size_t)(2 * row + 1) * N;
+  const double *h0 = D.dH + 2 * (size_t)(2 * row) * N, *h1 = D.dH + 2 * (size_t)(2 * row + 1) * N;
+  cd z = cd{0, 0}, e = cd{0, 0};
+  for (size_t c = lane; c < N; c += 64) {
+    // column c = (s-1)*4 + sn -> P[s][sn] at index 4 s + sn = c + 4
+    cd dp = cd{D.dP[2 * (c + 4)], D.dP[2 * (c + 4) + 1]}, p = cd{D.P[2 * (c + 4)], D.P[2 * (c + 4) + 1]};
+    z += cd{g0[2 * c], g0[2 * c + 1]} * dp - cd{h0[2 * c], h0[2 * c + 1]} * p;
+    e += cd{g1[2 * c], g1[2 * c + 1]} * dp - cd{h1[2 * c], h1[2 * c + 1]} * p;
+  }
+  z = cd{wave_sum(z.x), wave_sum(z.y)};
+  e = cd{wave_sum(e.x), wave_sum(e.y)};
+  if (lane != 0) return;
+  const double *dF = D.dF + 3 * row;
+  cd Pt = cd{D.P[2 * (4 * t + tn)], D.P[2 * (4 * t + tn) + 1]};
+  cd dPt = cd{D.dP[2 * (4 * t + tn)], D.dP[2 * (4 * t + tn) + 1]};
+  cd dPz = (z - Pt * dF[1]) * (1.0 / dF[0]);
+  cd dPe = (e - Pt * dF[2]) * (1.0 / dF[0]);
+  double vtz[3], vte[3], vn[3], a[9];
+  node_tangents(M, *tab, atd, tn, vtz, vte);
+  const double sig = at < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < 3; i++) vn[i] = sig * M.wen[12 * atd + 3 * tn + i];
+  double an = abs3(vn);
+  if (an != 0.0) { vn[0] /= an; vn[1] /= an; vn[2] /= an; }
+  inv33(a, vn, vtz, vte);
+  cd pv[3];
+  for (int i = 0; i < 3; i++) {
+    pv[i] = -(dPt * a[3 * i] + dPz * a[3 * i + 1] + dPe * a[3 * i + 2]);
+    D.pv[2 * (12 * t + 3 * tn + i)] = pv[i].x;
+    D.pv[2 * (12 * t + 3 * tn + i) + 1] = pv[i].y;
+  }
+  if (open_out && M.md[atd] == 0) {
+    const double *r = M.ren + 12 * atd + 3 * tn;
+    cd pi, pvi[3];
+    pw_pv(W, r, pi, pvi);
+    double rtz[3], rte[3];
+    for (int i = 0; i < 3; i++) {
+      rtz[i] = rho_0 * vtz[i];
+      rte[i] = rho_0 * vte[i];
+    }
+    cd rhs[3];
+    rhs[0] = (pv[0] - pvi[0]) * vn[0] + (pv[1] - pvi[1]) * vn[1] + (pv[2] - pvi[2]) * vn[2];
+    rhs[1] = (pv[0] * rho_d - pvi[0] * rho_0) * vtz[0] + (pv[1] * rho_d - pvi[1] * rho_0) * vtz[1] +
+             (pv[2] * rho_d - pvi[2] * rho_0) * vtz[2];
+    rhs[2] = (pv[0] * rho_d - pvi[0] * rho_0) * vte[0] + (pv[1] * rho_d - pvi[1] * rho_0) * vte[1] +
+             (pv[2] * rho_d - pvi[2] * rho_0) * vte[2];
+    inv33(a, vn, rtz, rte);
+    const int en = M.eni[4 * atd + tn];
+    for (int i = 0; i < 3; i++) {
+      cd v = rhs[0] * a[3 * i] + rhs[1] * a[3 * i + 1] + rhs[2] * a[3 * i + 2];
+      open_out[2 * (3 * en + i)] = v.x;
+      open_out[2 * (3 * en + i) + 1] = v.y;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pv_open_domain(DevMesh M, DevDomain D0, const double *__restrict__ tpv) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (s > D0.Ne) return;
+  const int asd = abs(D0.sid[s]);
+  const int nl = M.tri[asd] ? 3 : 4;
+  for (int l = 0; l < nl; l++) {
+    const int en = M.eni[4 * asd + l];
+    for (int i = 0; i < 3; i++) {
+      D0.pv[2 * (12 * s + 3 * l + i)] = tpv[2 * (3 * en + i)];
+      D0.pv[2 * (12 * s + 3 * l + i) + 1] = tpv[2 * (3 * en + i) + 1];
+    }
+  }
+}
+
+// dpv system of one domain: A = G (compressed rows/cols, column-major), B = H pv
+__global__ void __launch_bounds__(256) k_dpv_A(DevMesh M, DevDomain D, double *__restrict__ A) {
+  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)D.N * D.N) return;
+  int row = (int)(gid / D.N), col = (int)(gid % D.N);
+  int r = D.rowmap[row], c = D.rowmap[col];
+  if (r < 0 || c < 0) return;
+  const double *g = D.G + 2 * ((size_t)row * D.N + col);
+  double *a = A + 2 * ((size_t)c * D.nn + r);
+  a[0] = g[0];
+  a[1] = g[1];
+}
+
+__global__ void __launch_bounds__(64) k_dpv_B(DevMesh M, DevDomain D, double *__restrict__ B) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  const int r = D.rowmap[row];
+  if (r < 0) return;
+  cd acc[3] = {};
+  for (int c = lane; c < D.N; c += 64) {
+    if (D.rowmap[c] < 0) continue;
+    cd h = cd{D.H[2 * ((size_t)row * D.N + c)], D.H[2 * ((size_t)row * D.N + c) + 1]};
+    for (int i = 0; i < 3; i++) acc[i] += h * cd{D.pv[2 * (12 + 3 * c + i)], D.pv[2 * (12 + 3 * c + i) + 1]};
+  }
+  for (int i = 0; i < 3; i++) {
+    acc[i] = cd{wave_sum(acc[i].x), wave_sum(acc[i].y)};
+    if (lane == 0) {
+      B[2 * ((size_t)i * D.nn + r)] = acc[i].x;
+      B[2 * ((size_t)i * D.nn + r) + 1] = acc[i].y;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_dpv_store(DevDomain D, const double *__restrict__ B) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= D.N) return;
+  int r = D.rowmap[c];
+  if (r < 0) return;
+  for (int i = 0; i < 3; i++) {
+    D.dpv[2 * (12 + 3 * c + i)] = B[2 * ((size_t)i * D.nn + r)];
+    D.dpv[2 * (12 + 3 * c + i) + 1] = B[2 * ((size_t)i * D.nn + r) + 1];
+  }
+}
+
+static void hk2_table(const bqd2_problem *p, double *tab) {
+  // tab[(m*16 + s)] = k2[m] / k2[s]
+  for (int m = 0; m <= p->m.MN; m++)
+    for (int s = 0; s <= p->m.MN; s++) {
+      double ar = p->k2[m][0], ai = p->k2[m][1], br = p->k2[s][0], bi = p->k2[s][1];
+      double den = br * br + bi * bi;
+      tab[2 * (m * 16 + s)] = (ar * br + ai * bi) / den;
+      tab[2 * (m * 16 + s) + 1] = (ai * br - ar * bi) / den;
+    }
+}
+
+extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
+  BQD2_CHECK(p, BQD2_E_ARG, "bqd2_solve_surface: null problem");
+  for (int d = 0; d <= p->m.MN; d++)
+    BQD2_CHECK(p->dom[d].Ne == 0 || p->dom[d].assembled, BQD2_E_STATE, "bqd2_solve_surface: domain %d not assembled", d);
+  const int na = p->na;
+  hipStream_t st = p->stream;
+  if (!p->A) {
+    int rc = bqd2_alloc(p, (void **)&p->A, (size_t)na * na * 16);
+    if (rc) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->b, (size_t)na * 16))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->ipiv, (size_t)na * sizeof(int)))) return rc;
+  }
+  double *hk2;
+  {
+    static thread_local double tab[16 * 16 * 2];
+    memset(tab, 0, sizeof(tab));
+    hk2_table(p, tab);
+    int rc = bqd2_alloc(p, (void **)&hk2, sizeof(tab));
+    if (rc) return rc;
+    BQD2_HIP(hipMemcpyAsync(hk2, tab, sizeof(tab), hipMemcpyHostToDevice, st));
+    BQD2_HIP(hipStreamSynchronize(st));
+  }
+  BQD2_HIP(hipMemsetAsync(p->A, 0, (size_t)na * na * 16, st));
+  BQD2_HIP(hipMemsetAsync(p->b, 0, (size_t)na * 16, st));
+  int rowoff = 0;
+  for (int d = 0; d <= p->m.MN; d++) {
+    DevDomain &D = p->dom[d];
+    if (D.Ne == 0) continue;
+    long long n = (long long)D.N * D.Ne;
+    hipLaunchKernelGGL(k_system_A, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, D, rowoff, p->A, na);
+    hipLaunchKernelGGL(k_system_b, dim3(D.N), dim3(64), 0, st, p->m, D, rowoff, p->A, na, hk2, p->b);
+    BQD2_HIP(hipGetLastError());
+    rowoff += D.nn;
+  }
+  BQD2_CHECK(rowoff == na, BQD2_E_STATE, "bqd2_solve_surface: %d system rows for %d unknowns", rowoff, na);
+  int rc = bqd2_zgetrf(na, p->A, p->ipiv, st);
+  if (rc) return rc;
+  rc = bqd2_zgetrs(na, 1, p->A, p->ipiv, p->b, st);
+  if (rc) return rc;
+  for (int d = 0; d <= p->m.MN; d++) {
+    DevDomain &D = p->dom[d];
+    if (D.Ne == 0) continue;
+    hipLaunchKernelGGL(k_store_solution, dim3(bqd2_blocks(D.Ne, 256)), dim3(256), 0, st, p->m, D, p->b, hk2);
+  }
+  BQD2_HIP(hipGetLastError());
+  if (x_host) BQD2_HIP(hipMemcpyAsync(x_host, p->b, (size_t)na * 16, hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipStreamSynchronize(st));
+  p->solved = true;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
+  BQD2_CHECK(p && p->solved, BQD2_E_STATE, "bqd2_solve_velocity: surface solution missing");
+  hipStream_t st = p->stream;
+  PwDev W;
+  memcpy(W.p, p->pw_p, sizeof(W.p));
+  W.rho0 = p->pw_rho0; W.c0 = p->pw_c0; W.k0 = p->pw_k0;
+  memcpy(W.tv, p->pw_tv, sizeof(W.tv));
+  memcpy(W.R, p->pw_R, sizeof(W.R));
+  const bool periodic = (p->m.ps == 1);
+  double *tpv = nullptr;
+  if (periodic) {
+    int rc = bqd2_alloc(p, (void **)&tpv, (size_t)p->m.NN * 3 * 16);
+    if (rc) return rc;
+    BQD2_HIP(hipMemsetAsync(tpv, 0, (size_t)p->m.NN * 3 * 16, st));
+  }
+  for (int d = periodic ? 1 : 0; d <= p->m.MN; d++) {
+    DevDomain &D = p->dom[d];
+    if (D.Ne == 0) continue;
+    hipLaunchKernelGGL(k_velocity_rows, dim3(D.N), dim3(64), 0, st, p->m, D, p->tab, W, p->rho0[d], p->rho0[0],
+                       periodic ? tpv : (double *)nullptr);
+  }
+  if (periodic && p->dom[0].Ne > 0)
+    hipLaunchKernelGGL(k_pv_open_domain, dim3(bqd2_blocks(p->dom[0].Ne, 256)), dim3(256), 0, st, p->m, p->dom[0], tpv);
+  BQD2_HIP(hipGetLastError());
+  // normal derivative of the velocity: G dpv = H pv per domain (3 right-hand sides)
+  for (int d = 0; d <= p->m.MN; d++) {
+    DevDomain &D = p->dom[d];
+    if (D.Ne == 0) continue;
+    double *A = nullptr, *B = nullptr;
+    int *ipiv = nullptr;
+    int rc = bqd2_alloc(p, (void **)&A, (size_t)D.nn * D.nn * 16);
+    if (!rc) rc = bqd2_alloc(p, (void **)&B, (size_t)D.nn * 3 * 16);
+    if (!rc) rc = bqd2_alloc(p, (void **)&ipiv, (size_t)D.nn * sizeof(int));
+    if (rc) return rc;
+    BQD2_HIP(hipMemsetAsync(A, 0, (size_t)D.nn * D.nn * 16, st));
+    long long n2 = (long long)D.N * D.N;
+    hipLaunchKernelGGL(k_dpv_A, dim3(bqd2_blocks(n2, 256)), dim3(256), 0, st, p->m, D, A);
+    hipLaunchKernelGGL(k_dpv_B, dim3(D.N), dim3(64), 0, st, p->m, D, B);
+    BQD2_HIP(hipGetLastError());
+    rc = bqd2_zgetrf(D.nn, A, ipiv, st);
+    if (rc) return rc;
+    rc = bqd2_zgetrs(D.nn, 3, A, ipiv, B, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dpv_store, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st, D, B);
+    BQD2_HIP(hipGetLastError());
+    BQD2_HIP(hipStreamSynchronize(st));
+    // the per-domain work buffers are released right away
+    for (void *q : {(void *)A, (void *)B, (void *)ipiv}) {
+      hipFree(q);
+      for (auto it = p->owned.begin(); it != p->owned.end(); ++it)
+        if (*it == q) { p->owned.erase(it); break; }
+    }
+  }
+  BQD2_HIP(hipStreamSynchronize(st));
+  return BQD2_OK;
+}

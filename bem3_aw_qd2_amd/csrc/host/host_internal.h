@@ -8,7 +8,12 @@ void aqd2_element_constants(BOUD *bd);
 void aqd2_quadrature_tables(BOUD *bd);
 /* position and unnormalised normal at (z, e) of element s > 0 */
 void aqd2_rw(const BOUD *bd, int s, double z, double e, double *r, double *w);
-/* release the GPU-side state attached to qd (device.c) */
+/* GPU-side state attached to an AQD2, kept in a side table keyed by its address
+ * (AQD2 has no spare field: its layout is fixed by ex.dat).  solve.c */
+struct bqd2_problem;
+struct bqd2_problem *aqd2_problem(AQD2 *qd, int *has_solution);
+void aqd2_push_solution(AQD2 *qd, struct bqd2_problem *p);
+void aqd2_mark_solution(AQD2 *qd, int on);
 void aqd2_release_device(AQD2 *qd);
 
 #endif

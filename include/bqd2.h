@@ -62,6 +62,79 @@ int bqd2_gf_params(const double *qd8, GfParams *p);
 int bqd2_gf_batch(const GfParams *p, int mode, int n, const double *d_r, double eps,
                   double *d_G, double *d_dG, double *d_d2G, int *d_terms, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Boundary-value problem on the device
+ * ------------------------------------------------------------------------ */
+
+/* Host description of one model, filled by the host library from an AQD2
+ * (include/bem3_aw_qd2.h).  Element arrays are indexed by element id 1..Ne
+ * (row 0 unused), as in AQD2; complex values are (re, im) pairs. */
+typedef struct {
+  int Ne, Nn, NN, MN, ps;
+  const double *cr;      /* (Ne+1) x 12 : cr[s][d][j] at 4d+j            */
+  const double *cw;      /* (Ne+1) x 9  : cw[s][d][j] at 3d+j            */
+  const double *ren;     /* (Ne+1) x 12 : node n, coordinate d at 3n+d   */
+  const double *wen;     /* (Ne+1) x 12                                  */
+  const int *ed;         /* (Ne+1) x 4  : corner node ids (ed[s][3]==0: triangle) */
+  const int *eni;        /* (Ne+1) x 4  : collocation node index          */
+  const int *md, *sd;    /* (Ne+1)      : main / sub domain               */
+  const double *Pi, *dPi;/* (Ne+1) x 4 complex: incident potential at nodes */
+  const int *dom_ne;     /* MN+1        : elements per domain             */
+  const int *dom_sid;    /* sum(dom_ne)+MN+1: per domain 1-based signed ids, entry 0 unused */
+  const double *k0;      /* MN+1 real wave numbers                        */
+  const double *k2;      /* MN+1 complex i omega rho                      */
+  const double *rho0;    /* MN+1 densities                                */
+  double qd[8];          /* QPD2 of the host medium                       */
+  double xli[12], wli[12], xhi[64], whi[64];
+  double zt_44[4], et_44[4], wt_44[4], zt_34[4], et_34[4], wt_34[4];
+  double zt_49[9], et_49[9], wt_49[9], zt_37[7], et_37[7], wt_37[7];
+  /* incident plane wave: amplitude (re, im), rho0, c0, k0, translation, rotation */
+  double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
+} bqd2_model;
+
+typedef struct bqd2_problem bqd2_problem;
+
+/* counters of the last assembly of one domain */
+typedef struct {
+  long long pairs_gl, pairs_gh, self_rows;  /* off-diagonal pairs per rule, self blocks */
+  long long gf_evals;                       /* distinct Green's function evaluations    */
+  long long gf_terms;                       /* lattice terms summed over them           */
+  double ms_decide, ms_pairs, ms_self;      /* kernel times (HIP events)                */
+} bqd2_assembly_stats;
+
+int bqd2_create(const bqd2_model *m, bqd2_problem **out);
+int bqd2_destroy(bqd2_problem *p);
+
+/* Coefficient matrices of domain did on the device (create_cmatrix_domain,
+ * d3qd2_src/bem3_aw_qd2_solve_bieq.c:133-218, cm.type = 2):
+ *   G, H [N x N], dG, dH [2N x N] (interior and isolated domains only), dF [N x 3],
+ * N = 4 Ne_d, row (t-1)*4+tn, column (s-1)*4+n. st may be NULL. */
+int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st);
+
+/* copy domain did's matrices to host buffers (any may be NULL) */
+int bqd2_download_domain(bqd2_problem *p, int did, double *G, double *H, double *dG, double *dH, double *dF);
+
+/* Build the 2NN system (create_matrix_csr_dac, :244-345, dense column-major),
+ * factor it (LU with partial pivoting) and solve; scatter the solution to the
+ * per-domain P, dP (tmatrix_bd_store, :370-427).  x (host, 2NN complex) may be NULL. */
+int bqd2_solve_surface(bqd2_problem *p, double *x);
+
+/* Particle velocity and its normal derivative on the boundary: q_solve_pv_bv /
+ * solve_pv_bv (:429-491, :655-750) and solve_dpv_bv (:533-653). */
+int bqd2_solve_velocity(bqd2_problem *p);
+
+/* per-domain solution arrays, (Ne_d+1) x 4 complex for P, dP and
+ * (Ne_d+1) x 4 x 3 complex for pv, dpv, entry 0 unused, as in SUBD */
+int bqd2_download_solution(bqd2_problem *p, int did, double *P, double *dP, double *pv, double *dpv);
+int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double *dP, const double *pv,
+                         const double *dpv);
+
+/* dense complex LU helpers on device buffers (column-major, leading dimension n),
+ * used by the solves above and exposed for tests: A is overwritten by its
+ * factors, ipiv (device, n ints) receives the pivots; B (n x nrhs) by X. */
+int bqd2_zgetrf(int n, double *d_A, int *d_ipiv, void *stream);
+int bqd2_zgetrs(int n, int nrhs, const double *d_A, const int *d_ipiv, double *d_B, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
