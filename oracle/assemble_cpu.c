@@ -342,3 +342,21 @@ double orc_dpv_row_residual(const AQD2 *qd, int d, int t, int tn)
   free(G); free(H);
   return worst;
 }
+
+/* per-source-element F, dF/dtz, dF/dte of one row (diagnostics for tests) */
+int orc_row_fparts(const AQD2 *qd, int did, int t, int tn, double *out)
+{
+  const BOUD *bd = &qd->bd;
+  int Ne = bd->sb[did].Ne, td = bd->sb[did].sid[t];
+  double vtz[3], vte[3];
+  orc_node_tangents(bd, td, tn, vtz, vte);
+  for (int s = 1; s <= Ne; s++) {
+    double complex CC[9], dCz[9], dCe[9];
+    long c = orc_coef_block(qd, did, td, tn, bd->sb[did].sid[s], vtz, vte, CC, dCz, dCe);
+    if (c < 0) return (int)c;
+    out[3 * (s - 1)] = creal(CC[8]);
+    out[3 * (s - 1) + 1] = creal(dCz[8]);
+    out[3 * (s - 1) + 2] = creal(dCe[8]);
+  }
+  return 0;
+}

@@ -61,6 +61,12 @@ def test_sample_rows_match_oracle(did):
         if did:
             eg = rel(dG[2 * r:2 * r + 2], odG.reshape(2, N))
             eh = rel(dH[2 * r:2 * r + 2], odH.reshape(2, N))
-            ef = np.abs(dF[r, 1:] - odF[1:]).max() / np.abs(odF[1:]).max()
-            assert eg < 1e-10 and eh < 1e-10 and ef < 1e-8, msg + " dG %.2e dH %.2e dF %.2e" % (eg, eh, ef)
+            # dF/dv summed over a closed surface cancels to ~1e-11 of its terms
+            # (the solid angle is constant on a smooth boundary): compare on the
+            # scale of the sum of |terms|
+            parts = np.zeros(3 * Ne)
+            oracle().orc_row_fparts(m.buf, did, t, tn, parts.ctypes.data_as(C.c_void_p))
+            scale = np.abs(parts.reshape(Ne, 3)[:, 1:]).sum(axis=0)
+            ef = (np.abs(dF[r, 1:] - odF[1:]) / scale).max()
+            assert eg < 1e-10 and eh < 1e-10 and ef < 1e-12, msg + " dG %.2e dH %.2e dF %.2e" % (eg, eh, ef)
     m.finalize()
