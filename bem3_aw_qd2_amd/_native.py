@@ -75,6 +75,12 @@ def host_lib():
         lib = _load("libbem3_aw_qd2.so")
         lib.aqd2_problem.restype = C.c_void_p
         lib.aqd2_problem.argtypes = [C.c_void_p, C.c_void_p]
+        lib.pv_batch_aqd2.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+        lib.pv_batch_aqd2.restype = None
+        lib.pv_t_bd_aqd2.restype = None
+        lib.force_FN_aqd2.restype = None
+        lib.surface_area_aqd2.restype = None
         _host = lib
     return _host
 

@@ -1,0 +1,699 @@
+// field.hip -- pressure and particle velocity off and on the boundary, and the
+// radiation force and torque (bilinear elements).
+//
+//   k_domain_id   point classification by signed solid angles (domain_id,
+//                 bem3_aw_qd2.c:1148-1257), one lane per point, the reference's
+//                 sequential order and early exits
+//   k_field       one wavefront per point: representation formula and its
+//                 gradient (pv_s_dbieq_aqd2, bem3_aw_qd2_field.c:227-297 with
+//                 q0_bil_dcoef_grad_* / bil_dcoef_grad_*), falling back to the
+//                 boundary-value form (pv_s_bieq_aqd2, :140-201) when
+//                 |grad F| > CBD_CDF, Bloch phase, incident field
+//   k_field_bd    one wavefront per boundary point (t, zeta, eta):
+//                 pv_s_bd_aqd2 / pv_t_bd_aqd2 (:394-469) with the singular self
+//                 element integrated in polar form (q0_bil_coef_bd / bil_coef_bd)
+//   k_force       radiation stress tensor on the boundary of the open domain
+//                 (force_FN_aqd2, bem3_aw_qd2_force.c:4-165), per element,
+//                 then an ordered sum
+#include <cstring>
+#include "elem_dev.h"
+
+#define CBD_CDF_D 10.0
+
+struct DomView {
+  int Ne;
+  const int *sid;
+  const double *P, *dP, *pv, *dpv;
+  double k, k2x, k2y;
+};
+
+struct FieldCtx {
+  DomView dom[16];
+  int MN, ps;
+  double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
+  double d1x, d1y, d2x, d2y, idd, vkx, vky;
+};
+
+__device__ __forceinline__ void incident_pv(const FieldCtx &X, const double *r, cd &p, cd *v) {
+  const double *R = X.pw_R;
+  double rc0 = r[0] - X.pw_tv[0], rc1 = r[1] - X.pw_tv[1], rc2 = r[2] - X.pw_tv[2];
+  double z = -R[6] * rc0 - R[7] * rc1 + R[8] * rc2;
+  cd ce = cis(X.pw_k0 * z);
+  p = cd{X.pw_p[0], X.pw_p[1]} * ce;
+  cd vz = p * (1.0 / (X.pw_rho0 * X.pw_c0));
+  for (int i = 0; i < 3; i++) v[i] = vz * R[3 * i + 2];
+}
+
+// ------------------------------------------------------------ domain id
+__device__ __forceinline__ int unit3(double *v) {
+  double a = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (a == 0.0) return -1;
+  v[0] /= a; v[1] /= a; v[2] /= a;
+  return 0;
+}
+__device__ __forceinline__ void cross3(double *r, const double *a, const double *b) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// signed solid angle of a triangle / quadrangle seen from the origin of the
+// corner vectors r (fid_calc_solid_angle, bem3_aw_qd2.c:1196-1257)
+__device__ double solid_angle(bool quad, double r[4][3], int &flag) {
+  flag = 0;
+  int st = 0;
+  for (int i = 0; i < (quad ? 4 : 3); i++) st += unit3(r[i]);
+  if (st < 0) { flag = -1; return 0.0; }
+  double n01[3], n12[3], n20[3], nt[3], om = 0.0;
+  cross3(n01, r[0], r[1]);
+  cross3(n12, r[1], r[2]);
+  cross3(n20, r[2], r[0]);
+  cross3(nt, n01, n20);
+  double a0 = atan2(abs3(nt), -dot3(n01, n20));
+  cross3(nt, n12, n01);
+  double a1 = atan2(abs3(nt), -dot3(n12, n01));
+  cross3(nt, n20, n12);
+  double a2 = atan2(abs3(nt), -dot3(n20, n12));
+  double D = dot3(r[0], n12);
+  if (D > 0.0) om += a0 + a1 + a2 - M_PI;
+  else if (D < 0.0) om -= a0 + a1 + a2 - M_PI;
+  else { flag = -1; return 0.0; }
+  if (quad) {
+    n01[0] = -n20[0]; n01[1] = -n20[1]; n01[2] = -n20[2];
+    cross3(n12, r[2], r[3]);
+    cross3(n20, r[3], r[0]);
+    cross3(nt, n01, n20);
+    a0 = atan2(abs3(nt), -dot3(n01, n20));
+    cross3(nt, n12, n01);
+    a1 = atan2(abs3(nt), -dot3(n12, n01));
+    cross3(nt, n20, n12);
+    a2 = atan2(abs3(nt), -dot3(n20, n12));
+    D = dot3(r[0], n12);
+    if (D > 0.0) om += a0 + a1 + a2 - M_PI;
+    else if (D < 0.0) om -= a0 + a1 + a2 - M_PI;
+    else { flag = -1; return 0.0; }
+  }
+  return om;
+}
+
+// corner positions come from the mesh nodes: rn (Nn+1) x 3 and ed (Ne+1) x 4
+__global__ void __launch_bounds__(256) k_domain_id(DevMesh M, FieldCtx X, const double *__restrict__ rn,
+                                                   const int *__restrict__ ed, int n, const double *__restrict__ pts,
+                                                   int *__restrict__ did_out, double *__restrict__ r0_out,
+                                                   int *__restrict__ l_out) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n) return;
+  double rt[3] = {pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2]};
+  int l1 = 0, l2 = 0;
+  if (X.ps == 1) {
+    l1 = (int)floor((rt[0] * X.d2y - rt[1] * X.d2x) * X.idd + 0.5);
+    l2 = (int)floor(-(rt[0] * X.d1y - rt[1] * X.d1x) * X.idd + 0.5);
+    rt[0] = rt[0] - (double)l1 * X.d1x - (double)l2 * X.d2x;
+    rt[1] = rt[1] - (double)l1 * X.d1y - (double)l2 * X.d2y;
+  }
+  r0_out[3 * ip] = rt[0]; r0_out[3 * ip + 1] = rt[1]; r0_out[3 * ip + 2] = rt[2];
+  l_out[2 * ip] = l1; l_out[2 * ip + 1] = l2;
+  double Og[16];
+  for (int d = 0; d <= X.MN; d++) Og[d] = 0.0;
+  int result = -1;
+  for (int d = 0; d <= X.MN && result < 0; d++) {
+    const DomView &V = X.dom[d];
+    bool hit = false;
+    for (int i = 1; i <= V.Ne; i++) {
+      int s = V.sid[i];
+      if (s <= 0) continue;
+      double rv[4][3];
+      for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 3; k++) rv[j][k] = rn[3 * ed[4 * s + j] + k] - rt[k];
+      int flag;
+      double om = solid_angle(!M.tri[s], rv, flag);
+      if (flag < 0) { hit = true; break; }
+      Og[d] += om;
+      Og[M.sd[s]] -= om;
+    }
+    if (hit) { result = d; break; }
+    if (d == 0 && fabs(Og[d]) < 2.0 * M_PI) result = d;
+    else if (Og[d] > 2.0 * M_PI) result = d;
+  }
+  did_out[ip] = result;
+}
+
+// ------------------------------------------------------------ source points
+// position, normal, weight and interpolation weights of quadrature point q of
+// element s for rule type (0: nodes, 1: 9-point, 2: GLN^2, 3: GHN^2)
+__device__ __forceinline__ void source_point(const DevMesh &M, const Tab &T, const ElemC &E, int s, int type, int q,
+                                             double *r, double *w, double &wt, double *Nv) {
+  if (type == 0) {
+    for (int d = 0; d < 3; d++) {
+      r[d] = M.ren[12 * s + 3 * q + d];
+      w[d] = M.wen[12 * s + 3 * q + d];
+    }
+    wt = T.wt_44[q];
+    for (int n = 0; n < 4; n++) Nv[n] = (n == q) ? 1.0 : 0.0;
+    return;
+  }
+  double z, e;
+  if (type == 1) {
+    z = T.zt_49[q]; e = T.et_49[q]; wt = T.wt_49[q];
+  } else if (type == 2) {
+    int i = q / 12, j = q % 12;
+    z = T.xli[i]; e = T.xli[j]; wt = T.wli[i] * T.wli[j];
+  } else {
+    int i = q / 64, j = q % 64;
+    z = T.xhi[i]; e = T.xhi[j]; wt = T.whi[i] * T.whi[j];
+  }
+  bil_rw(E, z, e, r, w);
+  for (int n = 0; n < 4; n++) Nv[n] = bil_N(n, z, e);
+}
+
+__device__ __forceinline__ int rule_points(int type) { return type == 0 ? 4 : type == 1 ? 9 : type == 2 ? 144 : 4096; }
+
+__device__ __forceinline__ cd ld(const double *a, size_t i) { return cd{a[2 * i], a[2 * i + 1]}; }
+
+// ------------------------------------------------------------ off-boundary field
+// out: p (2), pv (6) per point
+__global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P, int type,
+                                              int total, int n, const double *__restrict__ r0s,
+                                              const int *__restrict__ ls, const double *__restrict__ pts,
+                                              int *__restrict__ did_io, double *__restrict__ out,
+                                              long long *__restrict__ counters) {
+  const int ip = blockIdx.x, lane = threadIdx.x;
+  if (ip >= n) return;
+  const int did = did_io[ip];
+  if (did < 0) {
+    if (lane == 0) for (int i = 0; i < 8; i++) out[8 * ip + i] = 0.0;
+    return;
+  }
+  const Tab &T = *tab;
+  const DomView &V = X.dom[did];
+  const bool periodic = (X.ps == 1 && did == 0);
+  const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
+  const int npt = rule_points(type);
+  const double k = V.k;
+  cd Pa = {}, dPa[3] = {}, PVa[3] = {};
+  double F = 0.0, dF[3] = {0, 0, 0};
+  long long terms = 0, evals = 0;
+  int err = 0;
+  const long long items = (long long)V.Ne * npt;
+  for (long long it = lane; it < items; it += 64) {
+    const int s = (int)(it / npt) + 1, q = (int)(it % npt);
+    const int sd = V.sid[s], as = abs(sd);
+    ElemC E;
+    if (type != 0) load_elem(M, as, E);
+    double r[3], w[3], wt, Nv[4];
+    source_point(M, T, E, as, type, q, r, w, wt, Nv);
+    double vR[3] = {r[0] - r0[0], r[1] - r0[1], r[2] - r0[2]};
+    double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
+    double rdW = dot3(vR, w) * iR;
+    cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+    for (int m = 0; m < 4; m++) {
+      if (Nv[m] == 0.0) continue;
+      Pi += ld(V.P, 4 * s + m) * Nv[m];
+      dPi += ld(V.dP, 4 * s + m) * Nv[m];
+      for (int c = 0; c < 3; c++) {
+        pvi[c] += ld(V.pv, 12 * s + 3 * m + c) * Nv[m];
+        dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
+      }
+    }
+    cd a, b, dGp[3], dHp[3];
+    double sg = 1.0;
+    if (periodic) {
+      Sum10 gf;
+      int c = qpgf_eval<GF_V2>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+      if (c < 0) err = c;
+      else terms += c;
+      evals++;
+      a = gf.g * aW;
+      b = gf.dg[0] * w[0] + gf.dg[1] * w[1] + gf.dg[2] * w[2];
+      // Hessian as a symmetric matrix: xx, yy, zz, xy, yz, zx
+      const cd *h = gf.d2g;
+      cd Hw[3];
+      Hw[0] = h[0] * w[0] + h[3] * w[1] + h[5] * w[2];
+      Hw[1] = h[3] * w[0] + h[1] * w[1] + h[4] * w[2];
+      Hw[2] = h[5] * w[0] + h[4] * w[1] + h[2] * w[2];
+      for (int c2 = 0; c2 < 3; c2++) {
+        dGp[c2] = -(gf.dg[c2] * aW);
+        dHp[c2] = -Hw[c2];
+      }
+    } else {
+      cd ce = cis(k * aR), cb = cd{-1.0, k * aR};
+      a = ce * (BQD2_IFP * iR * aW);
+      b = cb * ce * (BQD2_IFP * rdW * iR * iR);
+      cd c3 = cd{k * k * aR * aR + 3.0 * cb.x, 3.0 * cb.y};
+      for (int c2 = 0; c2 < 3; c2++) {
+        dGp[c2] = cb * ce * (-BQD2_IFP * iR * iR * vR[c2] * iR * aW);
+        dHp[c2] = ce * (BQD2_IFP * iR * iR * iR) * (c3 * (vR[c2] * iR * rdW) - cb * w[c2]);
+      }
+      sg = sd < 0 ? -1.0 : 1.0;
+    }
+    Pa += (a * dPi - b * Pi * sg) * wt;
+    for (int c2 = 0; c2 < 3; c2++) {
+      dPa[c2] += (dGp[c2] * dPi - dHp[c2] * Pi * sg) * wt;
+      PVa[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wt;
+    }
+    F += sg * wt * rdW * iR * iR;
+    for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
+  }
+  Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
+  for (int c2 = 0; c2 < 3; c2++) {
+    dPa[c2] = cd{wave_sum(dPa[c2].x), wave_sum(dPa[c2].y)};
+    PVa[c2] = cd{wave_sum(PVa[c2].x), wave_sum(PVa[c2].y)};
+    dF[c2] = wave_sum(dF[c2]) * BQD2_IFP;
+  }
+  F = wave_sum(F) * BQD2_IFP;
+  terms = wave_sum_ll(terms);
+  evals = wave_sum_ll(evals);
+  int anyerr = __any(err != 0);
+  if (lane != 0) return;
+  atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
+  atomicAdd((unsigned long long *)&counters[1], (unsigned long long)evals);
+  if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+  const double iC = did == 0 ? 1.0 / (1.0 + F) : 1.0 / F;
+  const cd mk2 = cd{-V.k2x, -V.k2y};
+  cd p, pv[3];
+  int ret = did;
+  if (fabs(dF[0]) > CBD_CDF_D || fabs(dF[1]) > CBD_CDF_D || fabs(dF[2]) > CBD_CDF_D) {
+    // near the boundary: boundary-value form with the stored pv, dpv
+    p = mk2 * Pa * iC;
+    for (int c2 = 0; c2 < 3; c2++) pv[c2] = PVa[c2] * iC;
+  } else {
+    p = mk2 * Pa * iC;
+    for (int c2 = 0; c2 < 3; c2++) pv[c2] = -(dPa[c2] - Pa * dF[c2]) * iC;
+  }
+  const int l1 = ls[2 * ip], l2 = ls[2 * ip + 1];
+  if (l1 != 0 || l2 != 0) {
+    double arg = X.vkx * ((double)l1 * X.d1x + (double)l2 * X.d2x) + X.vky * ((double)l1 * X.d1y + (double)l2 * X.d2y);
+    cd ce = cis(arg);
+    p *= ce;
+    for (int c2 = 0; c2 < 3; c2++) pv[c2] *= ce;
+  }
+  if (total && did == 0) {
+    const double rr[3] = {pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2]};
+    cd pi, vi[3];
+    incident_pv(X, rr, pi, vi);
+    p += pi;
+    for (int c2 = 0; c2 < 3; c2++) pv[c2] += vi[c2];
+  }
+  out[8 * ip] = p.x;
+  out[8 * ip + 1] = p.y;
+  for (int c2 = 0; c2 < 3; c2++) {
+    out[8 * ip + 2 + 2 * c2] = pv[c2].x;
+    out[8 * ip + 3 + 2 * c2] = pv[c2].y;
+  }
+  did_io[ip] = ret;
+}
+
+// ------------------------------------------------------------ boundary points
+// one wavefront per boundary point (domain did, element position t, zeta, eta)
+__global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
+                                                 int type, int total, int did, int n, const int *__restrict__ tpos,
+                                                 const double *__restrict__ ze, double *__restrict__ out,
+                                                 long long *__restrict__ counters) {
+  const int ip = blockIdx.x, lane = threadIdx.x;
+  if (ip >= n) return;
+  const Tab &T = *tab;
+  const DomView &V = X.dom[did];
+  const bool periodic = (X.ps == 1 && did == 0);
+  const int t = tpos[ip];
+  const double zt = ze[2 * ip], et = ze[2 * ip + 1];
+  const int td = V.sid[t], at = abs(td);
+  ElemC Et;
+  load_elem(M, at, Et);
+  double rt[3], wt_[3];
+  bil_rw(Et, zt, et, rt, wt_);
+  const int npt = rule_points(type);
+  const double k = V.k;
+  cd Pa = {}, PVa[3] = {};
+  double F = 0.0;
+  long long terms = 0, evals = 0;
+  int err = 0;
+  // regular elements
+  const long long items = (long long)V.Ne * npt;
+  for (long long it = lane; it < items; it += 64) {
+    const int s = (int)(it / npt) + 1, q = (int)(it % npt);
+    if (s == t) continue;
+    const int sd = V.sid[s], as = abs(sd);
+    ElemC E;
+    if (type != 0) load_elem(M, as, E);
+    double r[3], w[3], wt, Nv[4];
+    source_point(M, T, E, as, type, q, r, w, wt, Nv);
+    double vR[3] = {r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]};
+    double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
+    double tD = dot3(vR, w) * iR * iR * iR;
+    cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+    for (int m = 0; m < 4; m++) {
+      if (Nv[m] == 0.0) continue;
+      Pi += ld(V.P, 4 * s + m) * Nv[m];
+      dPi += ld(V.dP, 4 * s + m) * Nv[m];
+      for (int c = 0; c < 3; c++) {
+        pvi[c] += ld(V.pv, 12 * s + 3 * m + c) * Nv[m];
+        dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
+      }
+    }
+    cd a, b;
+    double sg = 1.0;
+    if (periodic) {
+      Sum10 gf;
+      int c = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+      if (c < 0) err = c;
+      else terms += c;
+      evals++;
+      a = gf.g * aW;
+      b = gf.dg[0] * w[0] + gf.dg[1] * w[1] + gf.dg[2] * w[2];
+    } else {
+      cd ce = cis(k * aR), cb = cd{-1.0, k * aR};
+      a = ce * (BQD2_IFP * iR * aW);
+      b = cb * ce * (BQD2_IFP * tD);
+      sg = sd < 0 ? -1.0 : 1.0;
+    }
+    Pa += (a * dPi - b * Pi * sg) * wt;
+    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wt;
+    F += sg * wt * tD;
+  }
+  F *= BQD2_IFP;
+  // self element: polar rule about (zt, et)
+  {
+    const int sd = td, s = t;
+    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+    double r[5], th[5];
+    bil_corners(zt, et, r, th);
+    const bool plane = bil_is_plane(Et);
+    const double bdC = Et.cr[0][1] * Et.cw[0][2] + Et.cr[1][1] * Et.cw[1][2] + Et.cr[2][1] * Et.cw[2][2];
+    cd Ps = {}, PVs[3] = {};
+    double Fs = 0.0;
+    for (int p = lane; p < 4 * 64 * 2 * 12; p += 64) {
+      int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+      double tt = theta_node(th, j, hh, T.xhi[m]);
+      double rm = polar_rm(r, th, j, tt);
+      double c = cos(tt), sn = sin(tt);
+      double rr = 0.5 * rm * (T.xli[i] + 1.0);
+      double z = rr * c + zt, e = rr * sn + et, K[3], W[3];
+      polar_K(Et, c, sn, rr, zt, et, K);
+      for (int l = 0; l < 3; l++) W[l] = Et.cw[l][0] + Et.cw[l][1] * z + Et.cw[l][2] * e;
+      double aK = abs3(K), aW = abs3(W);
+      double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i];
+      cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+      for (int mm = 0; mm < 4; mm++) {
+        double Nn = bil_N(mm, z, e);
+        Pi += ld(V.P, 4 * s + mm) * Nn;
+        dPi += ld(V.dP, 4 * s + mm) * Nn;
+        for (int cc = 0; cc < 3; cc++) {
+          pvi[cc] += ld(V.pv, 12 * s + 3 * mm + cc) * Nn;
+          dpvi[cc] += ld(V.dpv, 12 * s + 3 * mm + cc) * Nn;
+        }
+      }
+      cd a, b;
+      if (periodic) {
+        double vR[3] = {rr * K[0], rr * K[1], rr * K[2]};
+        Sum10 gf;
+        int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+        if (cnt < 0) err = cnt;
+        else terms += cnt;
+        evals++;
+        a = gf.g * (aW * rr);
+        b = (gf.dg[0] * W[0] + gf.dg[1] * W[1] + gf.dg[2] * W[2]) * rr;
+      } else {
+        cd ce = cis(k * aK * rr), cb = cd{-1.0, k * rr * aK};
+        a = ce * (BQD2_IFP * aW / aK);
+        b = plane ? cd{0.0, 0.0} : cb * ce * (BQD2_IFP * bdC / (aK * aK * aK) * c * sn);
+      }
+      Ps += (a * dPi - b * Pi * sg) * wgt;
+      for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wgt;
+      if (!plane) Fs += wgt * (1.0 / (aK * aK * aK) * c * sn);
+    }
+    Pa += Ps;
+    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += PVs[c2];
+    F += sg * Fs * BQD2_IFP * bdC;
+  }
+  Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
+  for (int c2 = 0; c2 < 3; c2++) PVa[c2] = cd{wave_sum(PVa[c2].x), wave_sum(PVa[c2].y)};
+  F = wave_sum(F);
+  terms = wave_sum_ll(terms);
+  evals = wave_sum_ll(evals);
+  int anyerr = __any(err != 0);
+  if (lane != 0) return;
+  atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
+  atomicAdd((unsigned long long *)&counters[1], (unsigned long long)evals);
+  if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+  const double iC = did == 0 ? 1.0 / (1.0 + F) : 1.0 / F;
+  cd p = cd{-V.k2x, -V.k2y} * Pa * iC, pv[3];
+  for (int c2 = 0; c2 < 3; c2++) pv[c2] = PVa[c2] * iC;
+  if (total && did == 0) {
+    cd pi, vi[3];
+    incident_pv(X, rt, pi, vi);
+    p += pi;
+    for (int c2 = 0; c2 < 3; c2++) pv[c2] += vi[c2];
+  }
+  out[8 * ip] = p.x;
+  out[8 * ip + 1] = p.y;
+  for (int c2 = 0; c2 < 3; c2++) {
+    out[8 * ip + 2 + 2 * c2] = pv[c2].x;
+    out[8 * ip + 3 + 2 * c2] = pv[c2].y;
+  }
+}
+
+// ------------------------------------------------------------ force
+// per open-domain element: stress tensor at 4 nodes (type 0) or 9 points
+// (type 1, boundary field from k_field_bd in bdval) (bil_force_4p/9p)
+__global__ void __launch_bounds__(256) k_force_elem(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, int type,
+                                                    const double *__restrict__ bdval, double rcx, double rcy,
+                                                    double rcz, double *__restrict__ part) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const DomView &V = X.dom[0];
+  if (s > V.Ne) return;
+  const Tab &T = *tab;
+  const int as = abs(V.sid[s]);
+  const double c1 = 0.25 / (X.pw_c0 * X.pw_c0 * X.pw_rho0), c2 = 0.25 * X.pw_rho0, c3 = 0.5 * X.pw_rho0;
+  const double rc[3] = {rcx, rcy, rcz};
+  ElemC E;
+  load_elem(M, as, E);
+  double F[3] = {0, 0, 0}, N[3] = {0, 0, 0};
+  const int npt = type == 0 ? 4 : 9;
+  for (int i = 0; i < npt; i++) {
+    double r[3], w[3], wt;
+    cd p, v[3];
+    if (type == 0) {
+      for (int d = 0; d < 3; d++) {
+        r[d] = M.ren[12 * as + 3 * i + d];
+        w[d] = M.wen[12 * as + 3 * i + d];
+      }
+      wt = T.wt_44[i];
+      incident_pv(X, r, p, v);
+      p -= ld(V.P, 4 * s + i) * cd{X.dom[0].k2x, X.dom[0].k2y};
+      for (int d = 0; d < 3; d++) v[d] += ld(V.pv, 12 * s + 3 * i + d);
+    } else {
+      bil_rw(E, T.zt_49[i], T.et_49[i], r, w);
+      wt = T.wt_49[i];
+      const double *o = bdval + 8 * (9 * (size_t)(s - 1) + i);
+      p = cd{o[0], o[1]};
+      for (int d = 0; d < 3; d++) v[d] = cd{o[2 + 2 * d], o[3 + 2 * d]};
+    }
+    double pp = p.x * p.x + p.y * p.y;
+    double vv = v[0].x * v[0].x + v[0].y * v[0].y + v[1].x * v[1].x + v[1].y * v[1].y + v[2].x * v[2].x + v[2].y * v[2].y;
+    double t1 = c1 * pp - c2 * vv, Tm[9];
+    for (int l = 0; l < 3; l++)
+      for (int m = 0; m < 3; m++) Tm[3 * l + m] = c3 * (v[l].x * v[m].x + v[l].y * v[m].y);
+    Tm[0] += t1; Tm[4] += t1; Tm[8] += t1;
+    for (int l = 0; l < 3; l++) F[l] += (Tm[3 * l] * w[0] + Tm[3 * l + 1] * w[1] + Tm[3 * l + 2] * w[2]) * wt;
+    double d0 = r[0] - rc[0], d1 = r[1] - rc[1], d2 = r[2] - rc[2];
+    N[0] += ((d1 * Tm[6] - d2 * Tm[3]) * w[0] + (d1 * Tm[7] - d2 * Tm[4]) * w[1] + (d1 * Tm[8] - d2 * Tm[5]) * w[2]) * wt;
+    N[1] += ((d2 * Tm[0] - d0 * Tm[6]) * w[0] + (d2 * Tm[1] - d0 * Tm[7]) * w[1] + (d2 * Tm[2] - d0 * Tm[8]) * w[2]) * wt;
+    N[2] += ((d0 * Tm[3] - d1 * Tm[0]) * w[0] + (d0 * Tm[4] - d1 * Tm[1]) * w[1] + (d0 * Tm[5] - d1 * Tm[2]) * w[2]) * wt;
+  }
+  for (int d = 0; d < 3; d++) {
+    part[6 * (s - 1) + d] = F[d];
+    part[6 * (s - 1) + 3 + d] = N[d];
+  }
+}
+
+// ordered sum of the per-element parts (one wavefront, strided partials then a
+// fixed reduction tree: deterministic run to run)
+__global__ void __launch_bounds__(64) k_force_sum(int ne, const double *__restrict__ part, double *__restrict__ FN) {
+  const int lane = threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = lane; s < ne; s += 64)
+    for (int d = 0; d < 6; d++) acc[d] += part[6 * s + d];
+  for (int d = 0; d < 6; d++) {
+    double v = wave_sum(acc[d]);
+    if (lane == 0) FN[d] = v;
+  }
+}
+
+// ------------------------------------------------------------ entry points
+static FieldCtx make_ctx(const bqd2_problem *p) {
+  FieldCtx X;
+  memset(&X, 0, sizeof(X));
+  X.MN = p->m.MN;
+  X.ps = p->m.ps;
+  for (int d = 0; d <= p->m.MN; d++) {
+    const DevDomain &D = p->dom[d];
+    X.dom[d].Ne = D.Ne;
+    X.dom[d].sid = D.sid;
+    X.dom[d].P = D.P;
+    X.dom[d].dP = D.dP;
+    X.dom[d].pv = D.pv;
+    X.dom[d].dpv = D.dpv;
+    X.dom[d].k = p->k0[d];
+    X.dom[d].k2x = p->k2[d][0];
+    X.dom[d].k2y = p->k2[d][1];
+  }
+  memcpy(X.pw_p, p->pw_p, sizeof(X.pw_p));
+  X.pw_rho0 = p->pw_rho0;
+  X.pw_c0 = p->pw_c0;
+  X.pw_k0 = p->pw_k0;
+  memcpy(X.pw_tv, p->pw_tv, sizeof(X.pw_tv));
+  memcpy(X.pw_R, p->pw_R, sizeof(X.pw_R));
+  X.d1x = p->qd8[4]; X.d1y = p->qd8[5]; X.d2x = p->qd8[6]; X.d2y = p->qd8[7];
+  X.idd = 1.0 / (X.d1x * X.d2y - X.d2x * X.d1y);
+  X.vkx = p->qd8[1];
+  X.vky = p->qd8[2];
+  return X;
+}
+
+struct Scratch {  // per-call device buffers, released on scope exit
+  std::vector<void *> v;
+  ~Scratch() {
+    for (void *q : v) hipFree(q);
+  }
+  int get(void **q, size_t bytes) {
+    hipError_t e = hipMalloc(q, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+      bqd2_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+      return BQD2_E_NOMEM;
+    }
+    v.push_back(*q);
+    return BQD2_OK;
+  }
+};
+
+extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, double *out, int *did,
+                          long long *gf_evals) {
+  BQD2_CHECK(p && n >= 0 && pts && out, BQD2_E_ARG, "bqd2_field: bad arguments");
+  BQD2_CHECK(type >= 0 && type <= 3, BQD2_E_ARG, "bqd2_field: type must be 0..3 (4-point, 9-point, GLN, GHN)");
+  if (n == 0) return BQD2_OK;
+  hipStream_t st = p->stream;
+  Scratch S;
+  double *d_pts, *d_r0, *d_out;
+  int *d_l, *d_did;
+  int rc;
+  if ((rc = S.get((void **)&d_pts, (size_t)n * 24)) || (rc = S.get((void **)&d_r0, (size_t)n * 24)) ||
+      (rc = S.get((void **)&d_out, (size_t)n * 64)) || (rc = S.get((void **)&d_l, (size_t)n * 8)) ||
+      (rc = S.get((void **)&d_did, (size_t)n * 4)))
+    return rc;
+  BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
+  FieldCtx X = make_ctx(p);
+  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st));
+  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
+                     d_r0, d_l);
+  hipLaunchKernelGGL(k_field, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, n, d_r0, d_l, d_pts,
+                     d_did, d_out, p->d_counters);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
+  if (did) BQD2_HIP(hipMemcpyAsync(did, d_did, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  long long cnt[4];
+  BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipStreamSynchronize(st));
+  if (gf_evals) *gf_evals = cnt[1];
+  BQD2_CHECK(cnt[2] == 0, BQD2_E_GF, "bqd2_field: Green's function error at %lld points", cnt[2]);
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_field_bd(bqd2_problem *p, int did, int n, const int *t, const double *ze, int type, int total,
+                             double *out) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN && n >= 0 && t && ze && out, BQD2_E_ARG, "bqd2_field_bd: bad arguments");
+  BQD2_CHECK(type >= 0 && type <= 3, BQD2_E_ARG, "bqd2_field_bd: type must be 0..3");
+  if (n == 0) return BQD2_OK;
+  for (int i = 0; i < n; i++)
+    BQD2_CHECK(t[i] >= 1 && t[i] <= p->dom[did].Ne, BQD2_E_ARG, "bqd2_field_bd: element %d out of range", t[i]);
+  hipStream_t st = p->stream;
+  Scratch S;
+  int *d_t;
+  double *d_ze, *d_out;
+  int rc;
+  if ((rc = S.get((void **)&d_t, (size_t)n * 4)) || (rc = S.get((void **)&d_ze, (size_t)n * 16)) ||
+      (rc = S.get((void **)&d_out, (size_t)n * 64)))
+    return rc;
+  BQD2_HIP(hipMemcpyAsync(d_t, t, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  BQD2_HIP(hipMemcpyAsync(d_ze, ze, (size_t)n * 16, hipMemcpyHostToDevice, st));
+  FieldCtx X = make_ctx(p);
+  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st));
+  hipLaunchKernelGGL(k_field_bd, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, did, n, d_t, d_ze,
+                     d_out, p->d_counters);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
+  long long cnt[4];
+  BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipStreamSynchronize(st));
+  BQD2_CHECK(cnt[2] == 0, BQD2_E_GF, "bqd2_field_bd: Green's function error at %lld points", cnt[2]);
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F, double *N) {
+  BQD2_CHECK(p && rc && F && N, BQD2_E_ARG, "bqd2_force: bad arguments");
+  const DevDomain &D0 = p->dom[0];
+  const int ne = D0.Ne;
+  if (ne == 0) {
+    for (int i = 0; i < 3; i++) F[i] = N[i] = 0.0;
+    return BQD2_OK;
+  }
+  hipStream_t st = p->stream;
+  Scratch S;
+  double *d_bd = nullptr, *d_part, *d_FN;
+  int rc_;
+  if ((rc_ = S.get((void **)&d_part, (size_t)ne * 48)) || (rc_ = S.get((void **)&d_FN, 48))) return rc_;
+  FieldCtx X = make_ctx(p);
+  if (type != 0) {
+    // total field at the 9 Gauss points of every open-domain element (bil_force_9p)
+    std::vector<int> tpos((size_t)ne * 9);
+    std::vector<double> ze((size_t)ne * 18);
+    for (int s = 1; s <= ne; s++)
+      for (int i = 0; i < 9; i++) {
+        tpos[9 * (s - 1) + i] = s;
+        ze[2 * (9 * (s - 1) + i)] = p->htab.zt_49[i];
+        ze[2 * (9 * (s - 1) + i) + 1] = p->htab.et_49[i];
+      }
+    int *d_t;
+    double *d_ze;
+    if ((rc_ = S.get((void **)&d_bd, (size_t)ne * 9 * 64)) || (rc_ = S.get((void **)&d_t, tpos.size() * 4)) ||
+        (rc_ = S.get((void **)&d_ze, ze.size() * 8)))
+      return rc_;
+    BQD2_HIP(hipMemcpyAsync(d_t, tpos.data(), tpos.size() * 4, hipMemcpyHostToDevice, st));
+    BQD2_HIP(hipMemcpyAsync(d_ze, ze.data(), ze.size() * 8, hipMemcpyHostToDevice, st));
+    BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st));
+    hipLaunchKernelGGL(k_field_bd, dim3(ne * 9), dim3(64), 0, st, p->m, p->tab, X, p->gf, 1, 1, 0, ne * 9, d_t, d_ze,
+                       d_bd, p->d_counters);
+    BQD2_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_force_elem, dim3(bqd2_blocks(ne, 256)), dim3(256), 0, st, p->m, p->tab, X, type, d_bd, rc[0],
+                     rc[1], rc[2], d_part);
+  hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(64), 0, st, ne, d_part, d_FN);
+  BQD2_HIP(hipGetLastError());
+  double FN[6];
+  BQD2_HIP(hipMemcpyAsync(FN, d_FN, 48, hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < 3; i++) {
+    F[i] = FN[i];
+    N[i] = FN[3 + i];
+  }
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_domain_id(bqd2_problem *p, int n, const double *pts, int *did) {
+  BQD2_CHECK(p && n >= 0 && pts && did, BQD2_E_ARG, "bqd2_domain_id: bad arguments");
+  if (n == 0) return BQD2_OK;
+  hipStream_t st = p->stream;
+  Scratch S;
+  double *d_pts, *d_r0;
+  int *d_l, *d_did, rc;
+  if ((rc = S.get((void **)&d_pts, (size_t)n * 24)) || (rc = S.get((void **)&d_r0, (size_t)n * 24)) ||
+      (rc = S.get((void **)&d_l, (size_t)n * 8)) || (rc = S.get((void **)&d_did, (size_t)n * 4)))
+    return rc;
+  BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
+  FieldCtx X = make_ctx(p);
+  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
+                     d_r0, d_l);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipMemcpyAsync(did, d_did, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipStreamSynchronize(st));
+  return BQD2_OK;
+}

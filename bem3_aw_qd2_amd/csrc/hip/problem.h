@@ -46,6 +46,9 @@ struct bqd2_problem {
   Tab *tab;            // device
   Tab htab;            // host copy
   GfParams gf;
+  double qd8[8];
+  double *rn;         // (Nn+1) x 3 mesh nodes (device)
+  int *ed;            // (Ne+1) x 4 corner node ids (device)
   double k2[16][2];    // host copies of the media constants (MN+1 <= 16)
   double k0[16], rho0[16];
   double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];

@@ -50,6 +50,8 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
   TRY(upload(p, (double **)&p->m.ren, m->ren, ne1 * 12));
   TRY(upload(p, (double **)&p->m.wen, m->wen, ne1 * 12));
   TRY(upload(p, (int **)&p->m.tri, tri.data(), ne1));
+  TRY(upload(p, &p->rn, m->rn, ((size_t)m->Nn + 1) * 3));
+  TRY(upload(p, &p->ed, m->ed, ne1 * 4));
   TRY(upload(p, (int **)&p->m.eni, m->eni, ne1 * 4));
   TRY(upload(p, (int **)&p->m.md, m->md, ne1));
   TRY(upload(p, (int **)&p->m.sd, m->sd, ne1));
@@ -77,6 +79,7 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
   p->pw_rho0 = m->pw_rho0; p->pw_c0 = m->pw_c0; p->pw_k0 = m->pw_k0;
   memcpy(p->pw_tv, m->pw_tv, sizeof(p->pw_tv));
   memcpy(p->pw_R, m->pw_R, sizeof(p->pw_R));
+  memcpy(p->qd8, m->qd, sizeof(p->qd8));
   if (m->ps == 1) TRY(bqd2_gf_params(m->qd, &p->gf));
   else memset(&p->gf, 0, sizeof(p->gf));
 

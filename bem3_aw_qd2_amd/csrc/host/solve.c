@@ -38,7 +38,7 @@ static double wall(void)
 
 /* flat copies of the AQD2 arrays handed to bqd2_create */
 typedef struct {
-  double *cr, *cw, *ren, *wen, *Pi, *dPi, *k0, *k2, *rho0;
+  double *rn, *cr, *cw, *ren, *wen, *Pi, *dPi, *k0, *k2, *rho0;
   int *ed, *eni, *md, *sd, *dom_ne, *dom_sid;
 } packed;
 
@@ -47,6 +47,9 @@ static void pack_model(const AQD2 *qd, bqd2_model *m, packed *pk)
   const BOUD *bd = &qd->bd;
   size_t ne1 = (size_t)bd->Ne + 1;
   memset(m, 0, sizeof(*m));
+  pk->rn = calloc(((size_t)bd->Nn + 1) * 3, sizeof(double));
+  for (int i = 0; i <= bd->Nn; i++)
+    for (int d = 0; d < 3; d++) pk->rn[3 * i + d] = bd->rn[i][d];
   pk->cr = calloc(ne1 * 12, sizeof(double));
   pk->cw = calloc(ne1 * 9, sizeof(double));
   pk->ren = calloc(ne1 * 12, sizeof(double));
@@ -102,6 +105,7 @@ static void pack_model(const AQD2 *qd, bqd2_model *m, packed *pk)
   m->NN = bd->NN;
   m->MN = qd->MN;
   m->ps = bd->ps;
+  m->rn = pk->rn;
   m->cr = pk->cr; m->cw = pk->cw; m->ren = pk->ren; m->wen = pk->wen;
   m->ed = pk->ed; m->eni = pk->eni; m->md = pk->md; m->sd = pk->sd;
   m->Pi = pk->Pi; m->dPi = pk->dPi;
@@ -130,7 +134,7 @@ static void pack_model(const AQD2 *qd, bqd2_model *m, packed *pk)
 
 static void free_packed(packed *pk)
 {
-  free(pk->cr); free(pk->cw); free(pk->ren); free(pk->wen); free(pk->Pi); free(pk->dPi);
+  free(pk->rn); free(pk->cr); free(pk->cw); free(pk->ren); free(pk->wen); free(pk->Pi); free(pk->dPi);
   free(pk->ed); free(pk->eni); free(pk->md); free(pk->sd); free(pk->dom_ne); free(pk->dom_sid);
   free(pk->k0); free(pk->k2); free(pk->rho0);
 }

@@ -106,3 +106,60 @@ class Model:
 
     def finalize(self):
         self.lib.finalize_aqd2(self.buf)
+
+    # ----------------------------------------------------------- field API
+    def _field1(self, fn, r, type_):
+        p = (C.c_double * 2)()
+        v = (C.c_double * 6)()
+        rt = (C.c_double * 3)(*r)
+        did = fn(p, v, rt, int(type_), self.buf)
+        return did, complex(p[0], p[1]), np.array([complex(v[2 * i], v[2 * i + 1]) for i in range(3)])
+
+    def pv_t(self, r, type_=0):
+        """total field (pv_t_aqd2): (domain id, p, v[3])"""
+        return self._field1(self.lib.pv_t_aqd2, r, type_)
+
+    def pv_s(self, r, type_=0):
+        """scattered field (pv_s_aqd2)"""
+        return self._field1(self.lib.pv_s_aqd2, r, type_)
+
+    def pv_i(self, r, type_=0):
+        """incident field (pv_i_aqd2)"""
+        return self._field1(self.lib.pv_i_aqd2, r, type_)
+
+    def domain_id(self, r):
+        rt = (C.c_double * 3)(*r)
+        return self.lib.domain_id_aqd2(rt, self.buf)
+
+    def pv_batch(self, pts, type_=0, total=1):
+        """field at n points in one device call (pv_batch_aqd2):
+        returns p (n,), v (n, 3) complex and the domain ids (n,)."""
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        n = len(pts)
+        p = np.zeros(n, dtype=np.complex128)
+        v = np.zeros((n, 3), dtype=np.complex128)
+        did = np.zeros(n, dtype=np.int32)
+        self.lib.pv_batch_aqd2(n, pts.ctypes.data, int(type_), int(total), p.ctypes.data, v.ctypes.data,
+                               did.ctypes.data, self.buf)
+        return p, v, did
+
+    def pv_t_bd(self, did, t, zeta, eta, type_=0):
+        """total field on the boundary (pv_t_bd_aqd2): (p, v[3])"""
+        p = (C.c_double * 2)()
+        v = (C.c_double * 6)()
+        self.lib.pv_t_bd_aqd2(p, v, int(did), int(t), C.c_double(zeta), C.c_double(eta), int(type_), self.buf)
+        return complex(p[0], p[1]), np.array([complex(v[2 * i], v[2 * i + 1]) for i in range(3)])
+
+    def force(self, rc=(0.0, 0.0, 0.0), type_=0):
+        """radiation force and torque about rc (force_FN_aqd2)"""
+        F = (C.c_double * 3)()
+        N = (C.c_double * 3)()
+        r = (C.c_double * 3)(*rc)
+        self.lib.force_FN_aqd2(F, N, r, int(type_), self.buf)
+        return np.array(F[:]), np.array(N[:])
+
+    def surface_area(self, did, type_=0):
+        S = C.c_double()
+        self.lib.surface_area_aqd2(C.byref(S), int(did), int(type_), self.buf)
+        return S.value
+

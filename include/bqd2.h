@@ -71,6 +71,7 @@ int bqd2_gf_batch(const GfParams *p, int mode, int n, const double *d_r, double 
  * (row 0 unused), as in AQD2; complex values are (re, im) pairs. */
 typedef struct {
   int Ne, Nn, NN, MN, ps;
+  const double *rn;      /* (Nn+1) x 3  : mesh nodes                      */
   const double *cr;      /* (Ne+1) x 12 : cr[s][d][j] at 4d+j            */
   const double *cw;      /* (Ne+1) x 9  : cw[s][d][j] at 3d+j            */
   const double *ren;     /* (Ne+1) x 12 : node n, coordinate d at 3n+d   */
@@ -128,6 +129,27 @@ int bqd2_solve_velocity(bqd2_problem *p);
 int bqd2_download_solution(bqd2_problem *p, int did, double *P, double *dP, double *pv, double *dpv);
 int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double *dP, const double *pv,
                          const double *dpv);
+
+/* Pressure and particle velocity at n points (pts host n x 3): the batched form
+ * of pv_t_aqd2 (total = 1) / pv_s_aqd2 (total = 0), bem3_aw_qd2_field.c:103-297,
+ * with rule type 0: 4-point, 1: 9-point, 2: GLN^2, 3: GHN^2.  out (host n x 8):
+ * p (re, im), pv x3 (re, im); did (host n, may be NULL): the domain id the
+ * reference returns.  gf_evals (may be NULL): Green's function evaluations. */
+int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, double *out, int *did,
+               long long *gf_evals);
+
+/* Domain of each point (domain_id / domain_id_aqd2, bem3_aw_qd2.c:111-156, :1148-1194):
+ * solid-angle classification of the point reduced to the unit cell. */
+int bqd2_domain_id(bqd2_problem *p, int n, const double *pts, int *did);
+
+/* Same at boundary points (zeta, eta) of elements t (1-based position in
+ * domain did's list): pv_s_bd_aqd2 / pv_t_bd_aqd2 (:394-469). ze host n x 2. */
+int bqd2_field_bd(bqd2_problem *p, int did, int n, const int *t, const double *ze, int type, int total, double *out);
+
+/* Radiation force F and torque N about rc on the open-domain boundary:
+ * force_FN_aqd2 (bem3_aw_qd2_force.c:4-42); type 0 node values, otherwise the
+ * 9-point rule with boundary field evaluations. */
+int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F, double *N);
 
 /* dense complex LU helpers on device buffers (column-major, leading dimension n),
  * used by the solves above and exposed for tests: A is overwritten by its
