@@ -46,7 +46,9 @@ class GfParams(C.Structure):
 class AssemblyStats(C.Structure):
     _fields_ = [("pairs_gl", C.c_longlong), ("pairs_gh", C.c_longlong), ("self_rows", C.c_longlong),
                 ("gf_evals", C.c_longlong), ("gf_terms", C.c_longlong),
-                ("ms_decide", C.c_double), ("ms_pairs", C.c_double), ("ms_self", C.c_double)]
+                ("ms_decide", C.c_double), ("ms_pairs", C.c_double), ("ms_self", C.c_double),
+                ("pair_terms", C.c_longlong * 4), ("pair_evals", C.c_longlong),
+                ("self_terms", C.c_longlong * 4), ("self_evals", C.c_longlong)]
 
 
 def hip_lib():
@@ -61,6 +63,13 @@ def hip_lib():
         lib.bqd2_download_solution.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
         lib.bqd2_zgetrf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.bqd2_zgetrs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.bqd2_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        lib.bqd2_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
+        lib.bqd2_field_bd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_void_p]
+        lib.bqd2_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        lib.bqd2_domain_id.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         lib.bqd2_gf_params.argtypes = [C.POINTER(C.c_double), C.POINTER(GfParams)]
         lib.bqd2_gf_batch.argtypes = [C.POINTER(GfParams), C.c_int, C.c_int, C.c_void_p, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
 
   cd g[4] = {}, h[4] = {};
   double F = 0.0, F0 = 0.0, F1 = 0.0;
-  long long terms = 0;
+  GfTally tl{0, 0, 0, 0};
   int err = 0;
   const int npts = n * n;
   for (int p = lane; p < npts; p += WAVE) {
@@ -78,9 +78,8 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     PtGeo q;
     point_geo(E, z, e, rt, vtz, vte, q);
     Sum10 gf;
-    int c = qpgf_eval<GF_V1>(P, q.vR[0], q.vR[1], q.vR[2], 1.0e-15, gf);
+    int c = qpgf_eval<GF_V1>(P, q.vR[0], q.vR[1], q.vR[2], 1.0e-15, gf, tl);
     if (c < 0) err = c;
-    else terms += c;
     cd hn = gf.dg[0] * q.vW[0] + gf.dg[1] * q.vW[1] + gf.dg[2] * q.vW[2];
 #pragma unroll
     for (int ep = 0; ep < 4; ep++) {
@@ -98,7 +97,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     h[ep].x = wave_sum(h[ep].x); h[ep].y = wave_sum(h[ep].y);
   }
   F = wave_sum(F); F0 = wave_sum(F0); F1 = wave_sum(F1);
-  terms = wave_sum_ll(terms);
+  long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   int anyerr = __any(err != 0);
   if (lane == 0) {
     const size_t base = ((size_t)row * D.N + (size_t)(s - 1) * 4) * 2;
@@ -109,9 +108,8 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     }
     double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
     fp[0] = F * BQD2_IFP; fp[1] = F0 * BQD2_IFP; fp[2] = F1 * BQD2_IFP;
-    atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
-    atomicAdd((unsigned long long *)&counters[1], (unsigned long long)npts);
-    if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+    tally_add(counters + CNT_PAIRS, tsp, tpr, tev, tfd, npts);
+    if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   }
 }
 
@@ -234,7 +232,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
   double r[5], th[5];
   bil_corners(zt, et, r, th);
   cd g[4] = {}, h[4] = {};
-  long long terms = 0;
+  GfTally tl{0, 0, 0, 0};
   int err = 0;
   const int npts = 4 * 64 * 2 * 12;
   for (int p = lane; p < npts; p += WAVE) {
@@ -251,9 +249,8 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
     }
     double aW = abs3(vW);
     Sum10 gf;
-    int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+    int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
     if (cnt < 0) err = cnt;
-    else terms += cnt;
     cd hn = gf.dg[0] * vW[0] + gf.dg[1] * vW[1] + gf.dg[2] * vW[2];
     double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i] * rr;
 #pragma unroll
@@ -268,7 +265,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
     g[n] = cd{wave_sum(g[n].x), wave_sum(g[n].y)};
     h[n] = cd{wave_sum(h[n].x), wave_sum(h[n].y)};
   }
-  terms = wave_sum_ll(terms);
+  long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   int anyerr = __any(err != 0);
   const double F = self_F_wave(E, T, zt, et, lane);
   // central difference of F along the element parameters (q0_bil_dcoef_v2_cc.c:336-390)
@@ -292,9 +289,8 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
     fp[0] = F;
     fp[1] = 1.0 / abs3(Tz) * (Fzp - Fzm) / (2.0 * h_);
     fp[2] = 1.0 / abs3(Te) * (Fep - Fem) / (2.0 * h_);
-    atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
-    atomicAdd((unsigned long long *)&counters[1], (unsigned long long)npts);
-    if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+    tally_add(counters + CNT_SELF, tsp, tpr, tev, tfd, npts);
+    if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   }
 }
 
@@ -530,7 +526,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipEventCreate(&e1));
   BQD2_HIP(hipEventCreate(&e2));
   BQD2_HIP(hipEventCreate(&e3));
-  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st_));
+  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st_));
   BQD2_HIP(hipMemsetAsync(D.G, 0, (size_t)D.N * D.N * 16, st_));
   BQD2_HIP(hipMemsetAsync(D.H, 0, (size_t)D.N * D.N * 16, st_));
   if (D.dG) {
@@ -557,7 +553,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipEventRecord(e3, st_));
   BQD2_HIP(hipEventSynchronize(e3));
-  long long cnt[4];
+  long long cnt[CNT_N];
   BQD2_HIP(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
   std::vector<unsigned char> rule((size_t)npairs);
   BQD2_HIP(hipMemcpy(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost));
@@ -570,16 +566,24 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     BQD2_HIP(hipEventElapsedTime(&b, e1, e2));
     BQD2_HIP(hipEventElapsedTime(&c, e2, e3));
     st->ms_decide = a; st->ms_pairs = b; st->ms_self = c;
-    st->gf_terms = cnt[0];
-    st->gf_evals = cnt[1];
+    for (int i = 0; i < 4; i++) {
+      st->pair_terms[i] = cnt[CNT_PAIRS + i];
+      st->self_terms[i] = cnt[CNT_SELF + i];
+    }
+    st->pair_evals = cnt[CNT_PAIRS + 4];
+    st->self_evals = cnt[CNT_SELF + 4];
+    st->gf_evals = st->pair_evals + st->self_evals;
+    st->gf_terms = 0;
+    for (int i = 0; i < 4; i++) st->gf_terms += st->pair_terms[i] + st->self_terms[i];
     st->pairs_gh = ngh;
     long long valid_rows = 0;
     for (int row = 0; row < D.N; row++) valid_rows += 1;  // quads only: every slot is a node
     st->self_rows = valid_rows;
     st->pairs_gl = valid_rows * (D.Ne - 1) - ngh;
   }
-  hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
-  BQD2_CHECK(cnt[2] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)", cnt[2], did);
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+  BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)",
+             cnt[CNT_ERR], did);
   D.assembled = true;
   return BQD2_OK;
 }

@@ -192,7 +192,8 @@ __global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__
   const double k = V.k;
   cd Pa = {}, dPa[3] = {}, PVa[3] = {};
   double F = 0.0, dF[3] = {0, 0, 0};
-  long long terms = 0, evals = 0;
+  GfTally tl{0, 0, 0, 0};
+  long long evals = 0;
   int err = 0;
   const long long items = (long long)V.Ne * npt;
   for (long long it = lane; it < items; it += 64) {
@@ -219,9 +220,8 @@ __global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__
     double sg = 1.0;
     if (periodic) {
       Sum10 gf;
-      int c = qpgf_eval<GF_V2>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+      int c = qpgf_eval<GF_V2>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
       if (c < 0) err = c;
-      else terms += c;
       evals++;
       a = gf.g * aW;
       b = gf.dg[0] * w[0] + gf.dg[1] * w[1] + gf.dg[2] * w[2];
@@ -261,13 +261,12 @@ __global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__
     dF[c2] = wave_sum(dF[c2]) * BQD2_IFP;
   }
   F = wave_sum(F) * BQD2_IFP;
-  terms = wave_sum_ll(terms);
+  long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   evals = wave_sum_ll(evals);
   int anyerr = __any(err != 0);
   if (lane != 0) return;
-  atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
-  atomicAdd((unsigned long long *)&counters[1], (unsigned long long)evals);
-  if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+  tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
+  if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   const double iC = did == 0 ? 1.0 / (1.0 + F) : 1.0 / F;
   const cd mk2 = cd{-V.k2x, -V.k2y};
   cd p, pv[3];
@@ -325,7 +324,8 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
   const double k = V.k;
   cd Pa = {}, PVa[3] = {};
   double F = 0.0;
-  long long terms = 0, evals = 0;
+  GfTally tl{0, 0, 0, 0};
+  long long evals = 0;
   int err = 0;
   // regular elements
   const long long items = (long long)V.Ne * npt;
@@ -354,9 +354,8 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
     double sg = 1.0;
     if (periodic) {
       Sum10 gf;
-      int c = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+      int c = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
       if (c < 0) err = c;
-      else terms += c;
       evals++;
       a = gf.g * aW;
       b = gf.dg[0] * w[0] + gf.dg[1] * w[1] + gf.dg[2] * w[2];
@@ -406,9 +405,8 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
       if (periodic) {
         double vR[3] = {rr * K[0], rr * K[1], rr * K[2]};
         Sum10 gf;
-        int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf);
+        int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
         if (cnt < 0) err = cnt;
-        else terms += cnt;
         evals++;
         a = gf.g * (aW * rr);
         b = (gf.dg[0] * W[0] + gf.dg[1] * W[1] + gf.dg[2] * W[2]) * rr;
@@ -428,13 +426,12 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
   Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
   for (int c2 = 0; c2 < 3; c2++) PVa[c2] = cd{wave_sum(PVa[c2].x), wave_sum(PVa[c2].y)};
   F = wave_sum(F);
-  terms = wave_sum_ll(terms);
+  long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   evals = wave_sum_ll(evals);
   int anyerr = __any(err != 0);
   if (lane != 0) return;
-  atomicAdd((unsigned long long *)&counters[0], (unsigned long long)terms);
-  atomicAdd((unsigned long long *)&counters[1], (unsigned long long)evals);
-  if (anyerr) atomicAdd((unsigned long long *)&counters[2], 1ull);
+  tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
+  if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   const double iC = did == 0 ? 1.0 / (1.0 + F) : 1.0 / F;
   cd p = cd{-V.k2x, -V.k2y} * Pa * iC, pv[3];
   for (int c2 = 0; c2 < 3; c2++) pv[c2] = PVa[c2] * iC;
@@ -553,7 +550,7 @@ static FieldCtx make_ctx(const bqd2_problem *p) {
 struct Scratch {  // per-call device buffers, released on scope exit
   std::vector<void *> v;
   ~Scratch() {
-    for (void *q : v) hipFree(q);
+    for (void *q : v) (void)hipFree(q);
   }
   int get(void **q, size_t bytes) {
     hipError_t e = hipMalloc(q, bytes ? bytes : 16);
@@ -582,7 +579,7 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
     return rc;
   BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
-  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st));
+  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st));
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
   hipLaunchKernelGGL(k_field, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, n, d_r0, d_l, d_pts,
@@ -590,11 +587,11 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
   if (did) BQD2_HIP(hipMemcpyAsync(did, d_did, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-  long long cnt[4];
+  long long cnt[CNT_N];
   BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
-  if (gf_evals) *gf_evals = cnt[1];
-  BQD2_CHECK(cnt[2] == 0, BQD2_E_GF, "bqd2_field: Green's function error at %lld points", cnt[2]);
+  if (gf_evals) *gf_evals = cnt[CNT_FIELD + 4];
+  BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_field: Green's function error at %lld points", cnt[CNT_ERR]);
   return BQD2_OK;
 }
 
@@ -616,15 +613,15 @@ extern "C" int bqd2_field_bd(bqd2_problem *p, int did, int n, const int *t, cons
   BQD2_HIP(hipMemcpyAsync(d_t, t, (size_t)n * 4, hipMemcpyHostToDevice, st));
   BQD2_HIP(hipMemcpyAsync(d_ze, ze, (size_t)n * 16, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
-  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st));
+  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st));
   hipLaunchKernelGGL(k_field_bd, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, did, n, d_t, d_ze,
                      d_out, p->d_counters);
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
-  long long cnt[4];
+  long long cnt[CNT_N];
   BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
-  BQD2_CHECK(cnt[2] == 0, BQD2_E_GF, "bqd2_field_bd: Green's function error at %lld points", cnt[2]);
+  BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_field_bd: Green's function error at %lld points", cnt[CNT_ERR]);
   return BQD2_OK;
 }
 
@@ -659,7 +656,7 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
       return rc_;
     BQD2_HIP(hipMemcpyAsync(d_t, tpos.data(), tpos.size() * 4, hipMemcpyHostToDevice, st));
     BQD2_HIP(hipMemcpyAsync(d_ze, ze.data(), ze.size() * 8, hipMemcpyHostToDevice, st));
-    BQD2_HIP(hipMemsetAsync(p->d_counters, 0, 4 * sizeof(long long), st));
+    BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st));
     hipLaunchKernelGGL(k_field_bd, dim3(ne * 9), dim3(64), 0, st, p->m, p->tab, X, p->gf, 1, 1, 0, ne * 9, d_t, d_ze,
                        d_bd, p->d_counters);
     BQD2_HIP(hipGetLastError());
@@ -669,8 +666,11 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
   hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(64), 0, st, ne, d_part, d_FN);
   BQD2_HIP(hipGetLastError());
   double FN[6];
+  long long cnt[CNT_N] = {0};
   BQD2_HIP(hipMemcpyAsync(FN, d_FN, 48, hipMemcpyDeviceToHost, st));
+  if (type != 0) BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
+  BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_force: Green's function error at %lld boundary points", cnt[CNT_ERR]);
   for (int i = 0; i < 3; i++) {
     F[i] = FN[i];
     N[i] = FN[3 + i];

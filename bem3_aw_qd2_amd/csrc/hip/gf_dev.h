@@ -261,6 +261,12 @@ __device__ __forceinline__ int fd_term(const GfParams& P, double rx, double ry, 
 }
 
 // ---------------------------------------------------------------- rows
+// terms summed by one evaluation, split by kind (spatial images, propagating and
+// evanescent spectral orders of the Ewald sum, orders of the pure spectral sum)
+struct GfTally {
+  int sp, pr, ev, fd;
+};
+
 template <int M>
 __device__ int spatial_row(const GfParams& P, double rx, double ry, double rz, double E, double eps, int l1,
                            Sum10& S) {
@@ -290,18 +296,20 @@ __device__ int spatial_row(const GfParams& P, double rx, double ry, double rz, d
 // the reference counts propagating orders only in its outward loops
 template <int M>
 __device__ int spectral_row(const GfParams& P, double rx, double ry, double rz, double E, double eps, int n1,
-                            Sum10& S, int& flag) {
+                            Sum10& S, int& flag, int& nev) {
   int nc = (int)round(-((-P.vkx + n1 * P.g1x) * P.g2x + (-P.vky + n1 * P.g1y) * P.g2y) / P.g2n);
   Sum10 t;
   int k = spectral_term<M>(P, rx, ry, rz, E, n1, nc, t);
   if (k < 0) return k;
   sum_add<M>(S, t);
+  nev += k;
   int cc = 1, n2, prop = 0;
   for (n2 = nc + 1; n2 < nc + BQD2_EW_LIMIT; n2++) {
     cc++;
     k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, t);
     if (k < 0) return k;
     sum_add<M>(S, t);
+    nev += k;
     if (k > 0) {
       if (term_small_spectral<M>(t, S, eps)) break;
     } else {
@@ -314,6 +322,7 @@ __device__ int spectral_row(const GfParams& P, double rx, double ry, double rz, 
     k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, t);
     if (k < 0) return k;
     sum_add<M>(S, t);
+    nev += k;
     if (k > 0) {
       if (term_small_spectral<M>(t, S, eps)) break;
     } else {
@@ -362,7 +371,7 @@ __device__ __forceinline__ void sum_zero(Sum10& s) {
 
 // Ewald evaluation in the unit cell (d3_qpgf_d2.c:1671-1712, :2011-2052)
 template <int M>
-__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   double E = P.E0;
   double r2 = rx * rx + ry * ry + rz * rz;
   double k2 = P.k * P.k;
@@ -391,13 +400,13 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   }
   if (l1 == -BQD2_EW_LIMIT) return -1;
 
-  int flag = 0;
-  int c2 = spectral_row<M>(P, rx, ry, rz, E, eps, P.nc, S2, flag);
+  int flag = 0, nev = 0;
+  int c2 = spectral_row<M>(P, rx, ry, rz, E, eps, P.nc, S2, flag, nev);
   if (c2 < 0) return c2;
   int n1;
   for (n1 = P.nc + 1; n1 < P.nc + BQD2_EW_LIMIT; n1++) {
     old = S2;
-    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag);
+    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag, nev);
     if (c < 0) return c;
     c2 += c;
     if (flag && rows_conv_spectral<M>(S2, old, eps)) break;
@@ -405,13 +414,16 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   if (n1 == P.nc + BQD2_EW_LIMIT) return -2;
   for (n1 = P.nc - 1; n1 > P.nc - BQD2_EW_LIMIT; n1--) {
     old = S2;
-    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag);
+    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag, nev);
     if (c < 0) return c;
     c2 += c;
     if (flag && rows_conv_spectral<M>(S2, old, eps)) break;
   }
   if (n1 == P.nc - BQD2_EW_LIMIT) return -2;
 
+  tl.sp += c1;
+  tl.ev += nev;
+  tl.pr += c2 - nev;
   const double ifp = 0.079577471545947667884;  // 1/(4 pi)
   const double iep = 0.039788735772973833942;  // 1/(8 pi)
   double cdd = 0.25 * P.aidd;
@@ -434,7 +446,7 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
 
 // pure spectral evaluation in the unit cell (d3_qpgf_d2.c:484-533, :887-950)
 template <int M>
-__device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+__device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   if (rz == 0.0) return -4;
   Sum10 S, old;
   sum_zero(S);
@@ -457,6 +469,7 @@ __device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, doubl
     if (rows_conv_fd<M>(S, old, eps)) break;
   }
   if (n1 == P.nc - BQD2_FD_LIMIT) return -1;
+  tl.fd += cc;
   double a = P.aidd, sz = rz / fabs(rz);
   out.g = mul_i(S.g) * (0.5 * a);
   out.dg[0] = S.dg[0] * (-0.5 * a);
@@ -474,7 +487,7 @@ __device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, doubl
 // full evaluation at offset r (source minus target); returns the term count or
 // the reference's negative code (-1x pure spectral, -2x Ewald)
 template <int M>
-__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   bool use_fd = false;
   if (!P.ewald_only) {
     use_fd = (P.bN / P.bNp * exp(-P.bNp * fabs(rz)) < eps) && (P.bN / P.bNm * exp(-P.bNm * fabs(rz)) < eps);
@@ -484,7 +497,7 @@ __device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, dou
   double tx = rx - l1 * P.d1x - l2 * P.d2x;
   double ty = ry - l1 * P.d1y - l2 * P.d2y;
   sum_zero(out);
-  int err = use_fd ? fd_cell<M>(P, tx, ty, rz, eps, out) : ewald_cell<M>(P, tx, ty, rz, eps, out);
+  int err = use_fd ? fd_cell<M>(P, tx, ty, rz, eps, out, tl) : ewald_cell<M>(P, tx, ty, rz, eps, out, tl);
   if (l1 != 0 || l2 != 0) {
     double arg = -(P.vkx * (l1 * P.d1x + l2 * P.d2x) + P.vky * (l1 * P.d1y + l2 * P.d2y));
     cd cf = cis(arg);
@@ -499,3 +512,10 @@ __device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, dou
   if (err < 0) err -= use_fd ? 10 : 20;
   return err;
 }
+
+template <int M>
+__device__ __forceinline__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out) {
+  GfTally tl{0, 0, 0, 0};
+  return qpgf_eval<M>(P, rx, ry, rz, eps, out, tl);
+}
+

@@ -255,7 +255,7 @@ extern "C" int bqd2_zgetrf(int n, double *A, int *ipiv, void *stream) {
   hipLaunchKernelGGL(k_check_diag, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, n, A, d_flag);
   BQD2_HIP(hipMemcpyAsync(&h_flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
-  hipFree(d_flag);
+  (void)hipFree(d_flag);
   BQD2_CHECK(h_flag == n, BQD2_E_SINGULAR, "bqd2_zgetrf: zero pivot at column %d", h_flag);
   return BQD2_OK;
 }

@@ -64,3 +64,17 @@ struct bqd2_problem {
 };
 
 int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);
+
+// device counter slots: errors, then per kernel family the summed Green's
+// function terms by kind (spatial, propagating, evanescent, pure spectral) and
+// the number of evaluations
+enum { CNT_ERR = 0, CNT_PAIRS = 1, CNT_SELF = 6, CNT_FIELD = 11, CNT_N = 16 };
+
+__device__ __forceinline__ void tally_add(long long *c, long long sp, long long pr, long long ev, long long fd,
+                                          long long evals) {
+  atomicAdd((unsigned long long *)&c[0], (unsigned long long)sp);
+  atomicAdd((unsigned long long *)&c[1], (unsigned long long)pr);
+  atomicAdd((unsigned long long *)&c[2], (unsigned long long)ev);
+  atomicAdd((unsigned long long *)&c[3], (unsigned long long)fd);
+  atomicAdd((unsigned long long *)&c[4], (unsigned long long)evals);
+}

@@ -83,7 +83,7 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
   if (m->ps == 1) TRY(bqd2_gf_params(m->qd, &p->gf));
   else memset(&p->gf, 0, sizeof(p->gf));
 
-  TRY(bqd2_alloc(p, (void **)&p->d_counters, 8 * sizeof(long long)));
+  TRY(bqd2_alloc(p, (void **)&p->d_counters, CNT_N * sizeof(long long)));
   BQD2_HIP(hipStreamCreate(&p->stream));
 
   size_t off = 0;
@@ -136,9 +136,9 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
 
 extern "C" int bqd2_destroy(bqd2_problem *p) {
   if (!p) return BQD2_OK;
-  if (p->stream) hipStreamSynchronize(p->stream);
-  for (void *ptr : p->owned) hipFree(ptr);
-  if (p->stream) hipStreamDestroy(p->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (void *ptr : p->owned) (void)hipFree(ptr);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return BQD2_OK;
 }
@@ -181,3 +181,11 @@ extern "C" int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, c
   if (dpv) BQD2_HIP(hipMemcpy(D.dpv, dpv, n1 * 12 * 16, hipMemcpyHostToDevice));
   return BQD2_OK;
 }
+
+extern "C" int bqd2_counters(bqd2_problem *p, long long *out, int n) {
+  BQD2_CHECK(p && out && n > 0 && n <= CNT_N, BQD2_E_ARG, "bqd2_counters: bad arguments");
+  BQD2_HIP(hipStreamSynchronize(p->stream));
+  BQD2_HIP(hipMemcpy(out, p->d_counters, (size_t)n * sizeof(long long), hipMemcpyDeviceToHost));
+  return BQD2_OK;
+}
+

@@ -370,7 +370,7 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
     BQD2_HIP(hipStreamSynchronize(st));
     // the per-domain work buffers are released right away
     for (void *q : {(void *)A, (void *)B, (void *)ipiv}) {
-      hipFree(q);
+      (void)hipFree(q);
       for (auto it = p->owned.begin(); it != p->owned.end(); ++it)
         if (*it == q) { p->owned.erase(it); break; }
     }

@@ -83,6 +83,26 @@ class Model:
     def ptr(self):
         return self.buf
 
+    def _int(self, off):
+        return struct.unpack_from("i", self.buf.raw, off)[0]
+
+    def mn(self):
+        """media besides the host (domains 0..MN)"""
+        return self._int(OFF_MN)
+
+    def ne(self):
+        """boundary elements"""
+        return self._int(OFF_NE)
+
+    def nn(self):
+        """collocation nodes (the system has 2 NN unknowns)"""
+        return self._int(OFF_NNCOLL)
+
+    @staticmethod
+    def qpd2_offset():
+        """byte offset of the host medium's QPD2 inside the AQD2 image"""
+        return OFF_QD
+
     def dat_read(self, path):
         self.lib.dat_read_aqd2(path.encode(), self.buf)
         return self

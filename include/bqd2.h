@@ -100,7 +100,13 @@ typedef struct {
   long long pairs_gl, pairs_gh, self_rows;  /* off-diagonal pairs per rule, self blocks */
   long long gf_evals;                       /* distinct Green's function evaluations    */
   long long gf_terms;                       /* lattice terms summed over them           */
-  double ms_decide, ms_pairs, ms_self;      /* kernel times (HIP events)                */
+  double ms_decide, ms_pairs, ms_self;      /* kernel times (HIP events on the problem's stream):
+                                               rule decision, off-diagonal pairs, self blocks + row sums */
+  /* periodic domain only: terms by kind (spatial images, propagating and evanescent
+   * spectral orders, pure spectral orders) and evaluations, for the off-diagonal
+   * pair kernel and for the self-element kernel separately */
+  long long pair_terms[4], pair_evals;
+  long long self_terms[4], self_evals;
 } bqd2_assembly_stats;
 
 int bqd2_create(const bqd2_model *m, bqd2_problem **out);
@@ -111,6 +117,13 @@ int bqd2_destroy(bqd2_problem *p);
  *   G, H [N x N], dG, dH [2N x N] (interior and isolated domains only), dF [N x 3],
  * N = 4 Ne_d, row (t-1)*4+tn, column (s-1)*4+n. st may be NULL. */
 int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st);
+
+/* Device counters of the last assembly, field or force call: [0] failed Green's
+ * function evaluations; then for the off-diagonal pair kernel [1..5], the
+ * self-element kernel [6..10] and the field kernels [11..15]: terms summed by
+ * kind (spatial, propagating spectral, evanescent spectral, pure spectral) and
+ * evaluations.  n <= 16 values are copied. */
+int bqd2_counters(bqd2_problem *p, long long *out, int n);
 
 /* copy domain did's matrices to host buffers (any may be NULL) */
 int bqd2_download_domain(bqd2_problem *p, int did, double *G, double *H, double *dG, double *dH, double *dF);

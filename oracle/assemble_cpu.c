@@ -8,8 +8,9 @@
  * create_matrix_csr_dac (:244-345, dense here), the solution scatter
  * tmatrix_bd_store (:370-427), the particle-velocity recovery q_solve_pv_bv /
  * solve_pv_bv with dpdt_fcm and inv_matrix_33 (:429-530, :655-750) and the
- * normal-derivative solve solve_dpv_bv (:533-653).  The dense complex solves use
- * LU with partial pivoting (dense_cpu.c) in place of MKL PARDISO and LAPACKE_zgesv.
+ * normal-derivative solve solve_dpv_bv (:533-653).  The dense complex solves
+ * themselves (MKL PARDISO and LAPACKE_zgesv in the reference) are left to the
+ * caller: the tests use numpy's LAPACK.
  *
  * Layouts (row-major, complex interleaved):
  *   G, H   [N x N]    row (t-1)*4+tn, column (s-1)*4+n, N = 4 Ne_d
@@ -129,6 +130,27 @@ long orc_assemble_domain(const AQD2 *qd, int did, double complex *G, double comp
       if (c < 0) err = c;
       else total += c;
     }
+  }
+  return err < 0 ? err : total;
+}
+
+/* coefficient blocks of n (row, source element) pairs of domain did, computed
+ * in parallel and discarded: the timed sample of the CPU baseline.  row is
+ * (t-1)*4+tn, src the 1-based element position.  Returns GF terms or <0. */
+long orc_pair_sample(const AQD2 *qd, int did, int n, const int *row, const int *src)
+{
+  const BOUD *bd = &qd->bd;
+  long total = 0, err = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : total)
+  for (int i = 0; i < n; i++) {
+    int t = row[i] / 4 + 1, tn = row[i] % 4;
+    int td = bd->sb[did].sid[t];
+    double vtz[3], vte[3];
+    orc_node_tangents(bd, td, tn, vtz, vte);
+    double complex CC[9], dCz[9], dCe[9];
+    long c = orc_coef_block(qd, did, td, tn, bd->sb[did].sid[src[i]], vtz, vte, CC, dCz, dCe);
+    if (c < 0) err = c;
+    else total += c;
   }
   return err < 0 ? err : total;
 }

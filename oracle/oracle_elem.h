@@ -23,4 +23,21 @@ long orc_q0_bil_self(const BOUD *bd, int s, double zt, double et, double complex
 void orc_q0_bil_self_dF(const BOUD *bd, int s, double zt, double et, double *dFz, double *dFe);
 void orc_bil_self(const BOUD *bd, int s_signed, double zt, double et, double k, double complex *CC);
 void orc_bil_self_dPV(const BOUD *bd, int s_signed, double zt, double et, double k, int dir, double complex *dCC);
+
+/* assembly level (assemble_cpu.c) */
+long orc_coef_block(const AQD2 *qd, int did, int td, int tn, int sd, const double *vtz, const double *vte,
+                    double complex *CC, double complex *dCz, double complex *dCe);
+long orc_assemble_row(const AQD2 *qd, int did, int t, int tn, double complex *G, double complex *H,
+                      double complex *dG, double complex *dH, double *dF);
+long orc_assemble_domain(const AQD2 *qd, int did, double complex *G, double complex *H, double complex *dG,
+                         double complex *dH, double *dF);
+long orc_pair_sample(const AQD2 *qd, int did, int n, const int *row, const int *src);
+void orc_system_row(const AQD2 *qd, int did, const double complex *Grow, const double complex *Hrow,
+                    double complex *A, double complex *b);
+void orc_solution_vector(const AQD2 *qd, double complex *x);
+void orc_store_solution(AQD2 *qd, const double complex *X);
+void orc_inv33(double *ma, const double *vn, const double *vtz, const double *vte);
+void orc_dpdt(const AQD2 *qd, int d, int t, int tn, const double complex *dGrow, const double complex *dHrow,
+              const double *dF, double complex *dPz, double complex *dPe);
+void orc_node_pv(const AQD2 *qd, int d, int t, int tn, double complex dPz, double complex dPe, double complex *pv);
 #endif
