@@ -217,6 +217,9 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=384)
     ap.add_argument("--cpu-self", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="before the steps, copy a device buffer of this size once (a known-traffic kernel that "
+                         "calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the solve is not sharded across GPUs (requested --gpus %d)"
@@ -240,6 +243,13 @@ def main():
     req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam))
     log("bench: workload %s, %d elements, %d unknowns" % (args.workload, model.ne(), 2 * model.nn()))
 
+    if args.copy_mib:
+        a = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)  # elementwise kernel: reads and writes copy_mib MiB each
+        torch.cuda.synchronize()
+        log("copied %d MiB device to device" % args.copy_mib)
+        del a, b
     for i in range(args.warmup):
         t = req.run()
         log("warmup %d: %.3f s (assemble %.3f, solve %.3f, velocity %.3f, field %.3f, force %.3f)"
