@@ -8,6 +8,7 @@
  * 1-based pointer-to-pointer arrays, and the same ex.dat byte layout
  * (bem3_aw_qd2.c:185-381): a raw image of AQD2 followed by the arrays.
  */
+#include <stddef.h>
 #include "bem3_aw_qd2.h"
 #include "host_internal.h"
 
@@ -705,4 +706,16 @@ void domain_id_l_aqd2(int *l1, int *l2, double *rt, AQD2 *qd)
   double idd = 1.0 / (d1[0] * d2[1] - d2[0] * d1[1]);
   *l1 = (int)floor((rt[0] * d2[1] - rt[1] * d2[0]) * idd + 0.5);
   *l2 = (int)floor(-(rt[0] * d1[1] - rt[1] * d1[0]) * idd + 0.5);
+}
+
+/* sizes and offsets of the public structs as compiled here, for binding checks:
+ * sizeof(AQD2), offsetof(AQD2, bd), offsetof(BOUD, sb), sizeof(Apw),
+ * sizeof(QPD2), offsetof(AQD2, MN) */
+int aqd2_layout(long *out, int n)
+{
+  long v[6] = {(long)sizeof(AQD2), (long)offsetof(AQD2, bd), (long)offsetof(BOUD, sb), (long)sizeof(Apw),
+               (long)sizeof(QPD2), (long)offsetof(AQD2, MN)};
+  int m = n < 6 ? n : 6;
+  for (int i = 0; i < m; i++) out[i] = v[i];
+  return m;
 }

