@@ -48,7 +48,8 @@ class AssemblyStats(C.Structure):
                 ("gf_evals", C.c_longlong), ("gf_terms", C.c_longlong),
                 ("ms_decide", C.c_double), ("ms_pairs", C.c_double), ("ms_self", C.c_double),
                 ("pair_terms", C.c_longlong * 4), ("pair_evals", C.c_longlong),
-                ("self_terms", C.c_longlong * 4), ("self_evals", C.c_longlong)]
+                ("self_terms", C.c_longlong * 4), ("self_evals", C.c_longlong),
+                ("pairs_split", C.c_longlong)]
 
 
 def hip_lib():

@@ -30,7 +30,51 @@ __device__ __forceinline__ bool row_target(const DevMesh &M, const DevDomain &D,
   return !(M.tri[abs(td)] && tn == 3);
 }
 
-__global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+// A GH64 pair of the periodic domain may be split: the free-space kernel
+// e^{ikR}/(4 pi R) (the image l = 0 of the lattice sum) carries the whole
+// near-singular behaviour and is integrated with GH64, while the remainder
+// G - G0 (the other images and the Ewald corrections) is analytic over the
+// element and its GL12 integral equals the GH64 one to rounding.  This needs
+// every other image of the target far from the element, and no quadrature point
+// with a z offset of exactly zero: there the reference's spectral sum stops on
+// rounding residue (see gf_dev.h) and its value depends on the point set, so
+// such pairs keep the full GH64 evaluation of the lattice sum.
+__device__ __forceinline__ bool split_ok(const ElemC &E, const Tab &T, const double *rt, const GfParams &P) {
+  // element centre and radius
+  double c[3], rad = 0.0;
+  bil_rw_r(E, 0.0, 0.0, c);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double q[3];
+    bil_rw_r(E, (k == 1 || k == 2) ? 1.0 : -1.0, (k >= 2) ? 1.0 : -1.0, q);
+    double d = sqrt((q[0] - c[0]) * (q[0] - c[0]) + (q[1] - c[1]) * (q[1] - c[1]) + (q[2] - c[2]) * (q[2] - c[2]));
+    rad = fmax(rad, d);
+  }
+  for (int l1 = -2; l1 <= 2; l1++) {
+    for (int l2 = -2; l2 <= 2; l2++) {
+      if (l1 == 0 && l2 == 0) continue;
+      double dx = c[0] - rt[0] + l1 * P.d1x + l2 * P.d2x;
+      double dy = c[1] - rt[1] + l1 * P.d1y + l2 * P.d2y;
+      double dz = c[2] - rt[2];
+      if (sqrt(dx * dx + dy * dy + dz * dz) - rad < 4.0 * rad) return false;
+    }
+  }
+  for (int i = 0; i < 64; i++)
+    for (int j = 0; j < 64; j++) {
+      const double z = T.xhi[i], e = T.xhi[j];
+      if (E.cr[2][0] + E.cr[2][1] * z + E.cr[2][2] * e + E.cr[2][3] * z * e - rt[2] == 0.0) return false;
+    }
+  for (int i = 0; i < 12; i++)
+    for (int j = 0; j < 12; j++) {
+      const double z = T.xli[i], e = T.xli[j];
+      if (E.cr[2][0] + E.cr[2][1] * z + E.cr[2][2] * e + E.cr[2][3] * z * e - rt[2] == 0.0) return false;
+    }
+  return true;
+}
+
+// rule per pair: 0 GL12, 1 GH64, 2 GH64 split (periodic domain only)
+__global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                int periodic) {
   long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)D.N * D.Ne) return;
   int row = (int)(gid / D.Ne), s = (int)(gid % D.Ne) + 1;
@@ -45,6 +89,7 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     ElemC E;
     load_elem(M, as, E);
     rule = needs_gh(E, T, rt, vtz, vte) ? 1 : 0;
+    if (rule && periodic && split_ok(E, T, rt, P)) rule = 2;
   }
   D.rule[gid] = rule;
 }
@@ -57,10 +102,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   if (!row_target(M, D, row, t, tn, td) || s == t) return;
   const Tab &T = *tab;
   const int at = abs(td), as = D.sid[s];  // periodic domain: source ids are positive
-  const bool gh = D.rule[(long long)row * D.Ne + (s - 1)] != 0;
-  const int n = gh ? 64 : 12;
-  const double *x = gh ? T.xhi : T.xli;
-  const double *w = gh ? T.whi : T.wli;
+  const int rule = D.rule[(long long)row * D.Ne + (s - 1)];
   double rt[3] = {M.ren[12 * at + 3 * tn], M.ren[12 * at + 3 * tn + 1], M.ren[12 * at + 3 * tn + 2]};
   double vtz[3], vte[3];
   node_tangents(M, T, at, tn, vtz, vte);
@@ -71,25 +113,64 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   double F = 0.0, F0 = 0.0, F1 = 0.0;
   GfTally tl{0, 0, 0, 0};
   int err = 0;
-  const int npts = n * n;
-  for (int p = lane; p < npts; p += WAVE) {
-    const int i = p / n, j = p - i * n;
-    const double z = x[i], e = x[j], wt = w[i] * w[j];
-    PtGeo q;
-    point_geo(E, z, e, rt, vtz, vte, q);
-    Sum10 gf;
-    int c = qpgf_eval<GF_V1>(P, q.vR[0], q.vR[1], q.vR[2], 1.0e-15, gf, tl);
-    if (c < 0) err = c;
-    cd hn = gf.dg[0] * q.vW[0] + gf.dg[1] * q.vW[1] + gf.dg[2] * q.vW[2];
+  long long nev = 0;
+  if (rule == 2) {
+    // near-singular image on GH64 with the free-space kernel
+    for (int p = lane; p < 64 * 64; p += WAVE) {
+      const int i = p >> 6, j = p & 63;
+      const double z = T.xhi[i], e = T.xhi[j], wt = T.whi[i] * T.whi[j];
+      PtGeo q;
+      point_geo(E, z, e, rt, vtz, vte, q);
+      cd g0, dg0[3];
+      free_gf(P.k, q.vR, q.aR, q.iR, g0, dg0);
+      cd hn = dg0[0] * q.vW[0] + dg0[1] * q.vW[1] + dg0[2] * q.vW[2];
 #pragma unroll
-    for (int ep = 0; ep < 4; ep++) {
-      double Np = bil_N(ep, z, e);
-      g[ep] += gf.g * (wt * Np * q.aW);
-      h[ep] += hn * (wt * Np);
+      for (int ep = 0; ep < 4; ep++) {
+        double Np = bil_N(ep, z, e);
+        g[ep] += g0 * (wt * Np * q.aW);
+        h[ep] += hn * (wt * Np);
+      }
+      F += wt * q.rdW * q.iR * q.iR;
+      F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
+      F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
     }
-    F += wt * q.rdW * q.iR * q.iR;
-    F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
-    F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
+  }
+  {
+    // lattice sum on the pair's rule (GL12 for the remainder of a split pair)
+    const bool gh = rule == 1;
+    const int n = gh ? 64 : 12;
+    const double *x = gh ? T.xhi : T.xli;
+    const double *w = gh ? T.whi : T.wli;
+    const int npts = n * n;
+    nev = npts;
+    for (int p = lane; p < npts; p += WAVE) {
+      const int i = p / n, j = p - i * n;
+      const double z = x[i], e = x[j], wt = w[i] * w[j];
+      PtGeo q;
+      point_geo(E, z, e, rt, vtz, vte, q);
+      Sum10 gf;
+      int c = qpgf_eval<GF_V1>(P, q.vR[0], q.vR[1], q.vR[2], 1.0e-15, gf, tl);
+      if (c < 0) err = c;
+      if (rule == 2) {
+        cd g0, dg0[3];
+        free_gf(P.k, q.vR, q.aR, q.iR, g0, dg0);
+        gf.g -= g0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) gf.dg[d] -= dg0[d];
+      }
+      cd hn = gf.dg[0] * q.vW[0] + gf.dg[1] * q.vW[1] + gf.dg[2] * q.vW[2];
+#pragma unroll
+      for (int ep = 0; ep < 4; ep++) {
+        double Np = bil_N(ep, z, e);
+        g[ep] += gf.g * (wt * Np * q.aW);
+        h[ep] += hn * (wt * Np);
+      }
+      if (rule != 2) {
+        F += wt * q.rdW * q.iR * q.iR;
+        F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
+        F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
+      }
+    }
   }
 #pragma unroll
   for (int ep = 0; ep < 4; ep++) {
@@ -108,7 +189,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     }
     double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
     fp[0] = F * BQD2_IFP; fp[1] = F0 * BQD2_IFP; fp[2] = F1 * BQD2_IFP;
-    tally_add(counters + CNT_PAIRS, tsp, tpr, tev, tfd, npts);
+    tally_add(counters + CNT_PAIRS, tsp, tpr, tev, tfd, nev);
     if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   }
 }
@@ -536,7 +617,8 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipMemsetAsync(D.Fpair, 0, (size_t)D.N * D.Ne * 3 * 8, st_));
   BQD2_HIP(hipEventRecord(e0, st_));
   long long npairs = (long long)D.N * D.Ne;
-  hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, st_, p->m, p->tab, D);
+  hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, st_, p->m, p->tab, D, p->gf,
+                     periodic ? 1 : 0);
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipEventRecord(e1, st_));
   if (periodic) {
@@ -558,7 +640,11 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   std::vector<unsigned char> rule((size_t)npairs);
   BQD2_HIP(hipMemcpy(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost));
   long long ngh = 0;
-  for (unsigned char r : rule) ngh += r;
+  long long nsplit = 0;
+  for (unsigned char r : rule) {
+    ngh += (r != 0);
+    nsplit += (r == 2);
+  }
   if (st) {
     memset(st, 0, sizeof(*st));
     float a, b, c;
@@ -576,6 +662,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     st->gf_terms = 0;
     for (int i = 0; i < 4; i++) st->gf_terms += st->pair_terms[i] + st->self_terms[i];
     st->pairs_gh = ngh;
+    st->pairs_split = nsplit;
     long long valid_rows = 0;
     for (int row = 0; row < D.N; row++) valid_rows += 1;  // quads only: every slot is a node
     st->self_rows = valid_rows;

@@ -44,6 +44,21 @@ __device__ __forceinline__ void bil_rw(const ElemC &E, double z, double e, doubl
   }
 }
 
+__device__ __forceinline__ void bil_rw_r(const ElemC &E, double z, double e, double *r) {
+#pragma unroll
+  for (int i = 0; i < 3; i++) r[i] = E.cr[i][0] + E.cr[i][1] * z + E.cr[i][2] * e + E.cr[i][3] * z * e;
+}
+
+// free-space Helmholtz kernel e^{ikR}/(4 pi R) and its gradient with respect to
+// the offset vR (source minus target), |vR| = aR, 1/|vR| = iR
+__device__ __forceinline__ void free_gf(double k, const double *vR, double aR, double iR, cd &g, cd *dg) {
+  const cd e = cis(k * aR);
+  g = e * (BQD2_IFP * iR);
+  const cd f = cd{-1.0, k * aR} * g * (iR * iR);
+#pragma unroll
+  for (int i = 0; i < 3; i++) dg[i] = f * vR[i];
+}
+
 __device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 __device__ __forceinline__ double abs3(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
 

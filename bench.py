@@ -47,6 +47,9 @@ METRIC = "solves/sec + p50/p99 latency at <=1e-8 rel force error; GF evals/sec"
 # formulas of d3_qpgf_d2.c (:1814-1838 spatial, :1965-2009 spectral) with add/mul 1,
 # FMA 2, sqrt/div 8, exp 20, sincos 30, erfc 40, w(z) 120
 F_TERM = {"spatial": 256, "propagating": 288, "evanescent": 212, "pure_spectral": 96}
+# flops per GH64 point of a split pair: free-space kernel and gradient (sincos 30,
+# 18 mul/add/FMA) and its accumulation into the four shape functions (42)
+F_FREE = 90
 PEAK_FP64_TFLOPS = 78.6
 
 
@@ -134,7 +137,7 @@ class Request:
 
 def pair_flops(st):
     kinds = ("spatial", "propagating", "evanescent", "pure_spectral")
-    return sum(st.pair_terms[i] * F_TERM[k] for i, k in enumerate(kinds))
+    return sum(st.pair_terms[i] * F_TERM[k] for i, k in enumerate(kinds)) + st.pairs_split * 4096 * F_FREE
 
 
 def cpu_baseline(model, req, npairs, nself, seed=0):
@@ -286,7 +289,8 @@ def main():
             "flops_per_launch": float(np.mean(flops)),
             "terms_per_launch": {"spatial": int(st0.pair_terms[0]), "propagating": int(st0.pair_terms[1]),
                                  "evanescent": int(st0.pair_terms[2]), "pure_spectral": int(st0.pair_terms[3])},
-            "evals_per_launch": int(st0.pair_evals), "flops_per_term": F_TERM}
+            "evals_per_launch": int(st0.pair_evals), "split_pairs": int(st0.pairs_split),
+            "gh_pairs": int(st0.pairs_gh), "flops_per_term": F_TERM, "flops_per_free_point": F_FREE}
 
     cpu = None
     if not args.no_cpu:

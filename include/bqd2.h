@@ -107,6 +107,9 @@ typedef struct {
    * pair kernel and for the self-element kernel separately */
   long long pair_terms[4], pair_evals;
   long long self_terms[4], self_evals;
+  /* GH64 pairs of the periodic domain evaluated as the free-space kernel on
+   * GH64 plus the smooth remainder of the lattice sum on GL12 */
+  long long pairs_split;
 } bqd2_assembly_stats;
 
 int bqd2_create(const bqd2_model *m, bqd2_problem **out);
