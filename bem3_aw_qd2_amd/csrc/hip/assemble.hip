@@ -39,36 +39,19 @@ __device__ __forceinline__ bool row_target(const DevMesh &M, const DevDomain &D,
 // with a z offset of exactly zero: there the reference's spectral sum stops on
 // rounding residue (see gf_dev.h) and its value depends on the point set, so
 // such pairs keep the full GH64 evaluation of the lattice sum.
+// z offset of the point (z, e) of the element from the target, rounded as point_geo does
+__device__ __forceinline__ double offset_z(const ElemC &E, double z, double e, const double *rt) {
+  return E.cr[2][0] + E.cr[2][1] * z + E.cr[2][2] * e + E.cr[2][3] * z * e - rt[2];
+}
+
 __device__ __forceinline__ bool split_ok(const ElemC &E, const Tab &T, const double *rt, const GfParams &P) {
-  // element centre and radius
-  double c[3], rad = 0.0;
-  bil_rw_r(E, 0.0, 0.0, c);
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    double q[3];
-    bil_rw_r(E, (k == 1 || k == 2) ? 1.0 : -1.0, (k >= 2) ? 1.0 : -1.0, q);
-    double d = sqrt((q[0] - c[0]) * (q[0] - c[0]) + (q[1] - c[1]) * (q[1] - c[1]) + (q[2] - c[2]) * (q[2] - c[2]));
-    rad = fmax(rad, d);
-  }
-  for (int l1 = -2; l1 <= 2; l1++) {
-    for (int l2 = -2; l2 <= 2; l2++) {
-      if (l1 == 0 && l2 == 0) continue;
-      double dx = c[0] - rt[0] + l1 * P.d1x + l2 * P.d2x;
-      double dy = c[1] - rt[1] + l1 * P.d1y + l2 * P.d2y;
-      double dz = c[2] - rt[2];
-      if (sqrt(dx * dx + dy * dy + dz * dz) - rad < 4.0 * rad) return false;
-    }
-  }
+  if (!images_far(E, rt, P)) return false;
   for (int i = 0; i < 64; i++)
-    for (int j = 0; j < 64; j++) {
-      const double z = T.xhi[i], e = T.xhi[j];
-      if (E.cr[2][0] + E.cr[2][1] * z + E.cr[2][2] * e + E.cr[2][3] * z * e - rt[2] == 0.0) return false;
-    }
+    for (int j = 0; j < 64; j++)
+      if (offset_z(E, T.xhi[i], T.xhi[j], rt) == 0.0) return false;
   for (int i = 0; i < 12; i++)
-    for (int j = 0; j < 12; j++) {
-      const double z = T.xli[i], e = T.xli[j];
-      if (E.cr[2][0] + E.cr[2][1] * z + E.cr[2][2] * e + E.cr[2][3] * z * e - rt[2] == 0.0) return false;
-    }
+    for (int j = 0; j < 12; j++)
+      if (offset_z(E, T.xli[i], T.xli[j], rt) == 0.0) return false;
   return true;
 }
 
@@ -316,6 +299,22 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
   GfTally tl{0, 0, 0, 0};
   int err = 0;
   const int npts = 4 * 64 * 2 * 12;
+  // Same split as for near pairs: the free-space image on the polar rule, the
+  // smooth remainder of the lattice sum on GL12 over the element, unless another
+  // image is close or a point has a z offset of exactly zero.
+  const double rt[3] = {M.ren[12 * s + 3 * tn], M.ren[12 * s + 3 * tn + 1], M.ren[12 * s + 3 * tn + 2]};
+  bool zero = false;
+  for (int p = lane; p < npts; p += WAVE) {
+    int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0);
+    zero |= rr * (E.cr[2][1] * c + E.cr[2][2] * sn + E.cr[2][3] * (rr * c * sn + zt * sn + et * c)) == 0.0;
+  }
+  for (int p = lane; p < 144; p += WAVE) zero |= offset_z(E, T.xli[p / 12], T.xli[p % 12], rt) == 0.0;
+  const bool split = images_far(E, rt, P) && !__any(zero);
+  long long evals = split ? 144 : npts;
   for (int p = lane; p < npts; p += WAVE) {
     int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
     double tt = theta_node(th, j, hh, T.xhi[m]);
@@ -330,8 +329,13 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
     }
     double aW = abs3(vW);
     Sum10 gf;
-    int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
-    if (cnt < 0) err = cnt;
+    if (split) {
+      const double aR = abs3(vR);
+      free_gf(P.k, vR, aR, 1.0 / aR, gf.g, gf.dg);
+    } else {
+      int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+      if (cnt < 0) err = cnt;
+    }
     cd hn = gf.dg[0] * vW[0] + gf.dg[1] * vW[1] + gf.dg[2] * vW[2];
     double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i] * rr;
 #pragma unroll
@@ -339,6 +343,30 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
       double Nn = bil_N(n, z, e) * wgt;
       g[n] += gf.g * (Nn * aW);
       h[n] += hn * Nn;
+    }
+  }
+  if (split) {
+    for (int p = lane; p < 144; p += WAVE) {
+      const int i = p / 12, j = p - i * 12;
+      const double z = T.xli[i], e = T.xli[j], wt = T.wli[i] * T.wli[j];
+      double vR[3], vW[3];
+      bil_rw(E, z, e, vR, vW);
+#pragma unroll
+      for (int l = 0; l < 3; l++) vR[l] -= rt[l];
+      const double aR = abs3(vR), aW = abs3(vW);
+      Sum10 gf;
+      int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+      if (cnt < 0) err = cnt;
+      cd g0, dg0[3];
+      free_gf(P.k, vR, aR, 1.0 / aR, g0, dg0);
+      const cd gs = gf.g - g0;
+      const cd hn = (gf.dg[0] - dg0[0]) * vW[0] + (gf.dg[1] - dg0[1]) * vW[1] + (gf.dg[2] - dg0[2]) * vW[2];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const double Nn = bil_N(n, z, e) * wt;
+        g[n] += gs * (Nn * aW);
+        h[n] += hn * Nn;
+      }
     }
   }
 #pragma unroll
@@ -370,7 +398,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
     fp[0] = F;
     fp[1] = 1.0 / abs3(Tz) * (Fzp - Fzm) / (2.0 * h_);
     fp[2] = 1.0 / abs3(Te) * (Fep - Fem) / (2.0 * h_);
-    tally_add(counters + CNT_SELF, tsp, tpr, tev, tfd, npts);
+    tally_add(counters + CNT_SELF, tsp, tpr, tev, tfd, evals);
     if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   }
 }

@@ -105,6 +105,32 @@ __device__ __forceinline__ void point_geo(const ElemC &E, double z, double e, co
   g.Wdv1 = dot3(g.vW, v1);
 }
 
+// every lattice image other than l = 0 of the target at least four element radii
+// from the element: then the lattice sum minus the free-space kernel is analytic
+// over the element (the split of near and self integrals relies on it)
+__device__ __forceinline__ bool images_far(const ElemC &E, const double *rt, const GfParams &P) {
+  // element centre and radius
+  double c[3], rad = 0.0;
+  bil_rw_r(E, 0.0, 0.0, c);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double q[3];
+    bil_rw_r(E, (k == 1 || k == 2) ? 1.0 : -1.0, (k >= 2) ? 1.0 : -1.0, q);
+    double d = sqrt((q[0] - c[0]) * (q[0] - c[0]) + (q[1] - c[1]) * (q[1] - c[1]) + (q[2] - c[2]) * (q[2] - c[2]));
+    rad = fmax(rad, d);
+  }
+  for (int l1 = -2; l1 <= 2; l1++) {
+    for (int l2 = -2; l2 <= 2; l2++) {
+      if (l1 == 0 && l2 == 0) continue;
+      double dx = c[0] - rt[0] + l1 * P.d1x + l2 * P.d2x;
+      double dy = c[1] - rt[1] + l1 * P.d1y + l2 * P.d2y;
+      double dz = c[2] - rt[2];
+      if (sqrt(dx * dx + dy * dy + dz * dz) - rad < 4.0 * rad) return false;
+    }
+  }
+  return true;
+}
+
 // GL12 dF/dv0, dF/dv1 with the reference's exact operation order and summation
 // order (inner j sum, times w_i, then 1/(4 pi)), the quantities that
 // d3qd2_elem.c:536-537 / :851-852 compare with CBD_SDF = 0.5.  Products and sums

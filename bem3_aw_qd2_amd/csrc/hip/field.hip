@@ -380,6 +380,28 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
     const double bdC = Et.cr[0][1] * Et.cw[0][2] + Et.cr[1][1] * Et.cw[1][2] + Et.cr[2][1] * Et.cw[2][2];
     cd Ps = {}, PVs[3] = {};
     double Fs = 0.0;
+    // periodic domain: free-space image on the polar rule plus the smooth
+    // remainder of the lattice sum on GL12 (as in the assembly), unless another
+    // image is close or a point has a z offset of exactly zero
+    bool split = false;
+    if (periodic) {
+      bool zero = false;
+      for (int p = lane; p < 4 * 64 * 2 * 12; p += 64) {
+        int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+        double tt = theta_node(th, j, hh, T.xhi[m]);
+        double rm = polar_rm(r, th, j, tt);
+        double c = cos(tt), sn = sin(tt), K[3];
+        double rr = 0.5 * rm * (T.xli[i] + 1.0);
+        polar_K(Et, c, sn, rr, zt, et, K);
+        zero |= rr * K[2] == 0.0;
+      }
+      for (int p = lane; p < 144; p += 64) {
+        double q[3];
+        bil_rw_r(Et, T.xli[p / 12], T.xli[p % 12], q);
+        zero |= q[2] - rt[2] == 0.0;
+      }
+      split = images_far(Et, rt, P) && !__any(zero);
+    }
     for (int p = lane; p < 4 * 64 * 2 * 12; p += 64) {
       int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
       double tt = theta_node(th, j, hh, T.xhi[m]);
@@ -405,9 +427,14 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
       if (periodic) {
         double vR[3] = {rr * K[0], rr * K[1], rr * K[2]};
         Sum10 gf;
-        int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
-        if (cnt < 0) err = cnt;
-        evals++;
+        if (split) {
+          const double aR = abs3(vR);
+          free_gf(P.k, vR, aR, 1.0 / aR, gf.g, gf.dg);
+        } else {
+          int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+          if (cnt < 0) err = cnt;
+          evals++;
+        }
         a = gf.g * (aW * rr);
         b = (gf.dg[0] * W[0] + gf.dg[1] * W[1] + gf.dg[2] * W[2]) * rr;
       } else {
@@ -418,6 +445,36 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
       Ps += (a * dPi - b * Pi * sg) * wgt;
       for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wgt;
       if (!plane) Fs += wgt * (1.0 / (aK * aK * aK) * c * sn);
+    }
+    if (split) {
+      for (int p = lane; p < 144; p += 64) {
+        const int i = p / 12, j = p - i * 12;
+        const double z = T.xli[i], e = T.xli[j], wgt = T.wli[i] * T.wli[j];
+        double vR[3], W[3];
+        bil_rw(Et, z, e, vR, W);
+        for (int l = 0; l < 3; l++) vR[l] -= rt[l];
+        const double aR = abs3(vR), aW = abs3(W);
+        Sum10 gf;
+        int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+        if (cnt < 0) err = cnt;
+        evals++;
+        cd g0, dg0[3];
+        free_gf(P.k, vR, aR, 1.0 / aR, g0, dg0);
+        const cd a = (gf.g - g0) * aW;
+        const cd b = (gf.dg[0] - dg0[0]) * W[0] + (gf.dg[1] - dg0[1]) * W[1] + (gf.dg[2] - dg0[2]) * W[2];
+        cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+        for (int mm = 0; mm < 4; mm++) {
+          double Nn = bil_N(mm, z, e);
+          Pi += ld(V.P, 4 * s + mm) * Nn;
+          dPi += ld(V.dP, 4 * s + mm) * Nn;
+          for (int cc = 0; cc < 3; cc++) {
+            pvi[cc] += ld(V.pv, 12 * s + 3 * mm + cc) * Nn;
+            dpvi[cc] += ld(V.dpv, 12 * s + 3 * mm + cc) * Nn;
+          }
+        }
+        Ps += (a * dPi - b * Pi * sg) * wgt;
+        for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wgt;
+      }
     }
     Pa += Ps;
     for (int c2 = 0; c2 < 3; c2++) PVa[c2] += PVs[c2];
