@@ -9,7 +9,7 @@ for Im z >= 0.  The a_n are the Fourier coefficients of
 f(t) = exp(-t^2) (L^2 + t^2) sampled at t = L tan(theta/2), computed here in
 40-digit arithmetic (mpmath) and rounded to double once.  With N = 40 the
 relative error against an mpmath reference is below 1e-15 on the closed upper
-half-plane (checked by tests/test_faddeeva.py).
+half-plane (checked by tests/test_oracle_faddeeva.py).
 
 Usage: python gen_weideman.py > weideman_coef.h
 """

@@ -20,12 +20,12 @@
  * per quadrature point instead of once per shape function, which changes no
  * sum because each shape function keeps its own accumulator in the same order.
  *
- * Pins: tests/test_oracle_exdat.py checks rows of the assembled system against
+ * Pins: tests/test_oracle_sample.py checks rows of the assembled system against
  * the reference's own solution in ex.dat (residual of A x = b), the particle
  * velocity recovery against ex.dat's pv (pins dG, dH, dF including the
  * principal-value self terms) and the dpv system residual; tests/test_elem_math.py
- * checks the static F against solid angles, symmetry and brute-force
- * high-order quadrature on a flat element.
+ * checks the static F row sums against the solid angle of a closed surface
+ * (Gauss) and its vanishing tangential derivatives.
  */
 #include <math.h>
 #include <complex.h>

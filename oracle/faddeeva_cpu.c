@@ -9,11 +9,12 @@
  * 1497-1518) with N = 40 terms on Im z >= 0 and the reflection
  * w(z) = 2 exp(-z^2) - w(-z) below the real axis.
  *
- * Pins: tests/test_faddeeva.py checks it against the reference's own known-answer
- * table (Faddeeva.cc:1920-1980, committed as tests/golden/faddeeva_kat.json),
- * against the reference's Faddeeva_w compiled into oracle/_ref, and against an
- * mpmath 30-digit evaluation; plus the identities w(-conj z) = conj w(z) and
- * Re w(x) = exp(-x^2).
+ * Pins: tests/test_oracle_faddeeva.py checks it against the reference's own
+ * known-answer table (Faddeeva.cc:1918-2075, committed as
+ * tests/golden/faddeeva_kat.json) and against scipy.special.wofz, plus the
+ * identities w(-z) = 2 exp(-z^2) - w(z), w(-conj z) = conj w(z) and
+ * Re w(x) = exp(-x^2).  The Green's function tests compare whole lattice sums
+ * with the reference's, which calls its own Faddeeva_w.
  */
 #include <math.h>
 #include <complex.h>

@@ -13,11 +13,11 @@
  * mode selects which partial sums drive the stopping tests, exactly as the four
  * reference chains do.
  *
- * Pins: tests/test_qpgf_oracle.py compares every mode, value and term count
+ * Pins: tests/test_oracle_gf.py compares every mode, value and term count
  * with the reference's own d3_qpgf_d2.c compiled into oracle/_ref, and checks
- * quasi-periodicity G(r+R) = exp(i k.R) G(r), the Helmholtz equation
- * trace(Hessian) = -k^2 G, Ewald/spectral agreement at |z| > 0, and an
- * independent mpmath evaluation of the Ewald split at a few points.
+ * quasi-periodicity G(r+R) = exp(-i k.R) G(r), the Helmholtz equation
+ * trace(Hessian) = -k^2 G and the smoothness of G across the switch between
+ * the Ewald split and the pure spectral sum.
  */
 #include <math.h>
 #include <complex.h>
