@@ -16,6 +16,7 @@
 //                 (force_FN_aqd2, bem3_aw_qd2_force.c:4-165), per element,
 //                 then an ordered sum
 #include <cstring>
+#include <algorithm>
 #include "elem_dev.h"
 
 #define CBD_CDF_D 10.0
@@ -171,17 +172,28 @@ __device__ __forceinline__ int rule_points(int type) { return type == 0 ? 4 : ty
 __device__ __forceinline__ cd ld(const double *a, size_t i) { return cd{a[2 * i], a[2 * i + 1]}; }
 
 // ------------------------------------------------------------ off-boundary field
-// out: p (2), pv (6) per point
-__global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P, int type,
-                                              int total, int n, const double *__restrict__ r0s,
-                                              const int *__restrict__ ls, const double *__restrict__ pts,
-                                              int *__restrict__ did_io, double *__restrict__ out,
-                                              long long *__restrict__ counters) {
-  const int ip = blockIdx.x, lane = threadIdx.x;
+// The (element, quadrature point) items of one field point are cut into chunks
+// of a fixed size that depends only on the mesh and the rule (not on how many
+// points a call asks for), one wavefront per (point, chunk); k_field_fin sums a
+// point's chunks in order.  A single point is then spread over up to 64
+// wavefronts and gives the same bits alone as inside a batch.
+enum { FP_N = 18 };  // per chunk: Pa (2), dPa (6), PVa (6), F, dF (3)
+
+__host__ __device__ __forceinline__ long long field_chunk_items(long long items) {
+  const long long per = (items + 64 * 64 - 1) / (64 * 64);
+  return 64 * (per > 0 ? per : 1);
+}
+
+__global__ void __launch_bounds__(64) k_field_part(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
+                                                   int type, int n, int nchunk, const double *__restrict__ r0s,
+                                                   const int *__restrict__ did_io,
+                                                   double *__restrict__ part, long long *__restrict__ counters) {
+  const int ip = blockIdx.x, ic = blockIdx.y, lane = threadIdx.x;
   if (ip >= n) return;
   const int did = did_io[ip];
+  double *dst = part + ((size_t)ip * nchunk + ic) * FP_N;
   if (did < 0) {
-    if (lane == 0) for (int i = 0; i < 8; i++) out[8 * ip + i] = 0.0;
+    if (lane < FP_N) dst[lane] = 0.0;
     return;
   }
   const Tab &T = *tab;
@@ -196,7 +208,9 @@ __global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__
   long long evals = 0;
   int err = 0;
   const long long items = (long long)V.Ne * npt;
-  for (long long it = lane; it < items; it += 64) {
+  const long long chunk = field_chunk_items(items);
+  const long long i0 = (long long)ic * chunk, i1 = min(items, i0 + chunk);
+  for (long long it = i0 + lane; it < i1; it += 64) {
     const int s = (int)(it / npt) + 1, q = (int)(it % npt);
     const int sd = V.sid[s], as = abs(sd);
     ElemC E;
@@ -258,19 +272,59 @@ __global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__
   for (int c2 = 0; c2 < 3; c2++) {
     dPa[c2] = cd{wave_sum(dPa[c2].x), wave_sum(dPa[c2].y)};
     PVa[c2] = cd{wave_sum(PVa[c2].x), wave_sum(PVa[c2].y)};
-    dF[c2] = wave_sum(dF[c2]) * BQD2_IFP;
+    dF[c2] = wave_sum(dF[c2]);
   }
-  F = wave_sum(F) * BQD2_IFP;
+  F = wave_sum(F);
   long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   evals = wave_sum_ll(evals);
   int anyerr = __any(err != 0);
   if (lane != 0) return;
   tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
   if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
+  dst[0] = Pa.x;
+  dst[1] = Pa.y;
+  for (int c2 = 0; c2 < 3; c2++) {
+    dst[2 + 2 * c2] = dPa[c2].x;
+    dst[3 + 2 * c2] = dPa[c2].y;
+    dst[8 + 2 * c2] = PVa[c2].x;
+    dst[9 + 2 * c2] = PVa[c2].y;
+    dst[15 + c2] = dF[c2];
+  }
+  dst[14] = F;
+}
+
+// one thread per field point: chunk sums in order, then the representation
+// formula's normalisation, Bloch phase and incident field (out: p, pv per point)
+__global__ void __launch_bounds__(256) k_field_fin(FieldCtx X, int total, int n, int nchunk,
+                                                   const double *__restrict__ part, const int *__restrict__ ls,
+                                                   const double *__restrict__ pts, int *__restrict__ did_io,
+                                                   double *__restrict__ out) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n) return;
+  const int did = did_io[ip];
+  if (did < 0) {
+    for (int i = 0; i < 8; i++) out[8 * ip + i] = 0.0;
+    return;
+  }
+  const DomView &V = X.dom[did];
+  double acc[FP_N];
+  for (int i = 0; i < FP_N; i++) acc[i] = 0.0;
+  for (int c = 0; c < nchunk; c++) {
+    const double *q = part + ((size_t)ip * nchunk + c) * FP_N;
+    for (int i = 0; i < FP_N; i++) acc[i] += q[i];
+  }
+  const cd Pa = cd{acc[0], acc[1]};
+  cd dPa[3], PVa[3];
+  double dF[3];
+  for (int c2 = 0; c2 < 3; c2++) {
+    dPa[c2] = cd{acc[2 + 2 * c2], acc[3 + 2 * c2]};
+    PVa[c2] = cd{acc[8 + 2 * c2], acc[9 + 2 * c2]};
+    dF[c2] = acc[15 + c2] * BQD2_IFP;
+  }
+  const double F = acc[14] * BQD2_IFP;
   const double iC = did == 0 ? 1.0 / (1.0 + F) : 1.0 / F;
   const cd mk2 = cd{-V.k2x, -V.k2y};
   cd p, pv[3];
-  int ret = did;
   if (fabs(dF[0]) > CBD_CDF_D || fabs(dF[1]) > CBD_CDF_D || fabs(dF[2]) > CBD_CDF_D) {
     // near the boundary: boundary-value form with the stored pv, dpv
     p = mk2 * Pa * iC;
@@ -299,7 +353,6 @@ __global__ void __launch_bounds__(64) k_field(DevMesh M, const Tab *__restrict__
     out[8 * ip + 2 + 2 * c2] = pv[c2].x;
     out[8 * ip + 3 + 2 * c2] = pv[c2].y;
   }
-  did_io[ip] = ret;
 }
 
 // ------------------------------------------------------------ boundary points
@@ -604,18 +657,25 @@ static FieldCtx make_ctx(const bqd2_problem *p) {
   return X;
 }
 
-struct Scratch {  // per-call device buffers, released on scope exit
-  std::vector<void *> v;
-  ~Scratch() {
-    for (void *q : v) (void)hipFree(q);
+// per-call device buffers carved from the problem's workspace, which grows
+// on demand and is kept between calls (single points from a caller's loop
+// cost no allocation); calls on one problem are serialised by the caller
+static size_t pad256(size_t b) { return (b + 255) & ~(size_t)255; }
+struct Scratch {
+  char *base = nullptr;
+  size_t off = 0, cap = 0;
+  int init(CallCtx *c, size_t total) {
+    cap = total;
+    return bqd2_workspace(c, total, &base);
   }
   int get(void **q, size_t bytes) {
-    hipError_t e = hipMalloc(q, bytes ? bytes : 16);
-    if (e != hipSuccess) {
-      bqd2_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    const size_t b = pad256(bytes ? bytes : 16);
+    if (off + b > cap) {
+      bqd2_set_error("workspace overflow (%zu + %zu > %zu bytes)", off, b, cap);
       return BQD2_E_NOMEM;
     }
-    v.push_back(*q);
+    *q = base + off;
+    off += b;
     return BQD2_OK;
   }
 };
@@ -625,27 +685,42 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   BQD2_CHECK(p && n >= 0 && pts && out, BQD2_E_ARG, "bqd2_field: bad arguments");
   BQD2_CHECK(type >= 0 && type <= 3, BQD2_E_ARG, "bqd2_field: type must be 0..3 (4-point, 9-point, GLN, GHN)");
   if (n == 0) return BQD2_OK;
-  hipStream_t st = p->stream;
+  CallCtx *cx = bqd2_call_ctx(p);
+  if (!cx) return BQD2_E_HIP;
+  hipStream_t st = cx->st;
+  long long *const ctr = cx->cnt;
   Scratch S;
-  double *d_pts, *d_r0, *d_out;
+  double *d_pts, *d_r0, *d_out, *d_part;
   int *d_l, *d_did;
   int rc;
+  const size_t nn = (size_t)n;
+  const int npt = type == 0 ? 4 : type == 1 ? 9 : type == 2 ? 144 : 4096;
+  int nchunk = 1;
+  for (int d = 0; d <= p->m.MN; d++) {
+    const long long items = (long long)p->dom[d].Ne * npt, chunk = field_chunk_items(items);
+    nchunk = std::max(nchunk, (int)((items + chunk - 1) / chunk));
+  }
+  if ((rc = S.init(cx, pad256(nn * 24) * 2 + pad256(nn * 64) + pad256(nn * 8) + pad256(nn * 4) +
+                          pad256(nn * nchunk * FP_N * 8))))
+    return rc;
   if ((rc = S.get((void **)&d_pts, (size_t)n * 24)) || (rc = S.get((void **)&d_r0, (size_t)n * 24)) ||
       (rc = S.get((void **)&d_out, (size_t)n * 64)) || (rc = S.get((void **)&d_l, (size_t)n * 8)) ||
-      (rc = S.get((void **)&d_did, (size_t)n * 4)))
+      (rc = S.get((void **)&d_did, (size_t)n * 4)) || (rc = S.get((void **)&d_part, nn * nchunk * FP_N * 8)))
     return rc;
   BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
-  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st));
+  BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
-  hipLaunchKernelGGL(k_field, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, n, d_r0, d_l, d_pts,
-                     d_did, d_out, p->d_counters);
+  hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
+                     d_did, d_part, ctr);
+  hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, X, total, n, nchunk, d_part, d_l, d_pts,
+                     d_did, d_out);
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
   if (did) BQD2_HIP(hipMemcpyAsync(did, d_did, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   long long cnt[CNT_N];
-  BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
   if (gf_evals) *gf_evals = cnt[CNT_FIELD + 4];
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_field: Green's function error at %lld points", cnt[CNT_ERR]);
@@ -659,24 +734,28 @@ extern "C" int bqd2_field_bd(bqd2_problem *p, int did, int n, const int *t, cons
   if (n == 0) return BQD2_OK;
   for (int i = 0; i < n; i++)
     BQD2_CHECK(t[i] >= 1 && t[i] <= p->dom[did].Ne, BQD2_E_ARG, "bqd2_field_bd: element %d out of range", t[i]);
-  hipStream_t st = p->stream;
+  CallCtx *cx = bqd2_call_ctx(p);
+  if (!cx) return BQD2_E_HIP;
+  hipStream_t st = cx->st;
+  long long *const ctr = cx->cnt;
   Scratch S;
   int *d_t;
   double *d_ze, *d_out;
   int rc;
+  if ((rc = S.init(cx, pad256((size_t)n * 4) + pad256((size_t)n * 16) + pad256((size_t)n * 64)))) return rc;
   if ((rc = S.get((void **)&d_t, (size_t)n * 4)) || (rc = S.get((void **)&d_ze, (size_t)n * 16)) ||
       (rc = S.get((void **)&d_out, (size_t)n * 64)))
     return rc;
   BQD2_HIP(hipMemcpyAsync(d_t, t, (size_t)n * 4, hipMemcpyHostToDevice, st));
   BQD2_HIP(hipMemcpyAsync(d_ze, ze, (size_t)n * 16, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
-  BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st));
+  BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
   hipLaunchKernelGGL(k_field_bd, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, did, n, d_t, d_ze,
-                     d_out, p->d_counters);
+                     d_out, ctr);
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
   long long cnt[CNT_N];
-  BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_field_bd: Green's function error at %lld points", cnt[CNT_ERR]);
   return BQD2_OK;
@@ -690,10 +769,17 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
     for (int i = 0; i < 3; i++) F[i] = N[i] = 0.0;
     return BQD2_OK;
   }
-  hipStream_t st = p->stream;
+  CallCtx *cx = bqd2_call_ctx(p);
+  if (!cx) return BQD2_E_HIP;
+  hipStream_t st = cx->st;
+  long long *const ctr = cx->cnt;
   Scratch S;
   double *d_bd = nullptr, *d_part, *d_FN;
   int rc_;
+  const size_t nbd = (size_t)ne * 9;
+  if ((rc_ = S.init(cx, pad256((size_t)ne * 48) + pad256(48) +
+                           (type != 0 ? pad256(nbd * 64) + pad256(nbd * 4) + pad256(nbd * 16) : 0))))
+    return rc_;
   if ((rc_ = S.get((void **)&d_part, (size_t)ne * 48)) || (rc_ = S.get((void **)&d_FN, 48))) return rc_;
   FieldCtx X = make_ctx(p);
   if (type != 0) {
@@ -713,9 +799,9 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
       return rc_;
     BQD2_HIP(hipMemcpyAsync(d_t, tpos.data(), tpos.size() * 4, hipMemcpyHostToDevice, st));
     BQD2_HIP(hipMemcpyAsync(d_ze, ze.data(), ze.size() * 8, hipMemcpyHostToDevice, st));
-    BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st));
+    BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
     hipLaunchKernelGGL(k_field_bd, dim3(ne * 9), dim3(64), 0, st, p->m, p->tab, X, p->gf, 1, 1, 0, ne * 9, d_t, d_ze,
-                       d_bd, p->d_counters);
+                       d_bd, ctr);
     BQD2_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_force_elem, dim3(bqd2_blocks(ne, 256)), dim3(256), 0, st, p->m, p->tab, X, type, d_bd, rc[0],
@@ -725,7 +811,7 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
   double FN[6];
   long long cnt[CNT_N] = {0};
   BQD2_HIP(hipMemcpyAsync(FN, d_FN, 48, hipMemcpyDeviceToHost, st));
-  if (type != 0) BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  if (type != 0) BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_force: Green's function error at %lld boundary points", cnt[CNT_ERR]);
   for (int i = 0; i < 3; i++) {
@@ -738,10 +824,14 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
 extern "C" int bqd2_domain_id(bqd2_problem *p, int n, const double *pts, int *did) {
   BQD2_CHECK(p && n >= 0 && pts && did, BQD2_E_ARG, "bqd2_domain_id: bad arguments");
   if (n == 0) return BQD2_OK;
-  hipStream_t st = p->stream;
+  CallCtx *cx = bqd2_call_ctx(p);
+  if (!cx) return BQD2_E_HIP;
+  hipStream_t st = cx->st;
+  long long *const ctr = cx->cnt;
   Scratch S;
   double *d_pts, *d_r0;
   int *d_l, *d_did, rc;
+  if ((rc = S.init(cx, pad256((size_t)n * 24) * 2 + pad256((size_t)n * 8) + pad256((size_t)n * 4)))) return rc;
   if ((rc = S.get((void **)&d_pts, (size_t)n * 24)) || (rc = S.get((void **)&d_r0, (size_t)n * 24)) ||
       (rc = S.get((void **)&d_l, (size_t)n * 8)) || (rc = S.get((void **)&d_did, (size_t)n * 4)))
     return rc;

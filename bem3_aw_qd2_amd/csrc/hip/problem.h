@@ -1,5 +1,7 @@
 // problem.h -- device-resident state of one boundary-value problem.
 #pragma once
+#include <mutex>
+#include <thread>
 #include <vector>
 #include "common.h"
 #include "gf_dev.h"
@@ -41,6 +43,16 @@ struct DevDomain {
   bool assembled;
 };
 
+// execution context of one calling host thread for the field and force entry
+// points: its own stream, grow-only workspace and counters, so that callers
+// that evaluate points from several threads overlap on the device
+struct CallCtx {
+  hipStream_t st;
+  char *ws;
+  size_t ws_bytes;
+  long long *cnt;
+};
+
 struct bqd2_problem {
   DevMesh m;
   Tab *tab;            // device
@@ -61,9 +73,19 @@ struct bqd2_problem {
   int *ipiv;
   int na;
   bool solved;
+  // the creating thread uses main (the problem's stream and counters); other
+  // threads get a context of their own on first use
+  CallCtx main;
+  std::thread::id owner;
+  std::mutex ctx_mu;
+  std::vector<std::pair<std::thread::id, CallCtx *>> ctxs;
 };
 
 int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);
+// the calling thread's context (nullptr on failure, error set)
+CallCtx *bqd2_call_ctx(bqd2_problem *p);
+// at least `bytes` of device workspace at *base (previous contents are lost)
+int bqd2_workspace(CallCtx *c, size_t bytes, char **base);
 
 // device counter slots: errors, then per kernel family the summed Green's
 // function terms by kind (spatial, propagating, evanescent, pure spectral) and

@@ -130,6 +130,11 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
   p->A = p->b = nullptr;
   p->ipiv = nullptr;
   p->solved = false;
+  p->main.st = p->stream;
+  p->main.ws = nullptr;
+  p->main.ws_bytes = 0;
+  p->main.cnt = p->d_counters;
+  p->owner = std::this_thread::get_id();
   *out = p;
   return BQD2_OK;
 }
@@ -137,7 +142,16 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
 extern "C" int bqd2_destroy(bqd2_problem *p) {
   if (!p) return BQD2_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (auto &tc : p->ctxs) {
+    CallCtx *c = tc.second;
+    (void)hipStreamSynchronize(c->st);
+    if (c->ws) (void)hipFree(c->ws);
+    (void)hipFree(c->cnt);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+  }
   for (void *ptr : p->owned) (void)hipFree(ptr);
+  if (p->main.ws) (void)hipFree(p->main.ws);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return BQD2_OK;
@@ -189,3 +203,44 @@ extern "C" int bqd2_counters(bqd2_problem *p, long long *out, int n) {
   return BQD2_OK;
 }
 
+int bqd2_workspace(CallCtx *c, size_t bytes, char **base) {
+  if (bytes > c->ws_bytes) {
+    BQD2_HIP(hipStreamSynchronize(c->st));
+    if (c->ws) BQD2_HIP(hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    size_t grow = bytes + bytes / 2;
+    hipError_t e = hipMalloc((void **)&c->ws, grow);
+    if (e != hipSuccess) {
+      bqd2_set_error("hipMalloc(%zu bytes) of the workspace failed: %s", grow, hipGetErrorString(e));
+      return BQD2_E_NOMEM;
+    }
+    c->ws_bytes = grow;
+  }
+  *base = c->ws;
+  return BQD2_OK;
+}
+
+CallCtx *bqd2_call_ctx(bqd2_problem *p) {
+  const std::thread::id me = std::this_thread::get_id();
+  if (me == p->owner) return &p->main;
+  std::lock_guard<std::mutex> g(p->ctx_mu);
+  for (auto &tc : p->ctxs)
+    if (tc.first == me) return tc.second;
+  CallCtx *c = new CallCtx();
+  c->ws = nullptr;
+  c->ws_bytes = 0;
+  if (hipStreamCreate(&c->st) != hipSuccess) {
+    bqd2_set_error("hipStreamCreate failed for a calling thread");
+    delete c;
+    return nullptr;
+  }
+  if (hipMalloc((void **)&c->cnt, CNT_N * sizeof(long long)) != hipSuccess) {
+    bqd2_set_error("hipMalloc of the thread's counters failed");
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return nullptr;
+  }
+  p->ctxs.push_back({me, c});
+  return c;
+}

@@ -4,7 +4,9 @@
  * Each call evaluates through libbqd2_hip.so (bqd2_field, bqd2_field_bd,
  * bqd2_domain_id); pv_batch_aqd2 is the batched entry point for grids.  Calls
  * from several host threads (example2.c drives pv_t_aqd2 from an OpenMP loop)
- * are serialised on one lock: the device work of a single point is one launch.
+ * run concurrently: the device library gives each calling thread its own
+ * stream and workspace.  One lock only guards the lazy upload of a solution
+ * read from ex.dat.
  */
 #include <pthread.h>
 #include "host_internal.h"
@@ -22,8 +24,10 @@ static void fail(const char *what, int rc)
 static struct bqd2_problem *ready(AQD2 *qd)
 {
   int has = 0;
+  pthread_mutex_lock(&g_field_lock);
   struct bqd2_problem *p = aqd2_problem(qd, &has);
   if (!has) aqd2_push_solution(qd, p);
+  pthread_mutex_unlock(&g_field_lock);
   return p;
 }
 
@@ -33,10 +37,8 @@ void pv_batch_aqd2(int n, const double *rt, int type, int total, double complex 
   if (n <= 0) return;
   double *out = malloc((size_t)n * 8 * sizeof(double));
   int *d = did ? did : malloc((size_t)n * sizeof(int));
-  pthread_mutex_lock(&g_field_lock);
   struct bqd2_problem *pr = ready(qd);
   int rc = bqd2_field(pr, n, rt, type, total, out, d, NULL);
-  pthread_mutex_unlock(&g_field_lock);
   if (rc) fail("bqd2_field", rc);
   for (int i = 0; i < n; i++) {
     if (p) p[i] = out[8 * i] + I * out[8 * i + 1];
@@ -98,10 +100,8 @@ int phi_t_aqd2(double complex *phi, double *rt, int type, AQD2 *qd)
 static int classify(double *rt, AQD2 *qd)
 {
   int did = -1;
-  pthread_mutex_lock(&g_field_lock);
   struct bqd2_problem *pr = aqd2_problem(qd, NULL);
   int rc = bqd2_domain_id(pr, 1, rt, &did);
-  pthread_mutex_unlock(&g_field_lock);
   if (rc) fail("bqd2_domain_id", rc);
   return did;
 }
@@ -140,10 +140,8 @@ static void bd_point(double complex *p, double complex *pv, int did, int t, doub
                      AQD2 *qd)
 {
   double out[8], ze[2] = {zt, et};
-  pthread_mutex_lock(&g_field_lock);
   struct bqd2_problem *pr = ready(qd);
   int rc = bqd2_field_bd(pr, did, 1, &t, ze, type, total, out);
-  pthread_mutex_unlock(&g_field_lock);
   if (rc) fail("bqd2_field_bd", rc);
   if (p) *p = out[0] + I * out[1];
   if (pv)
@@ -211,10 +209,8 @@ void pv_i_bd_aqd2(double complex *p, double complex *pv, int did, int t, double 
 /* ---------------------------------------------------------------- force */
 void force_FN_aqd2(double *F, double *N, double *rc, int type, AQD2 *qd)
 {
-  pthread_mutex_lock(&g_field_lock);
   struct bqd2_problem *pr = ready(qd);
   int rcode = bqd2_force(pr, rc, type, F, N);
-  pthread_mutex_unlock(&g_field_lock);
   if (rcode) fail("bqd2_force", rcode);
 }
 
