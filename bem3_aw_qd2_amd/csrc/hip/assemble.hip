@@ -39,19 +39,21 @@ __device__ __forceinline__ bool row_target(const DevMesh &M, const DevDomain &D,
 // with a z offset of exactly zero: there the reference's spectral sum stops on
 // rounding residue (see gf_dev.h) and its value depends on the point set, so
 // such pairs keep the full GH64 evaluation of the lattice sum.
-// z offset of the point (z, e) of the element from the target, rounded as point_geo does
-__device__ __forceinline__ double offset_z(const ElemC &E, double z, double e, const double *rt) {
-  return E.cr[2][0] + E.cr[2][1] * z + E.cr[2][2] * e + E.cr[2][3] * z * e - rt[2];
+// offset of the point (z, e) of the element from the target, rounded as point_geo does
+__device__ __forceinline__ bool offset_degenerate(const ElemC &E, double z, double e, const double *rt) {
+  double r[3];
+  bil_rw_r(E, z, e, r);
+  return z_degenerate(r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]);
 }
 
 __device__ __forceinline__ bool split_ok(const ElemC &E, const Tab &T, const double *rt, const GfParams &P) {
   if (!images_far(E, rt, P)) return false;
   for (int i = 0; i < 64; i++)
     for (int j = 0; j < 64; j++)
-      if (offset_z(E, T.xhi[i], T.xhi[j], rt) == 0.0) return false;
+      if (offset_degenerate(E, T.xhi[i], T.xhi[j], rt)) return false;
   for (int i = 0; i < 12; i++)
     for (int j = 0; j < 12; j++)
-      if (offset_z(E, T.xli[i], T.xli[j], rt) == 0.0) return false;
+      if (offset_degenerate(E, T.xli[i], T.xli[j], rt)) return false;
   return true;
 }
 
@@ -310,9 +312,13 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
     double rm = polar_rm(r, th, j, tt);
     double c = cos(tt), sn = sin(tt);
     double rr = 0.5 * rm * (T.xli[i] + 1.0);
-    zero |= rr * (E.cr[2][1] * c + E.cr[2][2] * sn + E.cr[2][3] * (rr * c * sn + zt * sn + et * c)) == 0.0;
+    double v[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+      v[l] = rr * (E.cr[l][1] * c + E.cr[l][2] * sn + E.cr[l][3] * (rr * c * sn + zt * sn + et * c));
+    zero |= z_degenerate(v[0], v[1], v[2]);
   }
-  for (int p = lane; p < 144; p += WAVE) zero |= offset_z(E, T.xli[p / 12], T.xli[p % 12], rt) == 0.0;
+  for (int p = lane; p < 144; p += WAVE) zero |= offset_degenerate(E, T.xli[p / 12], T.xli[p % 12], rt);
   const bool split = images_far(E, rt, P) && !__any(zero);
   long long evals = split ? 144 : npts;
   for (int p = lane; p < npts; p += WAVE) {

@@ -105,6 +105,15 @@ __device__ __forceinline__ void point_geo(const ElemC &E, double z, double e, co
   g.Wdv1 = dot3(g.vW, v1);
 }
 
+// An offset whose z component is zero or within rounding of zero relative to its
+// length: there the spectral part of the lattice sum ends on rounding residue
+// (gf_dev.h), so its value depends on the exact point.  Integrals that would
+// move the lattice sum to other points keep the reference's point set instead.
+#define BQD2_ZREL 1e-9
+__device__ __forceinline__ bool z_degenerate(double x, double y, double z) {
+  return fabs(z) <= BQD2_ZREL * sqrt(x * x + y * y + z * z);
+}
+
 // every lattice image other than l = 0 of the target at least four element radii
 // from the element: then the lattice sum minus the free-space kernel is analytic
 // over the element (the split of near and self integrals relies on it)

@@ -446,12 +446,12 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
         double c = cos(tt), sn = sin(tt), K[3];
         double rr = 0.5 * rm * (T.xli[i] + 1.0);
         polar_K(Et, c, sn, rr, zt, et, K);
-        zero |= rr * K[2] == 0.0;
+        zero |= z_degenerate(rr * K[0], rr * K[1], rr * K[2]);
       }
       for (int p = lane; p < 144; p += 64) {
         double q[3];
         bil_rw_r(Et, T.xli[p / 12], T.xli[p % 12], q);
-        zero |= q[2] - rt[2] == 0.0;
+        zero |= z_degenerate(q[0] - rt[0], q[1] - rt[1], q[2] - rt[2]);
       }
       split = images_far(Et, rt, P) && !__any(zero);
     }

@@ -23,6 +23,8 @@ CASES = {
     "tri1p8mm": (K1MHZ, np.pi / 6, np.pi / 12, (1.8e-3, 0.0), (0.9e-3, 1.5588457268119896e-3), 1e-3),
     # the reference's shipped sample: square 15 mm, 100 kHz, theta 0.3, psi 1e-8
     "sample15mm": (2 * np.pi * 1e5 / 1500.0, 0.3, 1e-8, (15e-3, 0.0), (0.0, 15e-3), 1e-2),
+    # the reference's verification sample: square 50 mm, 100 kHz (kd = 21, many propagating orders)
+    "veri50mm": (2 * np.pi * 1e5 / 1500.0, 0.3, 1e-8, (50e-3, 0.0), (0.0, 50e-3), 1e-2),
 }
 
 

@@ -4,7 +4,7 @@ linked against this repository's libbem3_aw_qd2.so (oracle/Makefile target
 with the tree), run on the GPU:
 
 * example1 on the reference's ex.dat must print the field and force of the
-  reference's example1.log (tests/golden/example1_log.json);
+  reference's example1.log (tests/golden/sample_example1.json);
 * aw_d3qd2_bv_solver on generated inputs must write the ex.dat that the
   library's own solve writes;
 * example2 drives pv_t_aqd2 per point from OpenMP threads: its intensity grid
@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 REF = os.path.join(ROOT, "oracle", "_ref")
 SAMPLE = os.path.join(HERE, "golden", "sample_ex.dat")
-LOG = json.load(open(os.path.join(HERE, "golden", "example1_log.json")))
+LOG = json.load(open(os.path.join(HERE, "golden", "sample_example1.json")))
 
 
 def driver(name):

@@ -4,7 +4,7 @@ The model and its surface solution are read from tests/golden/sample_ex.dat (the
 reference's ex.dat); example1.c of the reference evaluated the total field at
 r = (-0.03, 0.01, 0.002) and the force and torque about the origin for both
 quadrature settings and printed them to example1.log (14 digits), transcribed
-in tests/golden/example1_log.json.
+in tests/golden/sample_example1.json.
 
 Bar: p and v within 1e-11 normwise (the field is a sum over 600 elements of
 quasi-periodic Green's function values whose series are summed in a different
@@ -22,7 +22,7 @@ from bem3_aw_qd2_amd import exdat, host_lib
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 SAMPLE = os.path.join(HERE, "golden", "sample_ex.dat")
-LOG = json.load(open(os.path.join(HERE, "golden", "example1_log.json")))
+LOG = json.load(open(os.path.join(HERE, "golden", "sample_example1.json")))
 RADIUS = 5.0e-3
 
 

@@ -55,3 +55,18 @@ def test_dpv_rows_hold_reference_solution(sample, d, t, tn):
     o = oracle()
     o.orc_dpv_row_residual.restype = C.c_double
     assert o.orc_dpv_row_residual(m.buf, d, t, tn) < 1e-12
+
+
+@pytest.mark.parametrize("fname,did,t,tn", [("analysis_sample1_ex1.dat", 0, 5, 1), ("analysis_sample1_ex1.dat", 1, 9, 2),
+                                            ("analysis_sample2_ex2.dat", 2, 11, 0), ("analysis_sample2_ex2.dat", 3, 40, 3),
+                                            ("verification_veri.dat", 0, 333, 2)])
+def test_other_samples_rows(fname, did, t, tn):
+    """The same pin on the reference's other solved samples: a triangular
+    lattice, three media (domains 1-3) and a 50 mm lattice."""
+    m = exdat.Model(host_lib()).dat_read(os.path.join(HERE, "golden", fname))
+    o = oracle()
+    o.orc_row_residual.restype = C.c_double
+    r = o.orc_row_residual(m.buf, did, t, tn, None)
+    m.finalize()
+    assert 0.0 <= r < 1e-13, r
+
