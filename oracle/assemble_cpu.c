@@ -37,7 +37,36 @@ long orc_coef_block(const AQD2 *qd, int did, int td, int tn, int sd, const doubl
   int at = abs(td);
   const double *rt = bd->ren[at][tn];
   long terms = 0;
-  if (is_tri(bd, sd)) return -100; /* triangles: not in this oracle yet */
+  if (is_tri(bd, sd)) {
+    /* linear triangles (d3qd2_elem.c:541-570, :855-884) */
+    if (did == 0 && bd->ps == 1) {
+      if (td != sd) {
+        double dF0, dF1;
+        orc_lit_dF_GL(bd, rt, vtz, vte, sd, &dF0, &dF1);
+        if (fabs(dF0) > CBD_SDF || fabs(dF1) > CBD_SDF)
+          terms = orc_q0_lit_pair(bd, rt, vtz, vte, sd, GHN, bd->xhi, bd->whi, CC, dCz, dCe);
+        else
+          terms = orc_q0_lit_pair(bd, rt, vtz, vte, sd, GLN, bd->xli, bd->wli, CC, dCz, dCe);
+      } else {
+        terms = orc_q0_lit_self(bd, sd, bd->zt_34[tn], bd->et_34[tn], CC);
+        for (int l = 0; l < 9; l++) dCz[l] = dCe[l] = 0.0;
+      }
+    } else {
+      if (td != sd) {
+        double dF0, dF1;
+        orc_lit_dF_GL(bd, rt, vtz, vte, sd, &dF0, &dF1);
+        if (fabs(dF0) > CBD_SDF || fabs(dF1) > CBD_SDF)
+          orc_lit_pair(bd, rt, vtz, vte, sd, k, GHN, bd->xhi, bd->whi, CC, dCz, dCe);
+        else
+          orc_lit_pair(bd, rt, vtz, vte, sd, k, GLN, bd->xli, bd->wli, CC, dCz, dCe);
+      } else {
+        orc_lit_self(bd, sd, bd->zt_34[tn], bd->et_34[tn], k, CC);
+        orc_lit_self_dPV(bd, sd, bd->zt_34[tn], bd->et_34[tn], k, 0, dCz);
+        orc_lit_self_dPV(bd, sd, bd->zt_34[tn], bd->et_34[tn], k, 1, dCe);
+      }
+    }
+    return terms;
+  }
   if (did == 0 && bd->ps == 1) {
     if (td != sd) {
       double dF0, dF1;

@@ -24,6 +24,21 @@ void orc_q0_bil_self_dF(const BOUD *bd, int s, double zt, double et, double *dFz
 void orc_bil_self(const BOUD *bd, int s_signed, double zt, double et, double k, double complex *CC);
 void orc_bil_self_dPV(const BOUD *bd, int s_signed, double zt, double et, double k, int dir, double complex *dCC);
 
+/* linear triangles (lit_cpu.c) */
+double orc_lit_N(int n, double z, double e);
+void orc_lit_part(int part, double a, double b, double *z, double *e);
+void orc_lit_corners(double zt, double et, double *r, double *th);
+void orc_lit_dF_GL(const BOUD *bd, const double *rt, const double *v0, const double *v1, int s, double *dF0,
+                   double *dF1);
+long orc_q0_lit_pair(const BOUD *bd, const double *rt, const double *v0, const double *v1, int s, int n,
+                     const double *x, const double *w, double complex *CC, double complex *dC0,
+                     double complex *dC1);
+void orc_lit_pair(const BOUD *bd, const double *rt, const double *v0, const double *v1, int s, double k, int n,
+                  const double *x, const double *w, double complex *CC, double complex *dC0, double complex *dC1);
+long orc_q0_lit_self(const BOUD *bd, int s, double zt, double et, double complex *CC);
+void orc_lit_self(const BOUD *bd, int s_signed, double zt, double et, double k, double complex *CC);
+void orc_lit_self_dPV(const BOUD *bd, int s_signed, double zt, double et, double k, int dir, double complex *dCC);
+
 /* assembly level (assemble_cpu.c) */
 long orc_coef_block(const AQD2 *qd, int did, int td, int tn, int sd, const double *vtz, const double *vte,
                     double complex *CC, double complex *dCz, double complex *dCe);
