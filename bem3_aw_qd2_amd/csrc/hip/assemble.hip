@@ -46,14 +46,18 @@ __device__ __forceinline__ bool offset_degenerate(const ElemC &E, double z, doub
   return z_degenerate(r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]);
 }
 
-__device__ __forceinline__ bool split_ok(const ElemC &E, const Tab &T, const double *rt, const GfParams &P) {
+__device__ __forceinline__ bool split_ok(const ElemC &E, const Tab &T, const double *rt, const GfParams &P,
+                                         bool tri) {
   if (!images_far(E, rt, P)) return false;
-  for (int i = 0; i < 64; i++)
-    for (int j = 0; j < 64; j++)
-      if (offset_degenerate(E, T.xhi[i], T.xhi[j], rt)) return false;
-  for (int i = 0; i < 12; i++)
-    for (int j = 0; j < 12; j++)
-      if (offset_degenerate(E, T.xli[i], T.xli[j], rt)) return false;
+  double z, e, wt;
+  for (int p = 0; p < rule_npts(tri, 64); p++) {
+    rule_point(tri, 64, T.xhi, T.whi, p, z, e, wt);
+    if (offset_degenerate(E, z, e, rt)) return false;
+  }
+  for (int p = 0; p < rule_npts(tri, 12); p++) {
+    rule_point(tri, 12, T.xli, T.wli, p, z, e, wt);
+    if (offset_degenerate(E, z, e, rt)) return false;
+  }
   return true;
 }
 
@@ -73,8 +77,9 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     node_tangents(M, T, at, tn, vtz, vte);
     ElemC E;
     load_elem(M, as, E);
-    rule = needs_gh(E, T, rt, vtz, vte) ? 1 : 0;
-    if (rule && periodic && split_ok(E, T, rt, P)) rule = 2;
+    const bool tri = M.tri[as] != 0;
+    rule = (tri ? needs_gh_tri(E, T, rt, vtz, vte) : needs_gh(E, T, rt, vtz, vte)) ? 1 : 0;
+    if (rule && periodic && split_ok(E, T, rt, P, tri)) rule = 2;
   }
   D.rule[gid] = rule;
 }
@@ -93,6 +98,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   node_tangents(M, T, at, tn, vtz, vte);
   ElemC E;
   load_elem(M, as, E);
+  const bool tri = M.tri[as] != 0;
 
   cd g[4] = {}, h[4] = {};
   double F = 0.0, F0 = 0.0, F1 = 0.0;
@@ -101,9 +107,10 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   long long nev = 0;
   if (rule == 2) {
     // near-singular image on GH64 with the free-space kernel
-    for (int p = lane; p < 64 * 64; p += WAVE) {
-      const int i = p >> 6, j = p & 63;
-      const double z = T.xhi[i], e = T.xhi[j], wt = T.whi[i] * T.whi[j];
+    const int np64 = rule_npts(tri, 64);
+    for (int p = lane; p < np64; p += WAVE) {
+      double z, e, wt;
+      rule_point(tri, 64, T.xhi, T.whi, p, z, e, wt);
       PtGeo q;
       point_geo(E, z, e, rt, vtz, vte, q);
       cd g0, dg0[3];
@@ -111,7 +118,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
       cd hn = dg0[0] * q.vW[0] + dg0[1] * q.vW[1] + dg0[2] * q.vW[2];
 #pragma unroll
       for (int ep = 0; ep < 4; ep++) {
-        double Np = bil_N(ep, z, e);
+        double Np = shape_N(tri, ep, z, e);
         g[ep] += g0 * (wt * Np * q.aW);
         h[ep] += hn * (wt * Np);
       }
@@ -126,11 +133,11 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     const int n = gh ? 64 : 12;
     const double *x = gh ? T.xhi : T.xli;
     const double *w = gh ? T.whi : T.wli;
-    const int npts = n * n;
+    const int npts = rule_npts(tri, n);
     nev = npts;
     for (int p = lane; p < npts; p += WAVE) {
-      const int i = p / n, j = p - i * n;
-      const double z = x[i], e = x[j], wt = w[i] * w[j];
+      double z, e, wt;
+      rule_point(tri, n, x, w, p, z, e, wt);
       PtGeo q;
       point_geo(E, z, e, rt, vtz, vte, q);
       Sum10 gf;
@@ -146,7 +153,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
       cd hn = gf.dg[0] * q.vW[0] + gf.dg[1] * q.vW[1] + gf.dg[2] * q.vW[2];
 #pragma unroll
       for (int ep = 0; ep < 4; ep++) {
-        double Np = bil_N(ep, z, e);
+        double Np = shape_N(tri, ep, z, e);
         g[ep] += gf.g * (wt * Np * q.aW);
         h[ep] += hn * (wt * Np);
       }
@@ -166,14 +173,16 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   int anyerr = __any(err != 0);
   if (lane == 0) {
+    // the triangle rules carry 1/24 of the parts' area (q0_lit_dcoef_v2_cc.c:231-237)
+    const double sc = tri ? 1.0 / 24.0 : 1.0;
     const size_t base = ((size_t)row * D.N + (size_t)(s - 1) * 4) * 2;
 #pragma unroll
     for (int ep = 0; ep < 4; ep++) {
-      D.G[base + 2 * ep] = g[ep].x; D.G[base + 2 * ep + 1] = g[ep].y;
-      D.H[base + 2 * ep] = h[ep].x; D.H[base + 2 * ep + 1] = h[ep].y;
+      D.G[base + 2 * ep] = g[ep].x * sc; D.G[base + 2 * ep + 1] = g[ep].y * sc;
+      D.H[base + 2 * ep] = h[ep].x * sc; D.H[base + 2 * ep + 1] = h[ep].y * sc;
     }
     double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
-    fp[0] = F * BQD2_IFP; fp[1] = F0 * BQD2_IFP; fp[2] = F1 * BQD2_IFP;
+    fp[0] = F * (BQD2_IFP * sc); fp[1] = F0 * (BQD2_IFP * sc); fp[2] = F1 * (BQD2_IFP * sc);
     tally_add(counters + CNT_PAIRS, tsp, tpr, tev, tfd, nev);
     if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   }
@@ -196,13 +205,14 @@ __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restr
   node_tangents(M, T, at, tn, vtz, vte);
   ElemC E;
   load_elem(M, as, E);
+  const bool tri = M.tri[as] != 0;
 
   cd g[4] = {}, h[4] = {}, dg0[4] = {}, dg1[4] = {}, dh0[4] = {}, dh1[4] = {};
   double F = 0.0, F0 = 0.0, F1 = 0.0;
-  const int npts = n * n;
+  const int npts = rule_npts(tri, n);
   for (int p = lane; p < npts; p += WAVE) {
-    const int i = p / n, j = p - i * n;
-    const double z = x[i], e = x[j], wt = w[i] * w[j];
+    double z, e, wt;
+    rule_point(tri, n, x, w, p, z, e, wt);
     PtGeo q;
     point_geo(E, z, e, rt, vtz, vte, q);
     cd ce = cis(k * q.aR);
@@ -217,7 +227,7 @@ __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restr
     cd cdh1 = ce * (q.iR * q.iR * q.iR) * (c3 * (q.rdv1 * q.rdW) - cb * q.Wdv1);
 #pragma unroll
     for (int ep = 0; ep < 4; ep++) {
-      double Nw = wt * bil_N(ep, z, e);
+      double Nw = wt * shape_N(tri, ep, z, e);
       g[ep] += cg * Nw;
       h[ep] += chh * Nw;
       dg0[ep] += cdg * (Nw * q.rdv0);
@@ -229,20 +239,22 @@ __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restr
     F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
     F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
   }
-  // scaling (bil_dcoef_v2.c:207-218) and the sign of the sub side (:220-224)
+  // scaling (bil_dcoef_v2.c:207-218; triangles carry 1/24 of the parts' area,
+  // lit_dcoef_v2.c:248-262) and the sign of the sub side (:220-224)
   const double sg = sd < 0 ? -1.0 : 1.0;
+  const double ifp = BQD2_IFP * (tri ? 1.0 / 24.0 : 1.0);
 #pragma unroll
   for (int ep = 0; ep < 4; ep++) {
-    g[ep] = cd{wave_sum(g[ep].x), wave_sum(g[ep].y)} * BQD2_IFP;
-    h[ep] = cd{wave_sum(h[ep].x), wave_sum(h[ep].y)} * (BQD2_IFP * sg);
-    dg0[ep] = cd{wave_sum(dg0[ep].x), wave_sum(dg0[ep].y)} * (-BQD2_IFP);
-    dg1[ep] = cd{wave_sum(dg1[ep].x), wave_sum(dg1[ep].y)} * (-BQD2_IFP);
-    dh0[ep] = cd{wave_sum(dh0[ep].x), wave_sum(dh0[ep].y)} * (BQD2_IFP * sg);
-    dh1[ep] = cd{wave_sum(dh1[ep].x), wave_sum(dh1[ep].y)} * (BQD2_IFP * sg);
+    g[ep] = cd{wave_sum(g[ep].x), wave_sum(g[ep].y)} * ifp;
+    h[ep] = cd{wave_sum(h[ep].x), wave_sum(h[ep].y)} * (ifp * sg);
+    dg0[ep] = cd{wave_sum(dg0[ep].x), wave_sum(dg0[ep].y)} * (-ifp);
+    dg1[ep] = cd{wave_sum(dg1[ep].x), wave_sum(dg1[ep].y)} * (-ifp);
+    dh0[ep] = cd{wave_sum(dh0[ep].x), wave_sum(dh0[ep].y)} * (ifp * sg);
+    dh1[ep] = cd{wave_sum(dh1[ep].x), wave_sum(dh1[ep].y)} * (ifp * sg);
   }
-  F = wave_sum(F) * BQD2_IFP * sg;
-  F0 = wave_sum(F0) * BQD2_IFP * sg;
-  F1 = wave_sum(F1) * BQD2_IFP * sg;
+  F = wave_sum(F) * ifp * sg;
+  F0 = wave_sum(F0) * ifp * sg;
+  F1 = wave_sum(F1) * ifp * sg;
   if (lane == 0) {
     const size_t N = D.N, col = (size_t)(s - 1) * 4;
     const size_t base = ((size_t)row * N + col) * 2;
@@ -292,6 +304,7 @@ __global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restri
   if (!row_target(M, D, row, t, tn, td)) return;
   const Tab &T = *tab;
   const int s = abs(td);
+  if (M.tri[s]) return;  // k_q0_self_tri
   ElemC E;
   load_elem(M, s, E);
   const double zt = T.zt_44[tn], et = T.et_44[tn];
@@ -417,6 +430,7 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
   if (!row_target(M, D, row, t, tn, td)) return;
   const Tab &T = *tab;
   const int s = abs(td);
+  if (M.tri[s]) return;  // k_fs_self_tri
   const double sg = td < 0 ? -1.0 : 1.0;
   const double k = D.k;
   ElemC E;
@@ -606,6 +620,243 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
   }
 }
 
+// ------------------------------------------------------------------ triangle self elements
+// Polar rules about the node over the three sub-triangles to the edges, two
+// GH64 halves in theta and GL12 in r.  The element is flat: F, dF and the
+// free-space H vanish, and the area Jacobian is the constant
+// J = |cr1 x cr2| = 2|V|/(3 sqrt3), which takes the place of |W(z,e)|.
+
+// periodic self element (q0_lit_coef_bd, q0_lit_coef.c:297-372; dC = 0,
+// q0_lit_dcoef_v2_cc.c:331-343), with the same near/smooth split as the
+// quadrilateral self element
+__global__ void __launch_bounds__(WAVE) k_q0_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                      long long *__restrict__ counters) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td)) return;
+  const int s = abs(td);
+  if (!M.tri[s]) return;
+  const Tab &T = *tab;
+  ElemC E;
+  load_elem(M, s, E);
+  const double zt = T.zt_34[tn], et = T.et_34[tn];
+  double r[4], th[4];
+  lit_corners(zt, et, r, th);
+  const double V[3] = {E.cw[0][0], E.cw[1][0], E.cw[2][0]};
+  const double aV = abs3(V), J = 2.0 / (3.0 * BQD2_SQ3) * aV;
+  const double Wn[3] = {V[0] * (J / aV), V[1] * (J / aV), V[2] * (J / aV)};  // unit normal times J
+  const double rt[3] = {M.ren[12 * s + 3 * tn], M.ren[12 * s + 3 * tn + 1], M.ren[12 * s + 3 * tn + 2]};
+  const int npts = 3 * 64 * 2 * 12;
+  bool zero = false;
+  for (int p = lane; p < npts; p += WAVE) {
+    int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0), v[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) v[l] = rr * (E.cr[l][1] * c + E.cr[l][2] * sn);
+    zero |= z_degenerate(v[0], v[1], v[2]);
+  }
+  for (int p = lane; p < 3 * 144; p += WAVE) {
+    double z, e, wt;
+    rule_point(true, 12, T.xli, T.wli, p, z, e, wt);
+    zero |= offset_degenerate(E, z, e, rt);
+  }
+  const bool split = images_far(E, rt, P) && !__any(zero);
+  cd g[3] = {}, h[3] = {};
+  GfTally tl{0, 0, 0, 0};
+  int err = 0;
+  for (int p = lane; p < npts; p += WAVE) {
+    int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0);
+    double z = rr * c + zt, e = rr * sn + et, vR[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) vR[l] = rr * (E.cr[l][1] * c + E.cr[l][2] * sn);
+    Sum10 gf;
+    if (split) {
+      const double aR = abs3(vR);
+      free_gf(P.k, vR, aR, 1.0 / aR, gf.g, gf.dg);
+    } else {
+      int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+      if (cnt < 0) err = cnt;
+    }
+    cd hn = gf.dg[0] * Wn[0] + gf.dg[1] * Wn[1] + gf.dg[2] * Wn[2];
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i] * rr;
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      double Nn = lit_N(n, z, e) * wgt;
+      g[n] += gf.g * (Nn * J);
+      h[n] += hn * Nn;
+    }
+  }
+  if (split) {
+    // smooth remainder of the lattice sum on the triangle's GL12 parts
+    for (int p = lane; p < 3 * 144; p += WAVE) {
+      double z, e, wt;
+      rule_point(true, 12, T.xli, T.wli, p, z, e, wt);
+      double vR[3], vW[3];
+      bil_rw(E, z, e, vR, vW);
+#pragma unroll
+      for (int l = 0; l < 3; l++) vR[l] -= rt[l];
+      const double aR = abs3(vR);
+      Sum10 gf;
+      int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+      if (cnt < 0) err = cnt;
+      cd g0, dg0[3];
+      free_gf(P.k, vR, aR, 1.0 / aR, g0, dg0);
+      const cd gs = gf.g - g0;
+      const cd hs = (gf.dg[0] - dg0[0]) * V[0] + (gf.dg[1] - dg0[1]) * V[1] + (gf.dg[2] - dg0[2]) * V[2];
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        const double Nn = lit_N(n, z, e) * wt * (1.0 / 24.0);
+        g[n] += gs * (Nn * aV);
+        h[n] += hs * Nn;
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    g[n] = cd{wave_sum(g[n].x), wave_sum(g[n].y)};
+    h[n] = cd{wave_sum(h[n].x), wave_sum(h[n].y)};
+  }
+  long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
+  int anyerr = __any(err != 0);
+  if (lane == 0) {
+    const size_t base = ((size_t)row * D.N + (size_t)(t - 1) * 4) * 2;
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
+      D.H[base + 2 * n] = h[n].x; D.H[base + 2 * n + 1] = h[n].y;
+    }
+    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    fp[0] = 0.0; fp[1] = 0.0; fp[2] = 0.0;
+    tally_add(counters + CNT_SELF, tsp, tpr, tev, tfd, split ? 3 * 144 : npts);
+    if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
+  }
+}
+
+// free-space self element: G on the polar rule (lit_coef_bd, lit_coef.c:290-350)
+// and the principal-value dG/dv along both element parameters
+// (lit_dcoef_bd_tz0_PV / _te0_PV, lit_dcoef.c:461-637, :845-952)
+__global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td)) return;
+  const int s = abs(td);
+  if (!M.tri[s]) return;
+  const Tab &T = *tab;
+  const double k = D.k;
+  ElemC E;
+  load_elem(M, s, E);
+  const double zt = T.zt_34[tn], et = T.et_34[tn];
+  double r[4], th[4];
+  lit_corners(zt, et, r, th);
+  const double V[3] = {E.cw[0][0], E.cw[1][0], E.cw[2][0]};
+  const double J = 2.0 / (3.0 * BQD2_SQ3) * abs3(V);
+  double Tv[2][3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    Tv[0][l] = E.cr[l][1];
+    Tv[1][l] = E.cr[l][2];
+  }
+  cd g[3] = {};
+  for (int p = lane; p < 3 * 64 * 2 * 12; p += WAVE) {
+    int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * rm * (T.xli[i] + 1.0), K[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) K[l] = E.cr[l][1] * c + E.cr[l][2] * sn;
+    double aK = abs3(K);
+    cd ce = cis(k * aK * rr);
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i];
+    cd cg = ce * (J / aK * wgt);
+#pragma unroll
+    for (int n = 0; n < 3; n++) g[n] += cg * lit_N(n, rr * c + zt, rr * sn + et);
+  }
+  cd dG[2][3] = {};
+  // D_bar(eps): 3 sub-triangles x 128 theta x 64 r on [DEPS, rm]
+  for (int p = lane; p < 3 * 64 * 2 * 64; p += WAVE) {
+    int i = p % 64, q = p / 64, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+    double tt = theta_node(th, j, hh, T.xhi[m]);
+    double rm = polar_rm(r, th, j, tt);
+    double c = cos(tt), sn = sin(tt);
+    double rr = 0.5 * (rm - 0.1) * T.xhi[i] + 0.5 * (rm + 0.1);
+    double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * (rm - 0.1) * T.whi[i];
+    double K[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) K[l] = E.cr[l][1] * c + E.cr[l][2] * sn;
+    double aK = abs3(K), iK = 1.0 / aK;
+    double kR = k * rr * aK;
+    cd ce = cis(kR), cb = cd{-1.0, kR};
+    const double z = rr * c + zt, e = rr * sn + et;
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+      cd cg = cb * ce * (iK * iK / rr * (dot3(K, Tv[dir]) * iK) * J * wgt);
+#pragma unroll
+      for (int n = 0; n < 3; n++) dG[dir][n] += cg * lit_N(n, z, e);
+    }
+  }
+  // the disc r < DEPS in closed form (four quarters) and its principal value (two)
+  for (int p = lane; p < 6 * 64; p += WAVE) {
+    const int m = p % 64, qq = p / 64;
+    const bool pv = qq >= 4;
+    const double off = pv ? (qq == 4 ? 0.25 : 0.75) : (-0.75 + 0.5 * qq);
+    double tt = 0.25 * M_PI * T.xhi[m] + off * M_PI;
+    double c = cos(tt), sn = sin(tt), K[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) K[l] = E.cr[l][1] * c + E.cr[l][2] * sn;
+    double aK = abs3(K), iK = 1.0 / aK;
+    double keK = k * 0.1 * aK, cs, cc;
+    sincos(keK, &cs, &cc);
+    const double wq = 0.25 * M_PI * T.whi[m] * J;
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+      const double Kdv = dot3(K, Tv[dir]) * iK;
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        const double base = lit_dN(n, c, sn) * iK * iK * iK * Kdv * wq;
+        if (!pv) {
+          // i/k [ (keK + i) sin + (2 - i keK) cos - 2 ]
+          const cd f = cd{keK * cs + 2.0 * cc - 2.0, cs - keK * cc};
+          dG[dir][n] += mul_i(f) * (base / k);
+        } else {
+          dG[dir][n] -= cd{base * cs / k, 0.0};
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 3; n++) g[n] = cd{wave_sum(g[n].x), wave_sum(g[n].y)} * BQD2_IFP;
+  const double iT[2] = {1.0 / abs3(Tv[0]), 1.0 / abs3(Tv[1])};
+#pragma unroll
+  for (int dir = 0; dir < 2; dir++)
+#pragma unroll
+    for (int n = 0; n < 3; n++)
+      dG[dir][n] = cd{wave_sum(dG[dir][n].x), wave_sum(dG[dir][n].y)} * (-BQD2_IFP * iT[dir]);
+  if (lane == 0) {
+    const size_t N = D.N, col = (size_t)(t - 1) * 4;
+    const size_t base = ((size_t)row * N + col) * 2;
+    const size_t b0 = ((size_t)(2 * row) * N + col) * 2, b1 = ((size_t)(2 * row + 1) * N + col) * 2;
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
+      D.H[base + 2 * n] = 0.0; D.H[base + 2 * n + 1] = 0.0;
+      D.dG[b0 + 2 * n] = dG[0][n].x; D.dG[b0 + 2 * n + 1] = dG[0][n].y;
+      D.dG[b1 + 2 * n] = dG[1][n].x; D.dG[b1 + 2 * n + 1] = dG[1][n].y;
+      D.dH[b0 + 2 * n] = 0.0; D.dH[b0 + 2 * n + 1] = 0.0;
+      D.dH[b1 + 2 * n] = 0.0; D.dH[b1 + 2 * n + 1] = 0.0;
+    }
+    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    fp[0] = 0.0; fp[1] = 0.0; fp[2] = 0.0;
+  }
+}
+
 // per-row F sums in source order and the diagonal term (bem3_aw_qd2_solve_bieq.c:185-198)
 __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
   int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -662,8 +913,14 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   }
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipEventRecord(e2, st_));
-  if (periodic) hipLaunchKernelGGL(k_q0_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
-  else hipLaunchKernelGGL(k_fs_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+  if (periodic) {
+    hipLaunchKernelGGL(k_q0_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
+    if (D.has_tri)
+      hipLaunchKernelGGL(k_q0_self_tri, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
+  } else {
+    hipLaunchKernelGGL(k_fs_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+    if (D.has_tri) hipLaunchKernelGGL(k_fs_self_tri, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+  }
   BQD2_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_rows, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st_, p->m, D);
   BQD2_HIP(hipGetLastError());
@@ -697,8 +954,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     for (int i = 0; i < 4; i++) st->gf_terms += st->pair_terms[i] + st->self_terms[i];
     st->pairs_gh = ngh;
     st->pairs_split = nsplit;
-    long long valid_rows = 0;
-    for (int row = 0; row < D.N; row++) valid_rows += 1;  // quads only: every slot is a node
+    const long long valid_rows = D.nn;  // collocation nodes: 4 per quadrilateral, 3 per triangle
     st->self_rows = valid_rows;
     st->pairs_gl = valid_rows * (D.Ne - 1) - ngh;
   }

@@ -211,3 +211,119 @@ __device__ __forceinline__ double theta_node(const double *th, int j, int h, dou
 __device__ __forceinline__ double polar_rm(const double *r, const double *th, int j, double t) {
   return r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(t - th[j]) - r[j + 1] * sin(t - th[j + 1]));
 }
+
+// ------------------------------------------------------------ linear triangles
+// Reference triangle: corners (-1/2,-sqrt3/2), (1,0), (-1/2,sqrt3/2); r(z,e) =
+// cr0 + cr1 z + cr2 e (cr3 = 0), constant normal V = cw0 (cw1 = cw2 = 0), so the
+// bilinear helpers above evaluate triangles unchanged.  Nodes at 2/5 of the way
+// to the corners (d3qd2_elem.c:1035-1163).
+#define BQD2_SQ3 1.732050807568877293528
+#define BQD2_I_SPSQ3 3.062938307898844719507e-2  // 1/(6 pi sqrt3)
+
+__device__ __forceinline__ double lit_N(int n, double z, double e) {
+  const double ia = 0.5 / 0.2;
+  if (n == 0) return 1.0 / 3.0 * (1.0 - ia * (z + BQD2_SQ3 * e));
+  if (n == 1) return 1.0 / 3.0 * (1.0 + 2.0 * ia * z);
+  if (n == 2) return 1.0 / 3.0 * (1.0 - ia * (z - BQD2_SQ3 * e));
+  return 0.0;
+}
+
+// radial derivative term of N_n used by the closed-form disc of the principal
+// value (lit_dcoef.c:845-856)
+__device__ __forceinline__ double lit_dN(int n, double c, double s) {
+  const double ia = 0.5 / 0.2;
+  if (n == 0) return -2.0 / 3.0 * ia * (c + BQD2_SQ3 * s);
+  if (n == 1) return 4.0 / 3.0 * ia * c;
+  return -2.0 / 3.0 * ia * (c - BQD2_SQ3 * s);
+}
+
+// the three quadrilateral parts of the triangle as images of (a, b) in [-1,1]^2,
+// Jacobian factor 1 + a/4 + b/4 (q0_lit_dcoef_v2_cc.c:196-214)
+__device__ __forceinline__ void lit_part(int part, double a, double b, double &z, double &e) {
+  if (part == 0) {
+    z = 0.125 * (3.0 + 2.0 * a + 2.0 * b + a * b);
+    e = 0.125 * BQD2_SQ3 * (-a + b);
+  } else if (part == 1) {
+    z = 0.0625 * (-3.0 + a - 5.0 * b - a * b);
+    e = 0.0625 * BQD2_SQ3 * (3.0 + 3.0 * a + b + a * b);
+  } else {
+    z = 0.0625 * (-3.0 - 5.0 * a + b - a * b);
+    e = 0.0625 * BQD2_SQ3 * (-3.0 - a - 3.0 * b - a * b);
+  }
+}
+
+// corner distances and angles seen from (zt, et) (d3qd2_elem.c:1149-1163)
+__device__ __forceinline__ void lit_corners(double zt, double et, double *r, double *th) {
+  double ztp = 1.0 - zt, ztm = -0.5 - zt, etp = 0.5 * BQD2_SQ3 - et, etm = -0.5 * BQD2_SQ3 - et;
+  r[0] = sqrt(ztm * ztm + etm * etm);
+  r[1] = sqrt(ztp * ztp + et * et);
+  r[2] = sqrt(ztm * ztm + etp * etp);
+  r[3] = r[0];
+  th[0] = atan2(etm, ztm);
+  th[1] = atan2(-et, ztp);
+  th[2] = atan2(etp, ztm);
+  th[3] = th[0] + 2.0 * M_PI;
+}
+
+// interpolation function ep of a quadrilateral (0..3) or a triangle (0..2, 3 -> 0)
+__device__ __forceinline__ double shape_N(bool tri, int ep, double z, double e) {
+  return tri ? lit_N(ep, z, e) : bil_N(ep, z, e);
+}
+
+// points of the n x n Gauss rule on an element: (i, j) on the square, or
+// (part, i, j) on the triangle's three parts with the part's Jacobian factor
+__device__ __forceinline__ int rule_npts(bool tri, int n) { return tri ? 3 * n * n : n * n; }
+
+__device__ __forceinline__ void rule_point(bool tri, int n, const double *x, const double *w, int p, double &z,
+                                           double &e, double &wt) {
+  if (!tri) {
+    const int i = p / n, j = p - i * n;
+    z = x[i];
+    e = x[j];
+    wt = w[i] * w[j];
+  } else {
+    const int part = p / (n * n), q = p - part * n * n, i = q / n, j = q - i * n;
+    const double a = x[i], b = x[j];
+    lit_part(part, a, b, z, e);
+    wt = (1.0 + 0.25 * a + 0.25 * b) * w[j] * w[i];
+  }
+}
+
+// GL12 dF/dv0, dF/dv1 of a triangle pair with the reference's operation order
+// (q0_lit_dcoef_v2_cc.c:196-229, lit_dcoef_v2.c:218-262): three parts summed per
+// j, times the part weight and w_j, summed over j, times w_i, then the constant.
+__device__ __forceinline__ bool needs_gh_tri(const ElemC &E, const Tab &T, const double *rt, const double *v0,
+                                             const double *v1) {
+  const double V[3] = {E.cw[0][0], E.cw[1][0], E.cw[2][0]};
+  const double Vdv0 = V[0] * v0[0] + V[1] * v0[1] + V[2] * v0[2];
+  const double Vdv1 = V[0] * v1[0] + V[1] * v1[1] + V[2] * v1[2];
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = 0; i < 12; i++) {
+    const double a = T.xli[i];
+    double t0 = 0.0, t1 = 0.0;
+    for (int j = 0; j < 12; j++) {
+      const double b = T.xli[j];
+      const double wt = 1.0 + 0.25 * a + 0.25 * b;
+      double p0 = 0.0, p1 = 0.0;
+      for (int part = 0; part < 3; part++) {
+        double z, e, vR[3];
+        lit_part(part, a, b, z, e);
+#pragma unroll
+        for (int l = 0; l < 3; l++) vR[l] = E.cr[l][0] + E.cr[l][1] * z + E.cr[l][2] * e - rt[l];
+        const double aR = sqrt(vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]);
+        const double iR = 1.0 / aR;
+        const double rdV = (vR[0] * V[0] + vR[1] * V[1] + vR[2] * V[2]) * iR;
+        const double rdv0 = (vR[0] * v0[0] + vR[1] * v0[1] + vR[2] * v0[2]) * iR;
+        const double rdv1 = (vR[0] * v1[0] + vR[1] * v1[1] + vR[2] * v1[2]) * iR;
+        p0 += iR * iR * iR * (3.0 * rdv0 * rdV - Vdv0);
+        p1 += iR * iR * iR * (3.0 * rdv1 * rdV - Vdv1);
+      }
+      t0 += p0 * wt * T.wli[j];
+      t1 += p1 * wt * T.wli[j];
+    }
+    a0 += t0 * T.wli[i];
+    a1 += t1 * T.wli[i];
+  }
+  const double c = 0.0625 * BQD2_SQ3 * BQD2_I_SPSQ3;
+  return fabs(a0 * c) > 0.5 || fabs(a1 * c) > 0.5;
+}

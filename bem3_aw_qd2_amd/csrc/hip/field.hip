@@ -31,6 +31,7 @@ struct DomView {
 struct FieldCtx {
   DomView dom[16];
   int MN, ps;
+  int any_tri;  // the mesh has linear triangles: rules are sized for both kinds
   double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
   double d1x, d1y, d2x, d2y, idd, vkx, vky;
 };
@@ -141,33 +142,55 @@ __global__ void __launch_bounds__(256) k_domain_id(DevMesh M, FieldCtx X, const 
 
 // ------------------------------------------------------------ source points
 // position, normal, weight and interpolation weights of quadrature point q of
-// element s for rule type (0: nodes, 1: 9-point, 2: GLN^2, 3: GHN^2)
-__device__ __forceinline__ void source_point(const DevMesh &M, const Tab &T, const ElemC &E, int s, int type, int q,
+// element s for rule type (0: nodes, 1: 9-point / 7-point, 2: GLN^2, 3: GHN^2;
+// triangles use their three GL parts for 2 and 3).  Triangle weights carry the
+// reference's scale (q0_lit_coef.c:10-185: 1/2 for the node and 7-point rules,
+// 1/24 for the parts) so that every rule is summed the same way.  Returns false
+// for the unused tail of a rule sized for the other element kind.
+__device__ __forceinline__ bool source_point(const DevMesh &M, const Tab &T, const ElemC &E, int s, int type, int q,
                                              double *r, double *w, double &wt, double *Nv) {
+  const bool tri = M.tri[s] != 0;
   if (type == 0) {
+    if (q >= 4) return false;
     for (int d = 0; d < 3; d++) {
       r[d] = M.ren[12 * s + 3 * q + d];
       w[d] = M.wen[12 * s + 3 * q + d];
     }
-    wt = T.wt_44[q];
-    for (int n = 0; n < 4; n++) Nv[n] = (n == q) ? 1.0 : 0.0;
-    return;
+    if (!tri) {
+      wt = T.wt_44[q];
+      for (int n = 0; n < 4; n++) Nv[n] = (n == q) ? 1.0 : 0.0;
+    } else {
+      wt = 0.5 * T.wt_34[q];
+      for (int n = 0; n < 4; n++) Nv[n] = (q < 3) ? ((n == q) ? 1.0 : 0.0) : lit_N(n, T.zt_34[3], T.et_34[3]);
+    }
+    return true;
   }
   double z, e;
   if (type == 1) {
-    z = T.zt_49[q]; e = T.et_49[q]; wt = T.wt_49[q];
-  } else if (type == 2) {
-    int i = q / 12, j = q % 12;
-    z = T.xli[i]; e = T.xli[j]; wt = T.wli[i] * T.wli[j];
+    if (q >= (tri ? 7 : 9)) return false;
+    if (tri) {
+      z = T.zt_37[q]; e = T.et_37[q]; wt = 0.5 * T.wt_37[q];
+    } else {
+      z = T.zt_49[q]; e = T.et_49[q]; wt = T.wt_49[q];
+    }
   } else {
-    int i = q / 64, j = q % 64;
-    z = T.xhi[i]; e = T.xhi[j]; wt = T.whi[i] * T.whi[j];
+    const int n = type == 2 ? 12 : 64;
+    if (q >= rule_npts(tri, n)) return false;
+    rule_point(tri, n, type == 2 ? T.xli : T.xhi, type == 2 ? T.wli : T.whi, q, z, e, wt);
+    if (tri) wt *= 1.0 / 24.0;
   }
   bil_rw(E, z, e, r, w);
-  for (int n = 0; n < 4; n++) Nv[n] = bil_N(n, z, e);
+  for (int n = 0; n < 4; n++) Nv[n] = shape_N(tri, n, z, e);
+  return true;
 }
 
-__device__ __forceinline__ int rule_points(int type) { return type == 0 ? 4 : type == 1 ? 9 : type == 2 ? 144 : 4096; }
+// points per element of rule type, sized for the mesh's element kinds
+__host__ __device__ __forceinline__ int rule_points(int type, int any_tri) {
+  if (type == 0) return 4;
+  if (type == 1) return 9;
+  const int n2 = type == 2 ? 144 : 4096;
+  return any_tri ? 3 * n2 : n2;
+}
 
 __device__ __forceinline__ cd ld(const double *a, size_t i) { return cd{a[2 * i], a[2 * i + 1]}; }
 
@@ -200,7 +223,7 @@ __global__ void __launch_bounds__(64) k_field_part(DevMesh M, const Tab *__restr
   const DomView &V = X.dom[did];
   const bool periodic = (X.ps == 1 && did == 0);
   const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
-  const int npt = rule_points(type);
+  const int npt = rule_points(type, X.any_tri);
   const double k = V.k;
   cd Pa = {}, dPa[3] = {}, PVa[3] = {};
   double F = 0.0, dF[3] = {0, 0, 0};
@@ -216,7 +239,7 @@ __global__ void __launch_bounds__(64) k_field_part(DevMesh M, const Tab *__restr
     ElemC E;
     if (type != 0) load_elem(M, as, E);
     double r[3], w[3], wt, Nv[4];
-    source_point(M, T, E, as, type, q, r, w, wt, Nv);
+    if (!source_point(M, T, E, as, type, q, r, w, wt, Nv)) continue;
     double vR[3] = {r[0] - r0[0], r[1] - r0[1], r[2] - r0[2]};
     double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
     double rdW = dot3(vR, w) * iR;
@@ -373,7 +396,7 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
   load_elem(M, at, Et);
   double rt[3], wt_[3];
   bil_rw(Et, zt, et, rt, wt_);
-  const int npt = rule_points(type);
+  const int npt = rule_points(type, X.any_tri);
   const double k = V.k;
   cd Pa = {}, PVa[3] = {};
   double F = 0.0;
@@ -389,7 +412,7 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
     ElemC E;
     if (type != 0) load_elem(M, as, E);
     double r[3], w[3], wt, Nv[4];
-    source_point(M, T, E, as, type, q, r, w, wt, Nv);
+    if (!source_point(M, T, E, as, type, q, r, w, wt, Nv)) continue;
     double vR[3] = {r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]};
     double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
     double tD = dot3(vR, w) * iR * iR * iR;
@@ -423,8 +446,114 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
     F += sg * wt * tD;
   }
   F *= BQD2_IFP;
+  if (M.tri[at]) {
+    // self triangle: polar rule over the three sub-triangles about (zt, et)
+    // (lit_coef_bd, lit_coef.c:290-350; q0_lit_coef_bd, q0_lit_coef.c:187-262).
+    // The element is flat, so the free-space H and F vanish; the periodic H
+    // comes from the other lattice images.
+    const int s = t;
+    double r[4], th[4];
+    lit_corners(zt, et, r, th);
+    const double Vn[3] = {Et.cw[0][0], Et.cw[1][0], Et.cw[2][0]};
+    const double aV = abs3(Vn), J = 2.0 / (3.0 * BQD2_SQ3) * aV;
+    const double Wn[3] = {Vn[0] * (J / aV), Vn[1] * (J / aV), Vn[2] * (J / aV)};
+    const int npts = 3 * 64 * 2 * 12;
+    bool split = false;
+    if (periodic) {
+      bool zero = false;
+      for (int p = lane; p < npts; p += 64) {
+        int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+        double tt = theta_node(th, j, hh, T.xhi[m]);
+        double rm = polar_rm(r, th, j, tt);
+        double c = cos(tt), sn = sin(tt), v[3];
+        double rr = 0.5 * rm * (T.xli[i] + 1.0);
+        for (int l = 0; l < 3; l++) v[l] = rr * (Et.cr[l][1] * c + Et.cr[l][2] * sn);
+        zero |= z_degenerate(v[0], v[1], v[2]);
+      }
+      for (int p = lane; p < 3 * 144; p += 64) {
+        double z, e, wq, q[3];
+        rule_point(true, 12, T.xli, T.wli, p, z, e, wq);
+        bil_rw_r(Et, z, e, q);
+        zero |= z_degenerate(q[0] - rt[0], q[1] - rt[1], q[2] - rt[2]);
+      }
+      split = images_far(Et, rt, P) && !__any(zero);
+    }
+    cd Ps = {}, PVs[3] = {};
+    for (int p = lane; p < npts; p += 64) {
+      int i = p % 12, q = p / 12, hh = q % 2, m = (q / 2) % 64, j = q / 128;
+      double tt = theta_node(th, j, hh, T.xhi[m]);
+      double rm = polar_rm(r, th, j, tt);
+      double c = cos(tt), sn = sin(tt);
+      double rr = 0.5 * rm * (T.xli[i] + 1.0);
+      double z = rr * c + zt, e = rr * sn + et, K[3];
+      for (int l = 0; l < 3; l++) K[l] = Et.cr[l][1] * c + Et.cr[l][2] * sn;
+      double wgt = 0.25 * (th[j + 1] - th[j]) * T.whi[m] * 0.5 * rm * T.wli[i];
+      cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+      for (int mm = 0; mm < 3; mm++) {
+        double Nn = lit_N(mm, z, e);
+        Pi += ld(V.P, 4 * s + mm) * Nn;
+        dPi += ld(V.dP, 4 * s + mm) * Nn;
+        for (int cc = 0; cc < 3; cc++) {
+          pvi[cc] += ld(V.pv, 12 * s + 3 * mm + cc) * Nn;
+          dpvi[cc] += ld(V.dpv, 12 * s + 3 * mm + cc) * Nn;
+        }
+      }
+      cd a, b = {};
+      if (periodic) {
+        double vR[3] = {rr * K[0], rr * K[1], rr * K[2]};
+        Sum10 gf;
+        if (split) {
+          const double aR = abs3(vR);
+          free_gf(P.k, vR, aR, 1.0 / aR, gf.g, gf.dg);
+        } else {
+          int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+          if (cnt < 0) err = cnt;
+          evals++;
+        }
+        a = gf.g * (J * rr);
+        b = (gf.dg[0] * Wn[0] + gf.dg[1] * Wn[1] + gf.dg[2] * Wn[2]) * rr;
+      } else {
+        const double aK = abs3(K);
+        a = cis(k * aK * rr) * (BQD2_IFP * J / aK);
+      }
+      Ps += (a * dPi - b * Pi) * wgt;
+      for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2]) * wgt;
+    }
+    if (split) {
+      // smooth remainder of the lattice sum on the triangle's GL12 parts
+      for (int p = lane; p < 3 * 144; p += 64) {
+        double z, e, wgt, vR[3], W[3];
+        rule_point(true, 12, T.xli, T.wli, p, z, e, wgt);
+        wgt *= 1.0 / 24.0;
+        bil_rw(Et, z, e, vR, W);
+        for (int l = 0; l < 3; l++) vR[l] -= rt[l];
+        const double aR = abs3(vR);
+        Sum10 gf;
+        int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+        if (cnt < 0) err = cnt;
+        evals++;
+        cd g0, dg0[3];
+        free_gf(P.k, vR, aR, 1.0 / aR, g0, dg0);
+        const cd a = (gf.g - g0) * aV;
+        const cd b = (gf.dg[0] - dg0[0]) * Vn[0] + (gf.dg[1] - dg0[1]) * Vn[1] + (gf.dg[2] - dg0[2]) * Vn[2];
+        cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+        for (int mm = 0; mm < 3; mm++) {
+          double Nn = lit_N(mm, z, e);
+          Pi += ld(V.P, 4 * s + mm) * Nn;
+          dPi += ld(V.dP, 4 * s + mm) * Nn;
+          for (int cc = 0; cc < 3; cc++) {
+            pvi[cc] += ld(V.pv, 12 * s + 3 * mm + cc) * Nn;
+            dpvi[cc] += ld(V.dpv, 12 * s + 3 * mm + cc) * Nn;
+          }
+        }
+        Ps += (a * dPi - b * Pi) * wgt;
+        for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2]) * wgt;
+      }
+    }
+    Pa += Ps;
+    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += PVs[c2];
+  } else {
   // self element: polar rule about (zt, et)
-  {
     const int sd = td, s = t;
     const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
     double r[5], th[5];
@@ -563,35 +692,41 @@ __global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restric
 // per open-domain element: stress tensor at 4 nodes (type 0) or 9 points
 // (type 1, boundary field from k_field_bd in bdval) (bil_force_4p/9p)
 __global__ void __launch_bounds__(256) k_force_elem(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, int type,
-                                                    const double *__restrict__ bdval, double rcx, double rcy,
-                                                    double rcz, double *__restrict__ part) {
+                                                    const double *__restrict__ bdval, const int *__restrict__ bdoff,
+                                                    double rcx, double rcy, double rcz, double *__restrict__ part) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const DomView &V = X.dom[0];
   if (s > V.Ne) return;
   const Tab &T = *tab;
   const int as = abs(V.sid[s]);
+  const bool tri = M.tri[as] != 0;
   const double c1 = 0.25 / (X.pw_c0 * X.pw_c0 * X.pw_rho0), c2 = 0.25 * X.pw_rho0, c3 = 0.5 * X.pw_rho0;
   const double rc[3] = {rcx, rcy, rcz};
   ElemC E;
   load_elem(M, as, E);
+  const double *bv = bdval ? bdval + 8 * (size_t)bdoff[s - 1] : nullptr;
   double F[3] = {0, 0, 0}, N[3] = {0, 0, 0};
-  const int npt = type == 0 ? 4 : 9;
+  const int npt = type == 0 ? 4 : tri ? 7 : 9;
   for (int i = 0; i < npt; i++) {
     double r[3], w[3], wt;
     cd p, v[3];
-    if (type == 0) {
+    if (type == 0 && !(tri && i == 3)) {
       for (int d = 0; d < 3; d++) {
         r[d] = M.ren[12 * as + 3 * i + d];
         w[d] = M.wen[12 * as + 3 * i + d];
       }
-      wt = T.wt_44[i];
+      wt = tri ? T.wt_34[i] : T.wt_44[i];
       incident_pv(X, r, p, v);
       p -= ld(V.P, 4 * s + i) * cd{X.dom[0].k2x, X.dom[0].k2y};
       for (int d = 0; d < 3; d++) v[d] += ld(V.pv, 12 * s + 3 * i + d);
     } else {
-      bil_rw(E, T.zt_49[i], T.et_49[i], r, w);
-      wt = T.wt_49[i];
-      const double *o = bdval + 8 * (9 * (size_t)(s - 1) + i);
+      // boundary field at a Gauss point (triangles: the centroid of the 4-point
+      // rule, or the 7-point rule), computed beforehand by k_field_bd
+      const double z = type == 0 ? T.zt_34[3] : tri ? T.zt_37[i] : T.zt_49[i];
+      const double e = type == 0 ? T.et_34[3] : tri ? T.et_37[i] : T.et_49[i];
+      bil_rw(E, z, e, r, w);
+      wt = type == 0 ? T.wt_34[3] : tri ? T.wt_37[i] : T.wt_49[i];
+      const double *o = bv + 8 * (type == 0 ? 0 : i);
       p = cd{o[0], o[1]};
       for (int d = 0; d < 3; d++) v[d] = cd{o[2 + 2 * d], o[3 + 2 * d]};
     }
@@ -607,9 +742,10 @@ __global__ void __launch_bounds__(256) k_force_elem(DevMesh M, const Tab *__rest
     N[1] += ((d2 * Tm[0] - d0 * Tm[6]) * w[0] + (d2 * Tm[1] - d0 * Tm[7]) * w[1] + (d2 * Tm[2] - d0 * Tm[8]) * w[2]) * wt;
     N[2] += ((d0 * Tm[3] - d1 * Tm[0]) * w[0] + (d0 * Tm[4] - d1 * Tm[1]) * w[1] + (d0 * Tm[5] - d1 * Tm[2]) * w[2]) * wt;
   }
+  const double sc = tri ? 0.5 : 1.0;  // reference triangle area (lit_force_4p/7p)
   for (int d = 0; d < 3; d++) {
-    part[6 * (s - 1) + d] = F[d];
-    part[6 * (s - 1) + 3 + d] = N[d];
+    part[6 * (s - 1) + d] = F[d] * sc;
+    part[6 * (s - 1) + 3 + d] = N[d] * sc;
   }
 }
 
@@ -632,6 +768,7 @@ static FieldCtx make_ctx(const bqd2_problem *p) {
   memset(&X, 0, sizeof(X));
   X.MN = p->m.MN;
   X.ps = p->m.ps;
+  X.any_tri = p->any_tri;
   for (int d = 0; d <= p->m.MN; d++) {
     const DevDomain &D = p->dom[d];
     X.dom[d].Ne = D.Ne;
@@ -694,7 +831,7 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   int *d_l, *d_did;
   int rc;
   const size_t nn = (size_t)n;
-  const int npt = type == 0 ? 4 : type == 1 ? 9 : type == 2 ? 144 : 4096;
+  const int npt = rule_points(type, p->any_tri);
   int nchunk = 1;
   for (int d = 0; d <= p->m.MN; d++) {
     const long long items = (long long)p->dom[d].Ne * npt, chunk = field_chunk_items(items);
@@ -773,45 +910,57 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
   if (!cx) return BQD2_E_HIP;
   hipStream_t st = cx->st;
   long long *const ctr = cx->cnt;
+  // boundary-field points per element: the 9-point rule of quadrilaterals and
+  // the 7-point rule of triangles for type 1, the centroid of triangles for
+  // type 0 (bil_force_4p/9p, lit_force_4p/7p)
+  std::vector<int> off((size_t)ne + 1, 0);
+  for (int s = 1; s <= ne; s++)
+    off[s] = off[s - 1] + (p->tri0[s] ? (type == 0 ? 1 : 7) : (type == 0 ? 0 : 9));
+  const size_t nbd = (size_t)off[ne];
   Scratch S;
   double *d_bd = nullptr, *d_part, *d_FN;
+  int *d_off = nullptr;
   int rc_;
-  const size_t nbd = (size_t)ne * 9;
   if ((rc_ = S.init(cx, pad256((size_t)ne * 48) + pad256(48) +
-                           (type != 0 ? pad256(nbd * 64) + pad256(nbd * 4) + pad256(nbd * 16) : 0))))
+                           (nbd ? pad256(nbd * 64) + pad256(nbd * 4) + pad256(nbd * 16) + pad256(off.size() * 4)
+                                : 0))))
     return rc_;
   if ((rc_ = S.get((void **)&d_part, (size_t)ne * 48)) || (rc_ = S.get((void **)&d_FN, 48))) return rc_;
   FieldCtx X = make_ctx(p);
-  if (type != 0) {
-    // total field at the 9 Gauss points of every open-domain element (bil_force_9p)
-    std::vector<int> tpos((size_t)ne * 9);
-    std::vector<double> ze((size_t)ne * 18);
-    for (int s = 1; s <= ne; s++)
-      for (int i = 0; i < 9; i++) {
-        tpos[9 * (s - 1) + i] = s;
-        ze[2 * (9 * (s - 1) + i)] = p->htab.zt_49[i];
-        ze[2 * (9 * (s - 1) + i) + 1] = p->htab.et_49[i];
+  if (nbd) {
+    std::vector<int> tpos(nbd);
+    std::vector<double> ze(2 * nbd);
+    const Tab &T = p->htab;
+    for (int s = 1; s <= ne; s++) {
+      const bool tri = p->tri0[s] != 0;
+      for (int i = 0; i < off[s] - off[s - 1]; i++) {
+        const size_t j = (size_t)off[s - 1] + i;
+        tpos[j] = s;
+        ze[2 * j] = type == 0 ? T.zt_34[3] : tri ? T.zt_37[i] : T.zt_49[i];
+        ze[2 * j + 1] = type == 0 ? T.et_34[3] : tri ? T.et_37[i] : T.et_49[i];
       }
+    }
     int *d_t;
     double *d_ze;
-    if ((rc_ = S.get((void **)&d_bd, (size_t)ne * 9 * 64)) || (rc_ = S.get((void **)&d_t, tpos.size() * 4)) ||
-        (rc_ = S.get((void **)&d_ze, ze.size() * 8)))
+    if ((rc_ = S.get((void **)&d_bd, nbd * 64)) || (rc_ = S.get((void **)&d_t, nbd * 4)) ||
+        (rc_ = S.get((void **)&d_ze, nbd * 16)) || (rc_ = S.get((void **)&d_off, off.size() * 4)))
       return rc_;
-    BQD2_HIP(hipMemcpyAsync(d_t, tpos.data(), tpos.size() * 4, hipMemcpyHostToDevice, st));
-    BQD2_HIP(hipMemcpyAsync(d_ze, ze.data(), ze.size() * 8, hipMemcpyHostToDevice, st));
+    BQD2_HIP(hipMemcpyAsync(d_t, tpos.data(), nbd * 4, hipMemcpyHostToDevice, st));
+    BQD2_HIP(hipMemcpyAsync(d_ze, ze.data(), nbd * 16, hipMemcpyHostToDevice, st));
+    BQD2_HIP(hipMemcpyAsync(d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
     BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
-    hipLaunchKernelGGL(k_field_bd, dim3(ne * 9), dim3(64), 0, st, p->m, p->tab, X, p->gf, 1, 1, 0, ne * 9, d_t, d_ze,
-                       d_bd, ctr);
+    hipLaunchKernelGGL(k_field_bd, dim3((unsigned)nbd), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, 1, 0, (int)nbd,
+                       d_t, d_ze, d_bd, ctr);
     BQD2_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_force_elem, dim3(bqd2_blocks(ne, 256)), dim3(256), 0, st, p->m, p->tab, X, type, d_bd, rc[0],
-                     rc[1], rc[2], d_part);
+  hipLaunchKernelGGL(k_force_elem, dim3(bqd2_blocks(ne, 256)), dim3(256), 0, st, p->m, p->tab, X, type, d_bd, d_off,
+                     rc[0], rc[1], rc[2], d_part);
   hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(64), 0, st, ne, d_part, d_FN);
   BQD2_HIP(hipGetLastError());
   double FN[6];
   long long cnt[CNT_N] = {0};
   BQD2_HIP(hipMemcpyAsync(FN, d_FN, 48, hipMemcpyDeviceToHost, st));
-  if (type != 0) BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  if (nbd) BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_force: Green's function error at %lld boundary points", cnt[CNT_ERR]);
   for (int i = 0; i < 3; i++) {

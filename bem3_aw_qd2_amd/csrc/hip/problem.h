@@ -30,6 +30,7 @@ struct DevMesh {
 struct DevDomain {
   int did, Ne, N;      // N = 4 Ne
   int nn;              // collocation nodes of the domain (3 or 4 per element)
+  int has_tri;         // 1 when the domain has linear triangles
   int *sid;            // Ne+1 signed element ids
   int *rowmap;         // N: compressed node index of row (t-1)*4+tn, -1 for unused triangle slot
   double k;            // real wave number of the domain
@@ -65,6 +66,8 @@ struct bqd2_problem {
   double k0[16], rho0[16];
   double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
   std::vector<DevDomain> dom;
+  int any_tri;                // the mesh has linear triangles
+  std::vector<char> tri0;     // host: 1 where element position t of the open domain is a triangle
   std::vector<void *> owned;  // device allocations released by bqd2_destroy
   hipStream_t stream;
   long long *d_counters;      // device counters (GF terms)

@@ -33,9 +33,6 @@ static int upload(bqd2_problem *p, T **dst, const T *src, size_t n) {
 extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
   BQD2_CHECK(m && out, BQD2_E_ARG, "bqd2_create: null argument");
   BQD2_CHECK(m->Ne > 0 && m->MN >= 0 && m->MN < 16, BQD2_E_ARG, "bqd2_create: bad sizes Ne=%d MN=%d", m->Ne, m->MN);
-  for (int s = 1; s <= m->Ne; s++)
-    BQD2_CHECK(m->ed[4 * s + 3] != 0, BQD2_E_ARG,
-               "bqd2_create: element %d is a linear triangle; this build assembles bilinear elements only", s);
   bqd2_problem *p = new bqd2_problem();
   *out = nullptr;
   p->m.Ne = m->Ne;
@@ -99,8 +96,13 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     off += D.Ne + 1;
     int nn = 0;
     std::vector<int> rowmap(D.N > 0 ? D.N : 1, -1);
+    if (d == 0) {
+      p->tri0.assign((size_t)D.Ne + 1, 0);
+      for (int t = 1; t <= D.Ne; t++) p->tri0[t] = tri[abs(sid[t])] ? 1 : 0;
+    }
     for (int t = 1; t <= D.Ne; t++) {
       int nl = tri[abs(sid[t])] ? 3 : 4;
+      if (nl == 3) D.has_tri = p->any_tri = 1;
       for (int tn = 0; tn < nl; tn++) rowmap[(t - 1) * 4 + tn] = nn++;
     }
     D.nn = nn;

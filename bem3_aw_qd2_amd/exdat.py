@@ -48,16 +48,31 @@ def parse(raw):
     doms = []
     for d in range(nd):
         ne = struct.unpack_from("i", raw, off)[0]; off += 4
-        dd = {"Ne": ne}
+        dd = {"Ne": ne, "offsets": {}}
         dd["sid"] = np.frombuffer(raw, np.int32, ne + 1, off).copy(); off += 4 * (ne + 1)
         for name in ("P", "dP"):
+            dd["offsets"][name] = off
             dd[name] = np.frombuffer(raw, np.complex128, (ne + 1) * 4, off).reshape(ne + 1, 4).copy(); off += 64 * (ne + 1)
         for name in ("pv", "dpv"):
+            dd["offsets"][name] = off
             dd[name] = np.frombuffer(raw, np.complex128, (ne + 1) * 12, off).reshape(ne + 1, 4, 3).copy(); off += 192 * (ne + 1)
         doms.append(dd)
     out["dom"] = doms
     out["size"] = off
     return out
+
+
+def with_solution(raw, d, did, **arrays):
+    """Copy of an ex.dat byte string with domain did's P, dP, pv and/or dpv
+    replaced (arrays shaped as parse returns them)."""
+    out = bytearray(raw)
+    dom = d["dom"][did]
+    for name, a in arrays.items():
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == dom[name].shape, name
+        o = dom["offsets"][name]
+        out[o:o + a.nbytes] = a.tobytes()
+    return bytes(out)
 
 
 def read(path):

@@ -55,4 +55,11 @@ void orc_inv33(double *ma, const double *vn, const double *vtz, const double *vt
 void orc_dpdt(const AQD2 *qd, int d, int t, int tn, const double complex *dGrow, const double complex *dHrow,
               const double *dF, double complex *dPz, double complex *dPe);
 void orc_node_pv(const AQD2 *qd, int d, int t, int tn, double complex dPz, double complex dPe, double complex *pv);
+/* scattered field at a point off the boundary (field_cpu.c) */
+int orc_field_s(const AQD2 *qd, int did, const double *rt, int type, double complex *p, double complex *pv,
+                double *gradF);
+/* scattered field at a boundary point (field_cpu.c) */
+int orc_field_bd(const AQD2 *qd, int did, int t, double zt, double et, int type, double complex *p,
+                 double complex *pv);
+
 #endif
