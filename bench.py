@@ -135,6 +135,19 @@ class Request:
         return n + int(self.field_counts[15]) + int(self.force_counts[15])
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of k_q0_pairs from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known
+    size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "k_q0_pairs_%s.md" % workload)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), "profiles/k_q0_pairs_%s.md" % workload
+    return None, None
+
+
 def pair_flops(st):
     kinds = ("spatial", "propagating", "evanescent", "pure_spectral")
     return sum(st.pair_terms[i] * F_TERM[k] for i, k in enumerate(kinds)) + st.pairs_split * 4096 * F_FREE
@@ -283,8 +296,10 @@ def main():
 
     achieved = float(np.sum(flops) / (np.sum(ms_pairs) * 1e-3) / 1e12)
     st0 = req.stats[0]
-    roof = {"bound": "fp64-valu", "kernel": "k_q0_pairs", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+    traffic, traffic_src = measured_traffic(args.workload)
+    roof = {"bound": "mfma", "pipe": "FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "kernel": "k_q0_pairs", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
             "ms_per_launch": float(np.mean(ms_pairs)),
             "flops_per_launch": float(np.mean(flops)),
             "terms_per_launch": {"spatial": int(st0.pair_terms[0]), "propagating": int(st0.pair_terms[1]),
