@@ -5,8 +5,9 @@
 // Green's function once per quadrature point and shape function.  Here:
 //   k_decide      one lane per (row, s): the GL12 dF/dv test that picks GL12 or
 //                 GH64 (d3qd2_elem.c:536-537, :851-852), bit-exact;
-//   k_q0_pairs    one wavefront per (row, s) pair of the periodic domain: the
-//                 chosen Gauss-Legendre rule spread over 64 lanes, one
+//   k_q0_pairs    four (row, s) pairs of the periodic domain per wavefront,
+//                 16 lanes per pair, pairs grouped by rule on the host: the
+//                 chosen Gauss-Legendre rule spread over the pair's lanes, one
 //                 Green's function evaluation per point shared by the four
 //                 shape functions, cross-lane reduction
 //                 (q0_bil_dcoef_v2_cc_GL/GH, q0_bil_dcoef_v2_cc.c:129-281);
@@ -85,14 +86,34 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
 }
 
 // ------------------------------------------------------------------ periodic pairs
-__global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+// Four (row, s) pairs per wavefront, 16 lanes per pair.  The host sorts the
+// pairs by rule and element kind, so the pairs of one wavefront share their
+// loop bounds; a GL12 rule (144 points) then fills every lane, where one pair
+// per wavefront leaves a quarter of the lanes idle in its last pass.
+#define PPW 4
+#define LPP 16
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = LPP / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                   const int *__restrict__ list, long long nlist,
                                                    long long *__restrict__ counters) {
-  const int s = blockIdx.x + 1, row = blockIdx.y, lane = threadIdx.x;
+  const int lane = threadIdx.x, sl = lane % LPP;
+  const long long li = (long long)blockIdx.x * PPW + lane / LPP;
+  // padding entries (the end of a class) take the wavefront's first pair and
+  // add nothing: their loops are empty and they write nothing
+  const int pair = (li < nlist) ? list[li] : -1;
+  const bool on = pair >= 0;
+  const int gid = on ? pair : list[(long long)blockIdx.x * PPW];
+  const int row = gid / D.Ne, s = gid % D.Ne + 1;
   int t, tn, td;
-  if (!row_target(M, D, row, t, tn, td) || s == t) return;
+  row_target(M, D, row, t, tn, td);
   const Tab &T = *tab;
   const int at = abs(td), as = D.sid[s];  // periodic domain: source ids are positive
-  const int rule = D.rule[(long long)row * D.Ne + (s - 1)];
+  const int rule = D.rule[gid];
   double rt[3] = {M.ren[12 * at + 3 * tn], M.ren[12 * at + 3 * tn + 1], M.ren[12 * at + 3 * tn + 2]};
   double vtz[3], vte[3];
   node_tangents(M, T, at, tn, vtz, vte);
@@ -107,8 +128,8 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   long long nev = 0;
   if (rule == 2) {
     // near-singular image on GH64 with the free-space kernel
-    const int np64 = rule_npts(tri, 64);
-    for (int p = lane; p < np64; p += WAVE) {
+    const int np64 = on ? rule_npts(tri, 64) : 0;
+    for (int p = sl; p < np64; p += LPP) {
       double z, e, wt;
       rule_point(tri, 64, T.xhi, T.whi, p, z, e, wt);
       PtGeo q;
@@ -133,9 +154,9 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     const int n = gh ? 64 : 12;
     const double *x = gh ? T.xhi : T.xli;
     const double *w = gh ? T.whi : T.wli;
-    const int npts = rule_npts(tri, n);
-    nev = npts;
-    for (int p = lane; p < npts; p += WAVE) {
+    const int npts = on ? rule_npts(tri, n) : 0;
+    nev = (sl == 0) ? npts : 0;
+    for (int p = sl; p < npts; p += LPP) {
       double z, e, wt;
       rule_point(tri, n, x, w, p, z, e, wt);
       PtGeo q;
@@ -166,13 +187,14 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
   }
 #pragma unroll
   for (int ep = 0; ep < 4; ep++) {
-    g[ep].x = wave_sum(g[ep].x); g[ep].y = wave_sum(g[ep].y);
-    h[ep].x = wave_sum(h[ep].x); h[ep].y = wave_sum(h[ep].y);
+    g[ep].x = group_sum(g[ep].x); g[ep].y = group_sum(g[ep].y);
+    h[ep].x = group_sum(h[ep].x); h[ep].y = group_sum(h[ep].y);
   }
-  F = wave_sum(F); F0 = wave_sum(F0); F1 = wave_sum(F1);
+  F = group_sum(F); F0 = group_sum(F0); F1 = group_sum(F1);
   long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
+  nev = wave_sum_ll(nev);
   int anyerr = __any(err != 0);
-  if (lane == 0) {
+  if (on && sl == 0) {
     // the triangle rules carry 1/24 of the parts' area (q0_lit_dcoef_v2_cc.c:231-237)
     const double sc = tri ? 1.0 / 24.0 : 1.0;
     const size_t base = ((size_t)row * D.N + (size_t)(s - 1) * 4) * 2;
@@ -183,6 +205,8 @@ __global__ void __launch_bounds__(WAVE) k_q0_pairs(DevMesh M, const Tab *__restr
     }
     double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
     fp[0] = F * (BQD2_IFP * sc); fp[1] = F0 * (BQD2_IFP * sc); fp[2] = F1 * (BQD2_IFP * sc);
+  }
+  if (lane == 0) {
     tally_add(counters + CNT_PAIRS, tsp, tpr, tev, tfd, nev);
     if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   }
@@ -880,6 +904,29 @@ __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
   D.dF[3 * row + 2] = Fe;
 }
 
+// ------------------------------------------------------------------ pair list
+// (row, s) pairs of the periodic domain, row-major within each class, classes
+// (rule, element kind) one after the other, each padded with -1 to a multiple
+// of PPW so that a wavefront never mixes classes
+static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule, const std::vector<char> &tripos,
+                      std::vector<int> &out) {
+  std::vector<int> cls[6];
+  for (int row = 0; row < D.N; row++) {
+    const int t = row / 4 + 1, tn = row % 4;
+    if (tripos[t] && tn == 3) continue;
+    for (int s = 1; s <= D.Ne; s++) {
+      if (s == t) continue;
+      const int gid = row * D.Ne + (s - 1);
+      cls[2 * rule[gid] + (tripos[s] ? 1 : 0)].push_back(gid);
+    }
+  }
+  out.clear();
+  for (auto &c : cls) {
+    out.insert(out.end(), c.begin(), c.end());
+    while (out.size() % PPW) out.push_back(-1);
+  }
+}
+
 // ------------------------------------------------------------------ entry point
 extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_assemble: bad domain %d", did);
@@ -887,9 +934,10 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   if (D.Ne == 0) return BQD2_OK;
   const bool periodic = (did == 0 && p->m.ps == 1);
   hipStream_t st_ = p->stream;
-  hipEvent_t e0, e1, e2, e3;
+  hipEvent_t e0, e1, e1b, e2, e3;
   BQD2_HIP(hipEventCreate(&e0));
   BQD2_HIP(hipEventCreate(&e1));
+  BQD2_HIP(hipEventCreate(&e1b));
   BQD2_HIP(hipEventCreate(&e2));
   BQD2_HIP(hipEventCreate(&e3));
   BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st_));
@@ -906,8 +954,18 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
                      periodic ? 1 : 0);
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipEventRecord(e1, st_));
+  std::vector<unsigned char> rule((size_t)npairs);
+  BQD2_HIP(hipMemcpyAsync(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost, st_));
+  BQD2_HIP(hipStreamSynchronize(st_));
+  std::vector<int> plist;
+  if (periodic) pair_list(D, rule, p->tripos[did], plist);
+  BQD2_HIP(hipEventRecord(e1b, st_));
   if (periodic) {
-    hipLaunchKernelGGL(k_q0_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
+    if (!plist.empty()) {
+      BQD2_HIP(hipMemcpyAsync(D.plist, plist.data(), plist.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+      hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(plist.size() / PPW)), dim3(WAVE), 0, st_, p->m, p->tab, D,
+                         p->gf, D.plist, (long long)plist.size(), p->d_counters);
+    }
   } else {
     hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
   }
@@ -928,8 +986,6 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipEventSynchronize(e3));
   long long cnt[CNT_N];
   BQD2_HIP(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  std::vector<unsigned char> rule((size_t)npairs);
-  BQD2_HIP(hipMemcpy(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost));
   long long ngh = 0;
   long long nsplit = 0;
   for (unsigned char r : rule) {
@@ -940,7 +996,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     memset(st, 0, sizeof(*st));
     float a, b, c;
     BQD2_HIP(hipEventElapsedTime(&a, e0, e1));
-    BQD2_HIP(hipEventElapsedTime(&b, e1, e2));
+    BQD2_HIP(hipEventElapsedTime(&b, e1b, e2));
     BQD2_HIP(hipEventElapsedTime(&c, e2, e3));
     st->ms_decide = a; st->ms_pairs = b; st->ms_self = c;
     for (int i = 0; i < 4; i++) {
@@ -958,7 +1014,8 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     st->self_rows = valid_rows;
     st->pairs_gl = valid_rows * (D.Ne - 1) - ngh;
   }
-  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e1b);
+  (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)",
              cnt[CNT_ERR], did);
   D.assembled = true;

@@ -915,7 +915,7 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
   // type 0 (bil_force_4p/9p, lit_force_4p/7p)
   std::vector<int> off((size_t)ne + 1, 0);
   for (int s = 1; s <= ne; s++)
-    off[s] = off[s - 1] + (p->tri0[s] ? (type == 0 ? 1 : 7) : (type == 0 ? 0 : 9));
+    off[s] = off[s - 1] + (p->tripos[0][s] ? (type == 0 ? 1 : 7) : (type == 0 ? 0 : 9));
   const size_t nbd = (size_t)off[ne];
   Scratch S;
   double *d_bd = nullptr, *d_part, *d_FN;
@@ -932,7 +932,7 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
     std::vector<double> ze(2 * nbd);
     const Tab &T = p->htab;
     for (int s = 1; s <= ne; s++) {
-      const bool tri = p->tri0[s] != 0;
+      const bool tri = p->tripos[0][s] != 0;
       for (int i = 0; i < off[s] - off[s - 1]; i++) {
         const size_t j = (size_t)off[s - 1] + i;
         tpos[j] = s;

@@ -90,53 +90,55 @@ __global__ void __launch_bounds__(256) k_swap_rows(int n, int ncols, double *__r
   }
 }
 
-// X[k0:k0+kb, c] <- L11^{-1} X[k0:k0+kb, c] for unit-lower L11 = A[k0.., k0..]
-__global__ void __launch_bounds__(256) k_trsm_lunit(int n, const double *__restrict__ L, int k0, int kb,
-                                                    double *__restrict__ X, int ldx, int c0, int ncols) {
-  __shared__ double sL[NB * NB * 2];
-  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-    int r = e % kb, c = e / kb;
-    const double *a = L + 2 * ((size_t)(k0 + c) * n + k0 + r);
-    sL[2 * e] = a[0];
-    sL[2 * e + 1] = a[1];
+// Triangular solves with the kb x kb diagonal block T = A[k0.., k0..] of a
+// factored panel: X[k0:k0+kb, c] <- T^{-1} X[k0:k0+kb, c] for the columns c of
+// X, T unit lower (forward) or upper (backward).  One lane per column of X, 64
+// columns per workgroup; T and the 64 columns are staged in LDS (the lanes of a
+// wave read consecutive column entries, T by broadcast).
+template <bool UPPER>
+__global__ void __launch_bounds__(64) k_trsm(int n, const double *__restrict__ T, int k0, int kb,
+                                             double *__restrict__ X, int ldx, int c0, int ncols) {
+  __shared__ cd sT[NB * NB];  // column-major: (r, q) at q * NB + r
+  __shared__ cd sX[NB * 64];  // (r, lane) at r * 64 + lane
+  const int lane = threadIdx.x;
+  for (int e = lane; e < kb * kb; e += 64) {
+    const int r = e % kb, q = e / kb;
+    const double *a = T + 2 * ((size_t)(k0 + q) * n + k0 + r);
+    sT[q * NB + r] = cd{a[0], a[1]};
   }
-  __syncthreads();
-  int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c0 + ncols) return;
+  const int c = c0 + blockIdx.x * 64 + lane;
+  const bool on = c < c0 + ncols;
   double *x = X + 2 * ((size_t)c * ldx + k0);
-  cd v[NB];
-#pragma unroll 8
-  for (int r = 0; r < kb; r++) v[r] = cd{x[2 * r], x[2 * r + 1]};
-  for (int r = 0; r < kb; r++) {
-    cd acc = v[r];
-    for (int q = 0; q < r; q++) acc -= cd{sL[2 * (q * kb + r)], sL[2 * (q * kb + r) + 1]} * v[q];
-    v[r] = acc;
+  if (on)
+    for (int r = 0; r < kb; r++) sX[r * 64 + lane] = cd{x[2 * r], x[2 * r + 1]};
+  __syncthreads();
+  if (!on) return;
+  if (!UPPER) {
+    for (int q = 0; q < kb; q++) {
+      const cd vq = sX[q * 64 + lane];
+      for (int r = q + 1; r < kb; r++) sX[r * 64 + lane] -= sT[q * NB + r] * vq;
+    }
+  } else {
+    for (int q = kb - 1; q >= 0; q--) {
+      const cd vq = sX[q * 64 + lane] * crecip(sT[q * NB + q]);
+      sX[q * 64 + lane] = vq;
+      for (int r = 0; r < q; r++) sX[r * 64 + lane] -= sT[q * NB + r] * vq;
+    }
   }
-  for (int r = 0; r < kb; r++) { x[2 * r] = v[r].x; x[2 * r + 1] = v[r].y; }
+  for (int r = 0; r < kb; r++) {
+    const cd v = sX[r * 64 + lane];
+    x[2 * r] = v.x;
+    x[2 * r + 1] = v.y;
+  }
 }
 
-// X[k0:k0+kb, c] <- U11^{-1} X[k0:k0+kb, c] for upper U11 = A[k0.., k0..]
-__global__ void __launch_bounds__(256) k_trsm_upper(int n, const double *__restrict__ U, int k0, int kb,
-                                                    double *__restrict__ X, int ldx, int ncols) {
-  __shared__ double sU[NB * NB * 2];
-  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-    int r = e % kb, c = e / kb;
-    const double *a = U + 2 * ((size_t)(k0 + c) * n + k0 + r);
-    sU[2 * e] = a[0];
-    sU[2 * e + 1] = a[1];
-  }
-  __syncthreads();
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  double *x = X + 2 * ((size_t)c * ldx + k0);
-  cd v[NB];
-  for (int r = 0; r < kb; r++) v[r] = cd{x[2 * r], x[2 * r + 1]};
-  for (int r = kb - 1; r >= 0; r--) {
-    cd acc = v[r];
-    for (int q = r + 1; q < kb; q++) acc -= cd{sU[2 * (q * kb + r)], sU[2 * (q * kb + r) + 1]} * v[q];
-    v[r] = acc * crecip(cd{sU[2 * (r * kb + r)], sU[2 * (r * kb + r) + 1]});
-  }
-  for (int r = 0; r < kb; r++) { x[2 * r] = v[r].x; x[2 * r + 1] = v[r].y; }
+static void trsm(bool upper, int n, const double *T, int k0, int kb, double *X, int ldx, int c0, int ncols,
+                 hipStream_t st) {
+  if (ncols <= 0) return;
+  if (upper)
+    hipLaunchKernelGGL(k_trsm<true>, dim3(bqd2_blocks(ncols, 64)), dim3(64), 0, st, n, T, k0, kb, X, ldx, c0, ncols);
+  else
+    hipLaunchKernelGGL(k_trsm<false>, dim3(bqd2_blocks(ncols, 64)), dim3(64), 0, st, n, T, k0, kb, X, ldx, c0, ncols);
 }
 
 // ---------------------------------------------------------------- trailing update
@@ -242,7 +244,7 @@ extern "C" int bqd2_zgetrf(int n, double *A, int *ipiv, void *stream) {
     hipLaunchKernelGGL(k_swap_rows, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, n, n, A, n, k0, kb, ipiv, k0, c1);
     const int rest = n - c1;
     if (rest > 0) {
-      hipLaunchKernelGGL(k_trsm_lunit, dim3(bqd2_blocks(rest, 256)), dim3(256), 0, st, n, A, k0, kb, A, n, c1, rest);
+      trsm(false, n, A, k0, kb, A, n, c1, rest, st);
       int rc = gemm_sub(rest, rest, kb, A + 2 * ((size_t)k0 * n + c1), n, A + 2 * ((size_t)c1 * n + k0), n,
                         A + 2 * ((size_t)c1 * n + c1), n, st);
       if (rc) return rc;
@@ -271,7 +273,7 @@ extern "C" int bqd2_zgetrs(int n, int nrhs, const double *A, const int *ipiv, do
   // forward: L y = b
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = (n - k0 < NB) ? n - k0 : NB;
-    hipLaunchKernelGGL(k_trsm_lunit, dim3(bqd2_blocks(nrhs, 256)), dim3(256), 0, st, n, A, k0, kb, B, n, 0, nrhs);
+    trsm(false, n, A, k0, kb, B, n, 0, nrhs, st);
     int rest = n - k0 - kb;
     int rc = gemm_sub(rest, nrhs, kb, A + 2 * ((size_t)k0 * n + k0 + kb), n, B + 2 * (size_t)k0, n,
                       B + 2 * (size_t)(k0 + kb), n, st);
@@ -281,7 +283,7 @@ extern "C" int bqd2_zgetrs(int n, int nrhs, const double *A, const int *ipiv, do
   int last = ((n - 1) / NB) * NB;
   for (int k0 = last; k0 >= 0; k0 -= NB) {
     const int kb = (n - k0 < NB) ? n - k0 : NB;
-    hipLaunchKernelGGL(k_trsm_upper, dim3(bqd2_blocks(nrhs, 256)), dim3(256), 0, st, n, A, k0, kb, B, n, nrhs);
+    trsm(true, n, A, k0, kb, B, n, 0, nrhs, st);
     int rc = gemm_sub(k0, nrhs, kb, A + 2 * ((size_t)k0 * n), n, B + 2 * (size_t)k0, n, B, n, st);
     if (rc) return rc;
   }

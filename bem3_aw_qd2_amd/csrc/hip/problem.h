@@ -39,6 +39,7 @@ struct DevDomain {
   double *dF;          // N x 3
   double *Fpair;       // N x Ne x 3 : per pair F, dF/dtz, dF/dte
   unsigned char *rule; // N x Ne : 1 = GH64 rule
+  int *plist;          // N x Ne + 32 : (row, s) pairs sorted by rule and element kind
   double *P, *dP;      // (Ne+1) x 4 complex
   double *pv, *dpv;    // (Ne+1) x 4 x 3 complex
   bool assembled;
@@ -67,7 +68,7 @@ struct bqd2_problem {
   double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
   std::vector<DevDomain> dom;
   int any_tri;                // the mesh has linear triangles
-  std::vector<char> tri0;     // host: 1 where element position t of the open domain is a triangle
+  std::vector<std::vector<char>> tripos;  // host, per domain: 1 where element position t is a triangle
   std::vector<void *> owned;  // device allocations released by bqd2_destroy
   hipStream_t stream;
   long long *d_counters;      // device counters (GF terms)

@@ -96,10 +96,9 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     off += D.Ne + 1;
     int nn = 0;
     std::vector<int> rowmap(D.N > 0 ? D.N : 1, -1);
-    if (d == 0) {
-      p->tri0.assign((size_t)D.Ne + 1, 0);
-      for (int t = 1; t <= D.Ne; t++) p->tri0[t] = tri[abs(sid[t])] ? 1 : 0;
-    }
+    p->tripos.resize(m->MN + 1);
+    p->tripos[d].assign((size_t)D.Ne + 1, 0);
+    for (int t = 1; t <= D.Ne; t++) p->tripos[d][t] = tri[abs(sid[t])] ? 1 : 0;
     for (int t = 1; t <= D.Ne; t++) {
       int nl = tri[abs(sid[t])] ? 3 : 4;
       if (nl == 3) D.has_tri = p->any_tri = 1;
@@ -119,6 +118,7 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     TRY(bqd2_alloc(p, (void **)&D.dF, N * 3 * 8));
     TRY(bqd2_alloc(p, (void **)&D.Fpair, N * D.Ne * 3 * 8));
     TRY(bqd2_alloc(p, (void **)&D.rule, N * D.Ne));
+    TRY(bqd2_alloc(p, (void **)&D.plist, (N * D.Ne + 32) * sizeof(int)));
     TRY(bqd2_alloc(p, (void **)&D.P, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.dP, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.pv, ((size_t)D.Ne + 1) * 12 * 16));
