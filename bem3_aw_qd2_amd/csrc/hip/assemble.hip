@@ -321,7 +321,7 @@ __device__ double self_F_wave(const ElemC &E, const Tab &T, double zt, double et
   return wave_sum(acc) * BQD2_IFP * bdC;
 }
 
-__global__ void __launch_bounds__(WAVE) k_q0_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
                                                   long long *__restrict__ counters) {
   const int row = blockIdx.x, lane = threadIdx.x;
   int t, tn, td;
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
 // periodic self element (q0_lit_coef_bd, q0_lit_coef.c:297-372; dC = 0,
 // q0_lit_dcoef_v2_cc.c:331-343), with the same near/smooth split as the
 // quadrilateral self element
-__global__ void __launch_bounds__(WAVE) k_q0_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
                                                       long long *__restrict__ counters) {
   const int row = blockIdx.x, lane = threadIdx.x;
   int t, tn, td;

@@ -207,7 +207,7 @@ __host__ __device__ __forceinline__ long long field_chunk_items(long long items)
   return 64 * (per > 0 ? per : 1);
 }
 
-__global__ void __launch_bounds__(64) k_field_part(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_field_part(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
                                                    int type, int n, int nchunk, const double *__restrict__ r0s,
                                                    const int *__restrict__ did_io,
                                                    double *__restrict__ part, long long *__restrict__ counters) {
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(256) k_field_fin(FieldCtx X, int total, int n,
 
 // ------------------------------------------------------------ boundary points
 // one wavefront per boundary point (domain did, element position t, zeta, eta)
-__global__ void __launch_bounds__(64) k_field_bd(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_field_bd(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
                                                  int type, int total, int did, int n, const int *__restrict__ tpos,
                                                  const double *__restrict__ ze, double *__restrict__ out,
                                                  long long *__restrict__ counters) {

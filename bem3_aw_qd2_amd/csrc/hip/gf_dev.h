@@ -28,6 +28,9 @@
 #define BQD2_EW_LIMIT 100
 #define BQD2_FD_LIMIT 100
 #define EW_EXPM_D 3.0
+// |z| / E below which the Ewald spectral terms are formed exactly as the
+// reference forms them (its stopping rule reads rounding residue at z == 0)
+#define BQD2_ZFAST 2.0e-8
 
 enum { GF_QG = 0, GF_DQG = 1, GF_V1 = 2, GF_V2 = 3 };
 
@@ -127,9 +130,11 @@ __device__ __forceinline__ bool term_small_fd(const Sum10& t, const Sum10& s, do
 
 // ---------------------------------------------------------------- terms
 // spatial image (l1, l2): d3_qpgf_d2.c:1814-1838 (v1) and :2131-2170 (v2)
+// ce is the Bloch phase exp(i k_t . R_l) of the image, carried along the row by
+// the caller (one sincos per row instead of per image)
 template <int M>
 __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, double ry, double rz, double E,
-                                             int l1, int l2, Sum10& t) {
+                                             int l1, int l2, cd ce, Sum10& t) {
   double Rl0 = l1 * P.d1x + l2 * P.d2x;
   double Rl1 = l1 * P.d1y + l2 * P.d2y;
   double ra0 = rx + Rl0, ra1 = ry + Rl1;
@@ -137,7 +142,6 @@ __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, doubl
   double irl = 1.0 / rl;
   double iE = 1.0 / E;
   double ke = P.k * E, y = rl * 0.5 * iE;
-  cd ce = cis(P.vkx * Rl0 + P.vky * Rl1);
   double ef = exp(ke * ke - y * y);
   cd w = weideman_upper(ke, y);
   const double isp = 0.56418958354775628695;
@@ -163,13 +167,19 @@ __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, doubl
 
 // spectral order (n1, n2): d3_qpgf_d2.c:1965-2009 (v1) and :2349-2398 (v2).
 // returns 0 propagating, 1 evanescent, -3 Wood anomaly.
+// fast: ce is the planar phase exp(i a_n . r) carried along the row by the
+// caller, and the evanescent pair e^{+-b z} erfc(b E +- z/2E) is formed as
+// exp(-(b^2 E^2 + z^2/4E^2)) erfcx(b E +- z/2E) (the same value, one exp and two
+// scaled complementary error functions).  Used away from z == 0 only, where the
+// stopping rule reads well-conditioned terms; at z == 0 the terms are formed as
+// the reference forms them.
 template <int M>
 __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, double ry, double rz, double E,
-                                             int n1, int n2, Sum10& t) {
+                                             int n1, int n2, bool fast, cd cef, Sum10& t) {
   double an0 = -P.vkx + n1 * P.g1x + n2 * P.g2x;
   double an1 = -P.vky + n1 * P.g1y + n2 * P.g2y;
   double bn2 = P.k * P.k - an0 * an0 - an1 * an1;
-  cd ce = cis(an0 * rx + an1 * ry);
+  cd ce = fast ? cef : cis(an0 * rx + an1 * ry);
   double azve = rz / (2.0 * E);
   const double isp = 0.56418958354775628695;
   if (bn2 > 0.0) {
@@ -199,9 +209,16 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     return 0;
   } else if (bn2 < 0.0) {
     double bn = sqrt(-bn2);
-    double ebz = exp(bn * rz);
-    double f1 = ebz * erfc(bn * E + azve);
-    double f2 = 1.0 / ebz * erfc(bn * E - azve);
+    double f1, f2, efs = 0.0;
+    if (fast) {
+      efs = exp(bn2 * E * E - azve * azve);
+      f1 = efs * erfcx(bn * E + azve);
+      f2 = efs * erfcx(bn * E - azve);
+    } else {
+      double ebz = exp(bn * rz);
+      f1 = ebz * erfc(bn * E + azve);
+      f2 = 1.0 / ebz * erfc(bn * E - azve);
+    }
     double ib = 1.0 / bn;
     t.g = mul_i(ce) * (-(f1 + f2) * ib);
     if (M != GF_QG) {
@@ -209,7 +226,7 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
       t.dg[1] = t.g * an1;
       t.dg[2] = ce * (f1 - f2);
       if (M == GF_V2) {
-        double ef = exp(bn2 * E * E - azve * azve);
+        double ef = fast ? efs : exp(bn2 * E * E - azve * azve);
         t.d2g[0] = t.dg[0] * an0;
         t.d2g[1] = t.dg[1] * an1;
         t.d2g[2] = ce * (-bn * (f1 + f2) + 2.0 * isp / E * ef);
@@ -224,8 +241,9 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
 }
 
 // pure spectral order: d3_qpgf_d2.c:581-604, :1013-1043
+// ce: the planar phase exp(i a_n . r), carried along the row by the caller
 template <int M>
-__device__ __forceinline__ int fd_term(const GfParams& P, double rx, double ry, double rz, int n1, int n2,
+__device__ __forceinline__ int fd_term(const GfParams& P, double rx, double ry, double rz, int n1, int n2, cd ce,
                                        Sum10& t) {
   double an0 = -P.vkx + n1 * P.g1x + n2 * P.g2x;
   double an1 = -P.vky + n1 * P.g1y + n2 * P.g2y;
@@ -240,7 +258,7 @@ __device__ __forceinline__ int fd_term(const GfParams& P, double rx, double ry, 
     kind = 0;
   } else if (bn2 < 0.0) {
     bn = sqrt(-bn2);
-    e = cis(an0 * rx + an1 * ry) * exp(-bn * fabs(rz));
+    e = ce * exp(-bn * fabs(rz));
     t.g = mul_i(e) * (-1.0 / bn);
     kind = 1;
   } else {
@@ -267,97 +285,105 @@ struct GfTally {
   int sp, pr, ev, fd;
 };
 
+// Each row (and each sum of rows below) runs as one loop: the centre term, then
+// outward in +1 direction until the stopping rule fires, then in -1 direction
+// from the centre.  The order of the terms and the stopping rules are those of
+// the reference's three loops (centre, up, down); one loop keeps a single copy
+// of the term code and lets lanes in different phases run together.
+
+// cs: exp(i k_t . d2), the Bloch phase step from image l2 to l2 + 1
 template <int M>
 __device__ int spatial_row(const GfParams& P, double rx, double ry, double rz, double E, double eps, int l1,
-                           Sum10& S) {
-  int lc = -(int)round(((rx + l1 * P.d1x) * P.d2x + (ry + l1 * P.d1y) * P.d2y) / P.d2n);
+                           cd cs, Sum10& S) {
+  const int lc = -(int)round(((rx + l1 * P.d1x) * P.d2x + (ry + l1 * P.d1y) * P.d2y) / P.d2n);
+  const cd c0 = cis(P.vkx * (l1 * P.d1x + lc * P.d2x) + P.vky * (l1 * P.d1y + lc * P.d2y));
+  const cd csb = conj(cs);
   Sum10 t;
-  spatial_term<M>(P, rx, ry, rz, E, l1, lc, t);
-  sum_add<M>(S, t);
-  int cc = 1, l2;
-  for (l2 = lc + 1; l2 < lc + BQD2_EW_LIMIT; l2++) {
-    cc++;
-    spatial_term<M>(P, rx, ry, rz, E, l1, l2, t);
+  int l2 = lc, dir = 0, cc = 0;
+  cd ce = c0;
+  for (;;) {
+    spatial_term<M>(P, rx, ry, rz, E, l1, l2, ce, t);
     sum_add<M>(S, t);
-    if (term_small_spatial<M>(t, S, eps)) break;
-  }
-  if (l2 == lc + BQD2_EW_LIMIT) return -1;
-  for (l2 = lc - 1; l2 > lc - BQD2_EW_LIMIT; l2--) {
     cc++;
-    spatial_term<M>(P, rx, ry, rz, E, l1, l2, t);
-    sum_add<M>(S, t);
-    if (term_small_spatial<M>(t, S, eps)) break;
+    if (dir == 0) {
+      dir = 1;
+    } else if (term_small_spatial<M>(t, S, eps)) {
+      if (dir < 0) break;
+      dir = -1;
+      l2 = lc;
+      ce = c0;
+    } else if (dir > 0 ? l2 + 1 == lc + BQD2_EW_LIMIT : l2 - 1 == lc - BQD2_EW_LIMIT) {
+      return -1;
+    }
+    l2 += dir;
+    ce = ce * (dir > 0 ? cs : csb);
   }
-  if (l2 == lc - BQD2_EW_LIMIT) return -1;
   return cc;
 }
 
 // the row flag is 1 when no order other than the centre one was propagating:
-// the reference counts propagating orders only in its outward loops
+// the reference counts propagating orders only in its outward loops.
+// fast: planar phases carried along the row (cs = exp(i g2 . r) per step)
 template <int M>
 __device__ int spectral_row(const GfParams& P, double rx, double ry, double rz, double E, double eps, int n1,
-                            Sum10& S, int& flag, int& nev) {
-  int nc = (int)round(-((-P.vkx + n1 * P.g1x) * P.g2x + (-P.vky + n1 * P.g1y) * P.g2y) / P.g2n);
+                            bool fast, cd cs, Sum10& S, int& flag, int& nev) {
+  const int nc = (int)round(-((-P.vkx + n1 * P.g1x) * P.g2x + (-P.vky + n1 * P.g1y) * P.g2y) / P.g2n);
+  cd c0 = cd{1.0, 0.0};
+  if (fast) c0 = cis((-P.vkx + n1 * P.g1x + nc * P.g2x) * rx + (-P.vky + n1 * P.g1y + nc * P.g2y) * ry);
+  const cd csb = conj(cs);
   Sum10 t;
-  int k = spectral_term<M>(P, rx, ry, rz, E, n1, nc, t);
-  if (k < 0) return k;
-  sum_add<M>(S, t);
-  nev += k;
-  int cc = 1, n2, prop = 0;
-  for (n2 = nc + 1; n2 < nc + BQD2_EW_LIMIT; n2++) {
-    cc++;
-    k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, t);
+  int n2 = nc, dir = 0, cc = 0, prop = 0;
+  cd ce = c0;
+  for (;;) {
+    const int k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, fast, ce, t);
     if (k < 0) return k;
     sum_add<M>(S, t);
     nev += k;
-    if (k > 0) {
-      if (term_small_spectral<M>(t, S, eps)) break;
-    } else {
-      prop++;
-    }
-  }
-  if (n2 == nc + BQD2_EW_LIMIT) return -2;
-  for (n2 = nc - 1; n2 > nc - BQD2_EW_LIMIT; n2--) {
     cc++;
-    k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, t);
-    if (k < 0) return k;
-    sum_add<M>(S, t);
-    nev += k;
-    if (k > 0) {
-      if (term_small_spectral<M>(t, S, eps)) break;
+    if (dir == 0) {
+      dir = 1;
+    } else if (k > 0 && term_small_spectral<M>(t, S, eps)) {
+      if (dir < 0) break;
+      dir = -1;
+      n2 = nc;
+      ce = c0;
     } else {
-      prop++;
+      if (k == 0) prop++;
+      if (dir > 0 ? n2 + 1 == nc + BQD2_EW_LIMIT : n2 - 1 == nc - BQD2_EW_LIMIT) return -2;
     }
+    n2 += dir;
+    ce = ce * (dir > 0 ? cs : csb);
   }
-  if (n2 == nc - BQD2_EW_LIMIT) return -2;
   flag = (prop == 0);
   return cc;
 }
 
 template <int M>
-__device__ int fd_row(const GfParams& P, double rx, double ry, double rz, double eps, int n1, Sum10& S) {
-  int nc = (int)round((P.vkx * P.g2x + P.vky * P.g2y - n1 * P.g12) / P.g2n);
+__device__ int fd_row(const GfParams& P, double rx, double ry, double rz, double eps, int n1, cd cs, Sum10& S) {
+  const int nc = (int)round((P.vkx * P.g2x + P.vky * P.g2y - n1 * P.g12) / P.g2n);
+  const cd c0 = cis((-P.vkx + n1 * P.g1x + nc * P.g2x) * rx + (-P.vky + n1 * P.g1y + nc * P.g2y) * ry);
+  const cd csb = conj(cs);
   Sum10 t;
-  int k = fd_term<M>(P, rx, ry, rz, n1, nc, t);
-  if (k < 0) return k;
-  sum_add<M>(S, t);
-  int cc = 1, n2;
-  for (n2 = nc + 1; n2 < nc + BQD2_FD_LIMIT; n2++) {
-    k = fd_term<M>(P, rx, ry, rz, n1, n2, t);
+  int n2 = nc, dir = 0, cc = 0;
+  cd ce = c0;
+  for (;;) {
+    const int k = fd_term<M>(P, rx, ry, rz, n1, n2, ce, t);
     if (k < 0) return k;
     sum_add<M>(S, t);
     cc++;
-    if (k > 0 && term_small_fd<M>(t, S, eps)) break;
+    if (dir == 0) {
+      dir = 1;
+    } else if (k > 0 && term_small_fd<M>(t, S, eps)) {
+      if (dir < 0) break;
+      dir = -1;
+      n2 = nc;
+      ce = c0;
+    } else if (dir > 0 ? n2 + 1 == nc + BQD2_FD_LIMIT : n2 - 1 == nc - BQD2_FD_LIMIT) {
+      return -1;
+    }
+    n2 += dir;
+    ce = ce * (dir > 0 ? cs : csb);
   }
-  if (n2 == nc + BQD2_FD_LIMIT) return -1;
-  for (n2 = nc - 1; n2 > nc - BQD2_FD_LIMIT; n2--) {
-    k = fd_term<M>(P, rx, ry, rz, n1, n2, t);
-    if (k < 0) return k;
-    sum_add<M>(S, t);
-    cc++;
-    if (k > 0 && term_small_fd<M>(t, S, eps)) break;
-  }
-  if (n2 == nc - BQD2_FD_LIMIT) return -1;
   return cc;
 }
 
@@ -377,49 +403,49 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   double k2 = P.k * P.k;
   if (k2 * E * E - r2 / (4.0 * E * E) > EW_EXPM_D) E = sqrt(EW_EXPM_D + sqrt(k2 * r2 + EW_EXPM_D * EW_EXPM_D)) / (sqrt(2.0) * P.k);
 
+  // Bloch phase step of the images and planar phase step of the spectral
+  // orders along a row; the spectral recurrence is used away from z == 0
+  const cd css = cis(P.vkx * P.d2x + P.vky * P.d2y);
+  const cd csg = cis(P.g2x * rx + P.g2y * ry);
+  const bool fast = fabs(rz) >= BQD2_ZFAST * E;
   Sum10 S1, S2, old;
   sum_zero(S1);
   sum_zero(S2);
-  int c1 = spatial_row<M>(P, rx, ry, rz, E, eps, 0, S1);
-  if (c1 < 0) return -1;
-  int l1;
-  for (l1 = 1; l1 < BQD2_EW_LIMIT; l1++) {
+  int c1 = 0;
+  for (int l1 = 0, dir = 0;;) {
     old = S1;
-    int c = spatial_row<M>(P, rx, ry, rz, E, eps, l1, S1);
+    const int c = spatial_row<M>(P, rx, ry, rz, E, eps, l1, css, S1);
     if (c < 0) return -1;
     c1 += c;
-    if (rows_conv_spatial<M>(S1, old, eps)) break;
+    if (dir == 0) {
+      dir = 1;
+    } else if (rows_conv_spatial<M>(S1, old, eps)) {
+      if (dir < 0) break;
+      dir = -1;
+      l1 = 0;
+    } else if (dir > 0 ? l1 + 1 == BQD2_EW_LIMIT : l1 - 1 == -BQD2_EW_LIMIT) {
+      return -1;
+    }
+    l1 += dir;
   }
-  if (l1 == BQD2_EW_LIMIT) return -1;
-  for (l1 = -1; l1 > -BQD2_EW_LIMIT; l1--) {
-    old = S1;
-    int c = spatial_row<M>(P, rx, ry, rz, E, eps, l1, S1);
-    if (c < 0) return -1;
-    c1 += c;
-    if (rows_conv_spatial<M>(S1, old, eps)) break;
-  }
-  if (l1 == -BQD2_EW_LIMIT) return -1;
 
-  int flag = 0, nev = 0;
-  int c2 = spectral_row<M>(P, rx, ry, rz, E, eps, P.nc, S2, flag, nev);
-  if (c2 < 0) return c2;
-  int n1;
-  for (n1 = P.nc + 1; n1 < P.nc + BQD2_EW_LIMIT; n1++) {
+  int flag = 0, nev = 0, c2 = 0;
+  for (int n1 = P.nc, dir = 0;;) {
     old = S2;
-    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag, nev);
+    const int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, fast, csg, S2, flag, nev);
     if (c < 0) return c;
     c2 += c;
-    if (flag && rows_conv_spectral<M>(S2, old, eps)) break;
+    if (dir == 0) {
+      dir = 1;
+    } else if (flag && rows_conv_spectral<M>(S2, old, eps)) {
+      if (dir < 0) break;
+      dir = -1;
+      n1 = P.nc;
+    } else if (dir > 0 ? n1 + 1 == P.nc + BQD2_EW_LIMIT : n1 - 1 == P.nc - BQD2_EW_LIMIT) {
+      return -2;
+    }
+    n1 += dir;
   }
-  if (n1 == P.nc + BQD2_EW_LIMIT) return -2;
-  for (n1 = P.nc - 1; n1 > P.nc - BQD2_EW_LIMIT; n1--) {
-    old = S2;
-    int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, S2, flag, nev);
-    if (c < 0) return c;
-    c2 += c;
-    if (flag && rows_conv_spectral<M>(S2, old, eps)) break;
-  }
-  if (n1 == P.nc - BQD2_EW_LIMIT) return -2;
 
   tl.sp += c1;
   tl.ev += nev;
@@ -448,27 +474,26 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
 template <int M>
 __device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   if (rz == 0.0) return -4;
+  const cd csg = cis(P.g2x * rx + P.g2y * ry);
   Sum10 S, old;
   sum_zero(S);
-  int cc = fd_row<M>(P, rx, ry, rz, eps, P.nc, S);
-  if (cc < 0) return cc;
-  int n1;
-  for (n1 = P.nc + 1; n1 < P.nc + BQD2_FD_LIMIT; n1++) {
+  int cc = 0;
+  for (int n1 = P.nc, dir = 0;;) {
     old = S;
-    int c = fd_row<M>(P, rx, ry, rz, eps, n1, S);
+    const int c = fd_row<M>(P, rx, ry, rz, eps, n1, csg, S);
     if (c < 0) return c;
     cc += c;
-    if (rows_conv_fd<M>(S, old, eps)) break;
+    if (dir == 0) {
+      dir = 1;
+    } else if (rows_conv_fd<M>(S, old, eps)) {
+      if (dir < 0) break;
+      dir = -1;
+      n1 = P.nc;
+    } else if (dir > 0 ? n1 + 1 == P.nc + BQD2_FD_LIMIT : n1 - 1 == P.nc - BQD2_FD_LIMIT) {
+      return -1;
+    }
+    n1 += dir;
   }
-  if (n1 == P.nc + BQD2_FD_LIMIT) return -1;
-  for (n1 = P.nc - 1; n1 > P.nc - BQD2_FD_LIMIT; n1--) {
-    old = S;
-    int c = fd_row<M>(P, rx, ry, rz, eps, n1, S);
-    if (c < 0) return c;
-    cc += c;
-    if (rows_conv_fd<M>(S, old, eps)) break;
-  }
-  if (n1 == P.nc - BQD2_FD_LIMIT) return -1;
   tl.fd += cc;
   double a = P.aidd, sz = rz / fabs(rz);
   out.g = mul_i(S.g) * (0.5 * a);
