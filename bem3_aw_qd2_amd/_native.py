@@ -61,6 +61,8 @@ def hip_lib():
         lib.bqd2_download_domain.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
         lib.bqd2_solve_surface.argtypes = [C.c_void_p, C.c_void_p]
         lib.bqd2_solve_velocity.argtypes = [C.c_void_p]
+        lib.bqd2_set_solver.argtypes = [C.c_void_p, C.c_int]
+        lib.bqd2_solver_info.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.bqd2_download_solution.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
         lib.bqd2_zgetrf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.bqd2_zgetrs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]

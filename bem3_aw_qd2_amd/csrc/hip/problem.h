@@ -77,6 +77,14 @@ struct bqd2_problem {
   int *ipiv;
   int na;
   bool solved;
+  int solver;              // 0: GMRES, LU when it misses the tolerance; 1: LU only
+  double *hk2d;            // k2[m] / k2[s] table (16 x 16 complex), set once
+  double *xg;              // GMRES solution of the system (na)
+  double *tpv;             // open-domain node velocities (NN x 3)
+  double *dpvw;            // dpv right-hand sides and solutions on the row slots (2 x 3 x N)
+  int info_its[17];        // Krylov steps of the last solve: system, then dpv of domain d at 1 + d
+  double info_res[17];     // final scaled residuals
+  int info_lu;             // systems of the last solve that fell back to the LU
   // the creating thread uses main (the problem's stream and counters); other
   // threads get a context of their own on first use
   CallCtx main;
@@ -86,6 +94,11 @@ struct bqd2_problem {
 };
 
 int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);
+// restarted GMRES on nrhs <= 3 right-hand sides sharing the row-major matrix A
+// (gmres.hip); returns 0 when the scaled residual is below tol, 1 when not (the
+// caller falls back to the LU), < 0 on errors
+int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, const int *map, const double *B,
+               long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres);
 // the calling thread's context (nullptr on failure, error set)
 CallCtx *bqd2_call_ctx(bqd2_problem *p);
 // at least `bytes` of device workspace at *base (previous contents are lost)

@@ -42,7 +42,13 @@ __device__ __forceinline__ void inv33(double *ma, const double *vn, const double
 // A entries of domain D's rows: one thread per (row, s).  Main side: P column
 // H, dP column -G; sub side: dP column G (its P column hk2*H is written by
 // k_system_b together with the right-hand side).
-__global__ void __launch_bounds__(256) k_system_A(DevMesh M, DevDomain D, int rowoff, double *__restrict__ A, int na) {
+// element (r, c) of the dense system: column-major for the LU, row-major for GMRES
+__device__ __forceinline__ size_t sys_at(int rm, size_t r, size_t c, int na) {
+  return rm ? r * na + c : c * na + r;
+}
+
+__global__ void __launch_bounds__(256) k_system_A(DevMesh M, DevDomain D, int rowoff, double *__restrict__ A, int na,
+                                                  int rm) {
   long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)D.N * D.Ne) return;
   const int row = (int)(gid / D.Ne), s = (int)(gid % D.Ne) + 1;
@@ -57,9 +63,9 @@ __global__ void __launch_bounds__(256) k_system_A(DevMesh M, DevDomain D, int ro
   const double *Hr = D.H + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
   for (int l = 0; l < nl; l++) {
     const int col = M.eni[4 * asd + l];
-    double *b = A + 2 * ((size_t)(nn + col) * na + arow);
+    double *b = A + 2 * sys_at(rm, arow, (size_t)nn + col, na);
     if (main_side) {
-      double *a = A + 2 * ((size_t)col * na + arow);
+      double *a = A + 2 * sys_at(rm, arow, col, na);
       a[0] = Hr[2 * l];
       a[1] = Hr[2 * l + 1];
       b[0] = -Gr[2 * l];
@@ -73,7 +79,7 @@ __global__ void __launch_bounds__(256) k_system_A(DevMesh M, DevDomain D, int ro
 
 // sub-side P columns carry hk2 = k2[md]/k2[sd]; right-hand side per row
 __global__ void __launch_bounds__(64) k_system_b(DevMesh M, DevDomain D, int rowoff,
-                                                              double *__restrict__ A, int na,
+                                                              double *__restrict__ A, int na, int rm,
                                                               const double *__restrict__ hk2tab,
                                                               double *__restrict__ bvec) {
   const int row = blockIdx.x, lane = threadIdx.x;
@@ -92,7 +98,7 @@ __global__ void __launch_bounds__(64) k_system_b(DevMesh M, DevDomain D, int row
     for (int l = 0; l < nl; l++) {
       const int col = M.eni[4 * asd + l];
       cd h = hk2 * cd{Hr[2 * l], Hr[2 * l + 1]};
-      double *a = A + 2 * ((size_t)col * na + arow);
+      double *a = A + 2 * sys_at(rm, arow, col, na);
       a[0] = h.x; a[1] = h.y;
       if (mdid == 0) {
         cd pi = cd{M.Pi[2 * (4 * asd + l)], M.Pi[2 * (4 * asd + l) + 1]};
@@ -249,6 +255,36 @@ __global__ void __launch_bounds__(64) k_dpv_B(DevMesh M, DevDomain D, double *__
   }
 }
 
+// the same right-hand side on the N row slots of the domain (zero on unused
+// slots), B[i N + row], for GMRES on G itself
+__global__ void __launch_bounds__(64) k_dpv_B_slots(DevDomain D, double *__restrict__ B) {
+  const int row = blockIdx.x, lane = threadIdx.x;
+  const bool used = D.rowmap[row] >= 0;
+  cd acc[3] = {};
+  if (used)
+    for (int c = lane; c < D.N; c += 64) {
+      if (D.rowmap[c] < 0) continue;
+      cd h = cd{D.H[2 * ((size_t)row * D.N + c)], D.H[2 * ((size_t)row * D.N + c) + 1]};
+      for (int i = 0; i < 3; i++) acc[i] += h * cd{D.pv[2 * (12 + 3 * c + i)], D.pv[2 * (12 + 3 * c + i) + 1]};
+    }
+  for (int i = 0; i < 3; i++) {
+    acc[i] = cd{wave_sum(acc[i].x), wave_sum(acc[i].y)};
+    if (lane == 0) {
+      B[2 * ((size_t)i * D.N + row)] = acc[i].x;
+      B[2 * ((size_t)i * D.N + row) + 1] = acc[i].y;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_dpv_store_slots(DevDomain D, const double *__restrict__ X) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= D.N || D.rowmap[c] < 0) return;
+  for (int i = 0; i < 3; i++) {
+    D.dpv[2 * (12 + 3 * c + i)] = X[2 * ((size_t)i * D.N + c)];
+    D.dpv[2 * (12 + 3 * c + i) + 1] = X[2 * ((size_t)i * D.N + c) + 1];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_dpv_store(DevDomain D, const double *__restrict__ B) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= D.N) return;
@@ -271,28 +307,25 @@ static void hk2_table(const bqd2_problem *p, double *tab) {
     }
 }
 
-extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
-  BQD2_CHECK(p, BQD2_E_ARG, "bqd2_solve_surface: null problem");
-  for (int d = 0; d <= p->m.MN; d++)
-    BQD2_CHECK(p->dom[d].Ne == 0 || p->dom[d].assembled, BQD2_E_STATE, "bqd2_solve_surface: domain %d not assembled", d);
+// GMRES tolerance (scaled residual) and limits for the solves below
+#define GM_TOL 1.0e-15
+#define GM_RESTART 100
+#define GM_MAXIT 400
+
+static int ensure_hk2(bqd2_problem *p) {
+  if (p->hk2d) return BQD2_OK;
+  double tab[16 * 16 * 2];
+  memset(tab, 0, sizeof(tab));
+  hk2_table(p, tab);
+  int rc = bqd2_alloc(p, (void **)&p->hk2d, sizeof(tab));
+  if (rc) return rc;
+  BQD2_HIP(hipMemcpy(p->hk2d, tab, sizeof(tab), hipMemcpyHostToDevice));
+  return BQD2_OK;
+}
+
+// the dense system of all domains, column-major (rm = 0) or row-major (rm = 1)
+static int build_system(bqd2_problem *p, int rm, hipStream_t st) {
   const int na = p->na;
-  hipStream_t st = p->stream;
-  if (!p->A) {
-    int rc = bqd2_alloc(p, (void **)&p->A, (size_t)na * na * 16);
-    if (rc) return rc;
-    if ((rc = bqd2_alloc(p, (void **)&p->b, (size_t)na * 16))) return rc;
-    if ((rc = bqd2_alloc(p, (void **)&p->ipiv, (size_t)na * sizeof(int)))) return rc;
-  }
-  double *hk2;
-  {
-    static thread_local double tab[16 * 16 * 2];
-    memset(tab, 0, sizeof(tab));
-    hk2_table(p, tab);
-    int rc = bqd2_alloc(p, (void **)&hk2, sizeof(tab));
-    if (rc) return rc;
-    BQD2_HIP(hipMemcpyAsync(hk2, tab, sizeof(tab), hipMemcpyHostToDevice, st));
-    BQD2_HIP(hipStreamSynchronize(st));
-  }
   BQD2_HIP(hipMemsetAsync(p->A, 0, (size_t)na * na * 16, st));
   BQD2_HIP(hipMemsetAsync(p->b, 0, (size_t)na * 16, st));
   int rowoff = 0;
@@ -300,25 +333,100 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
     long long n = (long long)D.N * D.Ne;
-    hipLaunchKernelGGL(k_system_A, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, D, rowoff, p->A, na);
-    hipLaunchKernelGGL(k_system_b, dim3(D.N), dim3(64), 0, st, p->m, D, rowoff, p->A, na, hk2, p->b);
+    hipLaunchKernelGGL(k_system_A, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, D, rowoff, p->A, na, rm);
+    hipLaunchKernelGGL(k_system_b, dim3(D.N), dim3(64), 0, st, p->m, D, rowoff, p->A, na, rm, p->hk2d, p->b);
     BQD2_HIP(hipGetLastError());
     rowoff += D.nn;
   }
   BQD2_CHECK(rowoff == na, BQD2_E_STATE, "bqd2_solve_surface: %d system rows for %d unknowns", rowoff, na);
-  int rc = bqd2_zgetrf(na, p->A, p->ipiv, st);
-  if (rc) return rc;
-  rc = bqd2_zgetrs(na, 1, p->A, p->ipiv, p->b, st);
-  if (rc) return rc;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_set_solver(bqd2_problem *p, int mode) {
+  BQD2_CHECK(p && (mode == 0 || mode == 1), BQD2_E_ARG, "bqd2_set_solver: mode must be 0 (GMRES, LU fallback) or 1 (LU)");
+  p->solver = mode;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_solver_info(bqd2_problem *p, int *iters, double *relres, int *lu) {
+  BQD2_CHECK(p, BQD2_E_ARG, "bqd2_solver_info: null problem");
+  for (int i = 0; i <= p->m.MN + 1 && i < 17; i++) {
+    if (iters) iters[i] = p->info_its[i];
+    if (relres) relres[i] = p->info_res[i];
+  }
+  if (lu) *lu = p->info_lu;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
+  BQD2_CHECK(p, BQD2_E_ARG, "bqd2_solve_surface: null problem");
+  for (int d = 0; d <= p->m.MN; d++)
+    BQD2_CHECK(p->dom[d].Ne == 0 || p->dom[d].assembled, BQD2_E_STATE, "bqd2_solve_surface: domain %d not assembled", d);
+  const int na = p->na;
+  hipStream_t st = p->stream;
+  int rc;
+  if (!p->A) {
+    if ((rc = bqd2_alloc(p, (void **)&p->A, (size_t)na * na * 16))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->b, (size_t)na * 16))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->xg, (size_t)na * 16))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->ipiv, (size_t)na * sizeof(int)))) return rc;
+  }
+  if ((rc = ensure_hk2(p))) return rc;
+  p->info_lu = 0;
+  for (int i = 0; i < 17; i++) { p->info_its[i] = 0; p->info_res[i] = 0.0; }
+  const double *x = nullptr;
+  if (p->solver == 0) {
+    // Jacobi-scaled GMRES on the row-major system
+    if ((rc = build_system(p, 1, st))) return rc;
+    rc = bqd2_gmres(&p->main, na, 1, p->A, na, nullptr, p->b, na, p->xg, na, GM_TOL, GM_RESTART, GM_MAXIT,
+                    &p->info_its[0], &p->info_res[0]);
+    if (rc < 0) return rc;
+    if (rc == 0) x = p->xg;
+  }
+  if (!x) {
+    // dense LU with partial pivoting on the column-major system
+    p->info_lu++;
+    if ((rc = build_system(p, 0, st))) return rc;
+    if ((rc = bqd2_zgetrf(na, p->A, p->ipiv, st))) return rc;
+    if ((rc = bqd2_zgetrs(na, 1, p->A, p->ipiv, p->b, st))) return rc;
+    x = p->b;
+  }
   for (int d = 0; d <= p->m.MN; d++) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
-    hipLaunchKernelGGL(k_store_solution, dim3(bqd2_blocks(D.Ne, 256)), dim3(256), 0, st, p->m, D, p->b, hk2);
+    hipLaunchKernelGGL(k_store_solution, dim3(bqd2_blocks(D.Ne, 256)), dim3(256), 0, st, p->m, D, x, p->hk2d);
   }
   BQD2_HIP(hipGetLastError());
-  if (x_host) BQD2_HIP(hipMemcpyAsync(x_host, p->b, (size_t)na * 16, hipMemcpyDeviceToHost, st));
+  if (x_host) BQD2_HIP(hipMemcpyAsync(x_host, x, (size_t)na * 16, hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
   p->solved = true;
+  return BQD2_OK;
+}
+
+// G dpv = H pv of one domain by the dense LU on the compressed rows
+static int dpv_lu(bqd2_problem *p, DevDomain &D, hipStream_t st) {
+  double *A = nullptr, *B = nullptr;
+  int *ipiv = nullptr;
+  int rc = bqd2_alloc(p, (void **)&A, (size_t)D.nn * D.nn * 16);
+  if (!rc) rc = bqd2_alloc(p, (void **)&B, (size_t)D.nn * 3 * 16);
+  if (!rc) rc = bqd2_alloc(p, (void **)&ipiv, (size_t)D.nn * sizeof(int));
+  if (rc) return rc;
+  BQD2_HIP(hipMemsetAsync(A, 0, (size_t)D.nn * D.nn * 16, st));
+  long long n2 = (long long)D.N * D.N;
+  hipLaunchKernelGGL(k_dpv_A, dim3(bqd2_blocks(n2, 256)), dim3(256), 0, st, p->m, D, A);
+  hipLaunchKernelGGL(k_dpv_B, dim3(D.N), dim3(64), 0, st, p->m, D, B);
+  BQD2_HIP(hipGetLastError());
+  if ((rc = bqd2_zgetrf(D.nn, A, ipiv, st))) return rc;
+  if ((rc = bqd2_zgetrs(D.nn, 3, A, ipiv, B, st))) return rc;
+  hipLaunchKernelGGL(k_dpv_store, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st, D, B);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipStreamSynchronize(st));
+  // the work buffers of the fallback are released right away
+  for (void *q : {(void *)A, (void *)B, (void *)ipiv}) {
+    (void)hipFree(q);
+    for (auto it = p->owned.begin(); it != p->owned.end(); ++it)
+      if (*it == q) { p->owned.erase(it); break; }
+  }
   return BQD2_OK;
 }
 
@@ -331,48 +439,47 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
   memcpy(W.tv, p->pw_tv, sizeof(W.tv));
   memcpy(W.R, p->pw_R, sizeof(W.R));
   const bool periodic = (p->m.ps == 1);
-  double *tpv = nullptr;
+  int rc;
   if (periodic) {
-    int rc = bqd2_alloc(p, (void **)&tpv, (size_t)p->m.NN * 3 * 16);
-    if (rc) return rc;
-    BQD2_HIP(hipMemsetAsync(tpv, 0, (size_t)p->m.NN * 3 * 16, st));
+    if (!p->tpv && (rc = bqd2_alloc(p, (void **)&p->tpv, (size_t)p->m.NN * 3 * 16))) return rc;
+    BQD2_HIP(hipMemsetAsync(p->tpv, 0, (size_t)p->m.NN * 3 * 16, st));
   }
   for (int d = periodic ? 1 : 0; d <= p->m.MN; d++) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
     hipLaunchKernelGGL(k_velocity_rows, dim3(D.N), dim3(64), 0, st, p->m, D, p->tab, W, p->rho0[d], p->rho0[0],
-                       periodic ? tpv : (double *)nullptr);
+                       periodic ? p->tpv : (double *)nullptr);
   }
   if (periodic && p->dom[0].Ne > 0)
-    hipLaunchKernelGGL(k_pv_open_domain, dim3(bqd2_blocks(p->dom[0].Ne, 256)), dim3(256), 0, st, p->m, p->dom[0], tpv);
+    hipLaunchKernelGGL(k_pv_open_domain, dim3(bqd2_blocks(p->dom[0].Ne, 256)), dim3(256), 0, st, p->m, p->dom[0],
+                       p->tpv);
   BQD2_HIP(hipGetLastError());
   // normal derivative of the velocity: G dpv = H pv per domain (3 right-hand sides)
+  if (!p->dpvw) {
+    int nmax = 0;
+    for (int d = 0; d <= p->m.MN; d++) nmax = p->dom[d].N > nmax ? p->dom[d].N : nmax;
+    if ((rc = bqd2_alloc(p, (void **)&p->dpvw, (size_t)2 * 3 * nmax * 16 + 16))) return rc;
+  }
   for (int d = 0; d <= p->m.MN; d++) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
-    double *A = nullptr, *B = nullptr;
-    int *ipiv = nullptr;
-    int rc = bqd2_alloc(p, (void **)&A, (size_t)D.nn * D.nn * 16);
-    if (!rc) rc = bqd2_alloc(p, (void **)&B, (size_t)D.nn * 3 * 16);
-    if (!rc) rc = bqd2_alloc(p, (void **)&ipiv, (size_t)D.nn * sizeof(int));
-    if (rc) return rc;
-    BQD2_HIP(hipMemsetAsync(A, 0, (size_t)D.nn * D.nn * 16, st));
-    long long n2 = (long long)D.N * D.N;
-    hipLaunchKernelGGL(k_dpv_A, dim3(bqd2_blocks(n2, 256)), dim3(256), 0, st, p->m, D, A);
-    hipLaunchKernelGGL(k_dpv_B, dim3(D.N), dim3(64), 0, st, p->m, D, B);
-    BQD2_HIP(hipGetLastError());
-    rc = bqd2_zgetrf(D.nn, A, ipiv, st);
-    if (rc) return rc;
-    rc = bqd2_zgetrs(D.nn, 3, A, ipiv, B, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_dpv_store, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st, D, B);
-    BQD2_HIP(hipGetLastError());
-    BQD2_HIP(hipStreamSynchronize(st));
-    // the per-domain work buffers are released right away
-    for (void *q : {(void *)A, (void *)B, (void *)ipiv}) {
-      (void)hipFree(q);
-      for (auto it = p->owned.begin(); it != p->owned.end(); ++it)
-        if (*it == q) { p->owned.erase(it); break; }
+    bool done = false;
+    if (p->solver == 0) {
+      double *B = p->dpvw, *X = p->dpvw + 2 * (size_t)3 * D.N;
+      hipLaunchKernelGGL(k_dpv_B_slots, dim3(D.N), dim3(64), 0, st, D, B);
+      BQD2_HIP(hipGetLastError());
+      rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, GM_RESTART, GM_MAXIT,
+                      &p->info_its[1 + d], &p->info_res[1 + d]);
+      if (rc < 0) return rc;
+      if (rc == 0) {
+        hipLaunchKernelGGL(k_dpv_store_slots, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st, D, X);
+        BQD2_HIP(hipGetLastError());
+        done = true;
+      }
+    }
+    if (!done) {
+      p->info_lu++;
+      if ((rc = dpv_lu(p, D, st))) return rc;
     }
   }
   BQD2_HIP(hipStreamSynchronize(st));

@@ -121,6 +121,12 @@ class Request:
             self.F[typ], self.N[typ] = F[:], N[:]
         self.force_counts = self._counters()
         t5 = time.perf_counter()
+        its = (C.c_int * 17)()
+        res = (C.c_double * 17)()
+        lu = C.c_int()
+        check(hip.bqd2_solver_info(p, its, res, C.byref(lu)), "bqd2_solver_info")
+        self.solver = {"krylov_steps": list(its[:self.mn + 2]), "scaled_residual": list(res[:self.mn + 2]),
+                       "lu_fallbacks": lu.value}
         t.update(assemble=t1 - t0, solve=t2 - t1, velocity=t3 - t2, field=t4 - t3, force=t5 - t4, total=t5 - t0)
         return t
 
@@ -339,6 +345,7 @@ def main():
         "gf_evals_per_s": evals / elapsed,
         "force_rel_err": force_err,
         "phases_ms": ph,
+        "solver": req.solver,
         "force": {"type0": req.F[0].tolist(), "type1": req.F[1].tolist()},
         "torque": {"type0": req.N[0].tolist(), "type1": req.N[1].tolist()},
         "roofline": roof,

@@ -131,14 +131,27 @@ int bqd2_counters(bqd2_problem *p, long long *out, int n);
 /* copy domain did's matrices to host buffers (any may be NULL) */
 int bqd2_download_domain(bqd2_problem *p, int did, double *G, double *H, double *dG, double *dH, double *dF);
 
-/* Build the 2NN system (create_matrix_csr_dac, :244-345, dense column-major),
- * factor it (LU with partial pivoting) and solve; scatter the solution to the
- * per-domain P, dP (tmatrix_bd_store, :370-427).  x (host, 2NN complex) may be NULL. */
+/* Build the dense 2NN system (create_matrix_csr_dac, :244-345) and solve it; the
+ * reference uses MKL PARDISO (d3qd2_mkl_solver.c:18-148).  Default: GMRES with
+ * Jacobi row scaling to a scaled residual of 1e-15, and the dense LU with partial
+ * pivoting when that is not reached; bqd2_set_solver(p, 1) selects the LU alone.
+ * Scatters the solution to the per-domain P, dP (tmatrix_bd_store, :370-427).
+ * x (host, 2NN complex) may be NULL. */
 int bqd2_solve_surface(bqd2_problem *p, double *x);
 
 /* Particle velocity and its normal derivative on the boundary: q_solve_pv_bv /
- * solve_pv_bv (:429-491, :655-750) and solve_dpv_bv (:533-653). */
+ * solve_pv_bv (:429-491, :655-750) and solve_dpv_bv (:533-653; LAPACKE_zgesv
+ * there, GMRES with the LU fallback here, as for the system). */
 int bqd2_solve_velocity(bqd2_problem *p);
+
+/* No counterpart in the reference.  Solver of bqd2_solve_surface and
+ * bqd2_solve_velocity: 0 GMRES with the LU fallback (default), 1 LU only. */
+int bqd2_set_solver(bqd2_problem *p, int mode);
+
+/* No counterpart in the reference.  Krylov steps and final scaled residuals of
+ * the last solves: entry 0 the system, entry 1 + d the dpv system of domain d
+ * (MN + 2 entries, 0 where the LU ran); lu: systems that fell back to the LU. */
+int bqd2_solver_info(bqd2_problem *p, int *iters, double *relres, int *lu);
 
 /* per-domain solution arrays, (Ne_d+1) x 4 complex for P, dP and
  * (Ne_d+1) x 4 x 3 complex for pv, dpv, entry 0 unused, as in SUBD */
