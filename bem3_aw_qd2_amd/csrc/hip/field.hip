@@ -9,9 +9,10 @@
 //                 q0_bil_dcoef_grad_* / bil_dcoef_grad_*), falling back to the
 //                 boundary-value form (pv_s_bieq_aqd2, :140-201) when
 //                 |grad F| > CBD_CDF, Bloch phase, incident field
-//   k_field_bd    one wavefront per boundary point (t, zeta, eta):
-//                 pv_s_bd_aqd2 / pv_t_bd_aqd2 (:394-469) with the singular self
-//                 element integrated in polar form (q0_bil_coef_bd / bil_coef_bd)
+//   k_field_bd    chunks of a boundary point's (element, point) items, one
+//                 wavefront each, plus one wavefront for the point's own element
+//                 in polar form (q0_bil_coef_bd / bil_coef_bd); k_field_bd_fin
+//                 sums them as pv_s_bd_aqd2 / pv_t_bd_aqd2 (:394-469) do
 //   k_force       radiation stress tensor on the boundary of the open domain
 //                 (force_FN_aqd2, bem3_aw_qd2_force.c:4-165), per element,
 //                 then an ordered sum
@@ -379,12 +380,22 @@ __global__ void __launch_bounds__(256) k_field_fin(FieldCtx X, int total, int n,
 }
 
 // ------------------------------------------------------------ boundary points
-// one wavefront per boundary point (domain did, element position t, zeta, eta)
+// Boundary point (domain did, element position t, zeta, eta): the regular
+// (element, quadrature point) items are cut into chunks of BD_CHUNK, one
+// wavefront per (point, chunk), and one more wavefront per point takes the
+// point's own element on its polar rule; k_field_bd_fin sums the chunks in
+// order.  The chunk count depends only on the mesh and the rule.
+enum { BD_CHUNK = 1024, BP_N = 9 };  // per chunk: Pa (2), PVa (6), F
+
+__host__ __device__ __forceinline__ int bd_item_chunks(long long items) {
+  return (int)((items + BD_CHUNK - 1) / BD_CHUNK);
+}
+
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_field_bd(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
-                                                 int type, int total, int did, int n, const int *__restrict__ tpos,
-                                                 const double *__restrict__ ze, double *__restrict__ out,
+                                                 int type, int did, int n, const int *__restrict__ tpos,
+                                                 const double *__restrict__ ze, double *__restrict__ part,
                                                  long long *__restrict__ counters) {
-  const int ip = blockIdx.x, lane = threadIdx.x;
+  const int ip = blockIdx.x, ic = blockIdx.y, lane = threadIdx.x;
   if (ip >= n) return;
   const Tab &T = *tab;
   const DomView &V = X.dom[did];
@@ -405,7 +416,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   int err = 0;
   // regular elements
   const long long items = (long long)V.Ne * npt;
-  for (long long it = lane; it < items; it += 64) {
+  const int nci = bd_item_chunks(items);
+  const long long i0 = (long long)ic * BD_CHUNK, i1 = (ic < nci) ? min(items, i0 + BD_CHUNK) : 0;
+  for (long long it = i0 + lane; it < i1; it += 64) {
     const int s = (int)(it / npt) + 1, q = (int)(it % npt);
     if (s == t) continue;
     const int sd = V.sid[s], as = abs(sd);
@@ -446,7 +459,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     F += sg * wt * tD;
   }
   F *= BQD2_IFP;
-  if (M.tri[at]) {
+  if (ic != nci) {
+    // item chunk: no self element
+  } else if (M.tri[at]) {
     // self triangle: polar rule over the three sub-triangles about (zt, et)
     // (lit_coef_bd, lit_coef.c:290-350; q0_lit_coef_bd, q0_lit_coef.c:187-262).
     // The element is flat, so the free-space H and F vanish; the periodic H
@@ -671,10 +686,41 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   if (lane != 0) return;
   tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
   if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
+  double *dst = part + ((size_t)ip * (nci + 1) + ic) * BP_N;
+  dst[0] = Pa.x;
+  dst[1] = Pa.y;
+  for (int c2 = 0; c2 < 3; c2++) {
+    dst[2 + 2 * c2] = PVa[c2].x;
+    dst[3 + 2 * c2] = PVa[c2].y;
+  }
+  dst[8] = F;
+}
+
+// one thread per boundary point: the chunk sums in order, the normalisation
+// 1 / (1 + F) or 1 / F and, for the total field, the incident wave
+// (pv_s_bd_aqd2 / pv_t_bd_aqd2, bem3_aw_qd2_field.c:394-469)
+__global__ void __launch_bounds__(256) k_field_bd_fin(DevMesh M, FieldCtx X, int total, int did, int n, int nch,
+                                                      const int *__restrict__ tpos, const double *__restrict__ ze,
+                                                      const double *__restrict__ part, double *__restrict__ out) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n) return;
+  const DomView &V = X.dom[did];
+  cd Pa = {}, PVa[3] = {};
+  double F = 0.0;
+  for (int ic = 0; ic < nch; ic++) {
+    const double *q = part + ((size_t)ip * nch + ic) * BP_N;
+    Pa += cd{q[0], q[1]};
+    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += cd{q[2 + 2 * c2], q[3 + 2 * c2]};
+    F += q[8];
+  }
   const double iC = did == 0 ? 1.0 / (1.0 + F) : 1.0 / F;
   cd p = cd{-V.k2x, -V.k2y} * Pa * iC, pv[3];
   for (int c2 = 0; c2 < 3; c2++) pv[c2] = PVa[c2] * iC;
   if (total && did == 0) {
+    ElemC Et;
+    load_elem(M, abs(V.sid[tpos[ip]]), Et);
+    double rt[3], wt_[3];
+    bil_rw(Et, ze[2 * ip], ze[2 * ip + 1], rt, wt_);
     cd pi, vi[3];
     incident_pv(X, rt, pi, vi);
     p += pi;
@@ -686,6 +732,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     out[8 * ip + 2 + 2 * c2] = pv[c2].x;
     out[8 * ip + 3 + 2 * c2] = pv[c2].y;
   }
+}
+
+// boundary field at n points already on the device: partial sums in part
+// (n x (chunks + 1) x BP_N doubles), results in out (n x 8)
+static int launch_field_bd(bqd2_problem *p, hipStream_t st, FieldCtx &X, int type, int total, int did, int n,
+                           const int *d_t, const double *d_ze, double *d_part, double *d_out, long long *ctr) {
+  const int nch = bd_item_chunks((long long)p->dom[did].Ne * rule_points(type, p->any_tri)) + 1;
+  hipLaunchKernelGGL(k_field_bd, dim3(n, nch), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t, d_ze,
+                     d_part, ctr);
+  hipLaunchKernelGGL(k_field_bd_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, total, did, n, nch, d_t,
+                     d_ze, d_part, d_out);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
+static size_t bd_part_bytes(const bqd2_problem *p, int type, int did, int n) {
+  const int nch = bd_item_chunks((long long)p->dom[did].Ne * rule_points(type, p->any_tri)) + 1;
+  return (size_t)n * nch * BP_N * sizeof(double);
 }
 
 // ------------------------------------------------------------ force
@@ -877,19 +941,19 @@ extern "C" int bqd2_field_bd(bqd2_problem *p, int did, int n, const int *t, cons
   long long *const ctr = cx->cnt;
   Scratch S;
   int *d_t;
-  double *d_ze, *d_out;
+  double *d_ze, *d_out, *d_part;
   int rc;
-  if ((rc = S.init(cx, pad256((size_t)n * 4) + pad256((size_t)n * 16) + pad256((size_t)n * 64)))) return rc;
+  const size_t bpart = bd_part_bytes(p, type, did, n);
+  if ((rc = S.init(cx, pad256((size_t)n * 4) + pad256((size_t)n * 16) + pad256((size_t)n * 64) + pad256(bpart))))
+    return rc;
   if ((rc = S.get((void **)&d_t, (size_t)n * 4)) || (rc = S.get((void **)&d_ze, (size_t)n * 16)) ||
-      (rc = S.get((void **)&d_out, (size_t)n * 64)))
+      (rc = S.get((void **)&d_out, (size_t)n * 64)) || (rc = S.get((void **)&d_part, bpart)))
     return rc;
   BQD2_HIP(hipMemcpyAsync(d_t, t, (size_t)n * 4, hipMemcpyHostToDevice, st));
   BQD2_HIP(hipMemcpyAsync(d_ze, ze, (size_t)n * 16, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
   BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
-  hipLaunchKernelGGL(k_field_bd, dim3(n), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, total, did, n, d_t, d_ze,
-                     d_out, ctr);
-  BQD2_HIP(hipGetLastError());
+  if ((rc = launch_field_bd(p, st, X, type, total, did, n, d_t, d_ze, d_part, d_out, ctr))) return rc;
   BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
   long long cnt[CNT_N];
   BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
@@ -922,7 +986,8 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
   int *d_off = nullptr;
   int rc_;
   if ((rc_ = S.init(cx, pad256((size_t)ne * 48) + pad256(48) +
-                           (nbd ? pad256(nbd * 64) + pad256(nbd * 4) + pad256(nbd * 16) + pad256(off.size() * 4)
+                           (nbd ? pad256(nbd * 64) + pad256(nbd * 4) + pad256(nbd * 16) + pad256(off.size() * 4) +
+                                      pad256(bd_part_bytes(p, type, 0, (int)nbd))
                                 : 0))))
     return rc_;
   if ((rc_ = S.get((void **)&d_part, (size_t)ne * 48)) || (rc_ = S.get((void **)&d_FN, 48))) return rc_;
@@ -949,8 +1014,9 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
     BQD2_HIP(hipMemcpyAsync(d_ze, ze.data(), nbd * 16, hipMemcpyHostToDevice, st));
     BQD2_HIP(hipMemcpyAsync(d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
     BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
-    hipLaunchKernelGGL(k_field_bd, dim3((unsigned)nbd), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, 1, 0, (int)nbd,
-                       d_t, d_ze, d_bd, ctr);
+    double *d_bdpart;
+    if ((rc_ = S.get((void **)&d_bdpart, bd_part_bytes(p, type, 0, (int)nbd)))) return rc_;
+    if ((rc_ = launch_field_bd(p, st, X, type, 1, 0, (int)nbd, d_t, d_ze, d_bdpart, d_bd, ctr))) return rc_;
     BQD2_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_force_elem, dim3(bqd2_blocks(ne, 256)), dim3(256), 0, st, p->m, p->tab, X, type, d_bd, d_off,
