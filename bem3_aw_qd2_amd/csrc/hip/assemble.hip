@@ -17,6 +17,7 @@
 //   k_fs_self     rules about the node (q0_bil_coef.c:246-338, bil_coef.c:244-342)
 //                 and the principal-value derivatives (bil_dcoef.c:369-740);
 //   k_rows        per-row sum of F in source order and the 1+F / F diagonal.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #include "elem_dev.h"
@@ -70,6 +71,7 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
   int row = (int)(gid / D.Ne), s = (int)(gid % D.Ne) + 1;
   int t, tn, td;
   unsigned char rule = 0;
+  float key = 0.0f;
   if (row_target(M, D, row, t, tn, td) && s != t) {
     const Tab &T = *tab;
     int at = abs(td), as = abs(D.sid[s]);
@@ -81,8 +83,10 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     const bool tri = M.tri[as] != 0;
     rule = (tri ? needs_gh_tri(E, T, rt, vtz, vte) : needs_gh(E, T, rt, vtz, vte)) ? 1 : 0;
     if (rule && periodic && split_ok(E, T, rt, P, tri)) rule = 2;
+    key = (float)fabs(E.cr[2][0] - rt[2]);
   }
   D.rule[gid] = rule;
+  D.pkey[gid] = key;
 }
 
 // ------------------------------------------------------------------ periodic pairs
@@ -905,11 +909,14 @@ __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
 }
 
 // ------------------------------------------------------------------ pair list
-// (row, s) pairs of the periodic domain, row-major within each class, classes
-// (rule, element kind) one after the other, each padded with -1 to a multiple
-// of PPW so that a wavefront never mixes classes
-static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule, const std::vector<char> &tripos,
-                      std::vector<int> &out) {
+// (row, s) pairs of the periodic domain by class (rule, element kind), each
+// class padded with -1 to a multiple of PPW so that a wavefront never mixes
+// classes.  Within a class the pairs are ordered by the z offset of the element
+// centre from the target (ties in row-major order): the pairs of one wavefront
+// then take the same branch of the lattice sum (pure spectral beyond a height,
+// Ewald below) and about as many terms.
+static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule, const std::vector<float> &key,
+                      const std::vector<char> &tripos, std::vector<int> &out) {
   std::vector<int> cls[6];
   for (int row = 0; row < D.N; row++) {
     const int t = row / 4 + 1, tn = row % 4;
@@ -922,6 +929,7 @@ static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule
   }
   out.clear();
   for (auto &c : cls) {
+    std::stable_sort(c.begin(), c.end(), [&](int a, int b) { return key[a] < key[b]; });
     out.insert(out.end(), c.begin(), c.end());
     while (out.size() % PPW) out.push_back(-1);
   }
@@ -950,22 +958,42 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipMemsetAsync(D.Fpair, 0, (size_t)D.N * D.Ne * 3 * 8, st_));
   BQD2_HIP(hipEventRecord(e0, st_));
   long long npairs = (long long)D.N * D.Ne;
-  hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, st_, p->m, p->tab, D, p->gf,
-                     periodic ? 1 : 0);
-  BQD2_HIP(hipGetLastError());
+  // the rules and the pair list depend on the geometry and the lattice only:
+  // decided at the first assembly of the domain, reused afterwards
+  const bool first = (D.plist_n == 0);
+  if (first) {
+    hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, st_, p->m, p->tab, D, p->gf,
+                       periodic ? 1 : 0);
+    BQD2_HIP(hipGetLastError());
+  }
   BQD2_HIP(hipEventRecord(e1, st_));
-  std::vector<unsigned char> rule((size_t)npairs);
-  BQD2_HIP(hipMemcpyAsync(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost, st_));
-  BQD2_HIP(hipStreamSynchronize(st_));
-  std::vector<int> plist;
-  if (periodic) pair_list(D, rule, p->tripos[did], plist);
+  if (first) {
+    std::vector<unsigned char> rule((size_t)npairs);
+    BQD2_HIP(hipMemcpyAsync(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost, st_));
+    std::vector<float> key;
+    if (periodic) {
+      key.resize((size_t)npairs);
+      BQD2_HIP(hipMemcpyAsync(key.data(), D.pkey, (size_t)npairs * sizeof(float), hipMemcpyDeviceToHost, st_));
+    }
+    BQD2_HIP(hipStreamSynchronize(st_));
+    D.n_gh = D.n_split = 0;
+    for (unsigned char r : rule) {
+      D.n_gh += (r != 0);
+      D.n_split += (r == 2);
+    }
+    std::vector<int> plist;
+    if (periodic) {
+      pair_list(D, rule, key, p->tripos[did], plist);
+      if (!plist.empty())
+        BQD2_HIP(hipMemcpy(D.plist, plist.data(), plist.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    D.plist_n = periodic ? (int)plist.size() : 1;
+  }
   BQD2_HIP(hipEventRecord(e1b, st_));
   if (periodic) {
-    if (!plist.empty()) {
-      BQD2_HIP(hipMemcpyAsync(D.plist, plist.data(), plist.size() * sizeof(int), hipMemcpyHostToDevice, st_));
-      hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(plist.size() / PPW)), dim3(WAVE), 0, st_, p->m, p->tab, D,
-                         p->gf, D.plist, (long long)plist.size(), p->d_counters);
-    }
+    if (D.plist_n > 0)
+      hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(D.plist_n / PPW)), dim3(WAVE), 0, st_, p->m, p->tab, D,
+                         p->gf, D.plist, (long long)D.plist_n, p->d_counters);
   } else {
     hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
   }
@@ -986,12 +1014,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipEventSynchronize(e3));
   long long cnt[CNT_N];
   BQD2_HIP(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  long long ngh = 0;
-  long long nsplit = 0;
-  for (unsigned char r : rule) {
-    ngh += (r != 0);
-    nsplit += (r == 2);
-  }
+  const long long ngh = D.n_gh, nsplit = D.n_split;
   if (st) {
     memset(st, 0, sizeof(*st));
     float a, b, c;

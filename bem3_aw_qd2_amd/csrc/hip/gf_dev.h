@@ -135,6 +135,8 @@ __device__ __forceinline__ bool term_small_fd(const Sum10& t, const Sum10& s, do
 template <int M>
 __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, double ry, double rz, double E,
                                              int l1, int l2, cd ce, Sum10& t) {
+  // no z == 0 residue is read in the image sum: products may be fused
+#pragma clang fp contract(fast)
   double Rl0 = l1 * P.d1x + l2 * P.d2x;
   double Rl1 = l1 * P.d1y + l2 * P.d2y;
   double ra0 = rx + Rl0, ra1 = ry + Rl1;
@@ -211,6 +213,7 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     double bn = sqrt(-bn2);
     double f1, f2, efs = 0.0;
     if (fast) {
+#pragma clang fp contract(fast)
       efs = exp(bn2 * E * E - azve * azve);
       f1 = efs * erfcx(bn * E + azve);
       f2 = efs * erfcx(bn * E - azve);
@@ -245,6 +248,8 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
 template <int M>
 __device__ __forceinline__ int fd_term(const GfParams& P, double rx, double ry, double rz, int n1, int n2, cd ce,
                                        Sum10& t) {
+  // the pure spectral sum is never taken at z == 0: products may be fused
+#pragma clang fp contract(fast)
   double an0 = -P.vkx + n1 * P.g1x + n2 * P.g2x;
   double an1 = -P.vky + n1 * P.g1y + n2 * P.g2y;
   double bn2 = P.k * P.k - an0 * an0 - an1 * an1;

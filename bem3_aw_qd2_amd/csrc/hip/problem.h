@@ -40,6 +40,9 @@ struct DevDomain {
   double *Fpair;       // N x Ne x 3 : per pair F, dF/dtz, dF/dte
   unsigned char *rule; // N x Ne : 1 = GH64 rule
   int *plist;          // N x Ne + 32 : (row, s) pairs sorted by rule and element kind
+  float *pkey;         // N x Ne : |z| offset of the element centre from the target (pair order)
+  int plist_n;         // entries of plist, 0 until the first assembly (rules and list are cached)
+  long long n_gh, n_split;  // GH64 pairs, split ones among them
   double *P, *dP;      // (Ne+1) x 4 complex
   double *pv, *dpv;    // (Ne+1) x 4 x 3 complex
   bool assembled;

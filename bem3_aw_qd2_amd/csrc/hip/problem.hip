@@ -119,6 +119,7 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     TRY(bqd2_alloc(p, (void **)&D.Fpair, N * D.Ne * 3 * 8));
     TRY(bqd2_alloc(p, (void **)&D.rule, N * D.Ne));
     TRY(bqd2_alloc(p, (void **)&D.plist, (N * D.Ne + 32) * sizeof(int)));
+    TRY(bqd2_alloc(p, (void **)&D.pkey, N * D.Ne * sizeof(float)));
     TRY(bqd2_alloc(p, (void **)&D.P, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.dP, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.pv, ((size_t)D.Ne + 1) * 12 * 16));
