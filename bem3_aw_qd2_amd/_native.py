@@ -40,7 +40,7 @@ class GfParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("k", "vkx", "vky", "d1x", "d1y", "d2x", "d2y", "idd", "aidd",
                  "g1x", "g1y", "g2x", "g2y", "d2n", "g2n", "g12", "E0", "bN", "bNp", "bNm")] + \
-               [("nc", C.c_int), ("ewald_only", C.c_int)]
+               [("nc", C.c_int), ("ewald_only", C.c_int), ("fd_lift", C.c_double)]
 
 
 class AssemblyStats(C.Structure):

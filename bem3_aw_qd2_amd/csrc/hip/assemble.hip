@@ -912,7 +912,7 @@ __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
 // (row, s) pairs of the periodic domain by class (rule, element kind), each
 // class padded with -1 to a multiple of PPW so that a wavefront never mixes
 // classes.  Within a class the pairs are ordered by the z offset of the element
-// centre from the target (ties in row-major order): the pairs of one wavefront
+// centre from the target (256 levels, ties in row-major order): the pairs of one wavefront
 // then take the same branch of the lattice sum (pure spectral beyond a height,
 // Ewald below) and about as many terms.
 static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule, const std::vector<float> &key,
@@ -929,7 +929,18 @@ static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule
   }
   out.clear();
   for (auto &c : cls) {
-    std::stable_sort(c.begin(), c.end(), [&](int a, int b) { return key[a] < key[b]; });
+    // stable counting sort on the key quantised to 256 levels
+    float kmax = 0.0f;
+    for (int g : c) kmax = key[g] > kmax ? key[g] : kmax;
+    if (kmax > 0.0f && c.size() > 1) {
+      auto bucket = [&](int g) { int b = (int)(key[g] / kmax * 256.0f); return b > 255 ? 255 : b; };
+      std::vector<size_t> start(257, 0);
+      for (int g : c) start[bucket(g) + 1]++;
+      for (int b = 0; b < 256; b++) start[b + 1] += start[b];
+      std::vector<int> sorted(c.size());
+      for (int g : c) sorted[start[bucket(g)]++] = g;
+      c.swap(sorted);
+    }
     out.insert(out.end(), c.begin(), c.end());
     while (out.size() % PPW) out.push_back(-1);
   }

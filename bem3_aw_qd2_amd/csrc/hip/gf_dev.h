@@ -8,7 +8,10 @@
 // stopping rules (:471-482), and the Hessian variants (:2011-2398).  The
 // reference evaluates one point per call; here one lane evaluates one point and
 // the instance constants (lattice, reciprocal lattice, Ewald parameter, method
-// probe) are computed once on the host into GfParams.
+// probe) are computed once on the host into GfParams.  The reference's probe is
+// read at |z| / fd_lift: between its switch height and fd_lift times that, where
+// the pure spectral sum needs up to ~900 orders, the Ewald sum is taken instead
+// (the same value to ~1e-15; fd_lift = 1 restores the reference's choice).
 //
 // Faddeeva w(z): Weideman's rational approximation, N = 40 (SIAM J. Numer.
 // Anal. 31 (1994) 1497-1518), branch-free on Im z >= 0, reflection below.
@@ -520,7 +523,8 @@ template <int M>
 __device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   bool use_fd = false;
   if (!P.ewald_only) {
-    use_fd = (P.bN / P.bNp * exp(-P.bNp * fabs(rz)) < eps) && (P.bN / P.bNm * exp(-P.bNm * fabs(rz)) < eps);
+    const double zs = fabs(rz) / P.fd_lift;
+    use_fd = (P.bN / P.bNp * exp(-P.bNp * zs) < eps) && (P.bN / P.bNm * exp(-P.bNm * zs) < eps);
   }
   int l1 = (int)floor((rx * P.d2y - ry * P.d2x) * P.idd + 0.5);
   int l2 = (int)floor(-(rx * P.d1y - ry * P.d1x) * P.idd + 0.5);

@@ -75,10 +75,14 @@ def quiet_stdout():
         os.close(devnull)
 
 
-def grid_points(n, half):
+def grid_points(n, half, planes=1):
+    """n x n points over [-half, half]^2 in the y = 0 plane, then (planes = 3)
+    the z = 0 and x = 0 planes"""
     x = np.linspace(-half, half, n)
-    X, Z = np.meshgrid(x, x, indexing="ij")
-    return np.ascontiguousarray(np.stack([X.ravel(), np.zeros(n * n), Z.ravel()], axis=1))
+    A, B = np.meshgrid(x, x, indexing="ij")
+    a, b, o = A.ravel(), B.ravel(), np.zeros(n * n)
+    sets = [np.stack([a, o, b], axis=1), np.stack([a, b, o], axis=1), np.stack([o, a, b], axis=1)][:planes]
+    return np.ascontiguousarray(np.concatenate(sets, axis=0))
 
 
 class Request:
@@ -235,7 +239,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sphere384")
-    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--grid", type=int, default=0, help="field grid per plane (default 64, 256 for sphere3456)")
+    ap.add_argument("--planes", type=int, default=0, help="field planes y=0, z=0, x=0 (default 1, 3 for sphere3456)")
+    ap.add_argument("--triangles", action="store_true", help="the linear-triangle mesh of the workload")
     ap.add_argument("--cpu-pairs", type=int, default=384)
     ap.add_argument("--cpu-self", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -256,13 +262,16 @@ def main():
     hip = hip_lib()
     host = host_lib()
     cfg = mesh.CONFIGS[args.workload]
+    big = args.workload == "sphere3456"
+    args.grid = args.grid or (256 if big else 64)
+    args.planes = args.planes or (3 if big else 1)
     tmp = tempfile.mkdtemp(prefix="bqd2_bench_")
-    paths = mesh.write_config(args.workload, tmp)
+    paths = mesh.write_config(args.workload, tmp, triangles=args.triangles)
     with quiet_stdout():
         model = exdat.Model(host).read_inputs(*paths, out=os.path.join(tmp, "ex.dat")).initialize()
     prob = host.aqd2_problem(model.buf, None)
     lam = cfg["host"][1] / cfg["f"]
-    req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam))
+    req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam, args.planes))
     log("bench: workload %s, %d elements, %d unknowns" % (args.workload, model.ne(), 2 * model.nn()))
 
     if args.copy_mib:
@@ -333,12 +342,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (generated cube-sphere mesh, plane-wave incidence)",
-        "config": {"workload": "%s: %d-quad cube-sphere a=%.2g m, lattice d1=%s d2=%s m, f=%.3g Hz, "
-                               "theta=%.3g phi=%.4g; surface solve + %dx%d field grid (y=0, |x|,|z|<=3 lambda) "
+        "config": {"workload": "%s: %d-%s cube-sphere a=%.2g m, lattice d1=%s d2=%s m, f=%.3g Hz, "
+                               "theta=%.3g phi=%.4g; surface solve + %d x %dx%d field grid (%s, |.|<=3 lambda) "
                                "+ force/torque type 0 and 1"
-                               % (args.workload, model.ne(), cfg["radius"], cfg["d1"], cfg["d2"], cfg["f"],
-                                  cfg["theta"], cfg["psi"], args.grid, args.grid),
-                   "elements": model.ne(), "unknowns": 2 * model.nn(), "field_points": args.grid ** 2,
+                               % (args.workload, model.ne(), "triangle" if args.triangles else "quad",
+                                  cfg["radius"], cfg["d1"], cfg["d2"], cfg["f"], cfg["theta"], cfg["psi"],
+                                  args.planes, args.grid, args.grid, ", ".join(["y=0", "z=0", "x=0"][:args.planes])),
+                   "elements": model.ne(), "unknowns": 2 * model.nn(), "field_points": len(req.pts),
                    "parallelism": "1 GPU"},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),

@@ -46,6 +46,9 @@ typedef struct {
   double bN, bNp, bNm;         /* spectral-decay probe at orders +-S_LIMIT     */
   int nc;                      /* centre row n1 of the reciprocal sums         */
   int ewald_only;              /* 1: probe orders propagate, Ewald everywhere  */
+  double fd_lift;              /* the pure spectral sum is taken at heights    */
+                               /* |z| / fd_lift that pass the reference's probe */
+                               /* (1: the reference's choice; default 2.2)     */
 } GfParams;
 
 /* qd8 = {k, vk[0], vk[1], vk[2], vd1[0], vd1[1], vd2[0], vd2[1]} (QPD2 layout).
