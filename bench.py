@@ -3,16 +3,23 @@
 
 A step is one complete solve request on the device-resident problem:
   * assembly of the coefficient matrices of both domains;
-  * the dense complex solve of the 2NN system;
+  * the dense complex solve of the 2NN system (GMRES, dense LU as fallback);
   * recovery of the particle velocity and its normal derivative on the boundary;
-  * pressure and velocity on a 64 x 64 grid in the y = 0 plane;
+  * pressure and velocity on field grids;
   * radiation force and torque with both quadrature settings (type 0 and 1).
 The mesh and media are uploaded before the timed region (inputs resident in HBM).
 
-Workload sphere384: 384-quad cube-sphere of radius 0.5 mm (1200 kg/m^3,
-2400 m/s) in water (1000 kg/m^3, 1500 m/s) on a square 2.0 mm lattice, plane
-wave of 1 Pa at 1 MHz, theta = 1e-6 rad, phi = pi/4 (exact normal incidence
-makes the in-plane wave vector zero, which the reference's model rejects).
+Workloads (bem3_aw_qd2_amd.mesh.CONFIGS):
+  sphere3456 (default)  cube-sphere of 6 x 24 x 24 = 3456 bilinear quads, radius
+                0.5 mm (1200 kg/m^3, 2400 m/s) in water (1000 kg/m^3, 1500 m/s)
+                on a triangular 1.8 mm lattice (60 degrees), plane wave of 1 Pa at
+                1 MHz, theta = 30 deg, phi = 15 deg; field on 256 x 256 points in
+                each of the y = 0, z = 0 and x = 0 planes.  --triangles takes the
+                6912-triangle mesh of the same sphere.
+  sphere384     6 x 8 x 8 = 384 quads, same sphere and media on a square 2.0 mm
+                lattice, theta = 1e-6 rad, phi = pi/4 (exact normal incidence
+                makes the in-plane wave vector zero, which the reference's model
+                rejects); field on 64 x 64 points in the y = 0 plane.
 
 The JSON line also carries:
   roofline      the off-diagonal pair kernel of the periodic domain (k_q0_pairs),
@@ -24,7 +31,8 @@ The JSON line also carries:
                 a seeded sample of (row, element) coefficient blocks of both
                 domains plus self blocks, and Green's function evaluations for the
                 field and force, extrapolated to one full request by the counts
-                of the GPU run; the dense solve by numpy's LAPACK at full size.
+                of the GPU run; the dense solve by numpy's LAPACK at n <= 4096,
+                scaled by n^3.
 """
 import argparse
 import contextlib
@@ -218,27 +226,28 @@ def cpu_baseline(model, req, npairs, nself, seed=0):
     parts["field"] = req.field_counts[15] / rate[3]
     parts["force"] = req.force_counts[15] / rate[2]
     total += parts["field"] + parts["force"]
-    # dense solve at full size (LAPACK through numpy)
+    # dense solve (LAPACK through numpy) at up to n = 4096, scaled by n^3 to the system size
     na = 2 * model.nn()
-    A = rng.standard_normal((na, na)) + 1j * rng.standard_normal((na, na))
-    b = rng.standard_normal(na) + 0j
+    ns = min(na, 4096)
+    A = rng.standard_normal((ns, ns)) + 1j * rng.standard_normal((ns, ns))
+    b = rng.standard_normal(ns) + 0j
     t0 = time.perf_counter()
     np.linalg.solve(A, b)
-    parts["solve"] = time.perf_counter() - t0
+    parts["solve"] = (time.perf_counter() - t0) * (na / ns) ** 3
     total += parts["solve"]
     sample = ("oracle coefficient blocks of %d random off-diagonal (row, element) pairs and %d self blocks per "
               "domain, Green's function rates for the field and force evaluation counts, "
-              "numpy LAPACK solve at n=%d; extrapolated to one request: %s s"
-              % (npairs, nself, na, ", ".join("%s %.1f" % (k, v) for k, v in parts.items())))
+              "numpy LAPACK solve at n=%d scaled by n^3 to n=%d; extrapolated to one request: %s s"
+              % (npairs, nself, ns, na, ", ".join("%s %.1f" % (k, v) for k, v in parts.items())))
     return {"value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port", "sample": sample}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sphere384")
+    ap.add_argument("--workload", default="sphere3456")
     ap.add_argument("--grid", type=int, default=0, help="field grid per plane (default 64, 256 for sphere3456)")
     ap.add_argument("--planes", type=int, default=0, help="field planes y=0, z=0, x=0 (default 1, 3 for sphere3456)")
     ap.add_argument("--triangles", action="store_true", help="the linear-triangle mesh of the workload")
