@@ -93,12 +93,35 @@ __device__ __forceinline__ void sum_add(Sum10& s, const Sum10& t) {
   }
 }
 
+// the components a row-convergence test reads (per sum and mode); the sum as it
+// stood before the row is kept as these only
+struct Key2 {
+  cd a, b;
+};
+template <int M>
+__device__ __forceinline__ Key2 key_spatial(const Sum10& s) {
+  if (M == GF_QG || M == GF_V1) return Key2{s.g, cd{0, 0}};
+  if (M == GF_DQG) return Key2{s.dg[0], s.dg[1]};
+  return Key2{s.d2g[0], s.d2g[1]};
+}
+template <int M>
+__device__ __forceinline__ Key2 key_spectral(const Sum10& s) {
+  if (M == GF_QG) return Key2{s.g, cd{0, 0}};
+  if (M == GF_DQG || M == GF_V1) return Key2{s.dg[2], cd{0, 0}};
+  return Key2{s.d2g[0], s.d2g[1]};
+}
+template <int M>
+__device__ __forceinline__ Key2 key_fd(const Sum10& s) {
+  if (M == GF_QG) return Key2{s.g, cd{0, 0}};
+  if (M == GF_DQG || M == GF_V1) return Key2{s.dg[2], cd{0, 0}};
+  return Key2{s.d2g[2], cd{0, 0}};
+}
 // stopping quantities of each mode (d3_qpgf_d2.c, per chain)
 template <int M>
-__device__ __forceinline__ bool rows_conv_spatial(const Sum10& s, const Sum10& o, double eps) {
-  if (M == GF_QG || M == GF_V1) return conv_same(s.g, o.g, eps);
-  if (M == GF_DQG) return conv_same(s.dg[0], o.dg[0], eps) && conv_same(s.dg[1], o.dg[1], eps);
-  return conv_same(s.d2g[0], o.d2g[0], eps) && conv_same(s.d2g[1], o.d2g[1], eps);
+__device__ __forceinline__ bool rows_conv_spatial(const Sum10& s, const Key2& o, double eps) {
+  const Key2 k = key_spatial<M>(s);
+  if (M == GF_QG || M == GF_V1) return conv_same(k.a, o.a, eps);
+  return conv_same(k.a, o.a, eps) && conv_same(k.b, o.b, eps);
 }
 template <int M>
 __device__ __forceinline__ bool term_small_spatial(const Sum10& t, const Sum10& s, double eps) {
@@ -107,10 +130,10 @@ __device__ __forceinline__ bool term_small_spatial(const Sum10& t, const Sum10& 
   return conv_small(t.d2g[0], s.d2g[0], eps) && conv_small(t.d2g[1], s.d2g[1], eps);
 }
 template <int M>
-__device__ __forceinline__ bool rows_conv_spectral(const Sum10& s, const Sum10& o, double eps) {
-  if (M == GF_QG) return conv_same(s.g, o.g, eps);
-  if (M == GF_DQG || M == GF_V1) return conv_same(s.dg[2], o.dg[2], eps);
-  return conv_same(s.d2g[0], o.d2g[0], eps) && conv_same(s.d2g[1], o.d2g[1], eps);
+__device__ __forceinline__ bool rows_conv_spectral(const Sum10& s, const Key2& o, double eps) {
+  const Key2 k = key_spectral<M>(s);
+  if (M == GF_V2) return conv_same(k.a, o.a, eps) && conv_same(k.b, o.b, eps);
+  return conv_same(k.a, o.a, eps);
 }
 template <int M>
 __device__ __forceinline__ bool term_small_spectral(const Sum10& t, const Sum10& s, double eps) {
@@ -119,10 +142,8 @@ __device__ __forceinline__ bool term_small_spectral(const Sum10& t, const Sum10&
   return conv_small(t.d2g[0], s.d2g[0], eps) && conv_small(t.d2g[1], s.d2g[1], eps);
 }
 template <int M>
-__device__ __forceinline__ bool rows_conv_fd(const Sum10& s, const Sum10& o, double eps) {
-  if (M == GF_QG) return conv_same(s.g, o.g, eps);
-  if (M == GF_DQG || M == GF_V1) return conv_same(s.dg[2], o.dg[2], eps);
-  return conv_same(s.d2g[2], o.d2g[2], eps);
+__device__ __forceinline__ bool rows_conv_fd(const Sum10& s, const Key2& o, double eps) {
+  return conv_same(key_fd<M>(s).a, o.a, eps);
 }
 template <int M>
 __device__ __forceinline__ bool term_small_fd(const Sum10& t, const Sum10& s, double eps) {
@@ -416,12 +437,15 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   const cd css = cis(P.vkx * P.d2x + P.vky * P.d2y);
   const cd csg = cis(P.g2x * rx + P.g2y * ry);
   const bool fast = fabs(rz) >= BQD2_ZFAST * E;
-  Sum10 S1, S2, old;
-  sum_zero(S1);
-  sum_zero(S2);
+  const double ifp = 0.079577471545947667884;  // 1/(4 pi)
+  const double iep = 0.039788735772973833942;  // 1/(8 pi)
+  Key2 old;
   int c1 = 0;
+  {
+  Sum10 S1;
+  sum_zero(S1);
   for (int l1 = 0, dir = 0;;) {
-    old = S1;
+    old = key_spatial<M>(S1);
     const int c = spatial_row<M>(P, rx, ry, rz, E, eps, l1, css, S1);
     if (c < 0) return -1;
     c1 += c;
@@ -436,10 +460,28 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
     }
     l1 += dir;
   }
+  // the image sum's share of the result (S1 is not needed past this point)
+  if (M != GF_DQG) out.g = S1.g * ifp;
+  if (M != GF_QG) {
+    out.dg[0] = S1.dg[0] * (-ifp);
+    out.dg[1] = S1.dg[1] * (-ifp);
+    out.dg[2] = S1.dg[2] * (-rz * ifp);
+  }
+  if (M == GF_V2) {
+    out.d2g[0] = S1.d2g[0] * (-iep);
+    out.d2g[1] = S1.d2g[1] * (-iep);
+    out.d2g[2] = S1.d2g[2] * (-iep);
+    out.d2g[3] = S1.d2g[3] * (-iep);
+    out.d2g[4] = S1.d2g[4] * (-rz * iep);
+    out.d2g[5] = S1.d2g[5] * (-rz * iep);
+  }
+  }
 
+  Sum10 S2;
+  sum_zero(S2);
   int flag = 0, nev = 0, c2 = 0;
   for (int n1 = P.nc, dir = 0;;) {
-    old = S2;
+    old = key_spectral<M>(S2);
     const int c = spectral_row<M>(P, rx, ry, rz, E, eps, n1, fast, csg, S2, flag, nev);
     if (c < 0) return c;
     c2 += c;
@@ -458,22 +500,20 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   tl.sp += c1;
   tl.ev += nev;
   tl.pr += c2 - nev;
-  const double ifp = 0.079577471545947667884;  // 1/(4 pi)
-  const double iep = 0.039788735772973833942;  // 1/(8 pi)
   double cdd = 0.25 * P.aidd;
-  if (M != GF_DQG) out.g = S1.g * ifp + mul_i(S2.g) * cdd;
+  if (M != GF_DQG) out.g = out.g + mul_i(S2.g) * cdd;
   if (M != GF_QG) {
-    out.dg[0] = S1.dg[0] * (-ifp) - S2.dg[0] * cdd;
-    out.dg[1] = S1.dg[1] * (-ifp) - S2.dg[1] * cdd;
-    out.dg[2] = S1.dg[2] * (-rz * ifp) + S2.dg[2] * cdd;
+    out.dg[0] = out.dg[0] - S2.dg[0] * cdd;
+    out.dg[1] = out.dg[1] - S2.dg[1] * cdd;
+    out.dg[2] = out.dg[2] + S2.dg[2] * cdd;
   }
   if (M == GF_V2) {
-    out.d2g[0] = S1.d2g[0] * (-iep) - mul_i(S2.d2g[0]) * cdd;
-    out.d2g[1] = S1.d2g[1] * (-iep) - mul_i(S2.d2g[1]) * cdd;
-    out.d2g[2] = S1.d2g[2] * (-iep) - S2.d2g[2] * cdd;
-    out.d2g[3] = S1.d2g[3] * (-iep) - mul_i(S2.d2g[3]) * cdd;
-    out.d2g[4] = S1.d2g[4] * (-rz * iep) + mul_i(S2.d2g[4]) * cdd;
-    out.d2g[5] = S1.d2g[5] * (-rz * iep) + mul_i(S2.d2g[5]) * cdd;
+    out.d2g[0] = out.d2g[0] - mul_i(S2.d2g[0]) * cdd;
+    out.d2g[1] = out.d2g[1] - mul_i(S2.d2g[1]) * cdd;
+    out.d2g[2] = out.d2g[2] - S2.d2g[2] * cdd;
+    out.d2g[3] = out.d2g[3] - mul_i(S2.d2g[3]) * cdd;
+    out.d2g[4] = out.d2g[4] + mul_i(S2.d2g[4]) * cdd;
+    out.d2g[5] = out.d2g[5] + mul_i(S2.d2g[5]) * cdd;
   }
   return c1 + c2;
 }
@@ -483,11 +523,12 @@ template <int M>
 __device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   if (rz == 0.0) return -4;
   const cd csg = cis(P.g2x * rx + P.g2y * ry);
-  Sum10 S, old;
+  Sum10 S;
+  Key2 old;
   sum_zero(S);
   int cc = 0;
   for (int n1 = P.nc, dir = 0;;) {
-    old = S;
+    old = key_fd<M>(S);
     const int c = fd_row<M>(P, rx, ry, rz, eps, n1, csg, S);
     if (c < 0) return c;
     cc += c;
