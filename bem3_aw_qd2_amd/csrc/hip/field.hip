@@ -1,14 +1,16 @@
 // field.hip -- pressure and particle velocity off and on the boundary, and the
-// radiation force and torque (bilinear elements).
+// radiation force and torque (bilinear and linear-triangle elements).
 //
 //   k_domain_id   point classification by signed solid angles (domain_id,
 //                 bem3_aw_qd2.c:1148-1257), one lane per point, the reference's
 //                 sequential order and early exits
-//   k_field       one wavefront per point: representation formula and its
-//                 gradient (pv_s_dbieq_aqd2, bem3_aw_qd2_field.c:227-297 with
-//                 q0_bil_dcoef_grad_* / bil_dcoef_grad_*), falling back to the
-//                 boundary-value form (pv_s_bieq_aqd2, :140-201) when
-//                 |grad F| > CBD_CDF, Bloch phase, incident field
+//   k_field_part  partial sums of the representation formula and its gradient
+//                 (pv_s_dbieq_aqd2, bem3_aw_qd2_field.c:227-297 with
+//                 q0_bil_dcoef_grad_* / bil_dcoef_grad_*) over chunks of a
+//                 point's items, one wavefront per (point, chunk); k_field_fin
+//                 sums them, falls back to the boundary-value form
+//                 (pv_s_bieq_aqd2, :140-201) when |grad F| > CBD_CDF, applies the
+//                 Bloch phase and the incident field
 //   k_field_bd    chunks of a boundary point's (element, point) items, one
 //                 wavefront each, plus one wavefront for the point's own element
 //                 in polar form (q0_bil_coef_bd / bil_coef_bd); k_field_bd_fin
