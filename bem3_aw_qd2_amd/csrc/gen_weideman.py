@@ -9,7 +9,9 @@ for Im z >= 0.  The a_n are the Fourier coefficients of
 f(t) = exp(-t^2) (L^2 + t^2) sampled at t = L tan(theta/2), computed here in
 40-digit arithmetic (mpmath) and rounded to double once.  With N = 40 the
 relative error against an mpmath reference is below 1e-15 on the closed upper
-half-plane (checked by tests/test_oracle_faddeeva.py).
+half-plane (checked by tests/test_oracle_faddeeva.py).  A second table with
+N = 20 (WEID20_*) serves Im z >= 2.5, Re z <= 3, where its relative error is
+below 2e-15.
 
 Usage: python gen_weideman.py > weideman_coef.h
 """
@@ -38,19 +40,25 @@ def coefficients(n_terms):
     return L, out
 
 
-def main():
-    L, a = coefficients(N)
-    print("/* Generated by gen_weideman.py: Weideman (1994) rational approximation of")
-    print(" * the Faddeeva function, N = %d.  a[n] multiplies Z^n. */" % N)
-    print("#ifndef BQD2_WEIDEMAN_COEF_H_")
-    print("#define BQD2_WEIDEMAN_COEF_H_")
-    print("#define WEID_N %d" % N)
-    print("#define WEID_L %s" % mp.nstr(L, 20))
-    print("#define WEID_COEFS \\")
+def table(prefix, n):
+    L, a = coefficients(n)
+    print("#define %s_N %d" % (prefix, n))
+    print("#define %s_L %s" % (prefix, mp.nstr(L, 20)))
+    print("#define %s_COEFS \\" % prefix)
     for i, v in enumerate(a):
         sep = "," if i + 1 < len(a) else ""
         print("  %s%s \\" % (mp.nstr(v, 20, min_fixed=-1, max_fixed=-1), sep))
     print("")
+
+
+def main():
+    print("/* Generated by gen_weideman.py: Weideman (1994) rational approximation of")
+    print(" * the Faddeeva function, N = %d (WEID_*) and N = 20 (WEID20_*).  a[n]" % N)
+    print(" * multiplies Z^n. */")
+    print("#ifndef BQD2_WEIDEMAN_COEF_H_")
+    print("#define BQD2_WEIDEMAN_COEF_H_")
+    table("WEID", N)
+    table("WEID20", 20)
     print("#endif")
 
 

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
+#define BQD2_ERFCX_INLINE 1
 #include "elem_dev.h"
 
 #define WAVE 64

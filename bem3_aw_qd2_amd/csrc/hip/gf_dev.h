@@ -51,6 +51,42 @@ __device__ __forceinline__ cd weideman_upper(double x, double y) {
   return 2.0 * (p * iden * iden) + isp * iden;
 }
 
+// the same with N = 20, for Im z >= 2.5 and Re z <= 3 (relative error < 2e-15)
+__device__ __forceinline__ cd weideman20_upper(double x, double y) {
+  const double L = WEID20_L;
+  const double a[WEID20_N] = {WEID20_COEFS};
+  cd iden = crecip(cd{L + y, -x});
+  cd Z = cd{L - y, x} * iden;
+  cd p = cd{a[WEID20_N - 1], 0.0};
+#pragma unroll
+  for (int n = WEID20_N - 2; n >= 0; n--) p = p * Z + cd{a[n], 0.0};
+  const double isp = 0.56418958354775628695;
+  return 2.0 * (p * iden * iden) + isp * iden;
+}
+
+// erfcx(y) = exp(y^2) erfc(y) = w(i y): on the imaginary axis Weideman's Z is
+// real, so the N = 40 rational becomes a real recurrence; y < 0 by
+// erfcx(y) = 2 exp(y^2) - erfcx(-y).  Inlined where the translation unit asks
+// for it (BQD2_ERFCX_INLINE: the assembly kernels); called elsewhere, which
+// keeps the register pressure of the field kernels (Hessian sums) lower.
+#if defined(BQD2_ERFCX_INLINE)
+#define BQD2_ERFCX_ATTR __forceinline__
+#else
+#define BQD2_ERFCX_ATTR __noinline__
+#endif
+__device__ BQD2_ERFCX_ATTR double erfcx_w(double y) {
+  const double L = WEID_L;
+  const double a[WEID_N] = {WEID_COEFS};
+  const double t = fabs(y);
+  const double iden = 1.0 / (L + t), Z = (L - t) * iden;
+  double p = a[WEID_N - 1];
+#pragma unroll
+  for (int n = WEID_N - 2; n >= 0; n--) p = fma(p, Z, a[n]);
+  const double isp = 0.56418958354775628695;
+  const double v = 2.0 * p * iden * iden + isp * iden;
+  return y >= 0.0 ? v : 2.0 * exp(y * y) - v;
+}
+
 // w(x + i y) with the reference's conventions: real axis returns Re w = exp(-x^2)
 // exactly (Faddeeva.cc:684-686), lower half plane by reflection.
 __device__ __forceinline__ cd faddeeva_w(double x, double y) {
@@ -169,7 +205,7 @@ __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, doubl
   double iE = 1.0 / E;
   double ke = P.k * E, y = rl * 0.5 * iE;
   double ef = exp(ke * ke - y * y);
-  cd w = weideman_upper(ke, y);
+  cd w = (y >= 2.5 && ke <= 3.0) ? weideman20_upper(ke, y) : weideman_upper(ke, y);
   const double isp = 0.56418958354775628695;
   cd cer = ce * (ef * w.x);
   t.g = cer * irl;
@@ -239,8 +275,8 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     if (fast) {
 #pragma clang fp contract(fast)
       efs = exp(bn2 * E * E - azve * azve);
-      f1 = efs * erfcx(bn * E + azve);
-      f2 = efs * erfcx(bn * E - azve);
+      f1 = efs * erfcx_w(bn * E + azve);
+      f2 = efs * erfcx_w(bn * E - azve);
     } else {
       double ebz = exp(bn * rz);
       f1 = ebz * erfc(bn * E + azve);
