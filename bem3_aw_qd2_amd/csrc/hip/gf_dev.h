@@ -192,9 +192,10 @@ __device__ __forceinline__ bool term_small_fd(const Sum10& t, const Sum10& s, do
 // spatial image (l1, l2): d3_qpgf_d2.c:1814-1838 (v1) and :2131-2170 (v2)
 // ce is the Bloch phase exp(i k_t . R_l) of the image, carried along the row by
 // the caller (one sincos per row instead of per image)
+// ef: exp(k^2 E^2 - |r + R_l|^2 / 4E^2), also carried along the row by the caller
 template <int M>
 __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, double ry, double rz, double E,
-                                             int l1, int l2, cd ce, Sum10& t) {
+                                             int l1, int l2, cd ce, double ef, Sum10& t) {
   // no z == 0 residue is read in the image sum: products may be fused
 #pragma clang fp contract(fast)
   double Rl0 = l1 * P.d1x + l2 * P.d2x;
@@ -204,7 +205,6 @@ __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, doubl
   double irl = 1.0 / rl;
   double iE = 1.0 / E;
   double ke = P.k * E, y = rl * 0.5 * iE;
-  double ef = exp(ke * ke - y * y);
   cd w = (y >= 2.5 && ke <= 3.0) ? weideman20_upper(ke, y) : weideman_upper(ke, y);
   const double isp = 0.56418958354775628695;
   cd cer = ce * (ef * w.x);
@@ -237,7 +237,7 @@ __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, doubl
 // the reference forms them.
 template <int M>
 __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, double ry, double rz, double E,
-                                             int n1, int n2, bool fast, cd cef, Sum10& t) {
+                                             int n1, int n2, bool fast, cd cef, double eff, Sum10& t) {
   double an0 = -P.vkx + n1 * P.g1x + n2 * P.g2x;
   double an1 = -P.vky + n1 * P.g1y + n2 * P.g2y;
   double bn2 = P.k * P.k - an0 * an0 - an1 * an1;
@@ -249,7 +249,7 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     cd ff = cis(bn * rz);
     // ef and w kept apart as the reference does: at z == 0, ef*Re(w) - ff is
     // pure rounding and the row stopping rule reads it (products not fused)
-    double ef = exp(__dmul_rn(__dmul_rn(bn2, E), E) - __dmul_rn(azve, azve));
+    double ef = fast ? eff : exp(__dmul_rn(__dmul_rn(bn2, E), E) - __dmul_rn(azve, azve));
     cd w = faddeeva_w(bn * E, azve);
     cd F = cd{2.0 * ff.x, 2.0 * (__dmul_rn(ef, w.y) + ff.y)};
     double ib = 1.0 / bn;
@@ -274,7 +274,7 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     double f1, f2, efs = 0.0;
     if (fast) {
 #pragma clang fp contract(fast)
-      efs = exp(bn2 * E * E - azve * azve);
+      efs = eff;
       f1 = efs * erfcx_w(bn * E + azve);
       f2 = efs * erfcx_w(bn * E - azve);
     } else {
@@ -363,11 +363,20 @@ __device__ int spatial_row(const GfParams& P, double rx, double ry, double rz, d
   const int lc = -(int)round(((rx + l1 * P.d1x) * P.d2x + (ry + l1 * P.d1y) * P.d2y) / P.d2n);
   const cd c0 = cis(P.vkx * (l1 * P.d1x + lc * P.d2x) + P.vky * (l1 * P.d1y + lc * P.d2y));
   const cd csb = conj(cs);
+  // exp(k^2 E^2 - rl^2 / 4E^2) along the row: rl^2 is quadratic in l2, so the
+  // factor from one image to the next changes by the constant q each step
+  const double i4e = 0.25 / (E * E);
+  const double ax = rx + l1 * P.d1x + lc * P.d2x, ay = ry + l1 * P.d1y + lc * P.d2y;
+  const double bb = ax * P.d2x + ay * P.d2y;
+  const double e0 = exp(P.k * P.k * E * E - (ax * ax + ay * ay + rz * rz) * i4e);
+  const double q = exp(-2.0 * P.d2n * i4e);
+  const double fu0 = exp(-(2.0 * bb + P.d2n) * i4e), fd0 = exp((2.0 * bb - P.d2n) * i4e);
   Sum10 t;
   int l2 = lc, dir = 0, cc = 0;
   cd ce = c0;
+  double ef = e0, fs = fu0;
   for (;;) {
-    spatial_term<M>(P, rx, ry, rz, E, l1, l2, ce, t);
+    spatial_term<M>(P, rx, ry, rz, E, l1, l2, ce, ef, t);
     sum_add<M>(S, t);
     cc++;
     if (dir == 0) {
@@ -377,11 +386,15 @@ __device__ int spatial_row(const GfParams& P, double rx, double ry, double rz, d
       dir = -1;
       l2 = lc;
       ce = c0;
+      ef = e0;
+      fs = fd0;
     } else if (dir > 0 ? l2 + 1 == lc + BQD2_EW_LIMIT : l2 - 1 == lc - BQD2_EW_LIMIT) {
       return -1;
     }
     l2 += dir;
     ce = ce * (dir > 0 ? cs : csb);
+    ef = ef * fs;
+    fs = fs * q;
   }
   return cc;
 }
@@ -394,13 +407,24 @@ __device__ int spectral_row(const GfParams& P, double rx, double ry, double rz, 
                             bool fast, cd cs, Sum10& S, int& flag, int& nev) {
   const int nc = (int)round(-((-P.vkx + n1 * P.g1x) * P.g2x + (-P.vky + n1 * P.g1y) * P.g2y) / P.g2n);
   cd c0 = cd{1.0, 0.0};
-  if (fast) c0 = cis((-P.vkx + n1 * P.g1x + nc * P.g2x) * rx + (-P.vky + n1 * P.g1y + nc * P.g2y) * ry);
+  // fast: exp((k^2 - |a_n|^2) E^2 - z^2/4E^2) along the row, |a_n|^2 quadratic in n2
+  double e0 = 0.0, q = 0.0, fu0 = 0.0, fd0 = 0.0;
+  if (fast) {
+    const double ax = -P.vkx + n1 * P.g1x + nc * P.g2x, ay = -P.vky + n1 * P.g1y + nc * P.g2y;
+    c0 = cis(ax * rx + ay * ry);
+    const double e2 = E * E, azve = rz / (2.0 * E), bb = ax * P.g2x + ay * P.g2y;
+    e0 = exp((P.k * P.k - ax * ax - ay * ay) * e2 - azve * azve);
+    q = exp(-2.0 * P.g2n * e2);
+    fu0 = exp(-(2.0 * bb + P.g2n) * e2);
+    fd0 = exp((2.0 * bb - P.g2n) * e2);
+  }
   const cd csb = conj(cs);
   Sum10 t;
   int n2 = nc, dir = 0, cc = 0, prop = 0;
   cd ce = c0;
+  double ef = e0, fs = fu0;
   for (;;) {
-    const int k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, fast, ce, t);
+    const int k = spectral_term<M>(P, rx, ry, rz, E, n1, n2, fast, ce, ef, t);
     if (k < 0) return k;
     sum_add<M>(S, t);
     nev += k;
@@ -412,12 +436,16 @@ __device__ int spectral_row(const GfParams& P, double rx, double ry, double rz, 
       dir = -1;
       n2 = nc;
       ce = c0;
+      ef = e0;
+      fs = fd0;
     } else {
       if (k == 0) prop++;
       if (dir > 0 ? n2 + 1 == nc + BQD2_EW_LIMIT : n2 - 1 == nc - BQD2_EW_LIMIT) return -2;
     }
     n2 += dir;
     ce = ce * (dir > 0 ? cs : csb);
+    ef = ef * fs;
+    fs = fs * q;
   }
   flag = (prop == 0);
   return cc;
