@@ -135,9 +135,10 @@ CONFIGS = {
 }
 
 
-def write_config(name, directory, triangles=False):
+def write_config(name, directory, triangles=False, periodic=True):
     """Write pw.txt, periodicity.txt, medium.txt and the mesh of a workload into
-    directory; returns the four paths in read_aqd2 order."""
+    directory; returns the four paths in read_aqd2 order.  periodic=False writes
+    periodicity setting 0: the same sphere alone in free space."""
     import os
     c = CONFIGS[name]
     os.makedirs(directory, exist_ok=True)
@@ -145,7 +146,7 @@ def write_config(name, directory, triangles=False):
     elems = split_triangles(P, Q) if triangles else Q
     paths = [os.path.join(directory, x) for x in ("pw.txt", "periodicity.txt", "medium.txt", "mesh.msh")]
     write_pw(paths[0], c["host"][0], c["host"][1], c["f"], theta=c["theta"], psi=c["psi"])
-    write_periodicity(paths[1], c["d1"], c["d2"])
+    write_periodicity(paths[1], c["d1"], c["d2"], ps=1 if periodic else 0)
     write_medium(paths[2], [c["sphere"]])
     write_msh(paths[3], P, elems)
     return paths
