@@ -66,6 +66,13 @@ def hip_lib():
         lib.bqd2_download_solution.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
         lib.bqd2_zgetrf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.bqd2_zgetrs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.bqd2_cgetrf_wsize.restype = C.c_size_t
+        lib.bqd2_cgetrf_wsize.argtypes = [C.c_int]
+        lib.bqd2_cgetrf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.bqd2_cgetrs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.bqd2_cgemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
+                                   C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p]
+        lib.bqd2_lu32_info.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.bqd2_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         lib.bqd2_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p]

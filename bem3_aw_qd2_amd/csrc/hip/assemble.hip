@@ -1054,5 +1054,6 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)",
              cnt[CNT_ERR], did);
   D.assembled = true;
+  p->lu32_valid = false;
   return BQD2_OK;
 }

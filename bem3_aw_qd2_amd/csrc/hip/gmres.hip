@@ -12,6 +12,13 @@
 // the matrix run in lock step (one pass over the matrix per step for all).  The
 // caller falls back to the LU when the tolerance is not met.
 //
+// With a preconditioner (the complex float LU of the row-scaled matrix,
+// lu32.hip) the iteration is right-preconditioned: the Krylov space is built on
+// D A M^-1 and the update is X += M^-1 V y, so the residual monitored is still
+// the true scaled residual D (b - A X) of the FP64 system.  This is GMRES-based
+// iterative refinement: the float factors set the convergence rate, the FP64
+// matrix-vector products set the attainable accuracy.
+//
 // Matrices are row-major, lda >= n.  An optional row map marks unused rows
 // (map[r] < 0, the fourth slot of a triangle): they are solved as identity rows
 // with a zero right-hand side.
@@ -171,8 +178,24 @@ static void launch_mv(int nrhs, int n, const double *A, long long lda, const int
   else hipLaunchKernelGGL(k_gm_mv<3>, g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
 }
 
+// X_s += Z_s
+__global__ void __launch_bounds__(256) k_gm_add(int n, const double *__restrict__ Z, long long ldz,
+                                                double *__restrict__ X, long long ldx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (r >= n) return;
+  double *x = X + 2 * (size_t)s * ldx;
+  stc(x, r, ldc(x, r) + ldc(Z + 2 * (size_t)s * ldz, r));
+}
+
+int bqd2_jacobi(int n, const double *A, long long lda, const int *map, double *dinv, hipStream_t st) {
+  hipLaunchKernelGGL(k_gm_dinv, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, n, A, lda, map, dinv);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
 int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, const int *map, const double *B,
-               long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres) {
+               long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres,
+               const Precond *pc) {
   BQD2_CHECK(n > 0 && nrhs >= 1 && nrhs <= 3 && m >= 2, BQD2_E_ARG, "bqd2_gmres: bad arguments");
   hipStream_t st = cx->st;
   // workspace: V (nrhs x (m+1) x n), W, R, Db (nrhs x n), dinv (n), h (nrhs x (2(m+1)+1)), y (nrhs x m)
@@ -180,12 +203,13 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
   const int hstride = 2 * (m + 1) + 2;
   const size_t bV = (size_t)nrhs * vstride * 16, bN = (size_t)nrhs * n * 16, bD = (size_t)n * 16;
   const size_t bH = (size_t)nrhs * hstride * 16, bY = (size_t)nrhs * m * 16;
-  const size_t total = bV + 3 * bN + bD + bH + bY + 7 * 256;
+  const size_t total = bV + 4 * bN + bD + bH + bY + 8 * 256;
   char *base;
   int rc = bqd2_workspace(cx, total, &base);
   if (rc) return rc;
   auto take = [&](size_t bytes) { double *q = (double *)base; base += (bytes + 255) / 256 * 256; return q; };
   double *V = take(bV), *W = take(bN), *R = take(bN), *Db = take(bN), *dinv = take(bD), *H = take(bH), *Y = take(bY);
+  double *Z = take(bN);  // M^-1 of the basis vector / of the update (preconditioned runs)
   const dim3 gv(bqd2_blocks(n, 256), nrhs), b256(256);
 
   hipLaunchKernelGGL(k_gm_dinv, dim3(bqd2_blocks(n, 256)), b256, 0, st, n, A, lda, map, dinv);
@@ -229,7 +253,14 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
     for (int s = 0; s < nrhs; s++) g[(size_t)s * (m + 1)] = beta[s];
     int k = 0;  // basis vectors of this cycle used in the update
     for (int j = 0; j < m && total_its < maxit; j++) {
-      launch_mv(nrhs, n, A, lda, map, dinv, V + 2 * (size_t)j * n, vstride, W, (long long)n, st);
+      if (pc) {
+        int rc2 = bqd2_lu32_apply(n, nrhs, pc->Af, pc->ws, V + 2 * (size_t)j * n, vstride, Z, (long long)n, pc->tmp,
+                                  st);
+        if (rc2) return rc2;
+        launch_mv(nrhs, n, A, lda, map, dinv, Z, (long long)n, W, (long long)n, st);
+      } else {
+        launch_mv(nrhs, n, A, lda, map, dinv, V + 2 * (size_t)j * n, vstride, W, (long long)n, st);
+      }
       // classical Gram-Schmidt, twice; h of pass p at H[s][p (m+1) + i], the norm at H[s][2 (m+1)]
       for (int pass = 0; pass < 2; pass++) {
         double *hp = H + 2 * (size_t)pass * (m + 1);
@@ -290,7 +321,16 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
       }
     }
     BQD2_HIP(hipMemcpyAsync(Y, yv.data(), bY, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gm_update, gv, b256, 0, st, n, k, V, (long long)n, vstride, Y, m, X, ldx);
+    if (pc) {
+      // X += M^-1 (V y)
+      BQD2_HIP(hipMemsetAsync(Z, 0, bN, st));
+      hipLaunchKernelGGL(k_gm_update, gv, b256, 0, st, n, k, V, (long long)n, vstride, Y, m, Z, (long long)n);
+      int rc2 = bqd2_lu32_apply(n, nrhs, pc->Af, pc->ws, Z, (long long)n, Z, (long long)n, pc->tmp, st);
+      if (rc2) return rc2;
+      hipLaunchKernelGGL(k_gm_add, gv, b256, 0, st, n, Z, (long long)n, X, ldx);
+    } else {
+      hipLaunchKernelGGL(k_gm_update, gv, b256, 0, st, n, k, V, (long long)n, vstride, Y, m, X, ldx);
+    }
     BQD2_HIP(hipGetLastError());
     BQD2_HIP(hipStreamSynchronize(st));
   }

@@ -1,0 +1,582 @@
+// lu32.hip -- dense LU in complex FP32 with the trailing updates on the MFMA
+// units, and the solves with its factors.
+//
+// The reference solves its boundary-value system directly in complex double
+// (MKL PARDISO, d3qd2_mkl_solver.c:18-148).  This factorisation is the
+// mixed-precision form of that solve: the row-scaled system is rounded to
+// complex float, factored here with partial pivoting (max |Re|+|Im|, LAPACK's
+// izamax convention), and applied as the preconditioner of GMRES on the FP64
+// system (gmres.hip), which restores the FP64 result.
+//
+// Layout: column-major, leading dimension n, complex numbers as float2 (re, im).
+// Right-looking blocked algorithm with panel width NB:
+//   panel     one launch per column (k_cstep): every workgroup owns a slice of
+//             the panel's rows; the launch takes the pivot from the candidates
+//             the previous launch left (one per workgroup), interchanges, scales
+//             and applies the rank-1 update to its rows, then leaves its
+//             candidate for the next column.  Launch boundaries are the only
+//             grid-wide synchronisation.
+//   swaps     the panel's interchanges on the other columns (k_cswap_cols)
+//   inverses  of the diagonal block's unit-lower and upper factors (k_ctrtri),
+//             kept for the solves
+//   U12       = L11^-1 A12, a GEMM with the explicit inverse
+//   A22      -= L21 U12, the O(n^3) part: k_cgemm on v_mfma_f32_32x32x2_f32
+//             with FP32 accumulation.
+// Solves (bqd2_cgetrs): permutation, then per diagonal block x_k = T_k b_k and
+// the update of the remaining rows, forward with L and backward with U; GEMV
+// kernels for up to 4 right-hand sides, the MFMA GEMM beyond.
+#include <vector>
+#include "common.h"
+
+#define NB 128     // panel width = K of the trailing update
+#define GMAXW 256  // panel workgroups at most
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 crecipf(float2 a) {
+  const float d = 1.0f / fmaf(a.x, a.x, a.y * a.y);
+  return make_float2(a.x * d, -a.y * d);
+}
+
+// device workspace of a factorisation of order n (bqd2_cgetrf_wsize bytes):
+//   T     ceil(n / NB) diagonal blocks x 2 (L^-1, U^-1), NB x NB column-major each
+//   perm  n ints: row i of P A is row perm[i] of A
+//   slot  2 x GMAXW pivot candidates (value, row) of the panel workgroups
+//   cand  2 x GMAXW candidate rows of NB entries
+//   rowj  2 x NB entries: row j of the panel before the interchange
+//   flag  first zero pivot (n when none)
+struct LuWs {
+  float2 *T;
+  int *perm;
+  float2 *slot;  // (value, row as float bits)
+  float2 *cand;
+  float2 *rowj;
+  int *flag;
+};
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+static LuWs lu_ws(int n, void *base) {
+  char *q = (char *)base;
+  LuWs w;
+  const size_t nbk = (size_t)(n + NB - 1) / NB;
+  w.T = (float2 *)q; q += align256(nbk * 2 * NB * NB * sizeof(float2));
+  w.perm = (int *)q; q += align256((size_t)n * sizeof(int));
+  w.slot = (float2 *)q; q += align256(2 * GMAXW * sizeof(float2));
+  w.cand = (float2 *)q; q += align256((size_t)2 * GMAXW * NB * sizeof(float2));
+  w.rowj = (float2 *)q; q += align256(2 * NB * sizeof(float2));
+  w.flag = (int *)q;
+  return w;
+}
+
+extern "C" size_t bqd2_cgetrf_wsize(int n) {
+  if (n <= 0) return 0;
+  const size_t nbk = (size_t)(n + NB - 1) / NB;
+  return align256(nbk * 2 * NB * NB * sizeof(float2)) + align256((size_t)n * sizeof(int)) +
+         align256(2 * GMAXW * sizeof(float2)) + align256((size_t)2 * GMAXW * NB * sizeof(float2)) +
+         align256(2 * NB * sizeof(float2)) + 256;
+}
+
+// ---------------------------------------------------------------- panel
+// Launch for column j of the panel [k0, k0 + kb) (j = k0 - 1: the first
+// candidates only).  Workgroup w owns rows [k0 + w chunk, k0 + (w+1) chunk).
+// Reads the candidates of parity (j - k0) & 1, writes those of the other.  A
+// column whose candidates are all NaN keeps its row (p = j) and is flagged.
+__device__ __forceinline__ bool better(float v, int r, float bv, int br) { return v > bv || (v == bv && r < br); }
+
+__global__ void __launch_bounds__(256) k_cstep(int n, float2 *__restrict__ A, int k0, int kb, int j, int chunk,
+                                               int nw, int *__restrict__ ipiv, LuWs W) {
+  __shared__ float2 su[NB];
+  __shared__ float sv[4];
+  __shared__ int sr[4];
+  const int tid = threadIdx.x, w = blockIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int r0 = k0 + w * chunk, r1 = min(n, r0 + chunk);
+  const size_t ld = (size_t)n;
+  if (j >= k0) {
+    const int par = (j - k0) & 1;
+    // pivot: the best of the workgroups' candidates (ties: the lowest row)
+    float bv = -1.0f;
+    int br = 0x7fffffff;
+    for (int q = tid; q < nw; q += 256) {
+      const float2 s = W.slot[par * GMAXW + q];
+      const int r = __float_as_int(s.y);
+      if (better(s.x, r, bv, br)) { bv = s.x; br = r; }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float ov = __shfl_xor(bv, off, 64);
+      const int orr = __shfl_xor(br, off, 64);
+      if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+    }
+    if (lane == 0) { sv[wv] = bv; sr[wv] = br; }
+    __syncthreads();
+    bv = sv[0]; br = sr[0];
+    for (int q = 1; q < 4; q++)
+      if (better(sv[q], sr[q], bv, br)) { bv = sv[q]; br = sr[q]; }
+    const bool none = (br == 0x7fffffff);
+    const int p = none ? j : br;
+    const float2 *u = none ? W.rowj + par * NB : W.cand + ((size_t)par * GMAXW + (p - k0) / chunk) * NB;
+    for (int c = tid; c < kb; c += 256) su[c] = u[c];
+    if (w == 0 && tid == 0) ipiv[j] = p;
+    // interchange rows j and p over the panel's columns
+    if (p != j) {
+      const float2 *rj = W.rowj + par * NB;
+      if (p >= r0 && p < r1)
+        for (int c = tid; c < kb; c += 256) A[(size_t)(k0 + c) * ld + p] = rj[c];
+      if (j >= r0 && j < r1)
+        for (int c = tid; c < kb; c += 256) A[(size_t)(k0 + c) * ld + j] = u[c];
+    }
+    __syncthreads();
+    const float2 piv = su[j - k0];
+    const bool zero = none || (piv.x == 0.0f && piv.y == 0.0f);
+    if (zero && w == 0 && tid == 0) atomicMin(W.flag, j);
+    const float2 ip = zero ? make_float2(0.0f, 0.0f) : crecipf(piv);
+    // scale the column below the diagonal and update the panel's later columns
+    for (int r = max(r0, j + 1) + tid; r < r1; r += 256) {
+      float2 *a = A + r;
+      const float2 l = cmulf(a[(size_t)j * ld], ip);
+      a[(size_t)j * ld] = l;
+      for (int c = j + 1; c < k0 + kb; c++) {
+        const float2 uc = su[c - k0];
+        float2 v = a[(size_t)c * ld];
+        v.x -= fmaf(l.x, uc.x, -l.y * uc.y);
+        v.y -= fmaf(l.x, uc.y, l.y * uc.x);
+        a[(size_t)c * ld] = v;
+      }
+    }
+  }
+  // candidate of this workgroup for column j + 1
+  const int jn = j + 1;
+  if (jn >= k0 + kb) return;
+  const int parn = (jn - k0) & 1;
+  float bv = -1.0f;
+  int br = 0x7fffffff;
+  for (int r = max(r0, jn) + tid; r < r1; r += 256) {
+    const float2 a = A[(size_t)jn * ld + r];
+    const float v = fabsf(a.x) + fabsf(a.y);
+    if (better(v, r, bv, br)) { bv = v; br = r; }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(bv, off, 64);
+    const int orr = __shfl_xor(br, off, 64);
+    if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+  }
+  __syncthreads();
+  if (lane == 0) { sv[wv] = bv; sr[wv] = br; }
+  __syncthreads();
+  bv = sv[0]; br = sr[0];
+  for (int q = 1; q < 4; q++)
+    if (better(sv[q], sr[q], bv, br)) { bv = sv[q]; br = sr[q]; }
+  if (tid == 0) W.slot[parn * GMAXW + w] = make_float2(bv, __int_as_float(br));
+  float2 *cd = W.cand + ((size_t)parn * GMAXW + w) * NB;
+  if (br != 0x7fffffff)
+    for (int c = tid; c < kb; c += 256) cd[c] = A[(size_t)(k0 + c) * ld + br];
+  if (jn >= r0 && jn < r1)
+    for (int c = tid; c < kb; c += 256) W.rowj[parn * NB + c] = A[(size_t)(k0 + c) * ld + jn];
+}
+
+// the interchanges of the panel [k0, k0 + kb) on the columns outside it
+__global__ void __launch_bounds__(256) k_cswap_cols(int n, float2 *__restrict__ A, int k0, int kb,
+                                                    const int *__restrict__ ipiv) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n - kb) return;
+  if (c >= k0) c += kb;
+  float2 *a = A + (size_t)c * n;
+  for (int j = k0; j < k0 + kb; j++) {
+    const int p = ipiv[j];
+    if (p == j) continue;
+    const float2 t = a[j];
+    a[j] = a[p];
+    a[p] = t;
+  }
+}
+
+// Inverses of the diagonal block D = A[k0.., k0..] (kb x kb): block 0 the
+// inverse of its unit lower part, block 1 of its upper part, written to
+// T (ldT = NB).  One lane per column of the inverse; D staged in LDS.
+__global__ void __launch_bounds__(NB) k_ctrtri(int n, const float2 *__restrict__ A, int k0, int kb,
+                                               float2 *__restrict__ T) {
+  __shared__ float2 sD[NB * NB];  // column-major (r, q) at q * NB + r
+  const int c = threadIdx.x;
+  for (int e = c; e < kb * kb; e += NB) {
+    const int r = e % kb, q = e / kb;
+    sD[q * NB + r] = A[(size_t)(k0 + q) * n + k0 + r];
+  }
+  __syncthreads();
+  float2 *X = T + (size_t)blockIdx.x * NB * NB + (size_t)c * NB;  // column c of the inverse
+  if (c >= kb) return;
+  if (blockIdx.x == 0) {
+    // L X = I, L unit lower: X[r][c] = -sum_{c <= q < r} L[r][q] X[q][c]
+    for (int r = 0; r < c; r++) X[r] = make_float2(0.0f, 0.0f);
+    X[c] = make_float2(1.0f, 0.0f);
+    for (int r = c + 1; r < kb; r++) {
+      float2 s = make_float2(0.0f, 0.0f);
+      for (int q = c; q < r; q++) {
+        const float2 l = sD[q * NB + r], x = X[q];
+        s.x += fmaf(l.x, x.x, -l.y * x.y);
+        s.y += fmaf(l.x, x.y, l.y * x.x);
+      }
+      X[r] = make_float2(-s.x, -s.y);
+    }
+  } else {
+    // U X = I: X[c][c] = 1 / U[c][c], X[r][c] = -(sum_{r < q <= c} U[r][q] X[q][c]) / U[r][r]
+    for (int r = c + 1; r < kb; r++) X[r] = make_float2(0.0f, 0.0f);
+    X[c] = crecipf(sD[c * NB + c]);
+    for (int r = c - 1; r >= 0; r--) {
+      float2 s = make_float2(0.0f, 0.0f);
+      for (int q = r + 1; q <= c; q++) {
+        const float2 u = sD[q * NB + r], x = X[q];
+        s.x += fmaf(u.x, x.x, -u.y * x.y);
+        s.y += fmaf(u.x, x.y, u.y * x.x);
+      }
+      const float2 v = cmulf(s, crecipf(sD[r * NB + r]));
+      X[r] = make_float2(-v.x, -v.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- MFMA GEMM
+// C[m x nc] -= A[m x kk] B[kk x nc] (SUB) or C = A B (SET), column-major complex
+// float.  Workgroup tile 128 x 128 of C, four waves of 64 x 64; K staged in LDS
+// in slices of 32 complex columns.  v_mfma_f32_32x32x2_f32 computes
+// D[i][j] += sum_{k<2} X[i][k] Y[k][j], with lane l holding X[l%32][l/32] and
+// Y[l/32][l%32] and D[i][j] in register r of lane l for j = l%32,
+// i = 8 (r/4) + 4 (l/32) + r%4.  Each complex k is one instruction per real and
+// imaginary part of C^T = B^T A^T: the two k slots carry the real and imaginary
+// parts, X = (Br, Bi) for both, Y = (Ar, -Ai) for Re C and (Ai, Ar) for Im C.
+// With C^T as the product, j runs along the rows of C, so the 32 lanes of a
+// register store 32 consecutive rows of one column.  B may be C when m <= 128:
+// a workgroup then reads only the columns it writes, and writes after its last read.
+#define GT 128
+#define GK 32
+#define GPAD 1
+template <int SET>
+__global__ void __launch_bounds__(256) k_cgemm(int m, int nc, int kk, const float2 *__restrict__ A, long long lda,
+                                               const float2 *B, long long ldb, float2 *C, long long ldc) {
+  __shared__ float2 sA[GK][GT + GPAD];  // (k, row)
+  __shared__ float2 sB[GK][GT + GPAD];  // (k, col)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave & 1, wc = wave >> 1;
+  const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
+  const int l32 = lane & 31;
+  const bool hi = lane >= 32;
+  floatx16 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) { accR[a][b][r] = 0.0f; accI[a][b][r] = 0.0f; }
+  for (int k0 = 0; k0 < kk; k0 += GK) {
+    __syncthreads();
+    {
+      // A slice: 128 rows x 32 k, consecutive threads on consecutive rows
+      const int r = tid & 127, kq = tid >> 7;
+      const int gr = row0 + r;
+#pragma unroll 4
+      for (int i = 0; i < GK / 2; i++) {
+        const int k = kq + 2 * i, gk = k0 + k;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (gr < m && gk < kk) v = A[(size_t)gk * lda + gr];
+        sA[k][r] = v;
+      }
+      // B slice: 32 k x 128 columns, consecutive threads on consecutive k
+      const int k = tid & 31, cq = tid >> 5;
+      const int gk = k0 + k;
+#pragma unroll 4
+      for (int i = 0; i < GT / 8; i++) {
+        const int c = cq + 8 * i, gc = col0 + c;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (gc < nc && gk < kk) v = B[(size_t)gc * ldb + gk];
+        sB[k][c] = v;
+      }
+    }
+    __syncthreads();
+    const int kend = min(GK, kk - k0);
+    for (int q = 0; q < kend; q++) {
+      float xb[2], yr[2], yi[2];
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const float2 v = sB[q][wc * 64 + b * 32 + l32];
+        xb[b] = hi ? v.y : v.x;
+      }
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const float2 v = sA[q][wr * 64 + a * 32 + l32];
+        yr[a] = hi ? -v.y : v.x;
+        yi[a] = hi ? v.x : v.y;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          accR[b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yr[a], accR[b][a], 0, 0, 0);
+          accI[b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yi[a], accI[b][a], 0, 0, 0);
+        }
+    }
+  }
+  // accR[b][a] register r: row l32 of row block a, column 8 (r/4) + 4 hi + r%4 of column block b
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int gr = row0 + wr * 64 + a * 32 + l32;
+      if (gr >= m) continue;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int gc = col0 + wc * 64 + b * 32 + 8 * (r >> 2) + 4 * (hi ? 1 : 0) + (r & 3);
+        if (gc >= nc) continue;
+        float2 *p = C + (size_t)gc * ldc + gr;
+        if (SET) {
+          *p = make_float2(accR[b][a][r], accI[b][a][r]);
+        } else {
+          float2 o = *p;
+          o.x -= accR[b][a][r];
+          o.y -= accI[b][a][r];
+          *p = o;
+        }
+      }
+    }
+}
+
+static int cgemm(bool set, int m, int nc, int kk, const float2 *A, long long lda, const float2 *B, long long ldb,
+                 float2 *C, long long ldc, hipStream_t st) {
+  if (m <= 0 || nc <= 0 || kk <= 0) return BQD2_OK;
+  dim3 grid(bqd2_blocks(m, GT), bqd2_blocks(nc, GT));
+  if (set) hipLaunchKernelGGL(k_cgemm<1>, grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+  else hipLaunchKernelGGL(k_cgemm<0>, grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
+// ---------------------------------------------------------------- GEMV (few right-hand sides)
+// Y[m x NR] -= A[m x kk] X[kk x NR] (SUB) or Y = A X (SET), kk <= NB; X staged
+// in LDS before any write, so Y may alias X when one workgroup covers m.
+template <int NR, int SET>
+__global__ void __launch_bounds__(256) k_cgemv(int m, int kk, const float2 *__restrict__ A, long long lda,
+                                               const float2 *X, long long ldx, float2 *Y, long long ldy) {
+  __shared__ float2 sx[NR][NB];
+  for (int e = threadIdx.x; e < NR * kk; e += blockDim.x) sx[e / kk][e % kk] = X[(size_t)(e / kk) * ldx + e % kk];
+  __syncthreads();
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  float2 acc[NR];
+#pragma unroll
+  for (int s = 0; s < NR; s++) acc[s] = make_float2(0.0f, 0.0f);
+  const float2 *a = A + r;
+  for (int k = 0; k < kk; k++) {
+    const float2 v = a[(size_t)k * lda];
+#pragma unroll
+    for (int s = 0; s < NR; s++) {
+      const float2 x = sx[s][k];
+      acc[s].x += fmaf(v.x, x.x, -v.y * x.y);
+      acc[s].y += fmaf(v.x, x.y, v.y * x.x);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NR; s++) {
+    float2 *y = Y + (size_t)s * ldy + r;
+    if (SET) *y = acc[s];
+    else { float2 o = *y; o.x -= acc[s].x; o.y -= acc[s].y; *y = o; }
+  }
+}
+
+template <int NR>
+static void gemv_nr(bool set, int m, int kk, const float2 *A, long long lda, const float2 *X, long long ldx, float2 *Y,
+                    long long ldy, hipStream_t st) {
+  const int bs = set ? NB : 256;  // SET is used in place on one diagonal block: one workgroup
+  dim3 g(bqd2_blocks(m, bs));
+  if (set) hipLaunchKernelGGL((k_cgemv<NR, 1>), g, dim3(bs), 0, st, m, kk, A, lda, X, ldx, Y, ldy);
+  else hipLaunchKernelGGL((k_cgemv<NR, 0>), g, dim3(bs), 0, st, m, kk, A, lda, X, ldx, Y, ldy);
+}
+
+// Y -= A X or Y = A X for nrhs right-hand sides (SET only in place on <= NB rows)
+static int small_update(bool set, int m, int kk, int nrhs, const float2 *A, long long lda, const float2 *X,
+                        long long ldx, float2 *Y, long long ldy, hipStream_t st) {
+  if (m <= 0 || kk <= 0) return BQD2_OK;
+  switch (nrhs) {
+    case 1: gemv_nr<1>(set, m, kk, A, lda, X, ldx, Y, ldy, st); break;
+    case 2: gemv_nr<2>(set, m, kk, A, lda, X, ldx, Y, ldy, st); break;
+    case 3: gemv_nr<3>(set, m, kk, A, lda, X, ldx, Y, ldy, st); break;
+    case 4: gemv_nr<4>(set, m, kk, A, lda, X, ldx, Y, ldy, st); break;
+    default: return cgemm(set, m, nrhs, kk, A, lda, X, ldx, Y, ldy, st);
+  }
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
+// perm from the interchanges: row i of P A is row perm[i] of A (one lane: the
+// interchanges are sequential)
+__global__ void k_perm(int n, const int *__restrict__ ipiv, int *__restrict__ perm) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int i = 0; i < n; i++) perm[i] = i;
+  for (int i = 0; i < n; i++) {
+    const int p = ipiv[i];
+    const int t = perm[i];
+    perm[i] = perm[p];
+    perm[p] = t;
+  }
+}
+
+__global__ void k_init_flag(int *flag, int n) {
+  if (threadIdx.x == 0) *flag = n;
+}
+
+extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream) {
+  BQD2_CHECK(n > 0 && d_A && d_ipiv && d_ws, BQD2_E_ARG, "bqd2_cgetrf: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  float2 *A = (float2 *)d_A;
+  LuWs W = lu_ws(n, d_ws);
+  hipLaunchKernelGGL(k_init_flag, dim3(1), dim3(64), 0, st, W.flag, n);
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    const int rows = n - k0;
+    int nw = bqd2_blocks(rows, 128);
+    if (nw > GMAXW) nw = GMAXW;
+    const int chunk = bqd2_blocks(rows, nw);
+    nw = bqd2_blocks(rows, chunk);
+    for (int j = k0 - 1; j < k0 + kb; j++)
+      hipLaunchKernelGGL(k_cstep, dim3(nw), dim3(256), 0, st, n, A, k0, kb, j, chunk, nw, d_ipiv, W);
+    BQD2_HIP(hipGetLastError());
+    if (n > kb)
+      hipLaunchKernelGGL(k_cswap_cols, dim3(bqd2_blocks(n - kb, 256)), dim3(256), 0, st, n, A, k0, kb, d_ipiv);
+    float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB;
+    hipLaunchKernelGGL(k_ctrtri, dim3(2), dim3(NB), 0, st, n, A, k0, kb, Tk);
+    BQD2_HIP(hipGetLastError());
+    const int rest = n - k0 - kb;
+    if (rest > 0) {
+      // U12 = L11^-1 A12 in place (one row tile: each workgroup reads only its columns)
+      float2 *A12 = A + (size_t)(k0 + kb) * n + k0;
+      int rc = cgemm(true, kb, rest, kb, Tk, NB, A12, n, A12, n, st);
+      if (!rc) rc = cgemm(false, rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n, A12, n,
+                          A + (size_t)(k0 + kb) * n + k0 + kb, n, st);
+      if (rc) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_perm, dim3(1), dim3(64), 0, st, n, d_ipiv, W.perm);
+  int h_flag = n;
+  BQD2_HIP(hipMemcpyAsync(&h_flag, W.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  BQD2_HIP(hipStreamSynchronize(st));
+  BQD2_CHECK(h_flag == n, BQD2_E_SINGULAR, "bqd2_cgetrf: zero pivot at column %d", h_flag);
+  return BQD2_OK;
+}
+
+// X (n x nrhs, column-major, leading dimension ldb) <- U^-1 L^-1 P X
+static int cgetrs_perm(int n, int nrhs, const float2 *A, const LuWs &W, float2 *X, long long ldx, hipStream_t st) {
+  // forward: L y = b, by diagonal blocks
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    const float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB;
+    int rc = small_update(true, kb, kb, nrhs, Tk, NB, X + k0, ldx, X + k0, ldx, st);
+    if (!rc) rc = small_update(false, n - k0 - kb, kb, nrhs, A + (size_t)k0 * n + k0 + kb, n, X + k0, ldx,
+                               X + k0 + kb, ldx, st);
+    if (rc) return rc;
+  }
+  // backward: U x = y
+  for (int k0 = ((n - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    const float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB + NB * NB;
+    int rc = small_update(true, kb, kb, nrhs, Tk, NB, X + k0, ldx, X + k0, ldx, st);
+    if (!rc) rc = small_update(false, k0, kb, nrhs, A + (size_t)k0 * n, n, X + k0, ldx, X, ldx, st);
+    if (rc) return rc;
+  }
+  return BQD2_OK;
+}
+
+// complex float right-hand sides: out[i] = in[perm[i]]
+__global__ void __launch_bounds__(256) k_gather_f(int n, const int *__restrict__ perm, const float2 *__restrict__ in,
+                                                  long long ldi, float2 *__restrict__ out, long long ldo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (i >= n) return;
+  out[(size_t)s * ldo + i] = in[(size_t)s * ldi + perm[i]];
+}
+
+extern "C" int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B, float *d_tmp,
+                           void *stream) {
+  BQD2_CHECK(n > 0 && nrhs > 0 && d_A && d_ws && d_B && d_tmp, BQD2_E_ARG, "bqd2_cgetrs: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  LuWs W = lu_ws(n, (void *)d_ws);
+  float2 *B = (float2 *)d_B, *T = (float2 *)d_tmp;
+  hipLaunchKernelGGL(k_gather_f, dim3(bqd2_blocks(n, 256), nrhs), dim3(256), 0, st, n, W.perm, B, (long long)n, T,
+                     (long long)n);
+  int rc = cgetrs_perm(n, nrhs, (const float2 *)d_A, W, T, n, st);
+  if (rc) return rc;
+  BQD2_HIP(hipMemcpyAsync(B, T, (size_t)n * nrhs * sizeof(float2), hipMemcpyDeviceToDevice, st));
+  return BQD2_OK;
+}
+
+// ---------------------------------------------------------------- FP64 <-> FP32 for the refinement
+// Af (column-major complex float) = rows of the row-major complex double A, each
+// scaled by dinv[r]; 32 x 32 tiles through LDS so that reads and writes coalesce.
+__global__ void __launch_bounds__(256) k_rm64_to_cm32(int n, const double *__restrict__ A, long long lda,
+                                                      const double *__restrict__ dinv, float2 *__restrict__ Af) {
+  __shared__ float2 t[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (r < n && c < n) {
+      const double *a = A + 2 * ((size_t)r * lda + c);
+      const double dr = dinv ? dinv[2 * r] : 1.0, di = dinv ? dinv[2 * r + 1] : 0.0;
+      v = make_float2((float)(a[0] * dr - a[1] * di), (float)(a[0] * di + a[1] * dr));
+    }
+    t[i][tx] = v;
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int c = c0 + i, r = r0 + tx;
+    if (r < n && c < n) Af[(size_t)c * n + r] = t[tx][i];
+  }
+}
+
+// x (complex double, nrhs columns of leading dimension ld) -> P x in complex float
+__global__ void __launch_bounds__(256) k_gather_d2f(int n, const int *__restrict__ perm, const double *__restrict__ x,
+                                                    long long ldx, float2 *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (i >= n) return;
+  const double *v = x + 2 * ((size_t)s * ldx + perm[i]);
+  y[(size_t)s * n + i] = make_float2((float)v[0], (float)v[1]);
+}
+
+__global__ void __launch_bounds__(256) k_f2d(int n, const float2 *__restrict__ y, double *__restrict__ x,
+                                             long long ldx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (i >= n) return;
+  const float2 v = y[(size_t)s * n + i];
+  double *o = x + 2 * ((size_t)s * ldx + i);
+  o[0] = v.x;
+  o[1] = v.y;
+}
+
+// round the row-scaled system to complex float, column-major
+int bqd2_lu32_convert(int n, const double *A, long long lda, const double *dinv, float *Af, hipStream_t st) {
+  dim3 g(bqd2_blocks(n, 32), bqd2_blocks(n, 32));
+  hipLaunchKernelGGL(k_rm64_to_cm32, g, dim3(256), 0, st, n, A, lda, dinv, (float2 *)Af);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
+// out = M^-1 in for complex double vectors (nrhs columns, leading dimension ld),
+// M = P^T L U the complex float factors; tmp holds n x nrhs complex floats
+int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const double *in, long long ldi, double *out,
+                    long long ldo, float *tmp, hipStream_t st) {
+  LuWs W = lu_ws(n, (void *)ws);
+  float2 *T = (float2 *)tmp;
+  dim3 g(bqd2_blocks(n, 256), nrhs);
+  hipLaunchKernelGGL(k_gather_d2f, g, dim3(256), 0, st, n, W.perm, in, ldi, T);
+  int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, n, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_f2d, g, dim3(256), 0, st, n, T, out, ldo);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, const float *d_B,
+                          long long ldb, float *d_C, long long ldc, void *stream) {
+  BQD2_CHECK(m >= 0 && nc >= 0 && kk >= 0 && d_A && d_B && d_C, BQD2_E_ARG, "bqd2_cgemm: bad arguments");
+  return cgemm(set != 0, m, nc, kk, (const float2 *)d_A, lda, (const float2 *)d_B, ldb, (float2 *)d_C, ldc,
+               (hipStream_t)stream);
+}

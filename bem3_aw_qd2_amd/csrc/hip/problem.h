@@ -80,7 +80,16 @@ struct bqd2_problem {
   int *ipiv;
   int na;
   bool solved;
-  int solver;              // 0: GMRES, LU when it misses the tolerance; 1: LU only
+  int solver;              // 0: GMRES, LU when it misses the tolerance; 1: LU only;
+                           // 2: complex float LU (MFMA) + FP64 GMRES refinement
+  // mixed-precision factorisation of the row-scaled system (solver 2); valid
+  // until the next assembly
+  float *Af;               // na x na complex float, column-major
+  void *luws;              // bqd2_cgetrf_wsize(na) bytes
+  float *lutmp;            // na x 3 complex float
+  double *dinv;            // na complex: inverse diagonal of the system
+  bool lu32_valid;
+  double ms_factor;        // time of the last factorisation (conversion + LU)
   double *hk2d;            // k2[m] / k2[s] table (16 x 16 complex), set once
   double *xg;              // GMRES solution of the system (na)
   double *tpv;             // open-domain node velocities (NN x 3)
@@ -97,11 +106,29 @@ struct bqd2_problem {
 };
 
 int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);
+
+// Right preconditioner of GMRES: the complex float LU factors (lu32.hip) of the
+// row-scaled matrix D A (D = the inverse diagonal), so that D A M^-1 is close to
+// the identity.  tmp: n x nrhs complex floats.
+struct Precond {
+  const float *Af;
+  const void *ws;
+  float *tmp;
+};
 // restarted GMRES on nrhs <= 3 right-hand sides sharing the row-major matrix A
-// (gmres.hip); returns 0 when the scaled residual is below tol, 1 when not (the
-// caller falls back to the LU), < 0 on errors
+// (gmres.hip), right-preconditioned by pc when it is not null; returns 0 when
+// the scaled residual is below tol, 1 when not (the caller falls back to
+// another solver), < 0 on errors
 int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, const int *map, const double *B,
-               long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres);
+               long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres,
+               const Precond *pc = nullptr);
+// lu32.hip: the row-scaled row-major complex double matrix rounded to column-major
+// complex float; out = M^-1 in with the factors (in may be out)
+int bqd2_lu32_convert(int n, const double *A, long long lda, const double *dinv, float *Af, hipStream_t st);
+int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const double *in, long long ldi, double *out,
+                    long long ldo, float *tmp, hipStream_t st);
+// gmres.hip: dinv[r] = 1 / A[r][r] (complex), 1 on rows with map[r] < 0
+int bqd2_jacobi(int n, const double *A, long long lda, const int *map, double *dinv, hipStream_t st);
 // the calling thread's context (nullptr on failure, error set)
 CallCtx *bqd2_call_ctx(bqd2_problem *p);
 // at least `bytes` of device workspace at *base (previous contents are lost)

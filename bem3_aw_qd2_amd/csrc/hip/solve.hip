@@ -343,8 +343,49 @@ static int build_system(bqd2_problem *p, int rm, hipStream_t st) {
 }
 
 extern "C" int bqd2_set_solver(bqd2_problem *p, int mode) {
-  BQD2_CHECK(p && (mode == 0 || mode == 1), BQD2_E_ARG, "bqd2_set_solver: mode must be 0 (GMRES, LU fallback) or 1 (LU)");
+  BQD2_CHECK(p && mode >= 0 && mode <= 2, BQD2_E_ARG,
+             "bqd2_set_solver: mode must be 0 (GMRES, LU fallback), 1 (LU) or 2 (complex float LU + GMRES refinement)");
   p->solver = mode;
+  return BQD2_OK;
+}
+
+// GMRES steps and restart length of the refinement (each step is one FP64
+// matrix-vector product and one application of the float factors)
+#define IR_RESTART 30
+#define IR_MAXIT 60
+
+// Complex float LU of the row-scaled row-major system in p->A, once per
+// assembly; the factors stay on the device for every later right-hand side.
+static int factor_lu32(bqd2_problem *p, hipStream_t st) {
+  const int na = p->na;
+  int rc;
+  if (!p->Af) {
+    if ((rc = bqd2_alloc(p, (void **)&p->Af, (size_t)na * na * 8))) return rc;
+    if ((rc = bqd2_alloc(p, &p->luws, bqd2_cgetrf_wsize(na)))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->lutmp, (size_t)na * 3 * 8))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->dinv, (size_t)na * 16))) return rc;
+  }
+  hipEvent_t e0, e1;
+  BQD2_HIP(hipEventCreate(&e0));
+  BQD2_HIP(hipEventCreate(&e1));
+  BQD2_HIP(hipEventRecord(e0, st));
+  if ((rc = bqd2_jacobi(na, p->A, na, nullptr, p->dinv, st))) return rc;
+  if ((rc = bqd2_lu32_convert(na, p->A, na, p->dinv, p->Af, st))) return rc;
+  if ((rc = bqd2_cgetrf(na, p->Af, p->ipiv, p->luws, st))) return rc;
+  BQD2_HIP(hipEventRecord(e1, st));
+  BQD2_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  BQD2_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->ms_factor = ms;
+  p->lu32_valid = true;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_lu32_info(bqd2_problem *p, double *ms_factor) {
+  BQD2_CHECK(p && ms_factor, BQD2_E_ARG, "bqd2_lu32_info: null argument");
+  *ms_factor = p->lu32_valid ? p->ms_factor : 0.0;
   return BQD2_OK;
 }
 
@@ -375,7 +416,18 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
   p->info_lu = 0;
   for (int i = 0; i < 17; i++) { p->info_its[i] = 0; p->info_res[i] = 0.0; }
   const double *x = nullptr;
-  if (p->solver == 0) {
+  if (p->solver == 2) {
+    // complex float LU of the scaled system, then GMRES on the FP64 system
+    // right-preconditioned by it; plain GMRES and the FP64 LU remain as fallbacks
+    if ((rc = build_system(p, 1, st))) return rc;
+    if (!p->lu32_valid && (rc = factor_lu32(p, st))) return rc;
+    Precond pc{p->Af, p->luws, p->lutmp};
+    rc = bqd2_gmres(&p->main, na, 1, p->A, na, nullptr, p->b, na, p->xg, na, GM_TOL, IR_RESTART, IR_MAXIT,
+                    &p->info_its[0], &p->info_res[0], &pc);
+    if (rc < 0) return rc;
+    if (rc == 0) x = p->xg;
+  }
+  if (!x && p->solver != 1) {
     // Jacobi-scaled GMRES on the row-major system
     if ((rc = build_system(p, 1, st))) return rc;
     rc = bqd2_gmres(&p->main, na, 1, p->A, na, nullptr, p->b, na, p->xg, na, GM_TOL, GM_RESTART, GM_MAXIT,
@@ -464,7 +516,7 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
     bool done = false;
-    if (p->solver == 0) {
+    if (p->solver != 1) {
       double *B = p->dpvw, *X = p->dpvw + 2 * (size_t)3 * D.N;
       hipLaunchKernelGGL(k_dpv_B_slots, dim3(D.N), dim3(64), 0, st, D, B);
       BQD2_HIP(hipGetLastError());

@@ -96,9 +96,11 @@ def grid_points(n, half, planes=1):
 class Request:
     """One solve request on a device-resident problem through the C ABI."""
 
-    def __init__(self, hip, prob, mn, pts):
-        from bem3_aw_qd2_amd import AssemblyStats
+    def __init__(self, hip, prob, mn, pts, mode=0):
+        from bem3_aw_qd2_amd import AssemblyStats, check
         self.hip, self.p, self.mn = hip, prob, mn
+        self.mode = mode
+        check(hip.bqd2_set_solver(prob, mode), "bqd2_set_solver")
         self.pts = pts
         self.n = len(pts)
         self.out = np.zeros((self.n, 8))
@@ -137,8 +139,11 @@ class Request:
         res = (C.c_double * 17)()
         lu = C.c_int()
         check(hip.bqd2_solver_info(p, its, res, C.byref(lu)), "bqd2_solver_info")
-        self.solver = {"krylov_steps": list(its[:self.mn + 2]), "scaled_residual": list(res[:self.mn + 2]),
-                       "lu_fallbacks": lu.value}
+        ms = C.c_double()
+        check(hip.bqd2_lu32_info(p, C.byref(ms)), "bqd2_lu32_info")
+        self.solver = {"mode": self.mode, "krylov_steps": list(its[:self.mn + 2]),
+                       "scaled_residual": list(res[:self.mn + 2]), "lu_fallbacks": lu.value,
+                       "lu32_factor_ms": ms.value}
         t.update(assemble=t1 - t0, solve=t2 - t1, velocity=t3 - t2, field=t4 - t3, force=t5 - t4, total=t5 - t0)
         return t
 
@@ -251,6 +256,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0, help="field grid per plane (default 64, 256 for sphere3456)")
     ap.add_argument("--planes", type=int, default=0, help="field planes y=0, z=0, x=0 (default 1, 3 for sphere3456)")
     ap.add_argument("--triangles", action="store_true", help="the linear-triangle mesh of the workload")
+    ap.add_argument("--solver", type=int, default=0,
+                    help="0: FP64 GMRES (default), 1: FP64 LU, 2: complex float LU on MFMA + FP64 GMRES refinement")
     ap.add_argument("--cpu-pairs", type=int, default=384)
     ap.add_argument("--cpu-self", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -280,7 +287,7 @@ def main():
         model = exdat.Model(host).read_inputs(*paths, out=os.path.join(tmp, "ex.dat")).initialize()
     prob = host.aqd2_problem(model.buf, None)
     lam = cfg["host"][1] / cfg["f"]
-    req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam, args.planes))
+    req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam, args.planes), args.solver)
     log("bench: workload %s, %d elements, %d unknowns" % (args.workload, model.ne(), 2 * model.nn()))
 
     if args.copy_mib:

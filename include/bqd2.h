@@ -148,7 +148,11 @@ int bqd2_solve_surface(bqd2_problem *p, double *x);
 int bqd2_solve_velocity(bqd2_problem *p);
 
 /* No counterpart in the reference.  Solver of bqd2_solve_surface and
- * bqd2_solve_velocity: 0 GMRES with the LU fallback (default), 1 LU only. */
+ * bqd2_solve_velocity: 0 GMRES with the LU fallback (default), 1 LU only,
+ * 2 mixed precision: the row-scaled system factored once per assembly in
+ * complex float on the MFMA units (bqd2_cgetrf) and used as the right
+ * preconditioner of GMRES on the FP64 system, which refines the solution to the
+ * same scaled residual as mode 0 (the dpv systems run as in mode 0). */
 int bqd2_set_solver(bqd2_problem *p, int mode);
 
 /* No counterpart in the reference.  Krylov steps and final scaled residuals of
@@ -188,6 +192,25 @@ int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F, double *N
  * factors, ipiv (device, n ints) receives the pivots; B (n x nrhs) by X. */
 int bqd2_zgetrf(int n, double *d_A, int *d_ipiv, void *stream);
 int bqd2_zgetrs(int n, int nrhs, const double *d_A, const int *d_ipiv, double *d_B, void *stream);
+
+/* Complex float LU with partial pivoting on the MFMA units, the factorisation of
+ * the mixed-precision solver (bqd2_set_solver(p, 2)); replaces the same MKL
+ * solves as bqd2_zgetrf.  Column-major, leading dimension n, (re, im) floats.
+ * d_ws: device workspace of bqd2_cgetrf_wsize(n) bytes that keeps the inverted
+ * diagonal blocks and the row permutation for bqd2_cgetrs.  bqd2_cgetrs solves
+ * in place on d_B (n x nrhs); d_tmp holds n x nrhs complex floats. */
+size_t bqd2_cgetrf_wsize(int n);
+int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream);
+int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B, float *d_tmp, void *stream);
+/* The trailing-update kernel of bqd2_cgetrf, exposed for tests: C -= A B
+ * (set = 0) or C = A B (set = 1), column-major complex float, A m x kk,
+ * B kk x nc, C m x nc. */
+int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, const float *d_B, long long ldb,
+               float *d_C, long long ldc, void *stream);
+/* No counterpart in the reference.  Milliseconds of the last complex float
+ * factorisation of the system (row scaling, rounding and bqd2_cgetrf), 0 when
+ * none has run since the last assembly. */
+int bqd2_lu32_info(bqd2_problem *p, double *ms_factor);
 
 #ifdef __cplusplus
 }
