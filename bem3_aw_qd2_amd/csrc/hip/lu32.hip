@@ -17,19 +17,22 @@
 //             candidate for the next column.  Launch boundaries are the only
 //             grid-wide synchronisation.
 //   swaps     the panel's interchanges on the other columns (k_cswap_cols)
-//   inverses  of the diagonal block's unit-lower and upper factors (k_ctrtri),
-//             kept for the solves
-//   U12       = L11^-1 A12, a GEMM with the explicit inverse
+//   U12       = L11^-1 A12 (k_ctrsm_l)
 //   A22      -= L21 U12, the O(n^3) part: k_cgemm on v_mfma_f32_32x32x2_f32
-//             with FP32 accumulation.
+//             with FP32 accumulation
+// After the last panel: inverses of the diagonal blocks' unit-lower (k_clinv)
+// and upper (k_ctrtri_u) factors, kept for the solves.  A look-ahead (the next
+// panel on a second stream during the update) gains nothing here: the update
+// occupies every CU and the panel's launches then wait for free slots.
 // Solves (bqd2_cgetrs): permutation, then per diagonal block x_k = T_k b_k and
 // the update of the remaining rows, forward with L and backward with U; GEMV
 // kernels for up to 4 right-hand sides, the MFMA GEMM beyond.
 #include <vector>
 #include "common.h"
 
-#define NB 128     // panel width = K of the trailing update
-#define GMAXW 256  // panel workgroups at most
+#define NB 128      // panel width = K of the trailing update
+#define GMAXW 1024  // panel workgroups at most
+#define PROWS 32    // panel rows per workgroup (more when GMAXW workgroups do not cover the rows)
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -83,81 +86,17 @@ extern "C" size_t bqd2_cgetrf_wsize(int n) {
 // ---------------------------------------------------------------- panel
 // Launch for column j of the panel [k0, k0 + kb) (j = k0 - 1: the first
 // candidates only).  Workgroup w owns rows [k0 + w chunk, k0 + (w+1) chunk).
+// Lane (i, g) = (tid % 32, tid / 32) holds, for each block of 32 of those rows,
+// row i of the block in the columns cf + g + 8 u, u < 16 (cf = j): the whole
+// remaining panel row in registers.  The first block is loaded before the pivot
+// is known, so that its latency overlaps the reduction of the candidates.
 // Reads the candidates of parity (j - k0) & 1, writes those of the other.  A
 // column whose candidates are all NaN keeps its row (p = j) and is flagged.
+#define PEL (NB / 8)
 __device__ __forceinline__ bool better(float v, int r, float bv, int br) { return v > bv || (v == bv && r < br); }
 
-__global__ void __launch_bounds__(256) k_cstep(int n, float2 *__restrict__ A, int k0, int kb, int j, int chunk,
-                                               int nw, int *__restrict__ ipiv, LuWs W) {
-  __shared__ float2 su[NB];
-  __shared__ float sv[4];
-  __shared__ int sr[4];
-  const int tid = threadIdx.x, w = blockIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int r0 = k0 + w * chunk, r1 = min(n, r0 + chunk);
-  const size_t ld = (size_t)n;
-  if (j >= k0) {
-    const int par = (j - k0) & 1;
-    // pivot: the best of the workgroups' candidates (ties: the lowest row)
-    float bv = -1.0f;
-    int br = 0x7fffffff;
-    for (int q = tid; q < nw; q += 256) {
-      const float2 s = W.slot[par * GMAXW + q];
-      const int r = __float_as_int(s.y);
-      if (better(s.x, r, bv, br)) { bv = s.x; br = r; }
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-      const float ov = __shfl_xor(bv, off, 64);
-      const int orr = __shfl_xor(br, off, 64);
-      if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
-    }
-    if (lane == 0) { sv[wv] = bv; sr[wv] = br; }
-    __syncthreads();
-    bv = sv[0]; br = sr[0];
-    for (int q = 1; q < 4; q++)
-      if (better(sv[q], sr[q], bv, br)) { bv = sv[q]; br = sr[q]; }
-    const bool none = (br == 0x7fffffff);
-    const int p = none ? j : br;
-    const float2 *u = none ? W.rowj + par * NB : W.cand + ((size_t)par * GMAXW + (p - k0) / chunk) * NB;
-    for (int c = tid; c < kb; c += 256) su[c] = u[c];
-    if (w == 0 && tid == 0) ipiv[j] = p;
-    // interchange rows j and p over the panel's columns
-    if (p != j) {
-      const float2 *rj = W.rowj + par * NB;
-      if (p >= r0 && p < r1)
-        for (int c = tid; c < kb; c += 256) A[(size_t)(k0 + c) * ld + p] = rj[c];
-      if (j >= r0 && j < r1)
-        for (int c = tid; c < kb; c += 256) A[(size_t)(k0 + c) * ld + j] = u[c];
-    }
-    __syncthreads();
-    const float2 piv = su[j - k0];
-    const bool zero = none || (piv.x == 0.0f && piv.y == 0.0f);
-    if (zero && w == 0 && tid == 0) atomicMin(W.flag, j);
-    const float2 ip = zero ? make_float2(0.0f, 0.0f) : crecipf(piv);
-    // scale the column below the diagonal and update the panel's later columns
-    for (int r = max(r0, j + 1) + tid; r < r1; r += 256) {
-      float2 *a = A + r;
-      const float2 l = cmulf(a[(size_t)j * ld], ip);
-      a[(size_t)j * ld] = l;
-      for (int c = j + 1; c < k0 + kb; c++) {
-        const float2 uc = su[c - k0];
-        float2 v = a[(size_t)c * ld];
-        v.x -= fmaf(l.x, uc.x, -l.y * uc.y);
-        v.y -= fmaf(l.x, uc.y, l.y * uc.x);
-        a[(size_t)c * ld] = v;
-      }
-    }
-  }
-  // candidate of this workgroup for column j + 1
-  const int jn = j + 1;
-  if (jn >= k0 + kb) return;
-  const int parn = (jn - k0) & 1;
-  float bv = -1.0f;
-  int br = 0x7fffffff;
-  for (int r = max(r0, jn) + tid; r < r1; r += 256) {
-    const float2 a = A[(size_t)jn * ld + r];
-    const float v = fabsf(a.x) + fabsf(a.y);
-    if (better(v, r, bv, br)) { bv = v; br = r; }
-  }
+__device__ __forceinline__ void block_best(float &bv, int &br, float *sv, int *sr) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int off = 32; off >= 1; off >>= 1) {
     const float ov = __shfl_xor(bv, off, 64);
     const int orr = __shfl_xor(br, off, 64);
@@ -169,6 +108,111 @@ __global__ void __launch_bounds__(256) k_cstep(int n, float2 *__restrict__ A, in
   bv = sv[0]; br = sr[0];
   for (int q = 1; q < 4; q++)
     if (better(sv[q], sr[q], bv, br)) { bv = sv[q]; br = sr[q]; }
+}
+
+__global__ void __launch_bounds__(256) k_cstep(int n, float2 *__restrict__ A, int k0, int kb, int j, int chunk,
+                                               int nw, int *__restrict__ ipiv, LuWs W) {
+  __shared__ float2 su[NB];     // pivot row over the panel's columns
+  __shared__ float2 sj[NB];     // row j before the interchange
+  __shared__ float2 sl[32];     // multipliers of the current block of rows
+  __shared__ float sv[4];
+  __shared__ int sr[4];
+  const int tid = threadIdx.x, w = blockIdx.x;
+  const int li = tid & 31, lg = tid >> 5;
+  const int r0 = k0 + w * chunk, r1 = min(n, r0 + chunk);
+  const size_t ld = (size_t)n;
+  const int kend = k0 + kb;
+  const bool step = j >= k0;
+  // rows [rs, r1) in blocks of 32, columns [cf, cend)
+  const int rs = step ? max(r0, j + 1) : r0;
+  const int cf = step ? j : k0;
+  const int cend = step ? kend : k0 + 1;
+  const int jn = j + 1;
+  float2 v[PEL];
+  auto load = [&](int rb) {
+    const int r = rb + li;
+#pragma unroll
+    for (int u = 0; u < PEL; u++) {
+      const int c = cf + lg + 8 * u;
+      if (r < r1 && c < cend) v[u] = A[(size_t)c * ld + r];
+    }
+  };
+  load(rs);
+  int p = j;
+  float2 ip = make_float2(0.0f, 0.0f);
+  if (step) {
+    const int par = (j - k0) & 1;
+    // pivot: the best of the workgroups' candidates (ties: the lowest row)
+    float bv = -1.0f;
+    int br = 0x7fffffff;
+    for (int q = tid; q < nw; q += 256) {
+      const float2 sc = W.slot[par * GMAXW + q];
+      const int r = __float_as_int(sc.y);
+      if (better(sc.x, r, bv, br)) { bv = sc.x; br = r; }
+    }
+    block_best(bv, br, sv, sr);
+    const bool none = (br == 0x7fffffff);
+    p = none ? j : br;
+    const float2 *rj = W.rowj + par * NB;
+    const float2 *prow = none ? rj : W.cand + ((size_t)par * GMAXW + (p - k0) / chunk) * NB;
+    for (int c = tid; c < kb; c += 256) { su[c] = prow[c]; sj[c] = rj[c]; }
+    if (w == 0 && tid == 0) ipiv[j] = p;
+    // interchange rows j and p over the panel's columns
+    if (p != j) {
+      if (p >= r0 && p < r1)
+        for (int c = tid; c < kb; c += 256) A[(size_t)(k0 + c) * ld + p] = rj[c];
+      if (j >= r0 && j < r1)
+        for (int c = tid; c < kb; c += 256) A[(size_t)(k0 + c) * ld + j] = prow[c];
+    }
+    __syncthreads();
+    const float2 piv = su[j - k0];
+    const bool zero = none || (piv.x == 0.0f && piv.y == 0.0f);
+    if (zero && w == 0 && tid == 0) atomicMin(W.flag, j);
+    ip = zero ? make_float2(0.0f, 0.0f) : crecipf(piv);
+  }
+  float bv = -1.0f;
+  int br = 0x7fffffff;
+  for (int rb = rs; rb < r1; rb += 32) {
+    if (rb != rs) load(rb);
+    const int r = rb + li;
+    const bool on = r < r1;
+    if (step) {
+      // row p now holds row j
+      if (on && r == p) {
+#pragma unroll
+        for (int u = 0; u < PEL; u++) {
+          const int c = cf + lg + 8 * u;
+          if (c < cend) v[u] = sj[c - k0];
+        }
+      }
+      // multiplier of row r (column j: lane group 0, u = 0)
+      if (lg == 0 && on) {
+        v[0] = cmulf(v[0], ip);
+        sl[li] = v[0];
+        A[(size_t)j * ld + r] = v[0];
+      }
+      __syncthreads();
+      const float2 l = sl[li];
+#pragma unroll
+      for (int u = 0; u < PEL; u++) {
+        const int c = cf + lg + 8 * u;
+        if (!on || c == j || c >= cend) continue;
+        const float2 uc = su[c - k0];
+        v[u].x -= fmaf(l.x, uc.x, -l.y * uc.y);
+        v[u].y -= fmaf(l.x, uc.y, l.y * uc.x);
+        A[(size_t)c * ld + r] = v[u];
+      }
+      __syncthreads();  // sl is rewritten by the next block
+    }
+    // candidate for column jn: lane group 1 (step) or 0 (first launch), u = 0
+    if (on && lg == (step ? 1 : 0) && jn < kend) {
+      const float a = fabsf(v[0].x) + fabsf(v[0].y);
+      if (better(a, r, bv, br)) { bv = a; br = r; }
+    }
+  }
+  if (jn >= kend) return;
+  const int parn = (jn - k0) & 1;
+  block_best(bv, br, sv, sr);  // its barriers also order this workgroup's stores above
   if (tid == 0) W.slot[parn * GMAXW + w] = make_float2(bv, __int_as_float(br));
   float2 *cd = W.cand + ((size_t)parn * GMAXW + w) * NB;
   if (br != 0x7fffffff)
@@ -193,48 +237,113 @@ __global__ void __launch_bounds__(256) k_cswap_cols(int n, float2 *__restrict__ 
   }
 }
 
-// Inverses of the diagonal block D = A[k0.., k0..] (kb x kb): block 0 the
-// inverse of its unit lower part, block 1 of its upper part, written to
-// T (ldT = NB).  One lane per column of the inverse; D staged in LDS.
-__global__ void __launch_bounds__(NB) k_ctrtri(int n, const float2 *__restrict__ A, int k0, int kb,
-                                               float2 *__restrict__ T) {
+// X <- L^-1 X for the ncols columns of X (column c at X + c ldx, rows 0..kb-1),
+// L the unit lower part of the kb x kb block at Ld (leading dimension n): the
+// inverse of a panel's L11 (X the identity).  One lane per column, 256 columns
+// per workgroup.  The strictly lower part of L is packed row by row in LDS
+// ((r, q), q < r, at r (r-1)/2 + q; every lane reads the same entry, a
+// broadcast).  The rows go in blocks of 32 held in registers: the block's
+// triangle is solved in registers, then each later row of the lane's column
+// takes one 32-term dot product with it.
+#define TB 32
+__device__ __forceinline__ void trsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *Xall, long long ldx,
+                                       int ncols, int c, float2 *sL) {
+  const int tid = threadIdx.x;
+  for (int q = 0; q + 1 < kb; q++)
+    for (int r = q + 1 + tid; r < kb; r += 256) sL[r * (r - 1) / 2 + q] = Ld[(size_t)q * n + r];
+  __syncthreads();
+  if (c >= ncols) return;
+  float2 *X = Xall + (size_t)c * ldx;
+  for (int b0 = 0; b0 < kb; b0 += TB) {
+    float2 x[TB];
+#pragma unroll
+    for (int t = 0; t < TB; t++) x[t] = (b0 + t < kb) ? X[b0 + t] : make_float2(0.0f, 0.0f);
+    // the block's triangle
+#pragma unroll
+    for (int t = 1; t < TB; t++) {
+      if (b0 + t >= kb) break;
+      const float2 *Lr = sL + (b0 + t) * (b0 + t - 1) / 2 + b0;
+      float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int q = 0; q < t; q++) {
+        const float2 l = Lr[q];
+        acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
+        acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
+      }
+      x[t].x -= acc.x;
+      x[t].y -= acc.y;
+    }
+#pragma unroll
+    for (int t = 0; t < TB; t++)
+      if (b0 + t < kb) X[b0 + t] = x[t];
+    // later rows: x_r -= L[r][b0:b0+TB] x_block
+    for (int r = b0 + TB; r < kb; r++) {
+      const float2 *Lr = sL + r * (r - 1) / 2 + b0;
+      float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int q = 0; q < TB; q++) {
+        const float2 l = Lr[q];
+        acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
+        acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
+      }
+      float2 y = X[r];
+      y.x -= acc.x;
+      y.y -= acc.y;
+      X[r] = y;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ctrsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *X,
+                                                 long long ldx, int ncols) {
+  __shared__ float2 sL[NB * (NB - 1) / 2];
+  trsm_l(n, Ld, kb, X, ldx, ncols, blockIdx.x * 256 + threadIdx.x, sL);
+}
+
+// L11^-1 of every diagonal block (block blockIdx.x) into T + 2 NB NB blockIdx.x,
+// which holds the identity: the solve's forward-sweep blocks
+__global__ void __launch_bounds__(256) k_clinv(int n, const float2 *__restrict__ A, float2 *T) {
+  __shared__ float2 sL[NB * (NB - 1) / 2];
+  const int k0 = blockIdx.x * NB, kb = min(NB, n - k0);
+  trsm_l(n, A + (size_t)k0 * n + k0, kb, T + (size_t)blockIdx.x * 2 * NB * NB, (long long)NB, kb, threadIdx.x, sL);
+}
+
+// Inverse of the upper factor of the diagonal block k0 = NB blockIdx.x (kb x kb),
+// after the factorisation (a block is final once its panel is done: later
+// interchanges touch only rows below it), written to T + (2 blockIdx.x + 1) NB NB
+// (ldT = NB).  One lane per column of the inverse; the block staged in LDS.
+__global__ void __launch_bounds__(NB) k_ctrtri_u(int n, const float2 *__restrict__ A, float2 *__restrict__ Tall) {
   __shared__ float2 sD[NB * NB];  // column-major (r, q) at q * NB + r
   const int c = threadIdx.x;
+  const int k0 = blockIdx.x * NB, kb = min(NB, n - k0);
   for (int e = c; e < kb * kb; e += NB) {
     const int r = e % kb, q = e / kb;
     sD[q * NB + r] = A[(size_t)(k0 + q) * n + k0 + r];
   }
   __syncthreads();
-  float2 *X = T + (size_t)blockIdx.x * NB * NB + (size_t)c * NB;  // column c of the inverse
   if (c >= kb) return;
-  if (blockIdx.x == 0) {
-    // L X = I, L unit lower: X[r][c] = -sum_{c <= q < r} L[r][q] X[q][c]
-    for (int r = 0; r < c; r++) X[r] = make_float2(0.0f, 0.0f);
-    X[c] = make_float2(1.0f, 0.0f);
-    for (int r = c + 1; r < kb; r++) {
-      float2 s = make_float2(0.0f, 0.0f);
-      for (int q = c; q < r; q++) {
-        const float2 l = sD[q * NB + r], x = X[q];
-        s.x += fmaf(l.x, x.x, -l.y * x.y);
-        s.y += fmaf(l.x, x.y, l.y * x.x);
-      }
-      X[r] = make_float2(-s.x, -s.y);
+  float2 *X = Tall + (size_t)(2 * blockIdx.x + 1) * NB * NB + (size_t)c * NB;  // column c of the inverse
+  // U X = I: X[c][c] = 1 / U[c][c], X[r][c] = -(sum_{r < q <= c} U[r][q] X[q][c]) / U[r][r]
+  for (int r = c + 1; r < kb; r++) X[r] = make_float2(0.0f, 0.0f);
+  X[c] = crecipf(sD[c * NB + c]);
+  for (int r = c - 1; r >= 0; r--) {
+    float2 acc = make_float2(0.0f, 0.0f);
+    for (int q = r + 1; q <= c; q++) {
+      const float2 u = sD[q * NB + r], x = X[q];
+      acc.x += fmaf(u.x, x.x, -u.y * x.y);
+      acc.y += fmaf(u.x, x.y, u.y * x.x);
     }
-  } else {
-    // U X = I: X[c][c] = 1 / U[c][c], X[r][c] = -(sum_{r < q <= c} U[r][q] X[q][c]) / U[r][r]
-    for (int r = c + 1; r < kb; r++) X[r] = make_float2(0.0f, 0.0f);
-    X[c] = crecipf(sD[c * NB + c]);
-    for (int r = c - 1; r >= 0; r--) {
-      float2 s = make_float2(0.0f, 0.0f);
-      for (int q = r + 1; q <= c; q++) {
-        const float2 u = sD[q * NB + r], x = X[q];
-        s.x += fmaf(u.x, x.x, -u.y * x.y);
-        s.y += fmaf(u.x, x.y, u.y * x.x);
-      }
-      const float2 v = cmulf(s, crecipf(sD[r * NB + r]));
-      X[r] = make_float2(-v.x, -v.y);
-    }
+    const float2 v = cmulf(acc, crecipf(sD[r * NB + r]));
+    X[r] = make_float2(-v.x, -v.y);
   }
+}
+
+// the L blocks of T (block k at T + 2 NB NB k) set to the identity
+__global__ void __launch_bounds__(256) k_ceye(int nbk, float2 *__restrict__ T) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)nbk * NB * NB) return;
+  const int r = e % NB, c = (e / NB) % NB;
+  T[(e / (NB * NB)) * 2 * NB * NB + e % (NB * NB)] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- MFMA GEMM
@@ -250,11 +359,11 @@ __global__ void __launch_bounds__(NB) k_ctrtri(int n, const float2 *__restrict__
 // register store 32 consecutive rows of one column.  B may be C when m <= 128:
 // a workgroup then reads only the columns it writes, and writes after its last read.
 #define GT 128
-#define GK 32
+#define GK 16
 #define GPAD 1
 template <int SET>
-__global__ void __launch_bounds__(256) k_cgemm(int m, int nc, int kk, const float2 *__restrict__ A, long long lda,
-                                               const float2 *B, long long ldb, float2 *C, long long ldc) {
+__global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const float2 *__restrict__ A, long long lda,
+                                                  const float2 *B, long long ldb, float2 *C, long long ldc) {
   __shared__ float2 sA[GK][GT + GPAD];  // (k, row)
   __shared__ float2 sB[GK][GT + GPAD];  // (k, col)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -262,6 +371,25 @@ __global__ void __launch_bounds__(256) k_cgemm(int m, int nc, int kk, const floa
   const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
   const int l32 = lane & 31;
   const bool hi = lane >= 32;
+  // global -> register staging of one K slice: A 128 rows x 16 k (consecutive
+  // lanes on consecutive rows), B 16 k x 128 columns (16 lanes per column)
+  const int ar = tid & 127, ak = tid >> 7;
+  const int bk = tid & 15, bc = tid >> 4;
+  float2 ra[GK / 2], rb[GT / 16];
+  auto load = [&](int k0) {
+    const int gr = row0 + ar;
+#pragma unroll
+    for (int i = 0; i < GK / 2; i++) {
+      const int gk = k0 + ak + 2 * i;
+      ra[i] = (gr < m && gk < kk) ? A[(size_t)gk * lda + gr] : make_float2(0.0f, 0.0f);
+    }
+    const int gk = k0 + bk;
+#pragma unroll
+    for (int i = 0; i < GT / 16; i++) {
+      const int gc = col0 + bc + 16 * i;
+      rb[i] = (gc < nc && gk < kk) ? B[(size_t)gc * ldb + gk] : make_float2(0.0f, 0.0f);
+    }
+  };
   floatx16 accR[2][2], accI[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -269,33 +397,17 @@ __global__ void __launch_bounds__(256) k_cgemm(int m, int nc, int kk, const floa
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 16; r++) { accR[a][b][r] = 0.0f; accI[a][b][r] = 0.0f; }
+  load(0);
   for (int k0 = 0; k0 < kk; k0 += GK) {
     __syncthreads();
-    {
-      // A slice: 128 rows x 32 k, consecutive threads on consecutive rows
-      const int r = tid & 127, kq = tid >> 7;
-      const int gr = row0 + r;
-#pragma unroll 4
-      for (int i = 0; i < GK / 2; i++) {
-        const int k = kq + 2 * i, gk = k0 + k;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (gr < m && gk < kk) v = A[(size_t)gk * lda + gr];
-        sA[k][r] = v;
-      }
-      // B slice: 32 k x 128 columns, consecutive threads on consecutive k
-      const int k = tid & 31, cq = tid >> 5;
-      const int gk = k0 + k;
-#pragma unroll 4
-      for (int i = 0; i < GT / 8; i++) {
-        const int c = cq + 8 * i, gc = col0 + c;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (gc < nc && gk < kk) v = B[(size_t)gc * ldb + gk];
-        sB[k][c] = v;
-      }
-    }
+#pragma unroll
+    for (int i = 0; i < GK / 2; i++) sA[ak + 2 * i][ar] = ra[i];
+#pragma unroll
+    for (int i = 0; i < GT / 16; i++) sB[bk][bc + 16 * i] = rb[i];
     __syncthreads();
-    const int kend = min(GK, kk - k0);
-    for (int q = 0; q < kend; q++) {
+    if (k0 + GK < kk) load(k0 + GK);  // next slice in flight during this one's MFMAs
+#pragma unroll 4
+    for (int q = 0; q < GK; q++) {
       float xb[2], yr[2], yi[2];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
@@ -433,29 +545,34 @@ extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *str
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = (n - k0 < NB) ? n - k0 : NB;
     const int rows = n - k0;
-    int nw = bqd2_blocks(rows, 128);
+    int nw = bqd2_blocks(rows, PROWS);
     if (nw > GMAXW) nw = GMAXW;
     const int chunk = bqd2_blocks(rows, nw);
     nw = bqd2_blocks(rows, chunk);
     for (int j = k0 - 1; j < k0 + kb; j++)
       hipLaunchKernelGGL(k_cstep, dim3(nw), dim3(256), 0, st, n, A, k0, kb, j, chunk, nw, d_ipiv, W);
-    BQD2_HIP(hipGetLastError());
     if (n > kb)
       hipLaunchKernelGGL(k_cswap_cols, dim3(bqd2_blocks(n - kb, 256)), dim3(256), 0, st, n, A, k0, kb, d_ipiv);
-    float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB;
-    hipLaunchKernelGGL(k_ctrtri, dim3(2), dim3(NB), 0, st, n, A, k0, kb, Tk);
     BQD2_HIP(hipGetLastError());
     const int rest = n - k0 - kb;
     if (rest > 0) {
-      // U12 = L11^-1 A12 in place (one row tile: each workgroup reads only its columns)
+      // U12 = L11^-1 A12 in place, then A22 -= L21 U12 on the MFMA units
       float2 *A12 = A + (size_t)(k0 + kb) * n + k0;
-      int rc = cgemm(true, kb, rest, kb, Tk, NB, A12, n, A12, n, st);
-      if (!rc) rc = cgemm(false, rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n, A12, n,
-                          A + (size_t)(k0 + kb) * n + k0 + kb, n, st);
+      hipLaunchKernelGGL(k_ctrsm_l, dim3(bqd2_blocks(rest, 256)), dim3(256), 0, st, n, A + (size_t)k0 * n + k0, kb,
+                         A12, (long long)n, rest);
+      BQD2_HIP(hipGetLastError());
+      int rc = cgemm(false, rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n, A12, n,
+                     A + (size_t)(k0 + kb) * n + k0 + kb, n, st);
       if (rc) return rc;
     }
   }
+  // inverses of the diagonal blocks' factors for the solves, and the row permutation
+  const int nbk = bqd2_blocks(n, NB);
+  hipLaunchKernelGGL(k_ceye, dim3(bqd2_blocks((long long)nbk * NB * NB, 256)), dim3(256), 0, st, nbk, W.T);
+  hipLaunchKernelGGL(k_clinv, dim3(nbk), dim3(256), 0, st, n, A, W.T);
+  hipLaunchKernelGGL(k_ctrtri_u, dim3(nbk), dim3(NB), 0, st, n, A, W.T);
   hipLaunchKernelGGL(k_perm, dim3(1), dim3(64), 0, st, n, d_ipiv, W.perm);
+  BQD2_HIP(hipGetLastError());
   int h_flag = n;
   BQD2_HIP(hipMemcpyAsync(&h_flag, W.flag, sizeof(int), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
