@@ -296,6 +296,74 @@ __global__ void __launch_bounds__(256) k_dpv_store(DevDomain D, const double *__
   }
 }
 
+// Incident potential at the collocation nodes for the problem's plane wave:
+// Pi = -p / k2, dPi = -(dp/dn) / k2 (init_boundary_data, bem3_aw_qd2.c:1104-1146,
+// with calc_apw_dpdn, aw_pw.c:178-199).  One thread per (element, node slot).
+__global__ void __launch_bounds__(256) k_incident(DevMesh M, PwDev W, cd k2, double *__restrict__ Pi,
+                                                  double *__restrict__ dPi) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 4 * M.Ne) return;
+  const int s = 1 + id / 4, j = id % 4;
+  const double *r = M.ren + 12 * (size_t)s + 3 * j, *w = M.wen + 12 * (size_t)s + 3 * j;
+  const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double n[3] = {w[0] / wn, w[1] / wn, w[2] / wn};
+  const double *R = W.R;
+  const double rc0 = r[0] - W.tv[0], rc1 = r[1] - W.tv[1], rc2 = r[2] - W.tv[2];
+  const double z = -R[6] * rc0 - R[7] * rc1 + R[8] * rc2;
+  const double nz = -R[6] * n[0] - R[7] * n[1] + R[8] * n[2];
+  const cd p = cd{W.p[0], W.p[1]} * cis(W.k0 * z);
+  const cd dp = mul_i(p) * (W.k0 * nz);
+  const cd cp = -crecip(k2);
+  const cd a = cp * p, b = cp * dp;
+  const size_t o = 2 * (4 * (size_t)s + j);
+  Pi[o] = a.x; Pi[o + 1] = a.y;
+  dPi[o] = b.x; dPi[o + 1] = b.y;
+}
+
+// No counterpart in the reference (one plane wave per model there).  A new
+// incident plane wave pw7 = {Re p, Im p, theta, psi, tx, ty, tz} (the fields of
+// pw.txt, aw_pw.c:3-77) on an assembled problem.  The operator depends on the
+// incidence only through the in-plane wave vector, modulo the reciprocal
+// lattice: the new one must equal the problem's up to a reciprocal lattice
+// vector (the opposite kz, or the direction of another propagating diffraction
+// order), otherwise BQD2_E_ARG.  The assembled matrices and the factors of the
+// mixed-precision solver are kept; the next bqd2_solve_surface uses the new
+// right-hand side.
+extern "C" int bqd2_set_incidence(bqd2_problem *p, const double *pw7) {
+  BQD2_CHECK(p && pw7, BQD2_E_ARG, "bqd2_set_incidence: null argument");
+  const double ct = cos(pw7[2]), st = sin(pw7[2]), cp = cos(pw7[3]), sp = sin(pw7[3]);
+  double R[9] = {ct * cp * cp + sp * sp, sp * cp * (ct - 1.0), st * cp,
+                 sp * cp * (ct - 1.0),   ct * sp * sp + cp * cp, st * sp,
+                 -st * cp,               -st * sp,               ct};
+  BQD2_CHECK(R[8] != 0.0, BQD2_E_ARG, "bqd2_set_incidence: kz must not be 0");
+  if (p->m.ps == 1) {
+    // (kx, ky) - (kx0, ky0) = m1 g1 + m2 g2 with integer m1, m2
+    const double dx = p->pw_k0 * R[2] - p->qd8[1], dy = p->pw_k0 * R[5] - p->qd8[2];
+    const GfParams &g = p->gf;
+    const double det = g.g1x * g.g2y - g.g1y * g.g2x;
+    const double m1 = (dx * g.g2y - dy * g.g2x) / det, m2 = (g.g1x * dy - g.g1y * dx) / det;
+    BQD2_CHECK(fabs(m1 - nearbyint(m1)) < 1e-9 && fabs(m2 - nearbyint(m2)) < 1e-9, BQD2_E_ARG,
+               "bqd2_set_incidence: in-plane wave vector (%g, %g) differs from the operator's (%g, %g) by "
+               "(%g, %g) reciprocal lattice vectors, not an integer combination",
+               p->pw_k0 * R[2], p->pw_k0 * R[5], p->qd8[1], p->qd8[2], m1, m2);
+  }
+  p->pw_p[0] = pw7[0];
+  p->pw_p[1] = pw7[1];
+  memcpy(p->pw_tv, pw7 + 4, sizeof(p->pw_tv));
+  memcpy(p->pw_R, R, sizeof(R));
+  PwDev W;
+  memcpy(W.p, p->pw_p, sizeof(W.p));
+  W.rho0 = p->pw_rho0; W.c0 = p->pw_c0; W.k0 = p->pw_k0;
+  memcpy(W.tv, p->pw_tv, sizeof(W.tv));
+  memcpy(W.R, p->pw_R, sizeof(W.R));
+  hipLaunchKernelGGL(k_incident, dim3(bqd2_blocks(4 * p->m.Ne, 256)), dim3(256), 0, p->stream, p->m, W,
+                     cd{p->k2[0][0], p->k2[0][1]}, (double *)p->m.Pi, (double *)p->m.dPi);
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipStreamSynchronize(p->stream));
+  p->solved = false;
+  return BQD2_OK;
+}
+
 static void hk2_table(const bqd2_problem *p, double *tab) {
   // tab[(m*16 + s)] = k2[m] / k2[s]
   for (int m = 0; m <= p->m.MN; m++)

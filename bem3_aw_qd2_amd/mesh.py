@@ -135,12 +135,14 @@ CONFIGS = {
 }
 
 
-def write_config(name, directory, triangles=False, periodic=True):
+def write_config(name, directory, triangles=False, periodic=True, **over):
     """Write pw.txt, periodicity.txt, medium.txt and the mesh of a workload into
     directory; returns the four paths in read_aqd2 order.  periodic=False writes
-    periodicity setting 0: the same sphere alone in free space."""
+    periodicity setting 0: the same sphere alone in free space.  Keyword
+    arguments replace entries of the workload (n, radius, d1, d2, f, theta, psi,
+    host, sphere)."""
     import os
-    c = CONFIGS[name]
+    c = dict(CONFIGS[name], **over)
     os.makedirs(directory, exist_ok=True)
     P, Q = cube_sphere(c["n"], c["radius"])
     elems = split_triangles(P, Q) if triangles else Q

@@ -160,6 +160,16 @@ int bqd2_set_solver(bqd2_problem *p, int mode);
  * (MN + 2 entries, 0 where the LU ran); lu: systems that fell back to the LU. */
 int bqd2_solver_info(bqd2_problem *p, int *iters, double *relres, int *lu);
 
+/* No counterpart in the reference (one plane wave per model there).  A new
+ * incident plane wave on an assembled problem, pw7 = {Re p, Im p, theta, psi,
+ * tx, ty, tz} as in pw.txt (aw_pw.c:3-77).  Its in-plane wave vector must equal
+ * the problem's modulo the reciprocal lattice (the mirrored direction pi - theta,
+ * or another propagating diffraction order), since the periodic operator depends
+ * on it (BQD2_E_ARG otherwise).  The assembled matrices and the mixed-precision
+ * factors are kept: the next bqd2_solve_surface only forms and solves the new
+ * right-hand side. */
+int bqd2_set_incidence(bqd2_problem *p, const double *pw7);
+
 /* per-domain solution arrays, (Ne_d+1) x 4 complex for P, dP and
  * (Ne_d+1) x 4 x 3 complex for pv, dpv, entry 0 unused, as in SUBD */
 int bqd2_download_solution(bqd2_problem *p, int did, double *P, double *dP, double *pv, double *dpv);
