@@ -73,6 +73,7 @@ struct bqd2_problem {
   int any_tri;                // the mesh has linear triangles
   std::vector<std::vector<char>> tripos;  // host, per domain: 1 where element position t is a triangle
   std::vector<void *> owned;  // device allocations released by bqd2_destroy
+  size_t alloc_bytes;         // their size in bytes
   hipStream_t stream;
   long long *d_counters;      // device counters (GF terms)
   // system
@@ -131,6 +132,13 @@ int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const doub
 int bqd2_jacobi(int n, const double *A, long long lda, const int *map, double *dinv, hipStream_t st);
 // the calling thread's context (nullptr on failure, error set)
 CallCtx *bqd2_call_ctx(bqd2_problem *p);
+// make the calling thread the problem's owner (its main context): for a caller
+// that hands a problem from thread to thread and never uses it from two at once
+void bqd2_rebind(bqd2_problem *p);
+// device bytes held by the problem: allocations and every context's workspace
+size_t bqd2_bytes(bqd2_problem *p);
+// new incident plane wave given by amplitude, translation and rotation (solve.hip)
+int bqd2_set_incidence_R(bqd2_problem *p, const double *pw_p, const double *tv, const double *R);
 // at least `bytes` of device workspace at *base (previous contents are lost)
 int bqd2_workspace(CallCtx *c, size_t bytes, char **base);
 

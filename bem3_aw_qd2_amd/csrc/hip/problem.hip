@@ -10,6 +10,22 @@ int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes) {
     return BQD2_E_NOMEM;
   }
   p->owned.push_back(*ptr);
+  p->alloc_bytes += bytes;
+  return BQD2_OK;
+}
+
+void bqd2_rebind(bqd2_problem *p) { p->owner = std::this_thread::get_id(); }
+
+size_t bqd2_bytes(bqd2_problem *p) {
+  std::lock_guard<std::mutex> g(p->ctx_mu);
+  size_t b = p->alloc_bytes + p->main.ws_bytes;
+  for (auto &tc : p->ctxs) b += tc.second->ws_bytes + CNT_N * sizeof(long long);
+  return b;
+}
+
+extern "C" int bqd2_device_bytes(bqd2_problem *p, size_t *bytes) {
+  BQD2_CHECK(p && bytes, BQD2_E_ARG, "bqd2_device_bytes: null argument");
+  *bytes = bqd2_bytes(p);
   return BQD2_OK;
 }
 

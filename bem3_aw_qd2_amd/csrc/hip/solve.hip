@@ -332,9 +332,13 @@ __global__ void __launch_bounds__(256) k_incident(DevMesh M, PwDev W, cd k2, dou
 extern "C" int bqd2_set_incidence(bqd2_problem *p, const double *pw7) {
   BQD2_CHECK(p && pw7, BQD2_E_ARG, "bqd2_set_incidence: null argument");
   const double ct = cos(pw7[2]), st = sin(pw7[2]), cp = cos(pw7[3]), sp = sin(pw7[3]);
-  double R[9] = {ct * cp * cp + sp * sp, sp * cp * (ct - 1.0), st * cp,
-                 sp * cp * (ct - 1.0),   ct * sp * sp + cp * cp, st * sp,
-                 -st * cp,               -st * sp,               ct};
+  const double R[9] = {ct * cp * cp + sp * sp, sp * cp * (ct - 1.0), st * cp,
+                       sp * cp * (ct - 1.0),   ct * sp * sp + cp * cp, st * sp,
+                       -st * cp,               -st * sp,               ct};
+  return bqd2_set_incidence_R(p, pw7, pw7 + 4, R);
+}
+
+int bqd2_set_incidence_R(bqd2_problem *p, const double *pw_p, const double *tv, const double *R) {
   BQD2_CHECK(R[8] != 0.0, BQD2_E_ARG, "bqd2_set_incidence: kz must not be 0");
   if (p->m.ps == 1) {
     // (kx, ky) - (kx0, ky0) = m1 g1 + m2 g2 with integer m1, m2
@@ -347,10 +351,10 @@ extern "C" int bqd2_set_incidence(bqd2_problem *p, const double *pw7) {
                "(%g, %g) reciprocal lattice vectors, not an integer combination",
                p->pw_k0 * R[2], p->pw_k0 * R[5], p->qd8[1], p->qd8[2], m1, m2);
   }
-  p->pw_p[0] = pw7[0];
-  p->pw_p[1] = pw7[1];
-  memcpy(p->pw_tv, pw7 + 4, sizeof(p->pw_tv));
-  memcpy(p->pw_R, R, sizeof(R));
+  p->pw_p[0] = pw_p[0];
+  p->pw_p[1] = pw_p[1];
+  memcpy(p->pw_tv, tv, sizeof(p->pw_tv));
+  memcpy(p->pw_R, R, sizeof(p->pw_R));
   PwDev W;
   memcpy(W.p, p->pw_p, sizeof(W.p));
   W.rho0 = p->pw_rho0; W.c0 = p->pw_c0; W.k0 = p->pw_k0;
@@ -542,6 +546,7 @@ static int dpv_lu(bqd2_problem *p, DevDomain &D, hipStream_t st) {
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipStreamSynchronize(st));
   // the work buffers of the fallback are released right away
+  p->alloc_bytes -= (size_t)D.nn * D.nn * 16 + (size_t)D.nn * 3 * 16 + (size_t)D.nn * sizeof(int);
   for (void *q : {(void *)A, (void *)B, (void *)ipiv}) {
     (void)hipFree(q);
     for (auto it = p->owned.begin(); it != p->owned.end(); ++it)

@@ -139,6 +139,31 @@ static void free_packed(packed *pk)
   free(pk->k0); free(pk->k2); free(pk->rho0);
 }
 
+/* the packed model and its arrays in one block (the model first) */
+typedef struct {
+  bqd2_model m;
+  packed pk;
+} model_box;
+
+void *aqd2_model_new(AQD2 *qd)
+{
+  model_box *b = calloc(1, sizeof(*b));
+  if (!b) {
+    printf("libbem3_aw_qd2: out of memory in aqd2_model_new. Exit...\n");
+    exit(1);
+  }
+  pack_model(qd, &b->m, &b->pk);
+  return &b->m;
+}
+
+void aqd2_model_free(void *model)
+{
+  if (!model) return;
+  model_box *b = (model_box *)model;
+  free_packed(&b->pk);
+  free(b);
+}
+
 /* the device problem of qd, created on first use */
 bqd2_problem *aqd2_problem(AQD2 *qd, int *has_solution)
 {

@@ -157,6 +157,13 @@ void pv_i_bd_aqd2(double complex *p, double complex *pv, int did, int t, double 
 void pv_batch_aqd2(int n, const double *rt, int type, int total,
                    double complex *p, double complex *pv, int *did, AQD2 *qd);
 
+/* ---- device description of an initialised model (no counterpart in the
+ * reference): the bqd2_model (include/bqd2.h) of qd in newly allocated packed
+ * arrays, for bqd2_server_submit and bqd2_footprint; release it with
+ * aqd2_model_free.  Returned as void * so that this header needs no bqd2.h. ---- */
+void *aqd2_model_new(AQD2 *qd);
+void aqd2_model_free(void *model);
+
 /* ---- plane wave (aw_pw.h:44-58) ---- */
 void read_data_apw(char *fname, Apw *pw);
 void print_data_apw(Apw *pw);

@@ -222,6 +222,66 @@ int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, 
  * none has run since the last assembly. */
 int bqd2_lu32_info(bqd2_problem *p, double *ms_factor);
 
+/* ------------------------------------------------------------------------
+ * Serving (no counterpart in the reference, which solves one model per process:
+ * aw_d3qd2_bv_solver.c:5-11, then example1.c:10-48 for fields and forces)
+ * ------------------------------------------------------------------------ */
+
+/* Device bytes that one request's problem may hold: matrices, system, solver
+ * factors and workspaces, field and force workspaces (an upper estimate; the
+ * server admits requests by it). */
+size_t bqd2_footprint(const bqd2_model *m, int solver, int npts, int force_type);
+/* device bytes a problem holds now (allocations and workspaces) */
+int bqd2_device_bytes(bqd2_problem *p, size_t *bytes);
+
+typedef struct bqd2_server bqd2_server;
+
+/* One request: the whole solve of a model (bqd2_create, bqd2_assemble of every
+ * domain, bqd2_solve_surface, bqd2_solve_velocity), then optionally the total
+ * field at npts points (rule type 0) and the radiation force about rc.  The
+ * model and the points are copied at submission.  op_key != 0 promises that
+ * requests with the same key share mesh, media, frequency, lattice and in-plane
+ * wave vector modulo the reciprocal lattice (the incidence may differ): the
+ * operator of the first stays resident and the others reuse it (checked). */
+typedef struct {
+  const bqd2_model *model;
+  unsigned long long op_key;
+  int solver;          /* bqd2_set_solver mode: 0 FP64 GMRES, 1 FP64 LU, 2 mixed precision */
+  int npts;            /* field points (host npts x 3), 0 for none */
+  const double *pts;
+  int force_type;      /* -1 no force, else the type of bqd2_force */
+  double rc[3];        /* torque centre */
+} bqd2_request;
+
+typedef struct {
+  int status;          /* 0 or the BQD2_E code of the failure (error holds the message) */
+  int reused;          /* 1: the operator was resident (no assembly, no factorisation) */
+  int krylov_steps;    /* GMRES steps of the surface solve */
+  int pad_;
+  double relres;       /* its final scaled residual */
+  double F[3], N[3];   /* force and torque (force_type >= 0) */
+  double t_queue;      /* seconds from submission to the start of execution */
+  double t_run;        /* seconds of execution */
+  double t_done;       /* completion time on the server's clock */
+  size_t device_bytes; /* device bytes held by the request's problem at completion */
+  char error[256];
+} bqd2_result;
+
+/* A server with `slots` concurrent requests (worker threads, each request on its
+ * problem's own stream) and a device-memory budget of hbm_bytes for resident
+ * and running problems. */
+int bqd2_server_create(size_t hbm_bytes, int slots, bqd2_server **out);
+/* Queue a request; field_out (host npts x 8: p, v as in bqd2_field) must stay
+ * valid until bqd2_server_wait returns for the ticket. */
+int bqd2_server_submit(bqd2_server *s, const bqd2_request *rq, double *field_out, long long *ticket);
+/* Block until the request is done; returns its status. */
+int bqd2_server_wait(bqd2_server *s, long long ticket, bqd2_result *res);
+/* counts: hits, misses, evictions, failed, resident operators, pending requests;
+ * bytes: accounted device bytes in use, budget (either may be NULL) */
+int bqd2_server_stats(bqd2_server *s, long long *counts, size_t *bytes);
+/* Finish the queued requests, release every resident problem. */
+int bqd2_server_destroy(bqd2_server *s);
+
 #ifdef __cplusplus
 }
 #endif
