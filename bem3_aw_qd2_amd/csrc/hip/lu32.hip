@@ -356,13 +356,14 @@ __global__ void __launch_bounds__(256) k_ceye(int nbk, float2 *__restrict__ T) {
 // imaginary part of C^T = B^T A^T: the two k slots carry the real and imaginary
 // parts, X = (Br, Bi) for both, Y = (Ar, -Ai) for Re C and (Ai, Ar) for Im C.
 // With C^T as the product, j runs along the rows of C, so the 32 lanes of a
-// register store 32 consecutive rows of one column.  B may be C when m <= 128:
-// a workgroup then reads only the columns it writes, and writes after its last read.
+// register store 32 consecutive rows of one column.  B may be C when m <= 128,
+// or A may be C when nc <= 128: a workgroup then reads only the columns (rows)
+// it writes, and writes after its last read.
 #define GT 128
 #define GK 16
 #define GPAD 1
 template <int SET>
-__global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const float2 *__restrict__ A, long long lda,
+__global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const float2 *A, long long lda,
                                                   const float2 *B, long long ldb, float2 *C, long long ldc) {
   __shared__ float2 sA[GK][GT + GPAD];  // (k, row)
   __shared__ float2 sB[GK][GT + GPAD];  // (k, col)
@@ -573,6 +574,17 @@ extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *str
   hipLaunchKernelGGL(k_ctrtri_u, dim3(nbk), dim3(NB), 0, st, n, A, W.T);
   hipLaunchKernelGGL(k_perm, dim3(1), dim3(64), 0, st, n, d_ipiv, W.perm);
   BQD2_HIP(hipGetLastError());
+  // the solves' form of the factors: each off-diagonal block times the inverse of
+  // its column's diagonal block (L_rk L_kk^-1 below, U_rk U_kk^-1 above), in place
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int kb = (n - k0 < NB) ? n - k0 : NB, rest = n - k0 - kb;
+    const float2 *TL = W.T + (size_t)(k0 / NB) * 2 * NB * NB;
+    float2 *Lb = A + (size_t)k0 * n + k0 + kb, *Ub = A + (size_t)k0 * n;
+    int rc = cgemm(true, rest, kb, kb, Lb, n, TL, NB, Lb, n, st);
+    if (!rc) rc = cgemm(true, k0, kb, kb, Ub, n, TL + NB * NB, NB, Ub, n, st);
+    if (rc) return rc;
+  }
+  BQD2_HIP(hipGetLastError());
   int h_flag = n;
   BQD2_HIP(hipMemcpyAsync(&h_flag, W.flag, sizeof(int), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
@@ -580,23 +592,129 @@ extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *str
   return BQD2_OK;
 }
 
-// X (n x nrhs, column-major, leading dimension ldb) <- U^-1 L^-1 P X
-static int cgetrs_perm(int n, int nrhs, const float2 *A, const LuWs &W, float2 *X, long long ldx, hipStream_t st) {
-  // forward: L y = b, by diagonal blocks
+// One block step of a triangular sweep.  The factorisation leaves the
+// off-diagonal blocks multiplied by the inverse of their column's diagonal
+// block (M_rk = L_rk L_kk^-1 below the diagonal, U_rk U_kk^-1 above), so the
+// step is y_r -= M_rk y_k for the rows still to be eliminated ([k0 + kb, n)
+// forward, [0, k0) backward) and x_k = T_k y_k, written to X by the last
+// workgroup.  Nobody writes rows k0.. of Y in this launch.  A workgroup takes
+// 64 rows; its wave w sums the columns 32 w .. 32 w + 31 of the block (lanes on
+// consecutive rows), and the four partial sums meet in LDS.
+template <int NR, bool UPPER>
+__global__ void __launch_bounds__(256) k_csweep(int n, const float2 *__restrict__ A, const float2 *__restrict__ Tk,
+                                                int k0, int kb, float2 *Y, float2 *X, long long ld) {
+  __shared__ float2 sy[NR][NB];
+  __shared__ float2 part[4][NR][64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int e = tid; e < NR * kb; e += 256) sy[e / kb][e % kb] = Y[(size_t)(e / kb) * ld + k0 + e % kb];
+  __syncthreads();
+  if (blockIdx.x == gridDim.x - 1) {
+    // x_k = T_k y_k
+    if (tid >= kb) return;
+    float2 acc[NR];
+#pragma unroll
+    for (int s = 0; s < NR; s++) acc[s] = make_float2(0.0f, 0.0f);
+#pragma unroll 8
+    for (int q = 0; q < kb; q++) {
+      const float2 t = Tk[(size_t)q * NB + tid];
+#pragma unroll
+      for (int s = 0; s < NR; s++) {
+        const float2 y = sy[s][q];
+        acc[s].x += fmaf(t.x, y.x, -t.y * y.y);
+        acc[s].y += fmaf(t.x, y.y, t.y * y.x);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NR; s++) X[(size_t)s * ld + k0 + tid] = acc[s];
+    return;
+  }
+  const int r0 = UPPER ? 0 : k0 + kb, r1 = UPPER ? k0 : n;
+  const int r = r0 + blockIdx.x * 64 + lane;
+  float2 acc[NR];
+#pragma unroll
+  for (int s = 0; s < NR; s++) acc[s] = make_float2(0.0f, 0.0f);
+  if (r < r1) {
+    const int q0 = 32 * w, q1 = min(kb, q0 + 32);
+    const float2 *a = A + (size_t)k0 * n + r;
+#pragma unroll 8
+    for (int q = q0; q < q1; q++) {
+      const float2 v = a[(size_t)q * n];
+#pragma unroll
+      for (int s = 0; s < NR; s++) {
+        const float2 x = sy[s][q];
+        acc[s].x += fmaf(v.x, x.x, -v.y * x.y);
+        acc[s].y += fmaf(v.x, x.y, v.y * x.x);
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NR; s++) part[w][s][lane] = acc[s];
+  __syncthreads();
+  if (w != 0 || r >= r1) return;
+#pragma unroll
+  for (int s = 0; s < NR; s++) {
+    float2 t = part[0][s][lane];
+    for (int v = 1; v < 4; v++) {
+      t.x += part[v][s][lane].x;
+      t.y += part[v][s][lane].y;
+    }
+    float2 *y = Y + (size_t)s * ld + r;
+    float2 o = *y;
+    o.x -= t.x;
+    o.y -= t.y;
+    *y = o;
+  }
+}
+
+template <int NR>
+static void sweep_nr(bool upper, int n, const float2 *A, const float2 *Tk, int k0, int kb, float2 *Y, float2 *X,
+                     long long ld, hipStream_t st) {
+  const int rows = upper ? k0 : n - k0 - kb;
+  dim3 g(bqd2_blocks(rows, 64) + 1);  // row workgroups + the one that forms x_k
+  if (upper) hipLaunchKernelGGL((k_csweep<NR, true>), g, dim3(256), 0, st, n, A, Tk, k0, kb, Y, X, ld);
+  else hipLaunchKernelGGL((k_csweep<NR, false>), g, dim3(256), 0, st, n, A, Tk, k0, kb, Y, X, ld);
+}
+
+// Y (n x nrhs, column-major, leading dimension ld, already permuted) <- U^-1 L^-1 Y;
+// X is a second n x nrhs buffer: the forward sweep writes its result there and
+// the backward sweep writes back into Y.  More than 4 right-hand sides: the
+// block steps as MFMA GEMMs, in place on Y.
+static int cgetrs_perm(int n, int nrhs, const float2 *A, const LuWs &W, float2 *Y, float2 *X, long long ld,
+                       hipStream_t st) {
+  const int nbk = bqd2_blocks(n, NB);
+  if (nrhs <= 4) {
+    for (int pass = 0; pass < 2; pass++) {
+      const bool upper = pass == 1;
+      float2 *in = upper ? X : Y, *out = upper ? Y : X;
+      for (int b = 0; b < nbk; b++) {
+        const int k0 = (upper ? nbk - 1 - b : b) * NB, kb = (n - k0 < NB) ? n - k0 : NB;
+        const float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB + (upper ? NB * NB : 0);
+        switch (nrhs) {
+          case 1: sweep_nr<1>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
+          case 2: sweep_nr<2>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
+          case 3: sweep_nr<3>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
+          default: sweep_nr<4>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
+        }
+      }
+    }
+    BQD2_HIP(hipGetLastError());
+    return BQD2_OK;
+  }
+  // forward: y_r -= M_rk y_k below, then y_k = T_k y_k (in place), by diagonal blocks
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = (n - k0 < NB) ? n - k0 : NB;
     const float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB;
-    int rc = small_update(true, kb, kb, nrhs, Tk, NB, X + k0, ldx, X + k0, ldx, st);
-    if (!rc) rc = small_update(false, n - k0 - kb, kb, nrhs, A + (size_t)k0 * n + k0 + kb, n, X + k0, ldx,
-                               X + k0 + kb, ldx, st);
+    int rc = small_update(false, n - k0 - kb, kb, nrhs, A + (size_t)k0 * n + k0 + kb, n, Y + k0, ld, Y + k0 + kb,
+                          ld, st);
+    if (!rc) rc = small_update(true, kb, kb, nrhs, Tk, NB, Y + k0, ld, Y + k0, ld, st);
     if (rc) return rc;
   }
-  // backward: U x = y
+  // backward: the same with the upper factor, last block first
   for (int k0 = ((n - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
     const int kb = (n - k0 < NB) ? n - k0 : NB;
     const float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB + NB * NB;
-    int rc = small_update(true, kb, kb, nrhs, Tk, NB, X + k0, ldx, X + k0, ldx, st);
-    if (!rc) rc = small_update(false, k0, kb, nrhs, A + (size_t)k0 * n, n, X + k0, ldx, X, ldx, st);
+    int rc = small_update(false, k0, kb, nrhs, A + (size_t)k0 * n, n, Y + k0, ld, Y, ld, st);
+    if (!rc) rc = small_update(true, kb, kb, nrhs, Tk, NB, Y + k0, ld, Y + k0, ld, st);
     if (rc) return rc;
   }
   return BQD2_OK;
@@ -618,7 +736,7 @@ extern "C" int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, 
   float2 *B = (float2 *)d_B, *T = (float2 *)d_tmp;
   hipLaunchKernelGGL(k_gather_f, dim3(bqd2_blocks(n, 256), nrhs), dim3(256), 0, st, n, W.perm, B, (long long)n, T,
                      (long long)n);
-  int rc = cgetrs_perm(n, nrhs, (const float2 *)d_A, W, T, n, st);
+  int rc = cgetrs_perm(n, nrhs, (const float2 *)d_A, W, T, B, n, st);
   if (rc) return rc;
   BQD2_HIP(hipMemcpyAsync(B, T, (size_t)n * nrhs * sizeof(float2), hipMemcpyDeviceToDevice, st));
   return BQD2_OK;
@@ -677,14 +795,14 @@ int bqd2_lu32_convert(int n, const double *A, long long lda, const double *dinv,
 }
 
 // out = M^-1 in for complex double vectors (nrhs columns, leading dimension ld),
-// M = P^T L U the complex float factors; tmp holds n x nrhs complex floats
+// M = P^T L U the complex float factors; tmp holds 2 n nrhs complex floats
 int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const double *in, long long ldi, double *out,
                     long long ldo, float *tmp, hipStream_t st) {
   LuWs W = lu_ws(n, (void *)ws);
   float2 *T = (float2 *)tmp;
   dim3 g(bqd2_blocks(n, 256), nrhs);
   hipLaunchKernelGGL(k_gather_d2f, g, dim3(256), 0, st, n, W.perm, in, ldi, T);
-  int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, n, st);
+  int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, T + (size_t)n * nrhs, n, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_f2d, g, dim3(256), 0, st, n, T, out, ldo);
   BQD2_HIP(hipGetLastError());

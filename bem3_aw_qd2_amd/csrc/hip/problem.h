@@ -87,7 +87,7 @@ struct bqd2_problem {
   // until the next assembly
   float *Af;               // na x na complex float, column-major
   void *luws;              // bqd2_cgetrf_wsize(na) bytes
-  float *lutmp;            // na x 3 complex float
+  float *lutmp;            // 2 x na x 3 complex float
   double *dinv;            // na complex: inverse diagonal of the system
   bool lu32_valid;
   double ms_factor;        // time of the last factorisation (conversion + LU)
@@ -110,7 +110,7 @@ int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);
 
 // Right preconditioner of GMRES: the complex float LU factors (lu32.hip) of the
 // row-scaled matrix D A (D = the inverse diagonal), so that D A M^-1 is close to
-// the identity.  tmp: n x nrhs complex floats.
+// the identity.  tmp: 2 n nrhs complex floats.
 struct Precond {
   const float *Af;
   const void *ws;

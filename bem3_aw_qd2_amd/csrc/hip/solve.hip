@@ -434,7 +434,7 @@ static int factor_lu32(bqd2_problem *p, hipStream_t st) {
   if (!p->Af) {
     if ((rc = bqd2_alloc(p, (void **)&p->Af, (size_t)na * na * 8))) return rc;
     if ((rc = bqd2_alloc(p, &p->luws, bqd2_cgetrf_wsize(na)))) return rc;
-    if ((rc = bqd2_alloc(p, (void **)&p->lutmp, (size_t)na * 3 * 8))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&p->lutmp, (size_t)na * 6 * 8))) return rc;
     if ((rc = bqd2_alloc(p, (void **)&p->dinv, (size_t)na * 16))) return rc;
   }
   hipEvent_t e0, e1;

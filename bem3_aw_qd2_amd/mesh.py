@@ -140,14 +140,14 @@ def write_config(name, directory, triangles=False, periodic=True, **over):
     directory; returns the four paths in read_aqd2 order.  periodic=False writes
     periodicity setting 0: the same sphere alone in free space.  Keyword
     arguments replace entries of the workload (n, radius, d1, d2, f, theta, psi,
-    host, sphere)."""
+    host, sphere) or give the plane wave's complex amplitude p = (re, im)."""
     import os
     c = dict(CONFIGS[name], **over)
     os.makedirs(directory, exist_ok=True)
     P, Q = cube_sphere(c["n"], c["radius"])
     elems = split_triangles(P, Q) if triangles else Q
     paths = [os.path.join(directory, x) for x in ("pw.txt", "periodicity.txt", "medium.txt", "mesh.msh")]
-    write_pw(paths[0], c["host"][0], c["host"][1], c["f"], theta=c["theta"], psi=c["psi"])
+    write_pw(paths[0], c["host"][0], c["host"][1], c["f"], p=c.get("p", (1.0, 0.0)), theta=c["theta"], psi=c["psi"])
     write_periodicity(paths[1], c["d1"], c["d2"], ps=1 if periodic else 0)
     write_medium(paths[2], [c["sphere"]])
     write_msh(paths[3], P, elems)

@@ -247,6 +247,160 @@ def cpu_baseline(model, req, npairs, nself, seed=0):
     return {"value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port", "sample": sample}
 
 
+def lu32_leg(hip, prob, na):
+    """The mixed-precision solve on the request's assembled system: the
+    row-scaled system factored in complex float on the MFMA units, then FP64
+    GMRES preconditioned by the factors; further incidences with the same
+    in-plane wave vector reuse the factors.  Compared with the FP64 GMRES
+    solution of the timed steps (surface unknowns and type-0 force)."""
+    from bem3_aw_qd2_amd import check
+    out = {}
+    for mode in (0, 2):
+        check(hip.bqd2_set_solver(prob, mode), "bqd2_set_solver")
+        t0 = time.perf_counter()
+        xs = np.zeros(na, np.complex128)
+        check(hip.bqd2_solve_surface(prob, xs.ctypes.data), "bqd2_solve_surface")
+        t1 = time.perf_counter()
+        check(hip.bqd2_solve_velocity(prob), "bqd2_solve_velocity")
+        F = (C.c_double * 3)()
+        N = (C.c_double * 3)()
+        check(hip.bqd2_force(prob, (C.c_double * 3)(0.0, 0.0, 0.0), 0, F, N), "bqd2_force")
+        its = (C.c_int * 17)()
+        res = (C.c_double * 17)()
+        lu = C.c_int()
+        check(hip.bqd2_solver_info(prob, its, res, C.byref(lu)), "bqd2_solver_info")
+        out[mode] = dict(x=xs, F=np.array(F[:]), t=t1 - t0, its=its[0], res=res[0])
+    # a second incidence on the same operator: the factors are reused
+    t0 = time.perf_counter()
+    check(hip.bqd2_solve_surface(prob, None), "bqd2_solve_surface")
+    t_reuse = time.perf_counter() - t0
+    ms = C.c_double()
+    check(hip.bqd2_lu32_info(prob, C.byref(ms)), "bqd2_lu32_info")
+    check(hip.bqd2_set_solver(prob, 0), "bqd2_set_solver")
+    flops = 8.0 / 3.0 * na ** 3
+    x0, x2 = out[0]["x"], out[2]["x"]
+    return {"unknowns": na, "factor_ms": ms.value, "factor_tflops_8n3_3": flops / (ms.value * 1e-3) / 1e12,
+            "peak_fp32_mfma_tflops": 157.3, "refinement_steps": out[2]["its"], "scaled_residual": out[2]["res"],
+            "solve_ms_with_factor": out[2]["t"] * 1e3, "solve_ms_factor_reused": t_reuse * 1e3,
+            "fp64_gmres_steps": out[0]["its"], "fp64_gmres_solve_ms": out[0]["t"] * 1e3,
+            "solution_rel_diff_vs_fp64": float(np.linalg.norm(x2 - x0) / np.linalg.norm(x0)),
+            "force_rel_diff_vs_fp64": float(np.linalg.norm(out[2]["F"] - out[0]["F"]) / np.linalg.norm(out[0]["F"]))}
+
+
+
+def serving_stream(nreq, rate, repeat, seed=0, args_mirror=0.0):
+    """A seeded stream of solve requests of mixed size (a scaled-down form of
+    the 384-6144-element serving workload): Poisson arrivals at `rate` per
+    second; each request is, with probability `repeat`, another incidence on an
+    operator already requested (a new amplitude and phase of the same plane
+    wave, or with probability `args_mirror` the mirrored direction pi - theta),
+    else a new operator: quad or
+    triangle cube-sphere of 6 n^2 quads (n in 4..8), square or triangular lattice
+    with period / radius in [2.5, 6], ka in [0.3, 4], sphere density ratio in
+    [0.8, 3], sound-speed ratio in [0.5, 2], random incidence.  Lattices with a
+    diffraction order within 1e-3 of grazing (a Wood anomaly) are redrawn."""
+    rng = np.random.default_rng(seed)
+    a, rho, c = 0.5e-3, 1000.0, 1500.0
+    ops, reqs, t = [], [], 0.0
+    while len(reqs) < nreq:
+        t += rng.exponential(1.0 / rate)
+        if ops and rng.random() < repeat:
+            k = int(rng.integers(len(ops)))
+            over = dict(ops[k]["over"])
+            mirrored = rng.random() < args_mirror
+            if mirrored:
+                over["theta"] = math.pi - over["theta"]
+            amp = complex(np.exp(1j * rng.uniform(0, 2 * math.pi)) * rng.uniform(0.5, 2.0))
+            over["p"] = (amp.real, amp.imag)
+            reqs.append(dict(t=t, key=k + 1, over=over, tri=ops[k]["tri"], mirrored=mirrored))
+            continue
+        while True:
+            period = a * rng.uniform(2.5, 6.0)
+            tri = rng.random() < 0.5
+            d1, d2 = (period, 0.0), ((0.5 * period, 0.5 * math.sqrt(3.0) * period) if rng.random() < 0.5
+                                     else (0.0, period))
+            ka = rng.uniform(0.3, 4.0)
+            f = ka * c / (2 * math.pi * a)
+            theta, psi = rng.uniform(0.05, 1.2), rng.uniform(0.05, 2 * math.pi - 0.05)
+            k = 2 * math.pi * f / c
+            kx, ky = k * math.sin(theta) * math.cos(psi), k * math.sin(theta) * math.sin(psi)
+            det = d1[0] * d2[1] - d1[1] * d2[0]
+            g1 = (2 * math.pi * d2[1] / det, -2 * math.pi * d2[0] / det)
+            g2 = (-2 * math.pi * d1[1] / det, 2 * math.pi * d1[0] / det)
+            near = min(abs(((kx + i * g1[0] + j * g2[0]) ** 2 + (ky + i * g1[1] + j * g2[1]) ** 2) / k ** 2 - 1.0)
+                       for i in range(-12, 13) for j in range(-12, 13))
+            if near > 1e-3:
+                break
+        over = dict(n=int(rng.integers(4, 9)), d1=d1, d2=d2, f=f, theta=theta, psi=psi,
+                    sphere=(rho * rng.uniform(0.8, 3.0), c * rng.uniform(0.5, 2.0)))
+        ops.append(dict(over=over, tri=tri))
+        reqs.append(dict(t=t, key=len(ops), over=over, tri=tri, mirrored=False))
+    return reqs
+
+
+def run_stream(reqs, models, slots, solver, keyed, budget, pts):
+    """Submit each request at its arrival time; returns per-request results and
+    the wall time from the first arrival to the last completion."""
+    from bem3_aw_qd2_amd import serve
+    with serve.Server(budget, slots) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for r, m in zip(reqs, models):
+            dt = r["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(m, key=r["key"] if keyed else 0, solver=solver, pts=pts, force_type=0))
+        res = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0 - reqs[0]["t"]
+        st = srv.stats()
+    for r in res:
+        if r["status"] != 0:
+            raise RuntimeError("serving request failed: " + r["error"])
+    return res, wall, st
+
+
+def serving_bench(args):
+    """The plain FP64 path (one request at a time, every operator assembled,
+    FP64 GMRES) against the serving form (resident operators, `slots` concurrent
+    requests, the mixed-precision solver) on the same request stream."""
+    from bem3_aw_qd2_amd import serve
+    reqs = serving_stream(args.serve_requests, args.serve_rate, args.serve_repeat, args_mirror=args.serve_mirror)
+    models = [serve.PackedModel("sphere384", triangles=r["tri"], **r["over"]) for r in reqs]
+    pts = grid_points(8, 1.5e-3)
+    budget = int(args.serve_budget_gib * 2 ** 30)
+    log("serving: %d requests, %d operators, rate %.1f/s" % (len(reqs), len(set(r["key"] for r in reqs)),
+                                                             args.serve_rate))
+    plain, wall_p, _ = run_stream(reqs, models, 1, 0, False, budget, pts)
+    fast, wall_f, st = run_stream(reqs, models, args.serve_slots, args.serve_solver, True, budget, pts)
+    lat_p = np.array([r["t_queue"] + r["t_run"] for r in plain]) * 1e3
+    lat_f = np.array([r["t_queue"] + r["t_run"] for r in fast]) * 1e3
+    errs = [float(np.linalg.norm(a["F"] - b["F"]) / np.linalg.norm(b["F"])) for a, b in zip(fast, plain)]
+    ferr = max(errs)
+    ferr_fresh = max([e for e, a in zip(errs, fast) if not a["reused"]] or [0.0])
+    ferr_reused = max([e for e, a, r in zip(errs, fast, reqs) if a["reused"] and not r["mirrored"]] or [0.0])
+    ferr_mirr = max([e for e, a, r in zip(errs, fast, reqs) if a["reused"] and r["mirrored"]] or [0.0])
+    elems = [m.ne for m in models]
+    return {
+        "requests": len(reqs), "operators": len(set(r["key"] for r in reqs)),
+        "elements_min_max": [int(min(elems)), int(max(elems))], "arrival_rate_per_s": args.serve_rate,
+        "repeat_fraction": args.serve_repeat,
+        "plain_fp64": {"solves_per_s": len(reqs) / wall_p, "p50_ms": float(np.percentile(lat_p, 50)),
+                       "p99_ms": float(np.percentile(lat_p, 99)), "slots": 1, "solver": "FP64 GMRES",
+                       "resident_operators": False},
+        "serving": {"solves_per_s": len(reqs) / wall_f, "p50_ms": float(np.percentile(lat_f, 50)),
+                    "p99_ms": float(np.percentile(lat_f, 99)), "slots": args.serve_slots,
+                    "solver": {0: "FP64 GMRES", 2: "complex float LU (MFMA) + FP64 GMRES refinement"}[args.serve_solver],
+                    "resident_operators": True, "hits": st["hits"], "misses": st["misses"],
+                    "evictions": st["evictions"]},
+        "speedup": wall_p / wall_f,
+        "force_rel_err_vs_plain": ferr,
+        "force_rel_err_vs_plain_fresh_operator": ferr_fresh,
+        "force_rel_err_vs_plain_resident_operator": ferr_reused,
+        "force_rel_err_vs_plain_resident_mirrored": ferr_mirr,
+        "mirrored_fraction": args.serve_mirror,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +415,20 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=384)
     ap.add_argument("--cpu-self", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-stream measurement")
+    ap.add_argument("--no-lu32", action="store_true", help="skip the mixed-precision solve of the request's system")
+    ap.add_argument("--serving-only", action="store_true", help="only the request-stream measurement")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--serve-repeat", type=float, default=0.75,
+                    help="probability that a request reuses an earlier request's operator")
+    ap.add_argument("--serve-mirror", type=float, default=0.0,
+                    help="probability that a repeated request takes the mirrored incidence pi - theta (its "
+                         "in-plane wave vector then differs from the resident operator's in the last bits, which "
+                         "moves the lattice sums at points level with a source element by up to 1e-6 in the force)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-solver", type=int, default=2)
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the steps, copy a device buffer of this size once (a known-traffic kernel that "
                          "calibrates profiler byte counters)")
@@ -274,6 +442,12 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.serving_only:
+        sv = serving_bench(args)
+        print(json.dumps({"metric": METRIC, "value": sv["serving"]["solves_per_s"], "unit": "solves/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "config": {"workload": "request stream"}, "serving": sv}), flush=True)
+        return
     from bem3_aw_qd2_amd import exdat, hip_lib, host_lib, mesh
     hip = hip_lib()
     host = host_lib()
@@ -377,8 +551,14 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not args.no_lu32:
+        log("mixed-precision solve of the request's system")
+        line["lu32"] = lu32_leg(hip, prob, 2 * model.nn())
     with quiet_stdout():
         model.finalize()
+    if not args.no_serving:
+        log("serving: request stream")
+        line["serving"] = serving_bench(args)
     print(json.dumps(line), flush=True)
 
 

@@ -207,8 +207,11 @@ int bqd2_zgetrs(int n, int nrhs, const double *d_A, const int *d_ipiv, double *d
  * the mixed-precision solver (bqd2_set_solver(p, 2)); replaces the same MKL
  * solves as bqd2_zgetrf.  Column-major, leading dimension n, (re, im) floats.
  * d_ws: device workspace of bqd2_cgetrf_wsize(n) bytes that keeps the inverted
- * diagonal blocks and the row permutation for bqd2_cgetrs.  bqd2_cgetrs solves
- * in place on d_B (n x nrhs); d_tmp holds n x nrhs complex floats. */
+ * diagonal blocks and the row permutation for bqd2_cgetrs.  d_A receives the
+ * factors in the form the solves use: the 128 x 128 diagonal blocks hold L and
+ * U, every off-diagonal block is multiplied by the inverse of its column's
+ * diagonal block.  bqd2_cgetrs solves in place on d_B (n x nrhs); d_tmp holds
+ * n x nrhs complex floats. */
 size_t bqd2_cgetrf_wsize(int n);
 int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream);
 int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B, float *d_tmp, void *stream);
