@@ -154,7 +154,35 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
   }
   {
-    // lattice sum on the pair's rule (GL12 for the remainder of a split pair)
+    // lattice sum on the pair's rule (GL12 for the remainder of a split pair).
+    // Only the offset and the accumulators stay in registers across the lattice
+    // sum: the pair's element, target node and tangents sit in LDS per pair, and
+    // the point's weights for the accumulation are formed before the sum and
+    // parked in LDS per lane (the empty asm statements keep the compiler from
+    // holding LDS values in registers across the sum instead).
+    __shared__ double sgeo[PPW][30];  // cr 12, cw 9, rt 3, vtz 3, vte 3
+    __shared__ double spt[11][WAVE];  // per lane: g weights 4, h weights 4, W 3
+    __shared__ double sacc[16][WAVE]; // per lane: g (re, im) 4, h (re, im) 4
+    const int grp = lane / LPP;
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      sacc[4 * ep][lane] = g[ep].x; sacc[4 * ep + 1][lane] = g[ep].y;
+      sacc[4 * ep + 2][lane] = h[ep].x; sacc[4 * ep + 3][lane] = h[ep].y;
+    }
+    if (sl == 0) {
+      double *G = sgeo[grp];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) G[4 * d + j] = E.cr[d][j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) G[12 + 3 * d + j] = E.cw[d][j];
+        G[21 + d] = rt[d];
+        G[24 + d] = vtz[d];
+        G[27 + d] = vte[d];
+      }
+    }
+    __syncthreads();
     const bool gh = rule == 1;
     const int n = gh ? 64 : 12;
     const double *x = gh ? T.xhi : T.xli;
@@ -162,32 +190,60 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
     const int npts = on ? rule_npts(tri, n) : 0;
     nev = (sl == 0) ? npts : 0;
     for (int p = sl; p < npts; p += LPP) {
+      asm volatile("" ::: "memory");
+      const double *G = sgeo[grp];
+      ElemC Ep;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) Ep.cr[d][j] = G[4 * d + j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) Ep.cw[d][j] = G[12 + 3 * d + j];
+      }
       double z, e, wt;
       rule_point(tri, n, x, w, p, z, e, wt);
       PtGeo q;
-      point_geo(E, z, e, rt, vtz, vte, q);
-      Sum10 gf;
-      int c = qpgf_eval<GF_V1>(P, q.vR[0], q.vR[1], q.vR[2], 1.0e-15, gf, tl);
-      if (c < 0) err = c;
-      if (rule == 2) {
-        cd g0, dg0[3];
-        free_gf(P.k, q.vR, q.aR, q.iR, g0, dg0);
-        gf.g -= g0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) gf.dg[d] -= dg0[d];
-      }
-      cd hn = gf.dg[0] * q.vW[0] + gf.dg[1] * q.vW[1] + gf.dg[2] * q.vW[2];
+      point_geo(Ep, z, e, G + 21, G + 24, G + 27, q);
 #pragma unroll
       for (int ep = 0; ep < 4; ep++) {
-        double Np = shape_N(tri, ep, z, e);
-        g[ep] += gf.g * (wt * Np * q.aW);
-        h[ep] += hn * (wt * Np);
+        const double Np = shape_N(tri, ep, z, e);
+        spt[ep][lane] = wt * Np * q.aW;
+        spt[4 + ep][lane] = wt * Np;
       }
+#pragma unroll
+      for (int d = 0; d < 3; d++) spt[8 + d][lane] = q.vW[d];
       if (rule != 2) {
         F += wt * q.rdW * q.iR * q.iR;
         F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
         F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
       }
+      const double vR[3] = {q.vR[0], q.vR[1], q.vR[2]};
+      asm volatile("" ::: "memory");
+      Sum10 gf;
+      int c = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+      if (c < 0) err = c;
+      asm volatile("" ::: "memory");
+      if (rule == 2) {
+        cd g0, dg0[3];
+        const double aR = abs3(vR);
+        free_gf(P.k, vR, aR, 1.0 / aR, g0, dg0);
+        gf.g -= g0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) gf.dg[d] -= dg0[d];
+      }
+      cd hn = gf.dg[0] * spt[8][lane] + gf.dg[1] * spt[9][lane] + gf.dg[2] * spt[10][lane];
+#pragma unroll
+      for (int ep = 0; ep < 4; ep++) {
+        const double cg = spt[ep][lane], ch = spt[4 + ep][lane];
+        sacc[4 * ep][lane] += gf.g.x * cg; sacc[4 * ep + 1][lane] += gf.g.y * cg;
+        sacc[4 * ep + 2][lane] += hn.x * ch; sacc[4 * ep + 3][lane] += hn.y * ch;
+      }
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      g[ep] = cd{sacc[4 * ep][lane], sacc[4 * ep + 1][lane]};
+      h[ep] = cd{sacc[4 * ep + 2][lane], sacc[4 * ep + 3][lane]};
     }
   }
 #pragma unroll
