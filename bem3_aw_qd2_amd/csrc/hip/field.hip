@@ -228,7 +228,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
   const int npt = rule_points(type, X.any_tri);
   const double k = V.k;
-  cd Pa = {}, dPa[3] = {}, PVa[3] = {};
+  // Accumulators and the per-item coefficients live in LDS (per lane), so that
+  // only the offset stays in registers across the Hessian lattice sum; the empty
+  // asm statements keep the compiler from caching LDS values across it.
+  //   sacc: Pa, dPa[3], PVa[3] (re, im)
+  //   scf:  alpha = wt aW dPi, beta = -wt sg Pi, gamma[c] = wt aW dpv_c,
+  //         delta[c] = -wt sg pv_c (re, im), W[3]
+  __shared__ double sacc[14][64];
+  __shared__ double scf[19][64];
+#pragma unroll
+  for (int i = 0; i < 14; i++) sacc[i][lane] = 0.0;
   double F = 0.0, dF[3] = {0, 0, 0};
   GfTally tl{0, 0, 0, 0};
   long long evals = 0;
@@ -256,43 +265,74 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
       }
     }
-    cd a, b, dGp[3], dHp[3];
-    double sg = 1.0;
-    if (periodic) {
-      Sum10 gf;
-      int c = qpgf_eval<GF_V2>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
-      if (c < 0) err = c;
-      evals++;
-      a = gf.g * aW;
-      b = gf.dg[0] * w[0] + gf.dg[1] * w[1] + gf.dg[2] * w[2];
-      // Hessian as a symmetric matrix: xx, yy, zz, xy, yz, zx
-      const cd *h = gf.d2g;
-      cd Hw[3];
-      Hw[0] = h[0] * w[0] + h[3] * w[1] + h[5] * w[2];
-      Hw[1] = h[3] * w[0] + h[1] * w[1] + h[4] * w[2];
-      Hw[2] = h[5] * w[0] + h[4] * w[1] + h[2] * w[2];
-      for (int c2 = 0; c2 < 3; c2++) {
-        dGp[c2] = -(gf.dg[c2] * aW);
-        dHp[c2] = -Hw[c2];
-      }
-    } else {
-      cd ce = cis(k * aR), cb = cd{-1.0, k * aR};
-      a = ce * (BQD2_IFP * iR * aW);
-      b = cb * ce * (BQD2_IFP * rdW * iR * iR);
-      cd c3 = cd{k * k * aR * aR + 3.0 * cb.x, 3.0 * cb.y};
-      for (int c2 = 0; c2 < 3; c2++) {
-        dGp[c2] = cb * ce * (-BQD2_IFP * iR * iR * vR[c2] * iR * aW);
-        dHp[c2] = ce * (BQD2_IFP * iR * iR * iR) * (c3 * (vR[c2] * iR * rdW) - cb * w[c2]);
-      }
-      sg = sd < 0 ? -1.0 : 1.0;
-    }
-    Pa += (a * dPi - b * Pi * sg) * wt;
-    for (int c2 = 0; c2 < 3; c2++) {
-      dPa[c2] += (dGp[c2] * dPi - dHp[c2] * Pi * sg) * wt;
-      PVa[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wt;
-    }
+    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
     F += sg * wt * rdW * iR * iR;
     for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
+    // per item: Pa += g alpha + (dg.W) beta, dPa_c += -dg_c alpha - (H W)_c beta,
+    // PVa_c += g gamma_c + (dg.W) delta_c, with G = g aW and H = dg.W the kernel
+    // values of the representation formula (free space or lattice sum)
+    const cd alpha = dPi * (wt * aW), beta = Pi * (-wt * sg);
+    cd gv, dgw, dgp[3], hwp[3];  // g, dg.W, dg_c, (H W)_c
+    if (periodic) {
+      scf[0][lane] = alpha.x; scf[1][lane] = alpha.y; scf[2][lane] = beta.x; scf[3][lane] = beta.y;
+      for (int c2 = 0; c2 < 3; c2++) {
+        const cd gm = dpvi[c2] * (wt * aW), dl = pvi[c2] * (-wt * sg);
+        scf[4 + 2 * c2][lane] = gm.x; scf[5 + 2 * c2][lane] = gm.y;
+        scf[10 + 2 * c2][lane] = dl.x; scf[11 + 2 * c2][lane] = dl.y;
+        scf[16 + c2][lane] = w[c2];
+      }
+      const double x0 = vR[0], x1 = vR[1], x2 = vR[2];
+      asm volatile("" ::: "memory");
+      Sum10 gf;
+      int c = qpgf_eval<GF_V2>(P, x0, x1, x2, 1.0e-15, gf, tl);
+      if (c < 0) err = c;
+      evals++;
+      asm volatile("" ::: "memory");
+      const double W0 = scf[16][lane], W1 = scf[17][lane], W2 = scf[18][lane];
+      // Hessian as a symmetric matrix: xx, yy, zz, xy, yz, zx
+      const cd *h = gf.d2g;
+      gv = gf.g;
+      dgw = gf.dg[0] * W0 + gf.dg[1] * W1 + gf.dg[2] * W2;
+      hwp[0] = h[0] * W0 + h[3] * W1 + h[5] * W2;
+      hwp[1] = h[3] * W0 + h[1] * W1 + h[4] * W2;
+      hwp[2] = h[5] * W0 + h[4] * W1 + h[2] * W2;
+      for (int c2 = 0; c2 < 3; c2++) dgp[c2] = gf.dg[c2];
+    } else {
+      // free space: G = e^{ikR}/(4 pi R) aW, dG/dr_c and (H W)_c in closed form
+      for (int c2 = 0; c2 < 3; c2++) {
+        const cd gm = dpvi[c2] * (wt * aW), dl = pvi[c2] * (-wt * sg);
+        scf[4 + 2 * c2][lane] = gm.x; scf[5 + 2 * c2][lane] = gm.y;
+        scf[10 + 2 * c2][lane] = dl.x; scf[11 + 2 * c2][lane] = dl.y;
+      }
+      cd ce = cis(k * aR), cb = cd{-1.0, k * aR};
+      gv = ce * (BQD2_IFP * iR);
+      dgw = cb * ce * (BQD2_IFP * rdW * iR * iR);
+      cd c3 = cd{k * k * aR * aR + 3.0 * cb.x, 3.0 * cb.y};
+      for (int c2 = 0; c2 < 3; c2++) {
+        dgp[c2] = cb * ce * (BQD2_IFP * iR * iR * vR[c2] * iR);
+        hwp[c2] = -(ce * (BQD2_IFP * iR * iR * iR) * (c3 * (vR[c2] * iR * rdW) - cb * w[c2]));
+      }
+    }
+    const cd al = periodic ? cd{scf[0][lane], scf[1][lane]} : alpha;
+    const cd be = periodic ? cd{scf[2][lane], scf[3][lane]} : beta;
+    {
+      const cd v = gv * al + dgw * be;
+      sacc[0][lane] += v.x; sacc[1][lane] += v.y;
+    }
+    for (int c2 = 0; c2 < 3; c2++) {
+      const cd d = -(dgp[c2] * al) - hwp[c2] * be;
+      sacc[2 + 2 * c2][lane] += d.x; sacc[3 + 2 * c2][lane] += d.y;
+      const cd gm = cd{scf[4 + 2 * c2][lane], scf[5 + 2 * c2][lane]};
+      const cd dl = cd{scf[10 + 2 * c2][lane], scf[11 + 2 * c2][lane]};
+      const cd pvv = gv * gm + dgw * dl;
+      sacc[8 + 2 * c2][lane] += pvv.x; sacc[9 + 2 * c2][lane] += pvv.y;
+    }
+  }
+  asm volatile("" ::: "memory");
+  cd Pa = cd{sacc[0][lane], sacc[1][lane]}, dPa[3], PVa[3];
+  for (int c2 = 0; c2 < 3; c2++) {
+    dPa[c2] = cd{sacc[2 + 2 * c2][lane], sacc[3 + 2 * c2][lane]};
+    PVa[c2] = cd{sacc[8 + 2 * c2][lane], sacc[9 + 2 * c2][lane]};
   }
   Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
   for (int c2 = 0; c2 < 3; c2++) {
