@@ -103,16 +103,18 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
-                                                   const int *__restrict__ list, long long nlist,
-                                                   long long *__restrict__ counters) {
+// The pairs PPW wg .. PPW wg + PPW - 1 of the list (one wavefront); wg is the
+// workgroup of a launch over one domain, or a task of the batched launch.
+__device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
+                                              const GfParams &P, const int *__restrict__ list, long long nlist,
+                                              long long *__restrict__ counters, long long wg) {
   const int lane = threadIdx.x, sl = lane % LPP;
-  const long long li = (long long)blockIdx.x * PPW + lane / LPP;
+  const long long li = wg * PPW + lane / LPP;
   // padding entries (the end of a class) take the wavefront's first pair and
   // add nothing: their loops are empty and they write nothing
   const int pair = (li < nlist) ? list[li] : -1;
   const bool on = pair >= 0;
-  const int gid = on ? pair : list[(long long)blockIdx.x * PPW];
+  const int gid = on ? pair : list[wg * PPW];
   const int row = gid / D.Ne, s = gid % D.Ne + 1;
   int t, tn, td;
   row_target(M, D, row, t, tn, td);
@@ -270,6 +272,37 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   if (lane == 0) {
     tally_add(counters + CNT_PAIRS, tsp, tpr, tev, tfd, nev);
     if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
+  }
+}
+
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                   const int *__restrict__ list, long long nlist,
+                                                   long long *__restrict__ counters) {
+  q0_pairs_wave(M, tab, D, P, list, nlist, counters, blockIdx.x);
+}
+
+// One instance of a batched assembly: the periodic domain of one problem.
+struct PairBatch {
+  DevMesh M;
+  DevDomain D;
+  GfParams P;
+  const Tab *tab;
+  long long *counters;
+  long long wg_end;  // tasks of this and the earlier instances
+};
+
+// The pair tasks of several problems in one persistent launch: workgroup b takes
+// the tasks b, b + gridDim.x, ... of the concatenated task lists (every
+// workgroup stops at the total; no workgroup waits for another).  A pair is
+// computed the same way whichever workgroup takes it.
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_pairs_batch(const PairBatch *__restrict__ inst, int ninst) {
+  const long long total = inst[ninst - 1].wg_end;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    int i = 0;
+    while (t >= inst[i].wg_end) i++;
+    const PairBatch &B = inst[i];
+    const long long t0 = i ? inst[i - 1].wg_end : 0;
+    q0_pairs_wave(B.M, B.tab, B.D, B.P, B.D.plist, (long long)B.D.plist_n, B.counters, t - t0);
   }
 }
 
@@ -1004,18 +1037,23 @@ static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule
 }
 
 // ------------------------------------------------------------------ entry point
-extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
-  BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_assemble: bad domain %d", did);
+// Assembly in three phases: clear the outputs and (first time) decide the rules
+// and build the pair list; the off-diagonal pairs; the self blocks, row sums and
+// counters.  bqd2_assemble runs them for one domain, bqd2_assemble_batch runs
+// the pairs of several problems' periodic domains as one launch.
+struct AsmPhase {
+  hipEvent_t e0, e1, e1b, e2, e3;
+};
+
+static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
   DevDomain &D = p->dom[did];
-  if (D.Ne == 0) return BQD2_OK;
   const bool periodic = (did == 0 && p->m.ps == 1);
   hipStream_t st_ = p->stream;
-  hipEvent_t e0, e1, e1b, e2, e3;
-  BQD2_HIP(hipEventCreate(&e0));
-  BQD2_HIP(hipEventCreate(&e1));
-  BQD2_HIP(hipEventCreate(&e1b));
-  BQD2_HIP(hipEventCreate(&e2));
-  BQD2_HIP(hipEventCreate(&e3));
+  BQD2_HIP(hipEventCreate(&ev.e0));
+  BQD2_HIP(hipEventCreate(&ev.e1));
+  BQD2_HIP(hipEventCreate(&ev.e1b));
+  BQD2_HIP(hipEventCreate(&ev.e2));
+  BQD2_HIP(hipEventCreate(&ev.e3));
   BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st_));
   BQD2_HIP(hipMemsetAsync(D.G, 0, (size_t)D.N * D.N * 16, st_));
   BQD2_HIP(hipMemsetAsync(D.H, 0, (size_t)D.N * D.N * 16, st_));
@@ -1024,7 +1062,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     BQD2_HIP(hipMemsetAsync(D.dH, 0, (size_t)2 * D.N * D.N * 16, st_));
   }
   BQD2_HIP(hipMemsetAsync(D.Fpair, 0, (size_t)D.N * D.Ne * 3 * 8, st_));
-  BQD2_HIP(hipEventRecord(e0, st_));
+  BQD2_HIP(hipEventRecord(ev.e0, st_));
   long long npairs = (long long)D.N * D.Ne;
   // the rules and the pair list depend on the geometry and the lattice only:
   // decided at the first assembly of the domain, reused afterwards
@@ -1034,7 +1072,7 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
                        periodic ? 1 : 0);
     BQD2_HIP(hipGetLastError());
   }
-  BQD2_HIP(hipEventRecord(e1, st_));
+  BQD2_HIP(hipEventRecord(ev.e1, st_));
   if (first) {
     std::vector<unsigned char> rule((size_t)npairs);
     BQD2_HIP(hipMemcpyAsync(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost, st_));
@@ -1057,16 +1095,14 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     }
     D.plist_n = periodic ? (int)plist.size() : 1;
   }
-  BQD2_HIP(hipEventRecord(e1b, st_));
-  if (periodic) {
-    if (D.plist_n > 0)
-      hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(D.plist_n / PPW)), dim3(WAVE), 0, st_, p->m, p->tab, D,
-                         p->gf, D.plist, (long long)D.plist_n, p->d_counters);
-  } else {
-    hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
-  }
-  BQD2_HIP(hipGetLastError());
-  BQD2_HIP(hipEventRecord(e2, st_));
+  BQD2_HIP(hipEventRecord(ev.e1b, st_));
+  return BQD2_OK;
+}
+
+static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *st) {
+  DevDomain &D = p->dom[did];
+  const bool periodic = (did == 0 && p->m.ps == 1);
+  hipStream_t st_ = p->stream;
   if (periodic) {
     hipLaunchKernelGGL(k_q0_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
     if (D.has_tri)
@@ -1078,17 +1114,17 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
   BQD2_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_rows, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st_, p->m, D);
   BQD2_HIP(hipGetLastError());
-  BQD2_HIP(hipEventRecord(e3, st_));
-  BQD2_HIP(hipEventSynchronize(e3));
+  BQD2_HIP(hipEventRecord(ev.e3, st_));
+  BQD2_HIP(hipEventSynchronize(ev.e3));
   long long cnt[CNT_N];
   BQD2_HIP(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
   const long long ngh = D.n_gh, nsplit = D.n_split;
   if (st) {
     memset(st, 0, sizeof(*st));
     float a, b, c;
-    BQD2_HIP(hipEventElapsedTime(&a, e0, e1));
-    BQD2_HIP(hipEventElapsedTime(&b, e1b, e2));
-    BQD2_HIP(hipEventElapsedTime(&c, e2, e3));
+    BQD2_HIP(hipEventElapsedTime(&a, ev.e0, ev.e1));
+    BQD2_HIP(hipEventElapsedTime(&b, ev.e1b, ev.e2));
+    BQD2_HIP(hipEventElapsedTime(&c, ev.e2, ev.e3));
     st->ms_decide = a; st->ms_pairs = b; st->ms_self = c;
     for (int i = 0; i < 4; i++) {
       st->pair_terms[i] = cnt[CNT_PAIRS + i];
@@ -1101,15 +1137,103 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
     for (int i = 0; i < 4; i++) st->gf_terms += st->pair_terms[i] + st->self_terms[i];
     st->pairs_gh = ngh;
     st->pairs_split = nsplit;
+    (void)periodic;
     const long long valid_rows = D.nn;  // collocation nodes: 4 per quadrilateral, 3 per triangle
     st->self_rows = valid_rows;
     st->pairs_gl = valid_rows * (D.Ne - 1) - ngh;
   }
-  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e1b);
-  (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+  (void)hipEventDestroy(ev.e0); (void)hipEventDestroy(ev.e1); (void)hipEventDestroy(ev.e1b);
+  (void)hipEventDestroy(ev.e2); (void)hipEventDestroy(ev.e3);
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)",
              cnt[CNT_ERR], did);
   D.assembled = true;
   p->lu32_valid = false;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_assemble: bad domain %d", did);
+  DevDomain &D = p->dom[did];
+  if (D.Ne == 0) return BQD2_OK;
+  const bool periodic = (did == 0 && p->m.ps == 1);
+  hipStream_t st_ = p->stream;
+  AsmPhase ev;
+  int rc = asm_begin(p, did, ev);
+  if (rc) return rc;
+  if (periodic) {
+    if (D.plist_n > 0)
+      hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(D.plist_n / PPW)), dim3(WAVE), 0, st_, p->m, p->tab, D,
+                         p->gf, D.plist, (long long)D.plist_n, p->d_counters);
+  } else {
+    hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+  }
+  BQD2_HIP(hipGetLastError());
+  BQD2_HIP(hipEventRecord(ev.e2, st_));
+  return asm_end(p, did, ev, st);
+}
+
+// No counterpart in the reference.  Domain did of every problem; the
+// off-diagonal pairs of the periodic domains of all of them in one persistent
+// launch on the first problem's stream (other domains as bqd2_assemble).
+extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_assembly_stats *st) {
+  BQD2_CHECK(ps && count > 0, BQD2_E_ARG, "bqd2_assemble_batch: no problems");
+  for (int i = 0; i < count; i++)
+    BQD2_CHECK(ps[i] && did >= 0 && did <= ps[i]->m.MN, BQD2_E_ARG, "bqd2_assemble_batch: bad problem %d or domain", i);
+  std::vector<AsmPhase> ev(count);
+  std::vector<int> batched;
+  for (int i = 0; i < count; i++) {
+    bqd2_problem *p = ps[i];
+    if (p->dom[did].Ne == 0) continue;
+    if (!(did == 0 && p->m.ps == 1)) {
+      int rc = bqd2_assemble(p, did, st ? st + i : nullptr);
+      if (rc) return rc;
+      continue;
+    }
+    int rc = asm_begin(p, did, ev[i]);
+    if (rc) return rc;
+    if (p->dom[did].plist_n > 0) batched.push_back(i);
+  }
+  if (!batched.empty()) {
+    std::vector<PairBatch> hb(batched.size());
+    long long tasks = 0;
+    for (size_t j = 0; j < batched.size(); j++) {
+      bqd2_problem *p = ps[batched[j]];
+      hb[j].M = p->m;
+      hb[j].D = p->dom[did];
+      hb[j].P = p->gf;
+      hb[j].tab = p->tab;
+      hb[j].counters = p->d_counters;
+      tasks += p->dom[did].plist_n / PPW;
+      hb[j].wg_end = tasks;
+    }
+    hipStream_t bs = ps[batched[0]]->stream;
+    PairBatch *d_inst = nullptr;
+    BQD2_HIP(hipMalloc(&d_inst, hb.size() * sizeof(PairBatch)));
+    BQD2_HIP(hipMemcpyAsync(d_inst, hb.data(), hb.size() * sizeof(PairBatch), hipMemcpyHostToDevice, bs));
+    // the launch waits for every problem's set-up, every problem waits for it
+    for (int i : batched) {
+      BQD2_HIP(hipStreamWaitEvent(bs, ev[i].e1b, 0));
+      BQD2_HIP(hipEventRecord(ev[i].e1b, bs));  // the pair time is the batch's
+    }
+    int dev = 0, ncu = 0;
+    BQD2_HIP(hipGetDevice(&dev));
+    BQD2_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long grid = std::min<long long>(tasks, (long long)ncu * 8);
+    hipLaunchKernelGGL(k_q0_pairs_batch, dim3((unsigned)grid), dim3(WAVE), 0, bs, d_inst, (int)hb.size());
+    BQD2_HIP(hipGetLastError());
+    for (int i : batched) {
+      BQD2_HIP(hipEventRecord(ev[i].e2, bs));
+      BQD2_HIP(hipStreamWaitEvent(ps[i]->stream, ev[i].e2, 0));
+    }
+    BQD2_HIP(hipStreamSynchronize(bs));
+    BQD2_HIP(hipFree(d_inst));
+  }
+  for (int i = 0; i < count; i++) {
+    bqd2_problem *p = ps[i];
+    if (p->dom[did].Ne == 0 || !(did == 0 && p->m.ps == 1)) continue;
+    if (p->dom[did].plist_n == 0) BQD2_HIP(hipEventRecord(ev[i].e2, p->stream));
+    int rc = asm_end(p, did, ev[i], st ? st + i : nullptr);
+    if (rc) return rc;
+  }
   return BQD2_OK;
 }

@@ -445,13 +445,25 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const int t = tpos[ip];
   const double zt = ze[2 * ip], et = ze[2 * ip + 1];
   const int td = V.sid[t], at = abs(td);
-  ElemC Et;
-  load_elem(M, at, Et);
-  double rt[3], wt_[3];
-  bil_rw(Et, zt, et, rt, wt_);
+  double rt[3];
+  {
+    ElemC Et;
+    load_elem(M, at, Et);
+    double wt_[3];
+    bil_rw(Et, zt, et, rt, wt_);
+  }
   const int npt = rule_points(type, X.any_tri);
   const double k = V.k;
-  cd Pa = {}, PVa[3] = {};
+  // Regular elements: accumulators and per-item coefficients in LDS (per lane),
+  // so that only the offset stays in registers across the lattice sum; the empty
+  // asm statements keep the compiler from caching LDS values across it.
+  //   bacc: Pa, PVa[3] (re, im)
+  //   bcf:  alpha = wt aW dPi, beta = -wt sg Pi, gamma[c] = wt aW dpv_c,
+  //         delta[c] = -wt sg pv_c (re, im), W[3]
+  __shared__ double bacc[8][64];
+  __shared__ double bcf[19][64];
+#pragma unroll
+  for (int i = 0; i < 8; i++) bacc[i][lane] = 0.0;
   double F = 0.0;
   GfTally tl{0, 0, 0, 0};
   long long evals = 0;
@@ -481,26 +493,47 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
       }
     }
-    cd a, b;
-    double sg = 1.0;
+    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+    F += sg * wt * tD;
+    // Pa += g alpha + (dg.W) beta, PVa_c += g gamma_c + (dg.W) delta_c with the
+    // kernel value g and its normal derivative dg.W (free space or lattice sum)
+    {
+      const cd al = dPi * (wt * aW), be = Pi * (-wt * sg);
+      bcf[0][lane] = al.x; bcf[1][lane] = al.y; bcf[2][lane] = be.x; bcf[3][lane] = be.y;
+      for (int c2 = 0; c2 < 3; c2++) {
+        const cd gm = dpvi[c2] * (wt * aW), dl = pvi[c2] * (-wt * sg);
+        bcf[4 + 2 * c2][lane] = gm.x; bcf[5 + 2 * c2][lane] = gm.y;
+        bcf[10 + 2 * c2][lane] = dl.x; bcf[11 + 2 * c2][lane] = dl.y;
+        bcf[16 + c2][lane] = w[c2];
+      }
+    }
+    cd gv, dgw;
     if (periodic) {
+      const double x0 = vR[0], x1 = vR[1], x2 = vR[2];
       Sum10 gf;
-      int c = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+      int c = qpgf_eval<GF_V1>(P, x0, x1, x2, 1.0e-15, gf, tl);
       if (c < 0) err = c;
       evals++;
-      a = gf.g * aW;
-      b = gf.dg[0] * w[0] + gf.dg[1] * w[1] + gf.dg[2] * w[2];
+      gv = gf.g;
+      dgw = gf.dg[0] * bcf[16][lane] + gf.dg[1] * bcf[17][lane] + gf.dg[2] * bcf[18][lane];
     } else {
       cd ce = cis(k * aR), cb = cd{-1.0, k * aR};
-      a = ce * (BQD2_IFP * iR * aW);
-      b = cb * ce * (BQD2_IFP * tD);
-      sg = sd < 0 ? -1.0 : 1.0;
+      gv = ce * (BQD2_IFP * iR);
+      dgw = cb * ce * (BQD2_IFP * tD);
     }
-    Pa += (a * dPi - b * Pi * sg) * wt;
-    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wt;
-    F += sg * wt * tD;
+    {
+      const cd v = gv * cd{bcf[0][lane], bcf[1][lane]} + dgw * cd{bcf[2][lane], bcf[3][lane]};
+      bacc[0][lane] += v.x; bacc[1][lane] += v.y;
+    }
+    for (int c2 = 0; c2 < 3; c2++) {
+      const cd v = gv * cd{bcf[4 + 2 * c2][lane], bcf[5 + 2 * c2][lane]} +
+                   dgw * cd{bcf[10 + 2 * c2][lane], bcf[11 + 2 * c2][lane]};
+      bacc[2 + 2 * c2][lane] += v.x; bacc[3 + 2 * c2][lane] += v.y;
+    }
   }
   F *= BQD2_IFP;
+  ElemC Et;
+  load_elem(M, at, Et);
   if (ic != nci) {
     // item chunk: no self element
   } else if (M.tri[at]) {
@@ -607,8 +640,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2]) * wgt;
       }
     }
-    Pa += Ps;
-    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += PVs[c2];
+    bacc[0][lane] += Ps.x; bacc[1][lane] += Ps.y;
+    for (int c2 = 0; c2 < 3; c2++) { bacc[2 + 2 * c2][lane] += PVs[c2].x; bacc[3 + 2 * c2][lane] += PVs[c2].y; }
   } else {
   // self element: polar rule about (zt, et)
     const int sd = td, s = t;
@@ -715,10 +748,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         for (int c2 = 0; c2 < 3; c2++) PVs[c2] += (a * dpvi[c2] - b * pvi[c2] * sg) * wgt;
       }
     }
-    Pa += Ps;
-    for (int c2 = 0; c2 < 3; c2++) PVa[c2] += PVs[c2];
+    bacc[0][lane] += Ps.x; bacc[1][lane] += Ps.y;
+    for (int c2 = 0; c2 < 3; c2++) { bacc[2 + 2 * c2][lane] += PVs[c2].x; bacc[3 + 2 * c2][lane] += PVs[c2].y; }
     F += sg * Fs * BQD2_IFP * bdC;
   }
+  asm volatile("" ::: "memory");
+  cd Pa = cd{bacc[0][lane], bacc[1][lane]}, PVa[3];
+  for (int c2 = 0; c2 < 3; c2++) PVa[c2] = cd{bacc[2 + 2 * c2][lane], bacc[3 + 2 * c2][lane]};
   Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
   for (int c2 = 0; c2 < 3; c2++) PVa[c2] = cd{wave_sum(PVa[c2].x), wave_sum(PVa[c2].y)};
   F = wave_sum(F);

@@ -124,6 +124,13 @@ int bqd2_destroy(bqd2_problem *p);
  * N = 4 Ne_d, row (t-1)*4+tn, column (s-1)*4+n. st may be NULL. */
 int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st);
 
+/* No counterpart in the reference.  bqd2_assemble of domain did for `count`
+ * problems, with the off-diagonal pairs of their periodic domains computed in
+ * one persistent launch (uneven instances share the GPU; each pair is computed
+ * exactly as bqd2_assemble computes it).  st: count entries or NULL; the pair
+ * times are those of the whole batch. */
+int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_assembly_stats *st);
+
 /* Device counters of the last assembly, field or force call: [0] failed Green's
  * function evaluations; then for the off-diagonal pair kernel [1..5], the
  * self-element kernel [6..10] and the field kernels [11..15]: terms summed by

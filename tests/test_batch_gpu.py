@@ -1,0 +1,70 @@
+"""Assembly of several problems' periodic domains in one persistent launch
+(bqd2_assemble_batch) against assembly one problem at a time (bqd2_assemble).
+
+Three uneven instances (24 quads on the 2 mm square lattice, 96 quads on the
+1.8 mm triangular lattice at theta = 30 deg, 108 triangles on a 2.4 mm square
+lattice at 0.8 MHz) and a free-space problem in the same batch.
+Each pair of a periodic domain is computed the same way whichever workgroup of
+the persistent launch takes it, so G, H and dF must be bit-identical to the
+one-at-a-time assembly, and so must the counters of Green's function terms."""
+import ctypes as C
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from bem3_aw_qd2_amd import AssemblyStats, exdat, hip_lib, host_lib, mesh
+
+pytestmark = pytest.mark.gpu
+INSTANCES = [
+    ("sphere24", {}, False, True),
+    ("sphere96", dict(d1=(1.8e-3, 0.0), d2=(0.9e-3, 0.9e-3 * math.sqrt(3.0)), theta=math.pi / 6,
+                      psi=math.pi / 12), False, True),
+    ("sphere24", dict(n=3, d1=(2.4e-3, 0.0), d2=(0.0, 2.4e-3), f=0.8e6, theta=0.4, psi=0.3), True, True),
+    ("sphere24", {}, False, False),
+]
+
+
+def problem(tmp, name, over, tri, periodic):
+    paths = mesh.write_config(name, tmp, triangles=tri, periodic=periodic, **over)
+    m = exdat.Model(host_lib()).read_inputs(*paths, out=os.path.join(tmp, "ex.dat")).initialize()
+    return m, host_lib().aqd2_problem(m.buf, None)
+
+
+def matrices(prob, n):
+    G = np.zeros(n * n * 2)
+    H = np.zeros(n * n * 2)
+    dF = np.zeros(n * 3)
+    assert hip_lib().bqd2_download_domain(prob, 0, G.ctypes.data, H.ctypes.data, None, None, dF.ctypes.data) == 0
+    return G, H, dF
+
+
+def test_batch_equals_one_at_a_time():
+    hip = hip_lib()
+    with tempfile.TemporaryDirectory() as tmp:
+        models, probs = [], []
+        for i, (name, over, tri, per) in enumerate(INSTANCES):
+            m, p = problem(os.path.join(tmp, str(i)), name, over, tri, per)
+            models.append(m)
+            probs.append(p)
+        single, sstats = [], []
+        for m, p in zip(models, probs):
+            st = AssemblyStats()
+            assert hip.bqd2_assemble(p, 0, C.byref(st)) == 0
+            single.append(matrices(p, 4 * m.ne()))
+            sstats.append(st)
+        arr = (C.c_void_p * len(probs))(*probs)
+        bst = (AssemblyStats * len(probs))()
+        assert hip.bqd2_assemble_batch(arr, len(probs), 0, bst) == 0
+        for i, (m, p) in enumerate(zip(models, probs)):
+            G, H, dF = matrices(p, 4 * m.ne())
+            same = [np.array_equal(a, b) for a, b in zip((G, H, dF), single[i])]
+            print("instance %d (%d elements): G, H, dF identical %s; terms %s / %s" %
+                  (i, m.ne(), same, list(bst[i].pair_terms), list(sstats[i].pair_terms)))
+            assert all(same)
+            assert list(bst[i].pair_terms) == list(sstats[i].pair_terms)
+            assert bst[i].pair_evals == sstats[i].pair_evals
+        for m in models:
+            m.finalize()
