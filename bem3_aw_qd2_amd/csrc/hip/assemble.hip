@@ -1147,6 +1147,7 @@ static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_assemble: %lld Green's function evaluations failed (domain %d)",
              cnt[CNT_ERR], did);
   D.assembled = true;
+  D.g_lu_valid = false;
   p->lu32_valid = false;
   return BQD2_OK;
 }

@@ -255,7 +255,7 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
     for (int j = 0; j < m && total_its < maxit; j++) {
       if (pc) {
         int rc2 = bqd2_lu32_apply(n, nrhs, pc->Af, pc->ws, V + 2 * (size_t)j * n, vstride, Z, (long long)n, pc->tmp,
-                                  st);
+                                  st, pc->graph);
         if (rc2) return rc2;
         launch_mv(nrhs, n, A, lda, map, dinv, Z, (long long)n, W, (long long)n, st);
       } else {
@@ -325,7 +325,7 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
       // X += M^-1 (V y)
       BQD2_HIP(hipMemsetAsync(Z, 0, bN, st));
       hipLaunchKernelGGL(k_gm_update, gv, b256, 0, st, n, k, V, (long long)n, vstride, Y, m, Z, (long long)n);
-      int rc2 = bqd2_lu32_apply(n, nrhs, pc->Af, pc->ws, Z, (long long)n, Z, (long long)n, pc->tmp, st);
+      int rc2 = bqd2_lu32_apply(n, nrhs, pc->Af, pc->ws, Z, (long long)n, Z, (long long)n, pc->tmp, st, pc->graph);
       if (rc2) return rc2;
       hipLaunchKernelGGL(k_gm_add, gv, b256, 0, st, n, Z, (long long)n, X, ldx);
     } else {

@@ -744,9 +744,11 @@ extern "C" int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, 
 
 // ---------------------------------------------------------------- FP64 <-> FP32 for the refinement
 // Af (column-major complex float) = rows of the row-major complex double A, each
-// scaled by dinv[r]; 32 x 32 tiles through LDS so that reads and writes coalesce.
+// scaled by dinv[r] (rows with map[r] < 0: identity rows); 32 x 32 tiles through
+// LDS so that reads and writes coalesce.
 __global__ void __launch_bounds__(256) k_rm64_to_cm32(int n, const double *__restrict__ A, long long lda,
-                                                      const double *__restrict__ dinv, float2 *__restrict__ Af) {
+                                                      const int *__restrict__ map, const double *__restrict__ dinv,
+                                                      float2 *__restrict__ Af) {
   __shared__ float2 t[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
@@ -754,9 +756,13 @@ __global__ void __launch_bounds__(256) k_rm64_to_cm32(int n, const double *__res
     const int r = r0 + i, c = c0 + tx;
     float2 v = make_float2(0.0f, 0.0f);
     if (r < n && c < n) {
-      const double *a = A + 2 * ((size_t)r * lda + c);
-      const double dr = dinv ? dinv[2 * r] : 1.0, di = dinv ? dinv[2 * r + 1] : 0.0;
-      v = make_float2((float)(a[0] * dr - a[1] * di), (float)(a[0] * di + a[1] * dr));
+      if (map && map[r] < 0) {
+        v = make_float2(r == c ? 1.0f : 0.0f, 0.0f);  // unused row: identity, as GMRES treats it
+      } else {
+        const double *a = A + 2 * ((size_t)r * lda + c);
+        const double dr = dinv ? dinv[2 * r] : 1.0, di = dinv ? dinv[2 * r + 1] : 0.0;
+        v = make_float2((float)(a[0] * dr - a[1] * di), (float)(a[0] * di + a[1] * dr));
+      }
     }
     t[i][tx] = v;
   }
@@ -787,23 +793,48 @@ __global__ void __launch_bounds__(256) k_f2d(int n, const float2 *__restrict__ y
 }
 
 // round the row-scaled system to complex float, column-major
-int bqd2_lu32_convert(int n, const double *A, long long lda, const double *dinv, float *Af, hipStream_t st) {
+int bqd2_lu32_convert(int n, const double *A, long long lda, const int *map, const double *dinv, float *Af,
+                      hipStream_t st) {
   dim3 g(bqd2_blocks(n, 32), bqd2_blocks(n, 32));
-  hipLaunchKernelGGL(k_rm64_to_cm32, g, dim3(256), 0, st, n, A, lda, dinv, (float2 *)Af);
+  hipLaunchKernelGGL(k_rm64_to_cm32, g, dim3(256), 0, st, n, A, lda, map, dinv, (float2 *)Af);
   BQD2_HIP(hipGetLastError());
   return BQD2_OK;
 }
 
 // out = M^-1 in for complex double vectors (nrhs columns, leading dimension ld),
 // M = P^T L U the complex float factors; tmp holds 2 n nrhs complex floats
+// Sweeps as a HIP graph (0: launched one by one; for tests that compare the two)
+static bool g_use_graphs = true;
+extern "C" int bqd2_set_graphs(int on) {
+  g_use_graphs = on != 0;
+  return BQD2_OK;
+}
+
 int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const double *in, long long ldi, double *out,
-                    long long ldo, float *tmp, hipStream_t st) {
+                    long long ldo, float *tmp, hipStream_t st, hipGraphExec_t *graph) {
   LuWs W = lu_ws(n, (void *)ws);
   float2 *T = (float2 *)tmp;
   dim3 g(bqd2_blocks(n, 256), nrhs);
   hipLaunchKernelGGL(k_gather_d2f, g, dim3(256), 0, st, n, W.perm, in, ldi, T);
-  int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, T + (size_t)n * nrhs, n, st);
-  if (rc) return rc;
+  if (graph && g_use_graphs) {
+    // the block steps of both sweeps read and write fixed buffers (the factors,
+    // the workspace and tmp): captured once per preconditioner and replayed
+    if (!*graph) {
+      hipGraph_t gr = nullptr;
+      BQD2_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, T + (size_t)n * nrhs, n, st);
+      hipError_t e = hipStreamEndCapture(st, &gr);
+      if (rc) return rc;
+      BQD2_HIP(e);
+      e = hipGraphInstantiate(graph, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      BQD2_HIP(e);
+    }
+    BQD2_HIP(hipGraphLaunch(*graph, st));
+  } else {
+    int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, T + (size_t)n * nrhs, n, st);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_f2d, g, dim3(256), 0, st, n, T, out, ldo);
   BQD2_HIP(hipGetLastError());
   return BQD2_OK;

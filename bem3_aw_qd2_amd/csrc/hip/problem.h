@@ -46,6 +46,14 @@ struct DevDomain {
   double *P, *dP;      // (Ne+1) x 4 complex
   double *pv, *dpv;    // (Ne+1) x 4 x 3 complex
   bool assembled;
+  // mixed-precision solver: complex float factors of the row-scaled G for the
+  // dpv systems (N x N, identity on unused rows), valid until the next assembly
+  float *gAf, *gtmp;   // N x N; 2 x N x 3
+  void *gws;           // bqd2_cgetrf_wsize(N)
+  double *gdinv;       // N complex
+  int *gipiv;          // N
+  bool g_lu_valid;
+  hipGraphExec_t g_graph;  // its triangular sweeps (3 right-hand sides) as a graph
 };
 
 // execution context of one calling host thread for the field and force entry
@@ -89,6 +97,7 @@ struct bqd2_problem {
   void *luws;              // bqd2_cgetrf_wsize(na) bytes
   float *lutmp;            // 2 x na x 3 complex float
   double *dinv;            // na complex: inverse diagonal of the system
+  hipGraphExec_t lu_graph; // the triangular sweeps of its factors as a graph
   bool lu32_valid;
   double ms_factor;        // time of the last factorisation (conversion + LU)
   double *hk2d;            // k2[m] / k2[s] table (16 x 16 complex), set once
@@ -115,6 +124,7 @@ struct Precond {
   const float *Af;
   const void *ws;
   float *tmp;
+  hipGraphExec_t *graph;  // the sweeps, captured at the first application (owned by the factors' holder)
 };
 // restarted GMRES on nrhs <= 3 right-hand sides sharing the row-major matrix A
 // (gmres.hip), right-preconditioned by pc when it is not null; returns 0 when
@@ -125,9 +135,10 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
                const Precond *pc = nullptr);
 // lu32.hip: the row-scaled row-major complex double matrix rounded to column-major
 // complex float; out = M^-1 in with the factors (in may be out)
-int bqd2_lu32_convert(int n, const double *A, long long lda, const double *dinv, float *Af, hipStream_t st);
+int bqd2_lu32_convert(int n, const double *A, long long lda, const int *map, const double *dinv, float *Af,
+                      hipStream_t st);
 int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const double *in, long long ldi, double *out,
-                    long long ldo, float *tmp, hipStream_t st);
+                    long long ldo, float *tmp, hipStream_t st, hipGraphExec_t *graph);
 // gmres.hip: dinv[r] = 1 / A[r][r] (complex), 1 on rows with map[r] < 0
 int bqd2_jacobi(int n, const double *A, long long lda, const int *map, double *dinv, hipStream_t st);
 // the calling thread's context (nullptr on failure, error set)

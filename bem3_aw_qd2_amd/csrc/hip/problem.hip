@@ -169,6 +169,9 @@ extern "C" int bqd2_destroy(bqd2_problem *p) {
     (void)hipStreamDestroy(c->st);
     delete c;
   }
+  if (p->lu_graph) (void)hipGraphExecDestroy(p->lu_graph);
+  for (auto &D : p->dom)
+    if (D.g_graph) (void)hipGraphExecDestroy(D.g_graph);
   for (void *ptr : p->owned) (void)hipFree(ptr);
   if (p->main.ws) (void)hipFree(p->main.ws);
   if (p->stream) (void)hipStreamDestroy(p->stream);

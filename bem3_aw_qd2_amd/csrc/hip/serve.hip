@@ -137,7 +137,13 @@ extern "C" size_t bqd2_footprint(const bqd2_model *m, int solver, int npts, int 
   }
   const size_t na = 2 * (size_t)m->NN;
   b += sq(na) * 16 + na * (16 + 16 + 4) + 8192 + (size_t)m->NN * 48 + 6 * nmax * 16 + 16;
-  if (solver == 2) b += sq(na) * 8 + bqd2_cgetrf_wsize((int)na) + na * (48 + 16);
+  if (solver == 2) {
+    b += sq(na) * 8 + bqd2_cgetrf_wsize((int)na) + na * (48 + 16);
+    for (int d = 0; d <= m->MN; d++) {  // float factors of each domain's G (dpv systems)
+      const size_t N = 4 * (size_t)m->dom_ne[d];
+      if (N) b += sq(N) * 8 + bqd2_cgetrf_wsize((int)N) + N * (48 + 16 + 4);
+    }
+  }
   if (solver == 1) b += sq(nmax) * 16;                      // dpv LU buffers
   // GMRES: basis of restart + 1 vectors for the system, 3 right-hand sides for dpv
   ws = std::max(ws, 101 * na * 16 + 8 * na * 16);

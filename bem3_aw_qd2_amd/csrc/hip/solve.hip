@@ -442,7 +442,7 @@ static int factor_lu32(bqd2_problem *p, hipStream_t st) {
   BQD2_HIP(hipEventCreate(&e1));
   BQD2_HIP(hipEventRecord(e0, st));
   if ((rc = bqd2_jacobi(na, p->A, na, nullptr, p->dinv, st))) return rc;
-  if ((rc = bqd2_lu32_convert(na, p->A, na, p->dinv, p->Af, st))) return rc;
+  if ((rc = bqd2_lu32_convert(na, p->A, na, nullptr, p->dinv, p->Af, st))) return rc;
   if ((rc = bqd2_cgetrf(na, p->Af, p->ipiv, p->luws, st))) return rc;
   BQD2_HIP(hipEventRecord(e1, st));
   BQD2_HIP(hipEventSynchronize(e1));
@@ -493,7 +493,7 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
     // right-preconditioned by it; plain GMRES and the FP64 LU remain as fallbacks
     if ((rc = build_system(p, 1, st))) return rc;
     if (!p->lu32_valid && (rc = factor_lu32(p, st))) return rc;
-    Precond pc{p->Af, p->luws, p->lutmp};
+    Precond pc{p->Af, p->luws, p->lutmp, &p->lu_graph};
     rc = bqd2_gmres(&p->main, na, 1, p->A, na, nullptr, p->b, na, p->xg, na, GM_TOL, IR_RESTART, IR_MAXIT,
                     &p->info_its[0], &p->info_res[0], &pc);
     if (rc < 0) return rc;
@@ -524,6 +524,25 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
   if (x_host) BQD2_HIP(hipMemcpyAsync(x_host, x, (size_t)na * 16, hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
   p->solved = true;
+  return BQD2_OK;
+}
+
+// complex float factors of domain D's row-scaled G (identity on unused rows),
+// once per assembly: the preconditioner of its dpv systems in the mixed solver
+static int factor_g32(bqd2_problem *p, DevDomain &D, hipStream_t st) {
+  const size_t N = D.N;
+  int rc;
+  if (!D.gAf) {
+    if ((rc = bqd2_alloc(p, (void **)&D.gAf, N * N * 8))) return rc;
+    if ((rc = bqd2_alloc(p, &D.gws, bqd2_cgetrf_wsize(D.N)))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&D.gtmp, N * 6 * 8))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&D.gdinv, N * 16))) return rc;
+    if ((rc = bqd2_alloc(p, (void **)&D.gipiv, N * sizeof(int)))) return rc;
+  }
+  if ((rc = bqd2_jacobi(D.N, D.G, D.N, D.rowmap, D.gdinv, st))) return rc;
+  if ((rc = bqd2_lu32_convert(D.N, D.G, D.N, D.rowmap, D.gdinv, D.gAf, st))) return rc;
+  if ((rc = bqd2_cgetrf(D.N, D.gAf, D.gipiv, D.gws, st))) return rc;
+  D.g_lu_valid = true;
   return BQD2_OK;
 }
 
@@ -593,8 +612,18 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
       double *B = p->dpvw, *X = p->dpvw + 2 * (size_t)3 * D.N;
       hipLaunchKernelGGL(k_dpv_B_slots, dim3(D.N), dim3(64), 0, st, D, B);
       BQD2_HIP(hipGetLastError());
-      rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, GM_RESTART, GM_MAXIT,
-                      &p->info_its[1 + d], &p->info_res[1 + d]);
+      rc = 1;
+      if (p->solver == 2) {
+        // complex float factors of G (once per assembly) as the preconditioner
+        if (!D.g_lu_valid && (rc = factor_g32(p, D, st))) return rc;
+        Precond pc{D.gAf, D.gws, D.gtmp, &D.g_graph};
+        rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
+                        &p->info_its[1 + d], &p->info_res[1 + d], &pc);
+        if (rc < 0) return rc;
+      }
+      if (rc == 1)
+        rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, GM_RESTART, GM_MAXIT,
+                        &p->info_its[1 + d], &p->info_res[1 + d]);
       if (rc < 0) return rc;
       if (rc == 0) {
         hipLaunchKernelGGL(k_dpv_store_slots, dim3(bqd2_blocks(D.N, 256)), dim3(256), 0, st, D, X);

@@ -227,6 +227,11 @@ int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B,
  * B kk x nc, C m x nc. */
 int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, const float *d_B, long long ldb,
                float *d_C, long long ldc, void *stream);
+/* No counterpart in the reference.  The mixed-precision solves replay their
+ * triangular sweeps as HIP graphs (captured once per factorisation); 0 launches
+ * them one by one (process-wide; for tests that compare the two). */
+int bqd2_set_graphs(int on);
+
 /* No counterpart in the reference.  Milliseconds of the last complex float
  * factorisation of the system (row scaling, rounding and bqd2_cgetrf), 0 when
  * none has run since the last assembly. */

@@ -9,7 +9,12 @@ of the FP64 GMRES solution, and the force of both rules within 1e-9 of the
 reference's example1.log (the 1e-8 force bar of the mixed form with a margin).
 The refinement must reach the FP64 solver's scaled residual (1e-15) in far
 fewer steps than unpreconditioned GMRES, and a second solve on the same
-assembly reuses the factors."""
+assembly reuses the factors.
+
+The triangular sweeps of the factors run as a HIP graph captured at the first
+application: three solves in a row (each replays the graph at every refinement
+step, with the GMRES kernels and host synchronisations in between) must give
+the same bits as the sweeps launched one by one."""
 import ctypes as C
 import json
 import os
@@ -73,3 +78,27 @@ def test_mixed_solution_matches_reference(tmp_path):
         ef, eg = np.linalg.norm(F - Fr) / np.linalg.norm(Fr), np.linalg.norm(F - Fg) / np.linalg.norm(Fg)
         print("force type %d: mixed-ref %.2e mixed-fp64 %.2e" % (t, ef, eg))
         assert ef <= 1e-9 and eg <= 1e-9
+
+
+def test_graph_replays_match_eager():
+    import numpy as np
+    m = exdat.Model(host_lib()).dat_read(SAMPLE)
+    prob = host_lib().aqd2_problem(m.buf, None)
+    hip = hip_lib()
+    assert hip.bqd2_set_solver(prob, 2) == 0
+    m.solve()
+    na = 2 * m.nn()
+    xs = []
+    try:
+        for graphs in (0, 1, 1, 1):
+            assert hip.bqd2_set_graphs(graphs) == 0
+            x = np.zeros(na, complex)
+            assert hip.bqd2_solve_surface(prob, x.ctypes.data) == 0
+            assert hip.bqd2_solve_velocity(prob) == 0
+            xs.append(x)
+    finally:
+        hip.bqd2_set_graphs(1)
+    m.finalize()
+    for i, x in enumerate(xs[1:]):
+        print("replay solve %d: max |x - x_eager| = %.3e" % (i, np.abs(x - xs[0]).max()))
+        assert np.array_equal(x, xs[0])
