@@ -288,6 +288,80 @@ def lu32_leg(hip, prob, na):
 
 
 
+def batch_leg(args, hip, host):
+    """A scaled-down form of the batched workload: `batch_instances` uneven
+    instances (6 n^2 quads, n in {8, 12, 16}: 384, 864 and 1536 quads of the
+    workload's 384-3456; ka in [0.5, 3]; square or triangular lattice with
+    period / radius in [3, 6]; seeded), each solved for 16 incidences that
+    share its operator (8 amplitudes and phases, each at theta and at the
+    mirrored pi - theta).  Assembly one instance at a time
+    against one batched launch of the periodic pairs; the 16 solves (surface,
+    velocity, type-0 force) by FP64 GMRES each against one complex float
+    factorisation reused by all 16 (mixed precision)."""
+    from bem3_aw_qd2_amd import check, exdat, mesh
+    rng = np.random.default_rng(1)
+    a, c = 0.5e-3, 1500.0
+    models, probs, tmp = [], [], tempfile.mkdtemp(prefix="bqd2_batch_")
+    for i in range(args.batch_instances):
+        n = int(rng.choice([8, 12, 16]))
+        period = a * rng.uniform(3.0, 6.0)
+        d2 = (0.5 * period, 0.5 * math.sqrt(3.0) * period) if rng.random() < 0.5 else (0.0, period)
+        ka = rng.uniform(0.5, 3.0)
+        theta, psi = rng.uniform(0.1, 1.0), rng.uniform(0.1, 1.4)
+        paths = mesh.write_config("sphere384", os.path.join(tmp, str(i)), n=n, d1=(period, 0.0), d2=d2,
+                                  f=ka * c / (2 * math.pi * a), theta=theta, psi=psi)
+        with quiet_stdout():
+            m = exdat.Model(host).read_inputs(*paths, out=os.path.join(tmp, "ex%d.dat" % i)).initialize()
+        models.append((m, theta, psi))
+        probs.append(host.aqd2_problem(m.buf, None))
+    arr = (C.c_void_p * len(probs))(*probs)
+
+    def assemble(batched):
+        t0 = time.perf_counter()
+        if batched:
+            check(hip.bqd2_assemble_batch(arr, len(probs), 0, None), "bqd2_assemble_batch")
+        else:
+            for p in probs:
+                check(hip.bqd2_assemble(p, 0, None), "bqd2_assemble")
+        for p in probs:
+            check(hip.bqd2_assemble(p, 1, None), "bqd2_assemble")
+        return time.perf_counter() - t0
+
+    t_one = assemble(False)
+    t_bat = assemble(True)
+    incid = [(amp, mir) for amp in np.exp(1j * np.linspace(0, 2 * math.pi, 8, endpoint=False)) *
+             np.linspace(0.5, 2.0, 8) for mir in (False, True)]
+
+    def solves(mode):
+        forces = []
+        t0 = time.perf_counter()
+        for p, (m, theta, psi) in zip(probs, models):
+            check(hip.bqd2_set_solver(p, mode), "bqd2_set_solver")
+            for amp, mir in incid:
+                pw7 = (C.c_double * 7)(amp.real, amp.imag, math.pi - theta if mir else theta, psi, 0.0, 0.0, 0.0)
+                check(hip.bqd2_set_incidence(p, pw7), "bqd2_set_incidence")
+                check(hip.bqd2_solve_surface(p, None), "bqd2_solve_surface")
+                check(hip.bqd2_solve_velocity(p), "bqd2_solve_velocity")
+                F = (C.c_double * 3)()
+                N = (C.c_double * 3)()
+                check(hip.bqd2_force(p, (C.c_double * 3)(0.0, 0.0, 0.0), 0, F, N), "bqd2_force")
+                forces.append(np.array(F[:]))
+        return time.perf_counter() - t0, forces
+
+    t_plain, f_plain = solves(0)
+    t_mixed, f_mixed = solves(2)
+    err = max(float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(f_mixed, f_plain))
+    nsol = len(probs) * len(incid)
+    out = {"instances": len(probs), "elements": [m.ne() for m, _, _ in models], "incidences_per_instance": len(incid),
+           "assemble_s_one_at_a_time": t_one, "assemble_s_batched": t_bat,
+           "plain_fp64_solves_per_s": nsol / t_plain, "factor_reuse_solves_per_s": nsol / t_mixed,
+           "solve_speedup": t_plain / t_mixed, "force_rel_diff": err}
+    with quiet_stdout():
+        for m, _, _ in models:
+            m.finalize()
+    return out
+
+
 def serving_stream(nreq, rate, repeat, seed=0, args_mirror=0.0):
     """A seeded stream of solve requests of mixed size (a scaled-down form of
     the 384-6144-element serving workload): Poisson arrivals at `rate` per
@@ -417,6 +491,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream measurement")
     ap.add_argument("--no-lu32", action="store_true", help="skip the mixed-precision solve of the request's system")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
+    ap.add_argument("--batch-instances", type=int, default=6)
     ap.add_argument("--serving-only", action="store_true", help="only the request-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests per second")
@@ -559,6 +635,9 @@ def main():
     if not args.no_serving:
         log("serving: request stream")
         line["serving"] = serving_bench(args)
+    if not args.no_batch:
+        log("batch: instances with shared factorisations")
+        line["batch"] = batch_leg(args, hip, host)
     print(json.dumps(line), flush=True)
 
 
