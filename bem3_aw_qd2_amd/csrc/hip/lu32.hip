@@ -666,13 +666,31 @@ __global__ void __launch_bounds__(256) k_csweep(int n, const float2 *__restrict_
   }
 }
 
+// One block step: launched on st, or (gr != nullptr) added to the graph gr
+// after the node *prev, which it then replaces.
 template <int NR>
-static void sweep_nr(bool upper, int n, const float2 *A, const float2 *Tk, int k0, int kb, float2 *Y, float2 *X,
-                     long long ld, hipStream_t st) {
+static int sweep_nr(bool upper, int n, const float2 *A, const float2 *Tk, int k0, int kb, float2 *Y, float2 *X,
+                    long long ld, hipStream_t st, hipGraph_t gr, hipGraphNode_t *prev) {
   const int rows = upper ? k0 : n - k0 - kb;
   dim3 g(bqd2_blocks(rows, 64) + 1);  // row workgroups + the one that forms x_k
-  if (upper) hipLaunchKernelGGL((k_csweep<NR, true>), g, dim3(256), 0, st, n, A, Tk, k0, kb, Y, X, ld);
-  else hipLaunchKernelGGL((k_csweep<NR, false>), g, dim3(256), 0, st, n, A, Tk, k0, kb, Y, X, ld);
+  void *fn = upper ? (void *)k_csweep<NR, true> : (void *)k_csweep<NR, false>;
+  if (!gr) {
+    if (upper) hipLaunchKernelGGL((k_csweep<NR, true>), g, dim3(256), 0, st, n, A, Tk, k0, kb, Y, X, ld);
+    else hipLaunchKernelGGL((k_csweep<NR, false>), g, dim3(256), 0, st, n, A, Tk, k0, kb, Y, X, ld);
+    return BQD2_OK;
+  }
+  void *args[] = {(void *)&n, (void *)&A, (void *)&Tk, (void *)&k0, (void *)&kb, (void *)&Y, (void *)&X, (void *)&ld};
+  hipKernelNodeParams kp = {};
+  kp.func = fn;
+  kp.gridDim = g;
+  kp.blockDim = dim3(256);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  hipGraphNode_t node;
+  BQD2_HIP(hipGraphAddKernelNode(&node, gr, *prev ? prev : nullptr, *prev ? 1 : 0, &kp));
+  *prev = node;
+  return BQD2_OK;
 }
 
 // Y (n x nrhs, column-major, leading dimension ld, already permuted) <- U^-1 L^-1 Y;
@@ -680,26 +698,30 @@ static void sweep_nr(bool upper, int n, const float2 *A, const float2 *Tk, int k
 // the backward sweep writes back into Y.  More than 4 right-hand sides: the
 // block steps as MFMA GEMMs, in place on Y.
 static int cgetrs_perm(int n, int nrhs, const float2 *A, const LuWs &W, float2 *Y, float2 *X, long long ld,
-                       hipStream_t st) {
+                       hipStream_t st, hipGraph_t gr = nullptr) {
   const int nbk = bqd2_blocks(n, NB);
   if (nrhs <= 4) {
+    hipGraphNode_t prev = nullptr;
     for (int pass = 0; pass < 2; pass++) {
       const bool upper = pass == 1;
       float2 *in = upper ? X : Y, *out = upper ? Y : X;
       for (int b = 0; b < nbk; b++) {
         const int k0 = (upper ? nbk - 1 - b : b) * NB, kb = (n - k0 < NB) ? n - k0 : NB;
         const float2 *Tk = W.T + (size_t)(k0 / NB) * 2 * NB * NB + (upper ? NB * NB : 0);
+        int rc;
         switch (nrhs) {
-          case 1: sweep_nr<1>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
-          case 2: sweep_nr<2>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
-          case 3: sweep_nr<3>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
-          default: sweep_nr<4>(upper, n, A, Tk, k0, kb, in, out, ld, st); break;
+          case 1: rc = sweep_nr<1>(upper, n, A, Tk, k0, kb, in, out, ld, st, gr, &prev); break;
+          case 2: rc = sweep_nr<2>(upper, n, A, Tk, k0, kb, in, out, ld, st, gr, &prev); break;
+          case 3: rc = sweep_nr<3>(upper, n, A, Tk, k0, kb, in, out, ld, st, gr, &prev); break;
+          default: rc = sweep_nr<4>(upper, n, A, Tk, k0, kb, in, out, ld, st, gr, &prev); break;
         }
+        if (rc) return rc;
       }
     }
     BQD2_HIP(hipGetLastError());
     return BQD2_OK;
   }
+  BQD2_CHECK(!gr, BQD2_E_ARG, "bqd2 sweeps: graphs take at most 4 right-hand sides");
   // forward: y_r -= M_rk y_k below, then y_k = T_k y_k (in place), by diagonal blocks
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = (n - k0 < NB) ? n - k0 : NB;
@@ -818,16 +840,16 @@ int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const doub
   hipLaunchKernelGGL(k_gather_d2f, g, dim3(256), 0, st, n, W.perm, in, ldi, T);
   if (graph && g_use_graphs) {
     // the block steps of both sweeps read and write fixed buffers (the factors,
-    // the workspace and tmp): captured once per preconditioner and replayed
+    // the workspace and tmp): made a graph once per preconditioner and replayed
     if (!*graph) {
+      // built node by node (no stream capture: a capture in progress would
+      // conflict with other threads' synchronous work on the null stream)
       hipGraph_t gr = nullptr;
-      BQD2_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, T + (size_t)n * nrhs, n, st);
-      hipError_t e = hipStreamEndCapture(st, &gr);
-      if (rc) return rc;
-      BQD2_HIP(e);
-      e = hipGraphInstantiate(graph, gr, nullptr, nullptr, 0);
+      BQD2_HIP(hipGraphCreate(&gr, 0));
+      int rc = cgetrs_perm(n, nrhs, (const float2 *)Af, W, T, T + (size_t)n * nrhs, n, st, gr);
+      hipError_t e = rc ? hipSuccess : hipGraphInstantiate(graph, gr, nullptr, nullptr, 0);
       (void)hipGraphDestroy(gr);
+      if (rc) return rc;
       BQD2_HIP(e);
     }
     BQD2_HIP(hipGraphLaunch(*graph, st));

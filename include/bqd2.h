@@ -228,7 +228,7 @@ int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B,
 int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, const float *d_B, long long ldb,
                float *d_C, long long ldc, void *stream);
 /* No counterpart in the reference.  The mixed-precision solves replay their
- * triangular sweeps as HIP graphs (captured once per factorisation); 0 launches
+ * triangular sweeps as HIP graphs (built once per factorisation); 0 launches
  * them one by one (process-wide; for tests that compare the two). */
 int bqd2_set_graphs(int on);
 

@@ -11,7 +11,7 @@ The refinement must reach the FP64 solver's scaled residual (1e-15) in far
 fewer steps than unpreconditioned GMRES, and a second solve on the same
 assembly reuses the factors.
 
-The triangular sweeps of the factors run as a HIP graph captured at the first
+The triangular sweeps of the factors run as a HIP graph built at the first
 application: three solves in a row (each replays the graph at every refinement
 step, with the GMRES kernels and host synchronisations in between) must give
 the same bits as the sweeps launched one by one."""
