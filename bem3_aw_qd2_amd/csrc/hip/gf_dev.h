@@ -18,6 +18,7 @@
 #pragma once
 #include "cplx.h"
 #include "../../../include/weideman_coef.h"
+#include "../../../include/erfcx_coef.h"
 #include "../../../include/bqd2.h"
 
 // Products and sums are rounded one at a time, as the reference's C does on
@@ -78,23 +79,25 @@ __device__ __forceinline__ cd weideman20_upper(double x, double y) {
 }
 
 // erfcx(y) = exp(y^2) erfc(y) = w(i y): on the imaginary axis Weideman's Z is
-// real, so the N = 40 rational becomes a real recurrence; y < 0 by
-// erfcx(y) = 2 exp(y^2) - erfcx(-y).  Inlined where the translation unit asks
-// for it (BQD2_ERFCX_INLINE: the assembly kernels); called elsewhere, which
-// keeps the register pressure of the field kernels (Hessian sums) lower.
+// real and covers (-1, 1] once, so a real polynomial in Z fitted to erfcx's
+// relative error (degree 20, < 7e-16 on t >= 0; coefficients from
+// gen_erfcx.py) replaces the N = 40 Fourier table at half the multiply-adds;
+// y < 0 by erfcx(y) = 2 exp(y^2) - erfcx(-y).  Inlined where the translation
+// unit asks for it (BQD2_ERFCX_INLINE: the assembly kernels); called elsewhere,
+// which keeps the register pressure of the field kernels (Hessian sums) lower.
 #if defined(BQD2_ERFCX_INLINE)
 #define BQD2_ERFCX_ATTR __forceinline__
 #else
 #define BQD2_ERFCX_ATTR __noinline__
 #endif
 __device__ BQD2_ERFCX_ATTR double erfcx_w(double y) {
-  const double L = WEID_L;
-  const double a[WEID_N] = {WEID_COEFS};
+  const double L = ERFCX_L;
+  const double c[ERFCX_N] = {ERFCX_COEFS};
   const double t = fabs(y);
   const double iden = rcp_pos(L + t), Z = (L - t) * iden;
-  double p = a[WEID_N - 1];
+  double p = c[ERFCX_N - 1];
 #pragma unroll
-  for (int n = WEID_N - 2; n >= 0; n--) p = fma(p, Z, a[n]);
+  for (int n = ERFCX_N - 2; n >= 0; n--) p = fma(p, Z, c[n]);
   const double isp = 0.56418958354775628695;
   const double v = 2.0 * p * iden * iden + isp * iden;
   return y >= 0.0 ? v : 2.0 * exp(y * y) - v;
