@@ -10,8 +10,8 @@ f(t) = exp(-t^2) (L^2 + t^2) sampled at t = L tan(theta/2), computed here in
 40-digit arithmetic (mpmath) and rounded to double once.  With N = 40 the
 relative error against an mpmath reference is below 1e-15 on the closed upper
 half-plane (checked by tests/test_oracle_faddeeva.py).  A second table with
-N = 20 (WEID20_*) serves Im z >= 2.5, Re z <= 3, where its relative error is
-below 2e-15.
+N = 24 (WEID24_*) serves 1.5 <= Im z <= 12, Re z <= 3, where its relative error
+is below 4e-15, the level of the N = 40 table there.
 
 Usage: python gen_weideman.py > weideman_coef.h
 """
@@ -53,12 +53,12 @@ def table(prefix, n):
 
 def main():
     print("/* Generated by gen_weideman.py: Weideman (1994) rational approximation of")
-    print(" * the Faddeeva function, N = %d (WEID_*) and N = 20 (WEID20_*).  a[n]" % N)
+    print(" * the Faddeeva function, N = %d (WEID_*) and N = 24 (WEID24_*).  a[n]" % N)
     print(" * multiplies Z^n. */")
     print("#ifndef BQD2_WEIDEMAN_COEF_H_")
     print("#define BQD2_WEIDEMAN_COEF_H_")
     table("WEID", N)
-    table("WEID20", 20)
+    table("WEID24", 24)
     print("#endif")
 
 
