@@ -76,6 +76,10 @@ def hip_lib():
         lib.bqd2_set_incidence.argtypes = [C.c_void_p, C.c_void_p]
         lib.bqd2_assemble_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         lib.bqd2_set_graphs.argtypes = [C.c_int]
+        lib.bqd2_set_autotune.argtypes = [C.c_int]
+        lib.bqd2_autotune_force.argtypes = [C.c_int, C.c_int]
+        lib.bqd2_autotune_info.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.bqd2_autotune_reset.argtypes = []
         lib.bqd2_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         lib.bqd2_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p]

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <vector>
 #include "elem_dev.h"
+#include "tune.h"
 
 typedef std::complex<double> zc;
 
@@ -54,12 +55,12 @@ __global__ void __launch_bounds__(256) k_gm_dinv(int n, const double *__restrict
 }
 
 // Y_s = dinv (A X_s) for s < NR (identity on unused rows); one wavefront per row
-template <int NR>
-__global__ void __launch_bounds__(256) k_gm_mv(int n, const double *__restrict__ A, long long lda,
+template <int NR, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_gm_mv(int n, const double *__restrict__ A, long long lda,
                                                const int *__restrict__ map, const double *__restrict__ dinv,
                                                const double *__restrict__ X, long long ldx,
                                                double *__restrict__ Y, long long ldy) {
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = blockIdx.x * WPB + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= n) return;
   if (map && map[r] < 0) {
     if (lane < NR) stc(Y + 2 * (size_t)lane * ldy, r, ldc(X + 2 * (size_t)lane * ldx, r));
@@ -170,12 +171,27 @@ __global__ void __launch_bounds__(256) k_gm_update(int n, int k, const double *_
   stc(x, r, ldc(x, r) + acc);
 }
 
+// WPB rows per workgroup (one wavefront each), as the tuner picks: each row's
+// sum runs in its own wavefront whatever the workgroup size
+template <int WPB>
+static void launch_mv_wpb(int nrhs, int n, const double *A, long long lda, const int *map, const double *dinv,
+                          const double *X, long long ldx, double *Y, long long ldy, hipStream_t st) {
+  dim3 g(bqd2_blocks(n, WPB)), b(64 * WPB);
+  if (nrhs == 1) hipLaunchKernelGGL((k_gm_mv<1, WPB>), g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
+  else if (nrhs == 2) hipLaunchKernelGGL((k_gm_mv<2, WPB>), g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
+  else hipLaunchKernelGGL((k_gm_mv<3, WPB>), g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
+}
+
 static void launch_mv(int nrhs, int n, const double *A, long long lda, const int *map, const double *dinv,
                       const double *X, long long ldx, double *Y, long long ldy, hipStream_t st) {
-  dim3 g(bqd2_blocks(n, 4)), b(256);
-  if (nrhs == 1) hipLaunchKernelGGL(k_gm_mv<1>, g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
-  else if (nrhs == 2) hipLaunchKernelGGL(k_gm_mv<2>, g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
-  else hipLaunchKernelGGL(k_gm_mv<3>, g, b, 0, st, n, A, lda, map, dinv, X, ldx, Y, ldy);
+  const TuneTicket tk = tune_begin(TUNE_GMV, 8LL * tune_class(n) + nrhs, st);
+  switch (tk.cand) {
+    case 1: launch_mv_wpb<2>(nrhs, n, A, lda, map, dinv, X, ldx, Y, ldy, st); break;
+    case 2: launch_mv_wpb<8>(nrhs, n, A, lda, map, dinv, X, ldx, Y, ldy, st); break;
+    case 3: launch_mv_wpb<1>(nrhs, n, A, lda, map, dinv, X, ldx, Y, ldy, st); break;
+    default: launch_mv_wpb<4>(nrhs, n, A, lda, map, dinv, X, ldx, Y, ldy, st);
+  }
+  tune_end(tk, st);
 }
 
 // X_s += Z_s

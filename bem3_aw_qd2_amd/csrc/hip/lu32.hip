@@ -29,6 +29,7 @@
 // kernels for up to 4 right-hand sides, the MFMA GEMM beyond.
 #include <vector>
 #include "common.h"
+#include "tune.h"
 
 #define NB 128      // panel width = K of the trailing update
 #define GMAXW 1024  // panel workgroups at most
@@ -246,12 +247,44 @@ __global__ void __launch_bounds__(256) k_cswap_cols(int n, float2 *__restrict__ 
 // triangle is solved in registers, then each later row of the lane's column
 // takes one 32-term dot product with it.
 #define TB 32
-__device__ __forceinline__ void trsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *Xall, long long ldx,
-                                       int ncols, int c, float2 *sL) {
-  const int tid = threadIdx.x;
-  for (int q = 0; q + 1 < kb; q++)
-    for (int r = q + 1 + tid; r < kb; r += 256) sL[r * (r - 1) / 2 + q] = Ld[(size_t)q * n + r];
+// element (r, q), r > q, of the unit lower factor: from the packed LDS copy, or
+// (GL) straight from the matrix, where all lanes of a wavefront read the same
+// address and the load goes through the scalar cache
+template <bool GL>
+__device__ __forceinline__ float2 lval(const float2 *__restrict__ Ld, int n, const float2 *sL, int r, int q) {
+  return GL ? Ld[(size_t)q * n + r] : sL[r * (r - 1) / 2 + q];
+}
+
+// the strictly lower triangle of the kb x kb block into sL, packed by rows: a
+// flat walk (consecutive lanes on consecutive rows of a column) with eight
+// loads in flight per lane, since a walk column by column waits for one load
+// per column
+__device__ __forceinline__ void stage_l(int n, const float2 *__restrict__ Ld, int kb, float2 *sL) {
+  const int tid = threadIdx.x, nt = blockDim.x, tot = kb * kb;
+  for (int e0 = tid; e0 < tot; e0 += 8 * nt) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * nt;
+      v[u] = (e < tot) ? Ld[(size_t)(e / kb) * n + e % kb] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * nt, q = e / kb, r = e % kb;
+      if (e < tot && r > q) sL[r * (r - 1) / 2 + q] = v[u];
+    }
+  }
   __syncthreads();
+}
+
+// column c of X (ncols columns, ldx apart) := L^-1 X for the unit lower kb x kb
+// factor at Ld, by forward substitution in 32-row blocks held in registers:
+// the block's triangle row by row, then x_r -= L[r][block] x_block for the rows
+// below.  L comes from sL (staged by stage_l) or, with GL, from Ld; the
+// arithmetic is the same either way.
+template <bool GL>
+__device__ __forceinline__ void trsm_cols(int n, const float2 *__restrict__ Ld, int kb, float2 *Xall, long long ldx,
+                                          int ncols, int c, const float2 *sL) {
   if (c >= ncols) return;
   float2 *X = Xall + (size_t)c * ldx;
   for (int b0 = 0; b0 < kb; b0 += TB) {
@@ -262,11 +295,10 @@ __device__ __forceinline__ void trsm_l(int n, const float2 *__restrict__ Ld, int
 #pragma unroll
     for (int t = 1; t < TB; t++) {
       if (b0 + t >= kb) break;
-      const float2 *Lr = sL + (b0 + t) * (b0 + t - 1) / 2 + b0;
       float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
       for (int q = 0; q < t; q++) {
-        const float2 l = Lr[q];
+        const float2 l = lval<GL>(Ld, n, sL, b0 + t, b0 + q);
         acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
         acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
       }
@@ -276,28 +308,65 @@ __device__ __forceinline__ void trsm_l(int n, const float2 *__restrict__ Ld, int
 #pragma unroll
     for (int t = 0; t < TB; t++)
       if (b0 + t < kb) X[b0 + t] = x[t];
-    // later rows: x_r -= L[r][b0:b0+TB] x_block
-    for (int r = b0 + TB; r < kb; r++) {
-      const float2 *Lr = sL + r * (r - 1) / 2 + b0;
-      float2 acc = make_float2(0.0f, 0.0f);
+    // later rows: x_r -= L[r][b0:b0+TB] x_block, RB rows of X loaded at once
+    // (a lane's column is strided in memory: one load per row would wait for
+    // each in turn)
+    constexpr int RB = 16;
+    for (int r0 = b0 + TB; r0 < kb; r0 += RB) {
+      float2 y[RB];
 #pragma unroll
-      for (int q = 0; q < TB; q++) {
-        const float2 l = Lr[q];
-        acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
-        acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
+      for (int u = 0; u < RB; u++) y[u] = (r0 + u < kb) ? X[r0 + u] : make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int u = 0; u < RB; u++) {
+        const int r = r0 + u;
+        if (r >= kb) break;
+        float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int q = 0; q < TB; q++) {
+          const float2 l = lval<GL>(Ld, n, sL, r, b0 + q);
+          acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
+          acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
+        }
+        y[u].x -= acc.x;
+        y[u].y -= acc.y;
+        X[r] = y[u];
       }
-      float2 y = X[r];
-      y.x -= acc.x;
-      y.y -= acc.y;
-      X[r] = y;
     }
   }
 }
 
-__global__ void __launch_bounds__(256) k_ctrsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *X,
-                                                 long long ldx, int ncols) {
-  __shared__ float2 sL[NB * (NB - 1) / 2];
-  trsm_l(n, Ld, kb, X, ldx, ncols, blockIdx.x * 256 + threadIdx.x, sL);
+// BS columns per workgroup (one lane each), L staged in LDS or (GL) read from
+// the matrix: the variants differ in occupancy (the LDS copy allows two
+// workgroups per CU), not in any column's arithmetic
+template <int BS, bool GL>
+__global__ void __launch_bounds__(BS) k_ctrsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *X,
+                                                long long ldx, int ncols) {
+  if (GL) {
+    trsm_cols<true>(n, Ld, kb, X, ldx, ncols, blockIdx.x * BS + threadIdx.x, nullptr);
+  } else {
+    __shared__ float2 sL[NB * (NB - 1) / 2];
+    stage_l(n, Ld, kb, sL);
+    trsm_cols<false>(n, Ld, kb, X, ldx, ncols, blockIdx.x * BS + threadIdx.x, sL);
+  }
+}
+
+// U12 = L11^-1 A12 over ncols columns, in the variant the tuner picks
+static void ctrsm_l(int n, const float2 *Ld, int kb, float2 *X, long long ldx, int ncols, hipStream_t st) {
+  const TuneTicket tk = tune_begin(TUNE_TRSM, tune_class(ncols), st);
+  switch (tk.cand) {
+    case 1:
+      hipLaunchKernelGGL((k_ctrsm_l<64, true>), dim3(bqd2_blocks(ncols, 64)), dim3(64), 0, st, n, Ld, kb, X, ldx,
+                         ncols);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_ctrsm_l<256, true>), dim3(bqd2_blocks(ncols, 256)), dim3(256), 0, st, n, Ld, kb, X,
+                         ldx, ncols);
+      break;
+    default:
+      hipLaunchKernelGGL((k_ctrsm_l<256, false>), dim3(bqd2_blocks(ncols, 256)), dim3(256), 0, st, n, Ld, kb, X,
+                         ldx, ncols);
+  }
+  tune_end(tk, st);
 }
 
 // L11^-1 of every diagonal block (block blockIdx.x) into T + 2 NB NB blockIdx.x,
@@ -305,7 +374,9 @@ __global__ void __launch_bounds__(256) k_ctrsm_l(int n, const float2 *__restrict
 __global__ void __launch_bounds__(256) k_clinv(int n, const float2 *__restrict__ A, float2 *T) {
   __shared__ float2 sL[NB * (NB - 1) / 2];
   const int k0 = blockIdx.x * NB, kb = min(NB, n - k0);
-  trsm_l(n, A + (size_t)k0 * n + k0, kb, T + (size_t)blockIdx.x * 2 * NB * NB, (long long)NB, kb, threadIdx.x, sL);
+  stage_l(n, A + (size_t)k0 * n + k0, kb, sL);
+  trsm_cols<false>(n, A + (size_t)k0 * n + k0, kb, T + (size_t)blockIdx.x * 2 * NB * NB, (long long)NB, kb,
+                   threadIdx.x, sL);
 }
 
 // Inverse of the upper factor of the diagonal block k0 = NB blockIdx.x (kb x kb),
@@ -316,6 +387,7 @@ __global__ void __launch_bounds__(NB) k_ctrtri_u(int n, const float2 *__restrict
   __shared__ float2 sD[NB * NB];  // column-major (r, q) at q * NB + r
   const int c = threadIdx.x;
   const int k0 = blockIdx.x * NB, kb = min(NB, n - k0);
+#pragma unroll 8
   for (int e = c; e < kb * kb; e += NB) {
     const int r = e % kb, q = e / kb;
     sD[q * NB + r] = A[(size_t)(k0 + q) * n + k0 + r];
@@ -359,72 +431,77 @@ __global__ void __launch_bounds__(256) k_ceye(int nbk, float2 *__restrict__ T) {
 // register store 32 consecutive rows of one column.  B may be C when m <= 128,
 // or A may be C when nc <= 128: a workgroup then reads only the columns (rows)
 // it writes, and writes after its last read.
+// WT x WT blocks of 32 x 32 per wave, four waves (2 x 2): a 64 WT square tile
+// per workgroup.  Every tile computes each element of C from the same MFMA
+// chain over k, so the tile size changes no bit of the result.
 #define GT 128
 #define GK 16
 #define GPAD 1
-template <int SET>
+template <int SET, int WT>
 __global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const float2 *A, long long lda,
                                                   const float2 *B, long long ldb, float2 *C, long long ldc) {
-  __shared__ float2 sA[GK][GT + GPAD];  // (k, row)
-  __shared__ float2 sB[GK][GT + GPAD];  // (k, col)
+  constexpr int T = 64 * WT;     // tile edge
+  constexpr int AKL = 256 / T;   // lanes along k when staging A
+  __shared__ float2 sA[GK][T + GPAD];  // (k, row)
+  __shared__ float2 sB[GK][T + GPAD];  // (k, col)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave & 1, wc = wave >> 1;
-  const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
+  const int row0 = blockIdx.x * T, col0 = blockIdx.y * T;
   const int l32 = lane & 31;
   const bool hi = lane >= 32;
-  // global -> register staging of one K slice: A 128 rows x 16 k (consecutive
-  // lanes on consecutive rows), B 16 k x 128 columns (16 lanes per column)
-  const int ar = tid & 127, ak = tid >> 7;
+  // global -> register staging of one K slice: A T rows x 16 k (consecutive
+  // lanes on consecutive rows), B 16 k x T columns (16 lanes per column)
+  const int ar = tid % T, ak = tid / T;
   const int bk = tid & 15, bc = tid >> 4;
-  float2 ra[GK / 2], rb[GT / 16];
+  float2 ra[GK / AKL], rb[T / 16];
   auto load = [&](int k0) {
     const int gr = row0 + ar;
 #pragma unroll
-    for (int i = 0; i < GK / 2; i++) {
-      const int gk = k0 + ak + 2 * i;
+    for (int i = 0; i < GK / AKL; i++) {
+      const int gk = k0 + ak + AKL * i;
       ra[i] = (gr < m && gk < kk) ? A[(size_t)gk * lda + gr] : make_float2(0.0f, 0.0f);
     }
     const int gk = k0 + bk;
 #pragma unroll
-    for (int i = 0; i < GT / 16; i++) {
+    for (int i = 0; i < T / 16; i++) {
       const int gc = col0 + bc + 16 * i;
       rb[i] = (gc < nc && gk < kk) ? B[(size_t)gc * ldb + gk] : make_float2(0.0f, 0.0f);
     }
   };
-  floatx16 accR[2][2], accI[2][2];
+  floatx16 accR[WT][WT], accI[WT][WT];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < WT; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < WT; b++)
 #pragma unroll
       for (int r = 0; r < 16; r++) { accR[a][b][r] = 0.0f; accI[a][b][r] = 0.0f; }
   load(0);
   for (int k0 = 0; k0 < kk; k0 += GK) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < GK / 2; i++) sA[ak + 2 * i][ar] = ra[i];
+    for (int i = 0; i < GK / AKL; i++) sA[ak + AKL * i][ar] = ra[i];
 #pragma unroll
-    for (int i = 0; i < GT / 16; i++) sB[bk][bc + 16 * i] = rb[i];
+    for (int i = 0; i < T / 16; i++) sB[bk][bc + 16 * i] = rb[i];
     __syncthreads();
     if (k0 + GK < kk) load(k0 + GK);  // next slice in flight during this one's MFMAs
 #pragma unroll 4
     for (int q = 0; q < GK; q++) {
-      float xb[2], yr[2], yi[2];
+      float xb[WT], yr[WT], yi[WT];
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        const float2 v = sB[q][wc * 64 + b * 32 + l32];
+      for (int b = 0; b < WT; b++) {
+        const float2 v = sB[q][wc * 32 * WT + b * 32 + l32];
         xb[b] = hi ? v.y : v.x;
       }
 #pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const float2 v = sA[q][wr * 64 + a * 32 + l32];
+      for (int a = 0; a < WT; a++) {
+        const float2 v = sA[q][wr * 32 * WT + a * 32 + l32];
         yr[a] = hi ? -v.y : v.x;
         yi[a] = hi ? v.x : v.y;
       }
 #pragma unroll
-      for (int b = 0; b < 2; b++)
+      for (int b = 0; b < WT; b++)
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
+        for (int a = 0; a < WT; a++) {
           accR[b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yr[a], accR[b][a], 0, 0, 0);
           accI[b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yi[a], accI[b][a], 0, 0, 0);
         }
@@ -432,14 +509,14 @@ __global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const f
   }
   // accR[b][a] register r: row l32 of row block a, column 8 (r/4) + 4 hi + r%4 of column block b
 #pragma unroll
-  for (int b = 0; b < 2; b++)
+  for (int b = 0; b < WT; b++)
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
-      const int gr = row0 + wr * 64 + a * 32 + l32;
+    for (int a = 0; a < WT; a++) {
+      const int gr = row0 + wr * 32 * WT + a * 32 + l32;
       if (gr >= m) continue;
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const int gc = col0 + wc * 64 + b * 32 + 8 * (r >> 2) + 4 * (hi ? 1 : 0) + (r & 3);
+        const int gc = col0 + wc * 32 * WT + b * 32 + 8 * (r >> 2) + 4 * (hi ? 1 : 0) + (r & 3);
         if (gc >= nc) continue;
         float2 *p = C + (size_t)gc * ldc + gr;
         if (SET) {
@@ -454,12 +531,21 @@ __global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const f
     }
 }
 
+// wt: 2 for 128 x 128 tiles (the in-place forms need them: a workgroup must
+// cover all of C's rows or columns, as the aliasing rule above states), 1 for
+// 64 x 64 tiles on distinct operands
 static int cgemm(bool set, int m, int nc, int kk, const float2 *A, long long lda, const float2 *B, long long ldb,
-                 float2 *C, long long ldc, hipStream_t st) {
+                 float2 *C, long long ldc, hipStream_t st, int wt = 2) {
   if (m <= 0 || nc <= 0 || kk <= 0) return BQD2_OK;
-  dim3 grid(bqd2_blocks(m, GT), bqd2_blocks(nc, GT));
-  if (set) hipLaunchKernelGGL(k_cgemm<1>, grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
-  else hipLaunchKernelGGL(k_cgemm<0>, grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+  const int T = 64 * wt;
+  dim3 grid(bqd2_blocks(m, T), bqd2_blocks(nc, T));
+  if (wt == 1) {
+    if (set) hipLaunchKernelGGL((k_cgemm<1, 1>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+    else hipLaunchKernelGGL((k_cgemm<0, 1>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+  } else {
+    if (set) hipLaunchKernelGGL((k_cgemm<1, 2>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+    else hipLaunchKernelGGL((k_cgemm<0, 2>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+  }
   BQD2_HIP(hipGetLastError());
   return BQD2_OK;
 }
@@ -559,11 +645,13 @@ extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *str
     if (rest > 0) {
       // U12 = L11^-1 A12 in place, then A22 -= L21 U12 on the MFMA units
       float2 *A12 = A + (size_t)(k0 + kb) * n + k0;
-      hipLaunchKernelGGL(k_ctrsm_l, dim3(bqd2_blocks(rest, 256)), dim3(256), 0, st, n, A + (size_t)k0 * n + k0, kb,
-                         A12, (long long)n, rest);
+      ctrsm_l(n, A + (size_t)k0 * n + k0, kb, A12, (long long)n, rest, st);
       BQD2_HIP(hipGetLastError());
+      // operands distinct from A22: either tile size, as the tuner picks
+      const TuneTicket tk = tune_begin(TUNE_CGEMM, tune_class(rest), st);
       int rc = cgemm(false, rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n, A12, n,
-                     A + (size_t)(k0 + kb) * n + k0 + kb, n, st);
+                     A + (size_t)(k0 + kb) * n + k0 + kb, n, st, tk.cand == 1 ? 1 : 2);
+      tune_end(tk, st);
       if (rc) return rc;
     }
   }

@@ -287,6 +287,31 @@ def lu32_leg(hip, prob, na):
             "force_rel_diff_vs_fp64": float(np.linalg.norm(out[2]["F"] - out[0]["F"]) / np.linalg.norm(out[0]["F"]))}
 
 
+AUTOTUNE_SITES = {0: ("lu32_trailing_update", ["tile128", "tile64"]),
+                  1: ("lu32_u12_trsm", ["lds_wg256", "global_wg64", "global_wg256"]),
+                  2: ("gmres_matvec", ["rows4", "rows2", "rows8", "rows1"])}
+
+
+def autotune_tables(hip):
+    """The online autotuner's classes on this device: per site, each size class
+    (half octaves of the dimension; GMRES: 8 class + right-hand sides) with the
+    candidate it settled on (null while exploring) and the fastest timing seen
+    per candidate."""
+    out = {}
+    for site, (name, cands) in AUTOTUNE_SITES.items():
+        keys = (C.c_longlong * 256)()
+        chosen = (C.c_int * 256)()
+        best = (C.c_float * 1024)()
+        nc = C.c_int()
+        cnt = hip.bqd2_autotune_info(site, 1, 256, keys, chosen, best, C.byref(nc))
+        rows = []
+        for i in range(min(max(cnt, 0), 256)):
+            rows.append({"class": int(keys[i]), "chosen": cands[chosen[i]] if chosen[i] >= 0 else None,
+                         "best_ms": {cands[c]: (round(best[4 * i + c], 4) if best[4 * i + c] >= 0 else None)
+                                     for c in range(nc.value)}})
+        out[name] = rows
+    return out
+
 
 def batch_leg(args, hip, host):
     """A scaled-down form of the batched workload: `batch_instances` uneven
@@ -492,6 +517,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream measurement")
     ap.add_argument("--no-lu32", action="store_true", help="skip the mixed-precision solve of the request's system")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
+    ap.add_argument("--autotune", type=int, default=1, choices=(0, 1),
+                    help="online autotuning of launch shapes (0: default shapes everywhere)")
     ap.add_argument("--batch-instances", type=int, default=6)
     ap.add_argument("--serving-only", action="store_true", help="only the request-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -524,8 +551,9 @@ def main():
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
                           "config": {"workload": "request stream"}, "serving": sv}), flush=True)
         return
-    from bem3_aw_qd2_amd import exdat, hip_lib, host_lib, mesh
+    from bem3_aw_qd2_amd import check, exdat, hip_lib, host_lib, mesh
     hip = hip_lib()
+    check(hip.bqd2_set_autotune(args.autotune), "bqd2_set_autotune")
     host = host_lib()
     cfg = mesh.CONFIGS[args.workload]
     big = args.workload == "sphere3456"
@@ -638,6 +666,7 @@ def main():
     if not args.no_batch:
         log("batch: instances with shared factorisations")
         line["batch"] = batch_leg(args, hip, host)
+    line["autotune"] = autotune_tables(hip)
     print(json.dumps(line), flush=True)
 
 
