@@ -184,7 +184,7 @@ static void launch_mv_wpb(int nrhs, int n, const double *A, long long lda, const
 
 static void launch_mv(int nrhs, int n, const double *A, long long lda, const int *map, const double *dinv,
                       const double *X, long long ldx, double *Y, long long ldy, hipStream_t st) {
-  const TuneTicket tk = tune_begin(TUNE_GMV, 8LL * tune_class(n) + nrhs, st);
+  const TuneTicket tk = tune_begin(TUNE_GMV, 8LL * tune_class(n) + nrhs, 16.0 * n * n, st);
   switch (tk.cand) {
     case 1: launch_mv_wpb<2>(nrhs, n, A, lda, map, dinv, X, ldx, Y, ldy, st); break;
     case 2: launch_mv_wpb<8>(nrhs, n, A, lda, map, dinv, X, ldx, Y, ldy, st); break;

@@ -247,14 +247,6 @@ __global__ void __launch_bounds__(256) k_cswap_cols(int n, float2 *__restrict__ 
 // triangle is solved in registers, then each later row of the lane's column
 // takes one 32-term dot product with it.
 #define TB 32
-// element (r, q), r > q, of the unit lower factor: from the packed LDS copy, or
-// (GL) straight from the matrix, where all lanes of a wavefront read the same
-// address and the load goes through the scalar cache
-template <bool GL>
-__device__ __forceinline__ float2 lval(const float2 *__restrict__ Ld, int n, const float2 *sL, int r, int q) {
-  return GL ? Ld[(size_t)q * n + r] : sL[r * (r - 1) / 2 + q];
-}
-
 // the strictly lower triangle of the kb x kb block into sL, packed by rows: a
 // flat walk (consecutive lanes on consecutive rows of a column) with eight
 // loads in flight per lane, since a walk column by column waits for one load
@@ -278,13 +270,10 @@ __device__ __forceinline__ void stage_l(int n, const float2 *__restrict__ Ld, in
 }
 
 // column c of X (ncols columns, ldx apart) := L^-1 X for the unit lower kb x kb
-// factor at Ld, by forward substitution in 32-row blocks held in registers:
-// the block's triangle row by row, then x_r -= L[r][block] x_block for the rows
-// below.  L comes from sL (staged by stage_l) or, with GL, from Ld; the
-// arithmetic is the same either way.
-template <bool GL>
-__device__ __forceinline__ void trsm_cols(int n, const float2 *__restrict__ Ld, int kb, float2 *Xall, long long ldx,
-                                          int ncols, int c, const float2 *sL) {
+// factor staged in sL by stage_l, by forward substitution in 32-row blocks held
+// in registers: the block's triangle row by row, then x_r -= L[r][block]
+// x_block for the rows below
+__device__ __forceinline__ void trsm_cols(int kb, float2 *Xall, long long ldx, int ncols, int c, const float2 *sL) {
   if (c >= ncols) return;
   float2 *X = Xall + (size_t)c * ldx;
   for (int b0 = 0; b0 < kb; b0 += TB) {
@@ -298,7 +287,7 @@ __device__ __forceinline__ void trsm_cols(int n, const float2 *__restrict__ Ld, 
       float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
       for (int q = 0; q < t; q++) {
-        const float2 l = lval<GL>(Ld, n, sL, b0 + t, b0 + q);
+        const float2 l = sL[(b0 + t) * (b0 + t - 1) / 2 + b0 + q];
         acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
         acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
       }
@@ -323,7 +312,7 @@ __device__ __forceinline__ void trsm_cols(int n, const float2 *__restrict__ Ld, 
         float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int q = 0; q < TB; q++) {
-          const float2 l = lval<GL>(Ld, n, sL, r, b0 + q);
+          const float2 l = sL[r * (r - 1) / 2 + b0 + q];
           acc.x += fmaf(l.x, x[q].x, -l.y * x[q].y);
           acc.y += fmaf(l.x, x[q].y, l.y * x[q].x);
         }
@@ -335,38 +324,12 @@ __device__ __forceinline__ void trsm_cols(int n, const float2 *__restrict__ Ld, 
   }
 }
 
-// BS columns per workgroup (one lane each), L staged in LDS or (GL) read from
-// the matrix: the variants differ in occupancy (the LDS copy allows two
-// workgroups per CU), not in any column's arithmetic
-template <int BS, bool GL>
-__global__ void __launch_bounds__(BS) k_ctrsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *X,
-                                                long long ldx, int ncols) {
-  if (GL) {
-    trsm_cols<true>(n, Ld, kb, X, ldx, ncols, blockIdx.x * BS + threadIdx.x, nullptr);
-  } else {
-    __shared__ float2 sL[NB * (NB - 1) / 2];
-    stage_l(n, Ld, kb, sL);
-    trsm_cols<false>(n, Ld, kb, X, ldx, ncols, blockIdx.x * BS + threadIdx.x, sL);
-  }
-}
-
-// U12 = L11^-1 A12 over ncols columns, in the variant the tuner picks
-static void ctrsm_l(int n, const float2 *Ld, int kb, float2 *X, long long ldx, int ncols, hipStream_t st) {
-  const TuneTicket tk = tune_begin(TUNE_TRSM, tune_class(ncols), st);
-  switch (tk.cand) {
-    case 1:
-      hipLaunchKernelGGL((k_ctrsm_l<64, true>), dim3(bqd2_blocks(ncols, 64)), dim3(64), 0, st, n, Ld, kb, X, ldx,
-                         ncols);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_ctrsm_l<256, true>), dim3(bqd2_blocks(ncols, 256)), dim3(256), 0, st, n, Ld, kb, X,
-                         ldx, ncols);
-      break;
-    default:
-      hipLaunchKernelGGL((k_ctrsm_l<256, false>), dim3(bqd2_blocks(ncols, 256)), dim3(256), 0, st, n, Ld, kb, X,
-                         ldx, ncols);
-  }
-  tune_end(tk, st);
+// 256 columns per workgroup, one lane each
+__global__ void __launch_bounds__(256) k_ctrsm_l(int n, const float2 *__restrict__ Ld, int kb, float2 *X,
+                                                 long long ldx, int ncols) {
+  __shared__ float2 sL[NB * (NB - 1) / 2];
+  stage_l(n, Ld, kb, sL);
+  trsm_cols(kb, X, ldx, ncols, blockIdx.x * 256 + threadIdx.x, sL);
 }
 
 // L11^-1 of every diagonal block (block blockIdx.x) into T + 2 NB NB blockIdx.x,
@@ -375,8 +338,7 @@ __global__ void __launch_bounds__(256) k_clinv(int n, const float2 *__restrict__
   __shared__ float2 sL[NB * (NB - 1) / 2];
   const int k0 = blockIdx.x * NB, kb = min(NB, n - k0);
   stage_l(n, A + (size_t)k0 * n + k0, kb, sL);
-  trsm_cols<false>(n, A + (size_t)k0 * n + k0, kb, T + (size_t)blockIdx.x * 2 * NB * NB, (long long)NB, kb,
-                   threadIdx.x, sL);
+  trsm_cols(kb, T + (size_t)blockIdx.x * 2 * NB * NB, (long long)NB, kb, threadIdx.x, sL);
 }
 
 // Inverse of the upper factor of the diagonal block k0 = NB blockIdx.x (kb x kb),
@@ -645,12 +607,13 @@ extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *str
     if (rest > 0) {
       // U12 = L11^-1 A12 in place, then A22 -= L21 U12 on the MFMA units
       float2 *A12 = A + (size_t)(k0 + kb) * n + k0;
-      ctrsm_l(n, A + (size_t)k0 * n + k0, kb, A12, (long long)n, rest, st);
+      hipLaunchKernelGGL(k_ctrsm_l, dim3(bqd2_blocks(rest, 256)), dim3(256), 0, st, n, A + (size_t)k0 * n + k0, kb,
+                         A12, (long long)n, rest);
       BQD2_HIP(hipGetLastError());
       // operands distinct from A22: either tile size, as the tuner picks
-      const TuneTicket tk = tune_begin(TUNE_CGEMM, tune_class(rest), st);
+      const TuneTicket tk = tune_begin(TUNE_CGEMM, tune_class(rest), 8.0 * rest * rest * kb, st);
       int rc = cgemm(false, rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n, A12, n,
-                     A + (size_t)(k0 + kb) * n + k0 + kb, n, st, tk.cand == 1 ? 1 : 2);
+                     A + (size_t)(k0 + kb) * n + k0 + kb, n, st, tk.cand == 1 ? 2 : 1);
       tune_end(tk, st);
       if (rc) return rc;
     }

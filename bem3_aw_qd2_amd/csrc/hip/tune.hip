@@ -2,10 +2,14 @@
 //
 // Exploration runs on the library's real calls, so it costs no extra work: a
 // call of a class still being explored launches the candidate with the fewest
-// calls so far and is bracketed by two HIP events.  Event pairs are read with
-// hipEventQuery at later tuner calls (never waiting); once every candidate of
-// a class holds kSamples timings the class is fixed to the candidate with the
-// smallest one (the minimum discards launches slowed by first-use code loading
+// timed calls so far, bracketed by two HIP events, until each candidate has
+// kSamples of them; later calls of the class take candidate 0 untimed until
+// the timings are in.  Event pairs are read with
+// hipEventQuery at later tuner calls (never waiting).  A class spans sizes
+// within a factor sqrt(2), so a timing counts as a rate: the caller's measure
+// of the call's work (flops, bytes) per millisecond.  Once every candidate of a
+// class holds kSamples rates the class is fixed to the candidate with the
+// highest one (the maximum discards launches slowed by first-use code loading
 // or by other streams).  Every candidate gives the same bits, so a result never
 // depends on the tuner's state.  All state is process-wide and guarded by one
 // mutex (the serving loop factors and solves on several threads).
@@ -18,7 +22,7 @@
 
 namespace {
 
-const int kCands[TUNE_NSITES] = {2, 3, 4};
+const int kCands[TUNE_NSITES] = {2, 4};
 const int kSamples = 3;
 const int kMaxPending = 512;
 
@@ -28,11 +32,12 @@ struct Entry {
   int chosen;          // -1 while exploring
   int issued[4];       // calls launched per candidate
   int n[4];            // timings read per candidate
-  float best[4];       // smallest timing per candidate (ms)
+  float best[4];       // highest rate per candidate (work per ms)
 };
 
 struct Pending {
   int entry, cand, gen;
+  double work;
   hipEvent_t e0, e1;
 };
 
@@ -41,7 +46,7 @@ std::vector<Entry> g_entries;
 std::vector<Pending> g_pending;      // indexed by slot; e0 == nullptr when free
 std::vector<hipEvent_t> g_events;    // recycled events
 int g_mode = 1;                      // 0: default candidates, 1: online tuning
-int g_force[TUNE_NSITES] = {-1, -1, -1};
+int g_force[TUNE_NSITES] = {-1, -1};
 int g_gen = 0;                       // table generation (bqd2_autotune_reset)
 
 hipEvent_t take_event() {
@@ -61,7 +66,7 @@ void settle(Entry &E) {
   int best = 0;
   for (int c = 0; c < nc; c++) {
     if (E.n[c] < kSamples) return;
-    if (E.best[c] < E.best[best]) best = c;
+    if (E.best[c] > E.best[best]) best = c;
   }
   E.chosen = best;
 }
@@ -77,8 +82,9 @@ void drain(bool wait) {
       // timings of a settled class (calls issued while it was explored) are
       // dropped: the choice stays the fastest of the timings it was made on
       Entry &E = g_entries[p.entry];
+      const float rate = (float)(p.work / (ms > 1e-6f ? ms : 1e-6f));
       if (E.chosen < 0) {
-        if (E.n[p.cand] == 0 || ms < E.best[p.cand]) E.best[p.cand] = ms;
+        if (E.n[p.cand] == 0 || rate > E.best[p.cand]) E.best[p.cand] = rate;
         E.n[p.cand]++;
         settle(E);
       }
@@ -108,7 +114,7 @@ int tune_class(long long n) {
   return (int)std::floor(2.0 * std::log2((double)n));
 }
 
-TuneTicket tune_begin(int site, long long key, hipStream_t st) {
+TuneTicket tune_begin(int site, long long key, double work, hipStream_t st) {
   TuneTicket t{0, -1};
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_force[site] >= 0) {
@@ -128,6 +134,9 @@ TuneTicket tune_begin(int site, long long key, hipStream_t st) {
   int c = 0;
   for (int i = 1; i < kCands[site]; i++)
     if (E.issued[i] < E.issued[c]) c = i;
+  // every candidate has its timed calls in flight (the host runs ahead of the
+  // device): the default candidate, untimed, until the timings are read
+  if (E.issued[c] >= kSamples) return t;
   t.cand = c;
   // time the call when a slot is free; otherwise run it untimed
   int slot = -1;
@@ -137,7 +146,7 @@ TuneTicket tune_begin(int site, long long key, hipStream_t st) {
       break;
     }
   if (slot < 0 && (int)g_pending.size() < kMaxPending) {
-    g_pending.push_back(Pending{0, 0, 0, nullptr, nullptr});
+    g_pending.push_back(Pending{0, 0, 0, 0.0, nullptr, nullptr});
     slot = (int)g_pending.size() - 1;
   }
   if (slot < 0) return t;
@@ -148,7 +157,7 @@ TuneTicket tune_begin(int site, long long key, hipStream_t st) {
   }
   E.issued[c]++;
   // e1 is recorded by tune_end; until then drain() passes the slot by
-  g_pending[slot] = Pending{ei, c, g_gen, e0, nullptr};
+  g_pending[slot] = Pending{ei, c, g_gen, work, e0, nullptr};
   t.slot = slot;
   return t;
 }
@@ -183,7 +192,7 @@ extern "C" int bqd2_autotune_force(int site, int cand) {
   return BQD2_OK;
 }
 
-extern "C" int bqd2_autotune_info(int site, int wait, int max, long long *keys, int *chosen, float *best_ms,
+extern "C" int bqd2_autotune_info(int site, int wait, int max, long long *keys, int *chosen, float *best_rate,
                                   int *ncand) {
   BQD2_CHECK(site >= 0 && site < TUNE_NSITES, BQD2_E_ARG, "bqd2_autotune_info: no site %d", site);
   std::lock_guard<std::mutex> lk(g_mu);
@@ -197,8 +206,8 @@ extern "C" int bqd2_autotune_info(int site, int wait, int max, long long *keys, 
     if (cnt < max) {
       if (keys) keys[cnt] = E.key;
       if (chosen) chosen[cnt] = E.chosen;
-      if (best_ms)
-        for (int c = 0; c < 4; c++) best_ms[4 * cnt + c] = (c < kCands[site] && E.n[c] > 0) ? E.best[c] : -1.0f;
+      if (best_rate)
+        for (int c = 0; c < 4; c++) best_rate[4 * cnt + c] = (c < kCands[site] && E.n[c] > 0) ? E.best[c] : -1.0f;
     }
     cnt++;
   }

@@ -287,16 +287,15 @@ def lu32_leg(hip, prob, na):
             "force_rel_diff_vs_fp64": float(np.linalg.norm(out[2]["F"] - out[0]["F"]) / np.linalg.norm(out[0]["F"]))}
 
 
-AUTOTUNE_SITES = {0: ("lu32_trailing_update", ["tile128", "tile64"]),
-                  1: ("lu32_u12_trsm", ["lds_wg256", "global_wg64", "global_wg256"]),
-                  2: ("gmres_matvec", ["rows4", "rows2", "rows8", "rows1"])}
+AUTOTUNE_SITES = {0: ("lu32_trailing_update", ["tile64", "tile128"]),
+                  1: ("gmres_matvec", ["rows4", "rows2", "rows8", "rows1"])}
 
 
 def autotune_tables(hip):
     """The online autotuner's classes on this device: per site, each size class
     (half octaves of the dimension; GMRES: 8 class + right-hand sides) with the
-    candidate it settled on (null while exploring) and the fastest timing seen
-    per candidate."""
+    candidate it settled on (null while exploring) and the highest rate seen per
+    candidate (trailing update: TFLOP/s; GMRES product: TB/s of the matrix)."""
     out = {}
     for site, (name, cands) in AUTOTUNE_SITES.items():
         keys = (C.c_longlong * 256)()
@@ -307,8 +306,8 @@ def autotune_tables(hip):
         rows = []
         for i in range(min(max(cnt, 0), 256)):
             rows.append({"class": int(keys[i]), "chosen": cands[chosen[i]] if chosen[i] >= 0 else None,
-                         "best_ms": {cands[c]: (round(best[4 * i + c], 4) if best[4 * i + c] >= 0 else None)
-                                     for c in range(nc.value)}})
+                         "best_rate": {cands[c]: (round(best[4 * i + c] / 1e9, 3) if best[4 * i + c] >= 0 else None)
+                                       for c in range(nc.value)}})
         out[name] = rows
     return out
 

@@ -232,24 +232,24 @@ int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, 
  * them one by one (process-wide; for tests that compare the two). */
 int bqd2_set_graphs(int on);
 
-/* No counterpart in the reference (the batching form's autotuner).  Three
+/* No counterpart in the reference (the batching form's autotuner).  Two
  * launches have several tile or workgroup sizes that give the same bits:
- *   site 0  trailing update of bqd2_cgetrf: 128 x 128 (0) or 64 x 64 (1) tiles;
- *   site 1  its U12 triangular solve: 256 columns per workgroup with L11 staged in LDS (0),
- *           or 64 (1) or 256 (2) columns per workgroup reading L11 through the scalar cache;
- *   site 2  GMRES matrix-vector product: 4 (0), 2 (1), 8 (2) or 1 (3) rows per workgroup.
+ *   site 0  trailing update of bqd2_cgetrf: 64 x 64 (0) or 128 x 128 (1) tiles;
+ *   site 1  GMRES matrix-vector product: 4 (0), 2 (1), 8 (2) or 1 (3) rows per workgroup.
  * With mode 1 (the default) the library times every candidate on its own calls
- * of each size class (half octaves of the dimension; for site 2 also the number
- * of right-hand sides), three times each, and then keeps the fastest for that
- * class and device; mode 0 uses candidate 0 everywhere.  bqd2_autotune_force
- * pins a site to one candidate (-1 releases it).  bqd2_autotune_info returns
- * the number of classes of a site on the current device and fills up to max
- * of them: key (the class), chosen (-1 while exploring) and best_ms[4 max]
- * (fastest timing per candidate, -1 where none yet); wait != 0 first waits for
- * timings in flight.  bqd2_autotune_reset forgets every class. */
+ * of each size class (half octaves of the dimension; for site 1 also the number
+ * of right-hand sides), three times each, as a rate (the call's work per
+ * millisecond: flops for site 0, bytes of the matrix for site 1), and then
+ * keeps the fastest for that class and device; mode 0 uses candidate 0
+ * everywhere.  bqd2_autotune_force pins a site to one candidate (-1 releases
+ * it).  bqd2_autotune_info returns the number of classes of a site on the
+ * current device and fills up to max of them: key (the class), chosen (-1
+ * while exploring) and best_rate[4 max] (highest rate per candidate, -1 where
+ * none yet); wait != 0 first waits for timings in flight.
+ * bqd2_autotune_reset forgets every class. */
 int bqd2_set_autotune(int mode);
 int bqd2_autotune_force(int site, int cand);
-int bqd2_autotune_info(int site, int wait, int max, long long *keys, int *chosen, float *best_ms, int *ncand);
+int bqd2_autotune_info(int site, int wait, int max, long long *keys, int *chosen, float *best_rate, int *ncand);
 int bqd2_autotune_reset(void);
 
 /* No counterpart in the reference.  Milliseconds of the last complex float
