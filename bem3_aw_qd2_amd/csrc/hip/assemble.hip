@@ -85,6 +85,19 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     rule = (tri ? needs_gh_tri(E, T, rt, vtz, vte) : needs_gh(E, T, rt, vtz, vte)) ? 1 : 0;
     if (rule && periodic && split_ok(E, T, rt, P, tri)) rule = 2;
     key = (float)fabs(E.cr[2][0] - rt[2]);
+    if (periodic) {
+      // the low six mantissa bits of the key carry the in-plane offset of the
+      // element centre reduced to the unit cell, in 64 levels up to the cell's
+      // half diagonal (a relative change of 2^-17 to the z offset)
+      const double dx = E.cr[0][0] - rt[0], dy = E.cr[1][0] - rt[1];
+      const double l1 = floor((dx * P.d2y - dy * P.d2x) * P.idd + 0.5);
+      const double l2 = floor(-(dx * P.d1y - dy * P.d1x) * P.idd + 0.5);
+      const double tx = dx - l1 * P.d1x - l2 * P.d2x, ty = dy - l1 * P.d1y - l2 * P.d2y;
+      const double sx = P.d1x + P.d2x, sy = P.d1y + P.d2y, mx = P.d1x - P.d2x, my = P.d1y - P.d2y;
+      const double rmax = 0.5 * sqrt(fmax(sx * sx + sy * sy, mx * mx + my * my));
+      const unsigned rq = (unsigned)fmin(63.0, sqrt(tx * tx + ty * ty) / rmax * 64.0);
+      key = __uint_as_float((__float_as_uint(key) & ~63u) | rq);
+    }
   }
   D.rule[gid] = rule;
   D.pkey[gid] = key;
@@ -1002,9 +1015,10 @@ __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
 // (row, s) pairs of the periodic domain by class (rule, element kind), each
 // class padded with -1 to a multiple of PPW so that a wavefront never mixes
 // classes.  Within a class the pairs are ordered by the z offset of the element
-// centre from the target (256 levels, ties in row-major order): the pairs of one wavefront
-// then take the same branch of the lattice sum (pure spectral beyond a height,
-// Ewald below) and about as many terms.
+// centre from the target (256 levels), then by its in-plane offset reduced to
+// the unit cell (64 levels, in the key's low mantissa bits), ties in row-major
+// order: the pairs of one wavefront then take the same branch of the lattice
+// sum (pure spectral beyond a height, Ewald below) and about as many terms.
 static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule, const std::vector<float> &key,
                       const std::vector<char> &tripos, std::vector<int> &out) {
   std::vector<int> cls[6];
@@ -1019,14 +1033,19 @@ static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule
   }
   out.clear();
   for (auto &c : cls) {
-    // stable counting sort on the key quantised to 256 levels
+    // stable counting sort on (z offset in 256 levels, in-plane offset in 64)
     float kmax = 0.0f;
     for (int g : c) kmax = key[g] > kmax ? key[g] : kmax;
     if (kmax > 0.0f && c.size() > 1) {
-      auto bucket = [&](int g) { int b = (int)(key[g] / kmax * 256.0f); return b > 255 ? 255 : b; };
-      std::vector<size_t> start(257, 0);
+      auto bucket = [&](int g) {
+        unsigned bits;
+        std::memcpy(&bits, &key[g], sizeof(bits));
+        int b = (int)(key[g] / kmax * 256.0f);
+        return 64 * (b > 255 ? 255 : b) + (int)(bits & 63u);
+      };
+      std::vector<size_t> start(256 * 64 + 1, 0);
       for (int g : c) start[bucket(g) + 1]++;
-      for (int b = 0; b < 256; b++) start[b + 1] += start[b];
+      for (int b = 0; b < 256 * 64; b++) start[b + 1] += start[b];
       std::vector<int> sorted(c.size());
       for (int g : c) sorted[start[bucket(g)]++] = g;
       c.swap(sorted);
