@@ -27,6 +27,7 @@
 struct DomView {
   int Ne;
   const int *sid;
+  const int *fperm;  // element positions in visiting order
   const double *P, *dP, *pv, *dpv;
   double k, k2x, k2y;
 };
@@ -246,7 +247,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const long long chunk = field_chunk_items(items);
   const long long i0 = (long long)ic * chunk, i1 = min(items, i0 + chunk);
   for (long long it = i0 + lane; it < i1; it += 64) {
-    const int s = (int)(it / npt) + 1, q = (int)(it % npt);
+    const int s = V.fperm[it / npt], q = (int)(it % npt);
     const int sd = V.sid[s], as = abs(sd);
     ElemC E;
     if (type != 0) load_elem(M, as, E);
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const int nci = bd_item_chunks(items);
   const long long i0 = (long long)ic * BD_CHUNK, i1 = (ic < nci) ? min(items, i0 + BD_CHUNK) : 0;
   for (long long it = i0 + lane; it < i1; it += 64) {
-    const int s = (int)(it / npt) + 1, q = (int)(it % npt);
+    const int s = V.fperm[it / npt], q = (int)(it % npt);
     if (s == t) continue;
     const int sd = V.sid[s], as = abs(sd);
     ElemC E;
@@ -915,6 +916,7 @@ static FieldCtx make_ctx(const bqd2_problem *p) {
     const DevDomain &D = p->dom[d];
     X.dom[d].Ne = D.Ne;
     X.dom[d].sid = D.sid;
+    X.dom[d].fperm = D.fperm;
     X.dom[d].P = D.P;
     X.dom[d].dP = D.dP;
     X.dom[d].pv = D.pv;

@@ -33,6 +33,7 @@ struct DevDomain {
   int has_tri;         // 1 when the domain has linear triangles
   int *sid;            // Ne+1 signed element ids
   int *rowmap;         // N: compressed node index of row (t-1)*4+tn, -1 for unused triangle slot
+  int *fperm;          // Ne: element positions in the field kernels' order (latitude bands, then azimuth)
   double k;            // real wave number of the domain
   double *G, *H;       // N x N complex
   double *dG, *dH;     // 2N x N complex (interior / isolated domains)

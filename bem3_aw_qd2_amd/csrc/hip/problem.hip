@@ -1,5 +1,9 @@
 // problem.hip -- creation, transfer and release of a device-resident problem.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <utility>
+#include <vector>
 #include "problem.h"
 
 int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes) {
@@ -123,6 +127,29 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     D.nn = nn;
     TRY(upload(p, &D.sid, sid, (size_t)D.Ne + 1));
     TRY(upload(p, &D.rowmap, rowmap.data(), rowmap.size()));
+    {
+      // the field kernels visit a point's elements in bands of the element
+      // centre's z (64 over the domain's extent), by azimuth within a band, so
+      // that the lanes of a wavefront hold neighbouring elements and their
+      // lattice sums need about as many terms
+      std::vector<int> ord(D.Ne > 0 ? D.Ne : 1);
+      double z0 = 1e300, z1 = -1e300;
+      for (int t = 1; t <= D.Ne; t++) {
+        const double z = m->cr[12 * (size_t)abs(sid[t]) + 8];
+        z0 = std::min(z0, z);
+        z1 = std::max(z1, z);
+      }
+      const double zs = (z1 > z0) ? 64.0 / (z1 - z0) : 0.0;
+      std::vector<std::pair<std::pair<int, double>, int>> key((size_t)D.Ne);
+      for (int t = 1; t <= D.Ne; t++) {
+        const double *c = m->cr + 12 * (size_t)abs(sid[t]);
+        const int band = std::min(63, (int)((c[8] - z0) * zs));
+        key[t - 1] = {{band, atan2(c[4], c[0])}, t};
+      }
+      std::stable_sort(key.begin(), key.end());
+      for (int i = 0; i < D.Ne; i++) ord[i] = key[i].second;
+      TRY(upload(p, &D.fperm, ord.data(), ord.size()));
+    }
     const size_t N = D.N;
     const bool periodic0 = (d == 0 && m->ps == 1);
     TRY(bqd2_alloc(p, (void **)&D.G, N * N * 16));
