@@ -129,7 +129,7 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     TRY(upload(p, &D.rowmap, rowmap.data(), rowmap.size()));
     {
       // the field kernels visit a point's elements in bands of the element
-      // centre's z (64 over the domain's extent), by azimuth within a band, so
+      // centre's z (32 over the domain's extent), by azimuth within a band, so
       // that the lanes of a wavefront hold neighbouring elements and their
       // lattice sums need about as many terms
       std::vector<int> ord(D.Ne > 0 ? D.Ne : 1);
@@ -139,11 +139,11 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
         z0 = std::min(z0, z);
         z1 = std::max(z1, z);
       }
-      const double zs = (z1 > z0) ? 64.0 / (z1 - z0) : 0.0;
+      const double zs = (z1 > z0) ? 32.0 / (z1 - z0) : 0.0;
       std::vector<std::pair<std::pair<int, double>, int>> key((size_t)D.Ne);
       for (int t = 1; t <= D.Ne; t++) {
         const double *c = m->cr + 12 * (size_t)abs(sid[t]);
-        const int band = std::min(63, (int)((c[8] - z0) * zs));
+        const int band = std::min(31, (int)((c[8] - z0) * zs));
         key[t - 1] = {{band, atan2(c[4], c[0])}, t};
       }
       std::stable_sort(key.begin(), key.end());
