@@ -234,9 +234,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   // asm statements keep the compiler from caching LDS values across it.
   //   sacc: Pa, dPa[3], PVa[3] (re, im)
   //   scf:  alpha = wt aW dPi, beta = -wt sg Pi, gamma[c] = wt aW dpv_c,
-  //         delta[c] = -wt sg pv_c (re, im), W[3]
+  //         delta[c] = -wt sg pv_c (re, im), W[3], the offset r[3]
   __shared__ double sacc[14][64];
-  __shared__ double scf[19][64];
+  __shared__ double scf[22][64];
 #pragma unroll
   for (int i = 0; i < 14; i++) sacc[i][lane] = 0.0;
   double F = 0.0, dF[3] = {0, 0, 0};
@@ -273,7 +273,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     // PVa_c += g gamma_c + (dg.W) delta_c, with G = g aW and H = dg.W the kernel
     // values of the representation formula (free space or lattice sum)
     const cd alpha = dPi * (wt * aW), beta = Pi * (-wt * sg);
-    cd gv, dgw, dgp[3], hwp[3];  // g, dg.W, dg_c, (H W)_c
+    // one item's share: g, dg.W, dg_c and (H W)_c against the coefficients
+    // al, be (scalar part) and gamma_c, delta_c (LDS)
+    auto accumulate = [&](const cd &gv, const cd &dgw, const cd *dgp, const cd *hwp, const cd &al, const cd &be) {
+      {
+        const cd v = gv * al + dgw * be;
+        sacc[0][lane] += v.x; sacc[1][lane] += v.y;
+      }
+      for (int c2 = 0; c2 < 3; c2++) {
+        const cd d = -(dgp[c2] * al) - hwp[c2] * be;
+        sacc[2 + 2 * c2][lane] += d.x; sacc[3 + 2 * c2][lane] += d.y;
+        const cd gm = cd{scf[4 + 2 * c2][lane], scf[5 + 2 * c2][lane]};
+        const cd dl = cd{scf[10 + 2 * c2][lane], scf[11 + 2 * c2][lane]};
+        const cd pvv = gv * gm + dgw * dl;
+        sacc[8 + 2 * c2][lane] += pvv.x; sacc[9 + 2 * c2][lane] += pvv.y;
+      }
+    };
     if (periodic) {
       scf[0][lane] = alpha.x; scf[1][lane] = alpha.y; scf[2][lane] = beta.x; scf[3][lane] = beta.y;
       for (int c2 = 0; c2 < 3; c2++) {
@@ -282,22 +297,28 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         scf[10 + 2 * c2][lane] = dl.x; scf[11 + 2 * c2][lane] = dl.y;
         scf[16 + c2][lane] = w[c2];
       }
-      const double x0 = vR[0], x1 = vR[1], x2 = vR[2];
-      asm volatile("" ::: "memory");
-      Sum10 gf;
-      int c = qpgf_eval<GF_V2>(P, x0, x1, x2, 1.0e-15, gf, tl);
-      if (c < 0) err = c;
+      scf[19][lane] = vR[0]; scf[20][lane] = vR[1]; scf[21][lane] = vR[2];
+      // the image share and the spectral share of the Ewald split in two
+      // evaluations (the result is linear in them), so that neither share is
+      // held in registers across the other's sum
+      for (int part = 1; part <= 2; part++) {
+        asm volatile("" ::: "memory");
+        const double x0 = scf[19][lane], x1 = scf[20][lane], x2 = scf[21][lane];
+        Sum10 gf;
+        int c = qpgf_eval<GF_V2>(P, x0, x1, x2, 1.0e-15, gf, tl, part);
+        if (c < 0) err = c;
+        asm volatile("" ::: "memory");
+        const double W0 = scf[16][lane], W1 = scf[17][lane], W2 = scf[18][lane];
+        // Hessian as a symmetric matrix: xx, yy, zz, xy, yz, zx
+        const cd *h = gf.d2g;
+        const cd dgw = gf.dg[0] * W0 + gf.dg[1] * W1 + gf.dg[2] * W2;
+        cd hwp[3];
+        hwp[0] = h[0] * W0 + h[3] * W1 + h[5] * W2;
+        hwp[1] = h[3] * W0 + h[1] * W1 + h[4] * W2;
+        hwp[2] = h[5] * W0 + h[4] * W1 + h[2] * W2;
+        accumulate(gf.g, dgw, gf.dg, hwp, cd{scf[0][lane], scf[1][lane]}, cd{scf[2][lane], scf[3][lane]});
+      }
       evals++;
-      asm volatile("" ::: "memory");
-      const double W0 = scf[16][lane], W1 = scf[17][lane], W2 = scf[18][lane];
-      // Hessian as a symmetric matrix: xx, yy, zz, xy, yz, zx
-      const cd *h = gf.d2g;
-      gv = gf.g;
-      dgw = gf.dg[0] * W0 + gf.dg[1] * W1 + gf.dg[2] * W2;
-      hwp[0] = h[0] * W0 + h[3] * W1 + h[5] * W2;
-      hwp[1] = h[3] * W0 + h[1] * W1 + h[4] * W2;
-      hwp[2] = h[5] * W0 + h[4] * W1 + h[2] * W2;
-      for (int c2 = 0; c2 < 3; c2++) dgp[c2] = gf.dg[c2];
     } else {
       // free space: G = e^{ikR}/(4 pi R) aW, dG/dr_c and (H W)_c in closed form
       for (int c2 = 0; c2 < 3; c2++) {
@@ -306,27 +327,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         scf[10 + 2 * c2][lane] = dl.x; scf[11 + 2 * c2][lane] = dl.y;
       }
       cd ce = cis(k * aR), cb = cd{-1.0, k * aR};
-      gv = ce * (BQD2_IFP * iR);
-      dgw = cb * ce * (BQD2_IFP * rdW * iR * iR);
+      const cd gv = ce * (BQD2_IFP * iR);
+      const cd dgw = cb * ce * (BQD2_IFP * rdW * iR * iR);
       cd c3 = cd{k * k * aR * aR + 3.0 * cb.x, 3.0 * cb.y};
+      cd dgp[3], hwp[3];
       for (int c2 = 0; c2 < 3; c2++) {
         dgp[c2] = cb * ce * (BQD2_IFP * iR * iR * vR[c2] * iR);
         hwp[c2] = -(ce * (BQD2_IFP * iR * iR * iR) * (c3 * (vR[c2] * iR * rdW) - cb * w[c2]));
       }
-    }
-    const cd al = periodic ? cd{scf[0][lane], scf[1][lane]} : alpha;
-    const cd be = periodic ? cd{scf[2][lane], scf[3][lane]} : beta;
-    {
-      const cd v = gv * al + dgw * be;
-      sacc[0][lane] += v.x; sacc[1][lane] += v.y;
-    }
-    for (int c2 = 0; c2 < 3; c2++) {
-      const cd d = -(dgp[c2] * al) - hwp[c2] * be;
-      sacc[2 + 2 * c2][lane] += d.x; sacc[3 + 2 * c2][lane] += d.y;
-      const cd gm = cd{scf[4 + 2 * c2][lane], scf[5 + 2 * c2][lane]};
-      const cd dl = cd{scf[10 + 2 * c2][lane], scf[11 + 2 * c2][lane]};
-      const cd pvv = gv * gm + dgw * dl;
-      sacc[8 + 2 * c2][lane] += pvv.x; sacc[9 + 2 * c2][lane] += pvv.y;
+      accumulate(gv, dgw, dgp, hwp, alpha, beta);
     }
   }
   asm volatile("" ::: "memory");

@@ -508,9 +508,13 @@ __device__ __forceinline__ void sum_zero(Sum10& s) {
   for (int i = 0; i < 6; i++) s.d2g[i] = cd{0, 0};
 }
 
-// Ewald evaluation in the unit cell (d3_qpgf_d2.c:1671-1712, :2011-2052)
+// Ewald evaluation in the unit cell (d3_qpgf_d2.c:1671-1712, :2011-2052).
+// parts selects the shares summed (1: images, 2: spectral orders, 3: both):
+// the result is linear in the two, so a caller may take them in two calls and
+// hold neither across the other's sum.
 template <int M>
-__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
+__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl,
+                          int parts = 3) {
   double E = P.E0;
   double r2 = rx * rx + ry * ry + rz * rz;
   double k2 = P.k * P.k;
@@ -525,7 +529,7 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   const double iep = 0.039788735772973833942;  // 1/(8 pi)
   Key2 old;
   int c1 = 0;
-  {
+  if (parts & 1) {
   Sum10 S1;
   sum_zero(S1);
   for (int l1 = 0, dir = 0;;) {
@@ -561,6 +565,10 @@ __device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, do
   }
   }
 
+  if (!(parts & 2)) {
+    tl.sp += c1;
+    return c1;
+  }
   Sum10 S2;
   sum_zero(S2);
   int flag = 0, nev = 0, c2 = 0;
@@ -643,9 +651,12 @@ __device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, doubl
 }
 
 // full evaluation at offset r (source minus target); returns the term count or
-// the reference's negative code (-1x pure spectral, -2x Ewald)
+// the reference's negative code (-1x pure spectral, -2x Ewald).  parts: the
+// shares of the Ewald split to sum (ewald_cell); a pure spectral evaluation is
+// all taken with the image share (parts & 1) and is zero for parts == 2.
 template <int M>
-__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
+__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl,
+                         int parts = 3) {
   bool use_fd = false;
   if (!P.ewald_only) {
     const double zs = fabs(rz) / P.fd_lift;
@@ -656,7 +667,8 @@ __device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, dou
   double tx = rx - l1 * P.d1x - l2 * P.d2x;
   double ty = ry - l1 * P.d1y - l2 * P.d2y;
   sum_zero(out);
-  int err = use_fd ? fd_cell<M>(P, tx, ty, rz, eps, out, tl) : ewald_cell<M>(P, tx, ty, rz, eps, out, tl);
+  if (use_fd && !(parts & 1)) return 0;
+  int err = use_fd ? fd_cell<M>(P, tx, ty, rz, eps, out, tl) : ewald_cell<M>(P, tx, ty, rz, eps, out, tl, parts);
   if (l1 != 0 || l2 != 0) {
     double arg = -(P.vkx * (l1 * P.d1x + l2 * P.d2x) + P.vky * (l1 * P.d1y + l2 * P.d2y));
     cd cf = cis(arg);
