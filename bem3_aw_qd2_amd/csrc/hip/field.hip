@@ -443,11 +443,15 @@ __host__ __device__ __forceinline__ int bd_item_chunks(long long items) {
   return (int)((items + BD_CHUNK - 1) / BD_CHUNK);
 }
 
+// SELF = 0: the item chunks (blockIdx.y = chunk); SELF = 1: the point's own
+// element (one wavefront per point, chunk slot nci).  Two instantiations keep
+// the polar rule's register needs out of the chunks' lattice-sum loop.
+template <int SELF>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_field_bd(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
                                                  int type, int did, int n, const int *__restrict__ tpos,
                                                  const double *__restrict__ ze, double *__restrict__ part,
                                                  long long *__restrict__ counters) {
-  const int ip = blockIdx.x, ic = blockIdx.y, lane = threadIdx.x;
+  const int ip = blockIdx.x, lane = threadIdx.x;
   if (ip >= n) return;
   const Tab &T = *tab;
   const DomView &V = X.dom[did];
@@ -481,7 +485,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   // regular elements
   const long long items = (long long)V.Ne * npt;
   const int nci = bd_item_chunks(items);
-  const long long i0 = (long long)ic * BD_CHUNK, i1 = (ic < nci) ? min(items, i0 + BD_CHUNK) : 0;
+  const int ic = SELF ? nci : (int)blockIdx.y;
+  const long long i0 = (long long)ic * BD_CHUNK, i1 = SELF ? 0 : min(items, i0 + BD_CHUNK);
   for (long long it = i0 + lane; it < i1; it += 64) {
     const int s = V.fperm[it / npt], q = (int)(it % npt);
     if (s == t) continue;
@@ -544,7 +549,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   F *= BQD2_IFP;
   ElemC Et;
   load_elem(M, at, Et);
-  if (ic != nci) {
+  if (!SELF) {
     // item chunk: no self element
   } else if (M.tri[at]) {
     // self triangle: polar rule over the three sub-triangles about (zt, et)
@@ -827,7 +832,10 @@ __global__ void __launch_bounds__(256) k_field_bd_fin(DevMesh M, FieldCtx X, int
 static int launch_field_bd(bqd2_problem *p, hipStream_t st, FieldCtx &X, int type, int total, int did, int n,
                            const int *d_t, const double *d_ze, double *d_part, double *d_out, long long *ctr) {
   const int nch = bd_item_chunks((long long)p->dom[did].Ne * rule_points(type, p->any_tri)) + 1;
-  hipLaunchKernelGGL(k_field_bd, dim3(n, nch), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t, d_ze,
+  if (nch > 1)
+    hipLaunchKernelGGL(k_field_bd<0>, dim3(n, nch - 1), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t,
+                       d_ze, d_part, ctr);
+  hipLaunchKernelGGL(k_field_bd<1>, dim3(n, 1), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t, d_ze,
                      d_part, ctr);
   hipLaunchKernelGGL(k_field_bd_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, total, did, n, nch, d_t,
                      d_ze, d_part, d_out);

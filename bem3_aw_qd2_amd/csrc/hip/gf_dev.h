@@ -83,15 +83,8 @@ __device__ __forceinline__ cd weideman24_upper(double x, double y) {
 // real and covers (-1, 1] once, so a real polynomial in Z fitted to erfcx's
 // relative error (degree 20, < 7e-16 on t >= 0; coefficients from
 // gen_erfcx.py) replaces the N = 40 Fourier table at half the multiply-adds;
-// y < 0 by erfcx(y) = 2 exp(y^2) - erfcx(-y).  Inlined where the translation
-// unit asks for it (BQD2_ERFCX_INLINE: the assembly kernels); called elsewhere,
-// which keeps the register pressure of the field kernels (Hessian sums) lower.
-#if defined(BQD2_ERFCX_INLINE)
-#define BQD2_ERFCX_ATTR __forceinline__
-#else
-#define BQD2_ERFCX_ATTR __noinline__
-#endif
-__device__ BQD2_ERFCX_ATTR double erfcx_w(double y) {
+// y < 0 by erfcx(y) = 2 exp(y^2) - erfcx(-y).
+__device__ __forceinline__ double erfcx_body(double y) {
   const double L = ERFCX_L;
   const double c[ERFCX_N] = {ERFCX_COEFS};
   const double t = fabs(y);
@@ -102,6 +95,19 @@ __device__ BQD2_ERFCX_ATTR double erfcx_w(double y) {
   const double isp = 0.56418958354775628695;
   const double v = 2.0 * p * iden * iden + isp * iden;
   return y >= 0.0 ? v : 2.0 * exp(y * y) - v;
+}
+__device__ __noinline__ double erfcx_call(double y) { return erfcx_body(y); }
+
+// Inlined in the value-and-gradient sums; in the Hessian sums (GF_V2) a call,
+// which keeps their register pressure lower, unless the translation unit asks
+// for it inline everywhere (BQD2_ERFCX_INLINE: the assembly kernels).
+template <int M>
+__device__ __forceinline__ double erfcx_w(double y) {
+#if defined(BQD2_ERFCX_INLINE)
+  return erfcx_body(y);
+#else
+  return M == GF_V2 ? erfcx_call(y) : erfcx_body(y);
+#endif
 }
 
 // w(x + i y) with the reference's conventions: real axis returns Re w = exp(-x^2)
@@ -294,8 +300,8 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
       ib = rsqrt_pos(-bn2);
       bn = -bn2 * ib;
       efs = eff;
-      f1 = efs * erfcx_w(bn * E + azve);
-      f2 = efs * erfcx_w(bn * E - azve);
+      f1 = efs * erfcx_w<M>(bn * E + azve);
+      f2 = efs * erfcx_w<M>(bn * E - azve);
     } else {
       bn = sqrt(-bn2);
       ib = 1.0 / bn;
