@@ -20,6 +20,7 @@
 #include "../../../include/weideman_coef.h"
 #include "../../../include/erfcx_coef.h"
 #include "../../../include/bqd2.h"
+#include "exp_cr.h"
 
 // Products and sums are rounded one at a time, as the reference's C does on
 // x86-64 (no contraction into FMA): the Ewald spectral sum at z == 0 decides
@@ -111,12 +112,13 @@ __device__ __forceinline__ double erfcx_w(double y) {
 }
 
 // w(x + i y) with the reference's conventions: real axis returns Re w = exp(-x^2)
-// exactly (Faddeeva.cc:684-686), lower half plane by reflection.
+// exactly (Faddeeva.cc:684-686), correctly rounded (exp_cr.h), lower half plane
+// by reflection.
 __device__ __forceinline__ cd faddeeva_w(double x, double y) {
   if (y > 0.0) return weideman_upper(x, y);
   if (y == 0.0) {
     cd w = weideman_upper(x, 0.0);
-    return cd{exp(-__dmul_rn(x, x)), w.y};
+    return cd{exp_cr(-__dmul_rn(x, x)), w.y};
   }
   // w(z) = 2 exp(-z^2) - w(-z)
   cd wm = weideman_upper(-x, -y);
@@ -272,8 +274,10 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     double bn = sqrt(bn2);
     cd ff = cis(bn * rz);
     // ef and w kept apart as the reference does: at z == 0, ef*Re(w) - ff is
-    // pure rounding and the row stopping rule reads it (products not fused)
-    double ef = fast ? eff : exp(__dmul_rn(__dmul_rn(bn2, E), E) - __dmul_rn(azve, azve));
+    // pure rounding and the row stopping rule reads it (products not fused;
+    // both exponentials correctly rounded, so the residue does not depend on
+    // which libm the reference was built with)
+    double ef = fast ? eff : exp_cr(__dmul_rn(__dmul_rn(bn2, E), E) - __dmul_rn(azve, azve));
     cd w = faddeeva_w(bn * E, azve);
     cd F = cd{2.0 * ff.x, 2.0 * (__dmul_rn(ef, w.y) + ff.y)};
     double ib = 1.0 / bn;

@@ -40,7 +40,7 @@ double complex orc_faddeeva_w(double complex z)
   /* on the real axis Re w(x) = exp(-x^2) exactly as the reference returns it
    * (Faddeeva.cc:684-686); the Ewald spectral sum at z = 0 subtracts it from
    * cos(0) and its stopping rule then sees that difference */
-  if (cimag(z) == 0.0) return exp(-creal(z) * creal(z)) + I * cimag(w_upper(z));
+  if (cimag(z) == 0.0) return orc_exp_z0(-creal(z) * creal(z)) + I * cimag(w_upper(z));
   if (cimag(z) > 0.0) return w_upper(z);
   /* w(z) = 2 exp(-z^2) - w(-z) */
   return 2.0 * cexp(-z * z) - w_upper(-z);

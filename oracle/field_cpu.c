@@ -91,11 +91,12 @@ int orc_field_s(const AQD2 *qd, int did, const double *rt, int type, double comp
 }
 
 /* scattered field at the boundary point (zt, et) of element position t of
- * domain did (pv_s_bd_aqd2, bem3_aw_qd2_field.c:394-430) for field types 2
- * (GLN) and 3 (GHN): the other elements on the element rule, the element
- * itself on its polar rule about the point (coef_bd / q0_coef_bd,
- * d3qd2_elem.c:184-229, 609-654).  p = -k2 P / C, v = (sum G dpv - H pv) / C.
- * Returns 0 or a negative Green's function status. */
+ * domain did (pv_s_bd_aqd2, bem3_aw_qd2_field.c:394-430) for field types 0, 1
+ * (the 4-/9-point and 4-/7-point rules of orc_coef_pts), 2 (GLN) and 3 (GHN):
+ * the other elements on the element rule, the element itself on its polar rule
+ * about the point (coef_bd / q0_coef_bd, d3qd2_elem.c:187-230, :684-726).
+ * p = -k2 P / C, v = (sum G dpv - H pv) / C.  Returns 0 or a negative Green's
+ * function status. */
 int orc_field_bd(const AQD2 *qd, int did, int t, double zt, double et, int type, double complex *p,
                  double complex *pv)
 {
@@ -116,7 +117,9 @@ int orc_field_bd(const AQD2 *qd, int did, int t, double zt, double et, int type,
     int tri = bd->ed[abs(sd)][3] == 0;
     double complex CC[9], d0[9], d1[9];
     long c = 0;
-    if (sd != td) {
+    if (sd != td && type <= 1) {
+      c = orc_coef_pts(qd, did, rt, sd, type, CC);
+    } else if (sd != td) {
       if (periodic)
         c = tri ? orc_q0_lit_pair(bd, rt, v0, v0, sd, n, x, w, CC, d0, d1)
                 : orc_q0_bil_pair(bd, rt, v0, v0, sd, n, x, w, CC, d0, d1);

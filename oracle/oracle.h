@@ -22,6 +22,12 @@ double complex orc_faddeeva_w(double complex z);
 double complex orc_scaled_w(double x, double y);
 void orc_faddeeva_w_batch(int n, const double *z, double *w);
 
+/* exp of the two factors whose rounding residue the z == 0 stopping rule of the
+ * Ewald spectral sum reads: libm's (mode 0, the default) or correctly rounded
+ * (mode 1) -- qpgf_cpu.c. */
+void orc_set_cr_exp(int on);
+double orc_exp_z0(double x);
+
 /* Quasi-periodic Green's function (qpgf_cpu.c).  mode: 0 value only
  * (d3hm_qpgf_d2_qG), 1 gradient only (_dqG), 2 value+gradient (_v1),
  * 3 value+gradient+Hessian (_v2).  Returns the number of lattice terms summed

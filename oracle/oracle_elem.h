@@ -62,4 +62,13 @@ int orc_field_s(const AQD2 *qd, int did, const double *rt, int type, double comp
 int orc_field_bd(const AQD2 *qd, int did, int t, double zt, double et, int type, double complex *p,
                  double complex *pv);
 
+/* point rules of field types 0 and 1, incident wave, boundary field and
+ * radiation force (force_cpu.c) */
+long orc_coef_pts(const AQD2 *ad, int did, const double *rt, int sd, int type, double complex *CC);
+void orc_apw_pv(const Apw *pw, const double *r, double complex *p, double complex *v);
+int orc_field_bd_total(const AQD2 *ad, int t, double zt, double et, int type, double complex *p,
+                       double complex *pv);
+int orc_force_elem(const AQD2 *ad, const double *rc, int type, int t, double *F, double *N);
+int orc_force(const AQD2 *ad, const double *rc, int type, double *F, double *N);
+
 #endif

@@ -22,7 +22,19 @@
 #include <math.h>
 #include <complex.h>
 #include <stdlib.h>
+#include <quadmath.h>
 #include "oracle.h"
+
+/* The exponentials whose product's rounding residue the z == 0 stopping rule
+ * reads (the propagating Ewald spectral order, d3_qpgf_d2.c:1976-1989, and
+ * Faddeeva's real axis, Faddeeva.cc:684-686).  Mode 0: this libm's exp, as the
+ * reference compiled here evaluates them.  Mode 1: correctly rounded, from
+ * libquadmath's 113-bit expq rounded once to double, the platform-independent
+ * statement of the same rule (glibc 2.35's exp misrounds about 0.08 % of
+ * arguments; the GPU evaluates these two exponentials correctly rounded). */
+static int cr_exp_mode = 0;
+void orc_set_cr_exp(int on) { cr_exp_mode = on; }
+double orc_exp_z0(double x) { return cr_exp_mode ? (double)expq((__float128)x) : exp(x); }
 
 enum { M_QG = 0, M_DQG = 1, M_V1 = 2, M_V2 = 3 };
 
@@ -152,7 +164,7 @@ static int spectral_term(int mode, const QPD2 *qd, const double *r, double E,
     /* exp(bn^2 E^2 - azve^2) and w(bn E + i azve) kept apart as in :1976-1985:
      * at z == 0 the difference ef*Re(w) - ff is pure rounding and the row
      * stopping rule depends on it */
-    double ef = exp(bn2 * E * E - azve * azve);
+    double ef = orc_exp_z0(bn2 * E * E - azve * azve);
     double complex w = orc_faddeeva_w(bn * E + I * azve);
     double complex F = 2.0 * (I * ef * cimag(w) + ff);
     double complex Gz = 2.0 * (ef * creal(w) - ff);

@@ -59,8 +59,18 @@ def ref_eval(mode, qd, pts, eps=1e-15):
     return G, dG, d2G, ret
 
 
-def oracle_eval(mode, qd, pts, eps=1e-15):
+def oracle_eval(mode, qd, pts, eps=1e-15, cr_exp=False):
+    """The oracle's Green's function at pts; cr_exp: the two exponentials whose
+    rounding residue the z == 0 stopping rule reads are correctly rounded."""
     lib = oracle()
+    lib.orc_set_cr_exp(C.c_int(1 if cr_exp else 0))
+    try:
+        return _oracle_eval(lib, mode, qd, pts, eps)
+    finally:
+        lib.orc_set_cr_exp(C.c_int(0))
+
+
+def _oracle_eval(lib, mode, qd, pts, eps):
     n = len(pts)
     r = np.ascontiguousarray(pts, dtype=np.float64)
     G = np.zeros(n, complex); dG = np.zeros((n, 3), complex); d2G = np.zeros((n, 6), complex)

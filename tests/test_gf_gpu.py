@@ -2,9 +2,13 @@
 
 Bar: every value within 1e-12 of the reference, normwise per point (the reference
 itself converges to eps = 1e-15 per lattice row).  With the reference's method
-choice (GfParams.fd_lift = 1) the term counts are identical except at z == 0
-exactly, where the reference's stopping rule reads the rounding residue of
-exp(a) * exp(-a) (see gf_dev.h) and the two libm's may round differently.  With
+choice (GfParams.fd_lift = 1) the term counts are identical.  At z == 0 exactly
+the reference's stopping rule reads the rounding residue of
+exp(bn^2 E^2) exp(-(bn E)^2) - 1; the GPU evaluates those two exponentials
+correctly rounded (exp_cr.h), and there it is compared with the oracle in the
+same mode: equal term counts and values within 1e-12 at every point, and equal
+term counts with the reference compiled here at all but the points where glibc's
+exp misrounds (the oracle's own test pins that difference to exp alone).  With
 the default lift (2.2: the Ewald sum up to 2.2 times the reference's switch
 height) the values stay within the same bar, and the points summed by the same
 method keep the reference's term counts."""
@@ -13,7 +17,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from tests.checkers import ref_eval
+from tests.checkers import oracle_eval, ref_eval
 from tests.lattices import CASES, points, qpd2
 
 pytestmark = pytest.mark.gpu
@@ -82,5 +86,41 @@ def test_gf_matches_reference(case, mode, lift):
     same = ~((az >= switch_height(q, 1.0)) & (az < switch_height(q, 2.2 if lift is None else lift)))
     print(case, mode, lift, "points summed differently from the reference:", int((~same).sum()))
     assert (tg[~z0 & same] == tr[~z0 & same]).all()
-    # at z == 0 at least most points agree bit-for-bit in their stopping decisions
-    assert (tg[z0] == tr[z0]).mean() > 0.8
+    # z == 0: the oracle with correctly rounded exponentials, point for point
+    Go, dGo, d2Go, to = oracle_eval(mode, q, pts[z0], cr_exp=True)
+    assert (tg[z0] == to).all()
+    eo = [relerr(Gg[z0], Go)] if mode != 1 else []
+    if mode >= 1:
+        eo.append(relerr(dGg[z0], dGo))
+    if mode == 3:
+        eo.append(relerr(d2Gg[z0], d2Go))
+    assert np.max(eo) < 1e-12
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_gf_z0(case, mode):
+    """1000 offsets with z == 0 exactly (level with the source) per lattice."""
+    q = qpd2(case)
+    k, th, ph, d1, d2, rm = CASES[case]
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-rm, rm, (1000, 3))
+    pts[:, 2] = 0.0
+    pts[:50, :2] *= 1e-3
+    Gg, dGg, d2Gg, tg = gpu_eval(mode, q, pts)
+    Go, dGo, d2Go, to = oracle_eval(mode, q, pts, cr_exp=True)
+    Gr, dGr, d2Gr, tr = ref_eval(mode, q, pts)
+    assert (tg > 0).all() and (tg == to).all(), int((tg != to).sum())
+    errs = [relerr(Gg, Go)] if mode != 1 else []
+    if mode >= 1:
+        errs.append(relerr(dGg, dGo))
+    if mode == 3:
+        errs.append(relerr(d2Gg, d2Go))
+    assert np.max(errs) < 1e-12
+    same = tg == tr
+    print(case, mode, "z == 0 points stopping where the reference built here stops:", int(same.sum()), "of", len(pts))
+    assert same.mean() >= 0.995
+    if mode != 1:
+        assert relerr(Gg[same], Gr[same]).max() < 1e-12
+    else:
+        assert relerr(dGg[same], dGr[same]).max() < 1e-12

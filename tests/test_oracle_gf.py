@@ -4,10 +4,14 @@ oracle/_ref/ (skipped when that build is absent), and against properties that
 pin it without the reference: quasi-periodicity, independence of the result
 from the Ewald parameter (the spectral form agrees with the Ewald split) and the
 Helmholtz equation through the Hessian trace."""
+import ctypes as C
+import math
+
+import mpmath
 import numpy as np
 import pytest
 
-from tests.checkers import oracle_eval, ref_eval
+from tests.checkers import oracle, oracle_eval, ref_eval
 from tests.lattices import CASES, points, qpd2
 
 
@@ -79,3 +83,45 @@ def test_ewald_and_spectral_forms_agree():
     # smoothness across the whole range: second differences are small
     d2 = np.abs(G[2:] - 2 * G[1:-1] + G[:-2])
     assert d2.max() < 1e-3 * np.abs(G).max()
+
+
+def test_z0_exp_modes():
+    """orc_exp_z0: mode 0 is this libm's exp (what the reference compiled here
+    calls), mode 1 the correctly rounded exp (mpmath at 200 bits, rounded once)."""
+    lib = oracle()
+    lib.orc_exp_z0.restype = C.c_double
+    lib.orc_exp_z0.argtypes = [C.c_double]
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-40.0, 40.0, 6000), rng.uniform(-1e-3, 1e-3, 1000)])
+    mpmath.mp.prec = 200
+    try:
+        lib.orc_set_cr_exp(C.c_int(0))
+        assert all(lib.orc_exp_z0(float(x)) == math.exp(float(x)) for x in xs)
+        lib.orc_set_cr_exp(C.c_int(1))
+        assert all(lib.orc_exp_z0(float(x)) == float(mpmath.exp(mpmath.mpf(float(x)))) for x in xs)
+    finally:
+        lib.orc_set_cr_exp(C.c_int(0))
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_z0_correctly_rounded_mode(case):
+    """At z == 0 exactly the stopping rule of the Ewald spectral sum reads the
+    rounding residue of exp(bn^2 E^2) exp(-(bn E)^2) - 1.  With those two
+    exponentials correctly rounded (the GPU's choice) the term counts equal the
+    reference's compiled here except where glibc's exp misrounds (about 0.08 %
+    of arguments; test_z0_exp_modes pins that the modes differ only in exp):
+    measured 3 of 16 000 evaluations over the four lattices, each off by 8.7e-4."""
+    q = qpd2(case)
+    k, th, ph, d1, d2, rm = CASES[case]
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(-rm, rm, (1000, 3))
+    pts[:, 2] = 0.0
+    for mode in (2, 3):
+        Gr, dGr, _, tr = ref_eval(mode, q, pts)
+        G0, dG0, _, t0 = oracle_eval(mode, q, pts, cr_exp=False)
+        G1, dG1, _, t1 = oracle_eval(mode, q, pts, cr_exp=True)
+        assert (t0 == tr).all()
+        assert rel(G0, Gr).max() < 1e-13
+        same = t1 == tr
+        assert same.mean() >= 0.995, (case, mode, int((~same).sum()))
+        assert rel(G1[same], Gr[same]).max() < 1e-13
