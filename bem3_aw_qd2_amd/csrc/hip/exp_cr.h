@@ -62,8 +62,8 @@ EXPCR_HD inline dd_t dd_add(dd_t a, dd_t b) {
   return dd_fast_two_sum(s.h, s.l + t.l);
 }
 
-EXPCR_HD inline double exp_cr(double x) {
-  if (!(x >= -700.0 && x <= 709.0)) return exp(x);
+// exp(x) as a double-double, -700 <= x <= 709
+EXPCR_HD inline dd_t exp_dd(double x) {
   const dd_t T[64] = EXPCR_T_INIT;
   const dd_t C[12] = EXPCR_C_INIT;
   const double kd = rint(x * EXPCR_INV);
@@ -79,6 +79,11 @@ EXPCR_HD inline double exp_cr(double x) {
   for (int n = 10; n >= 0; n--) p = dd_add(dd_mul(p, r), C[n]);
   const int j = k & 63, m = (k - j) / 64;
   const dd_t q = dd_mul(T[j], p);
-  // q.h is q.h + q.l rounded to nearest (fast two-sum normalisation)
-  return ldexp(q.h, m);
+  return dd_t{ldexp(q.h, m), ldexp(q.l, m)};
+}
+
+EXPCR_HD inline double exp_cr(double x) {
+  if (!(x >= -700.0 && x <= 709.0)) return exp(x);
+  // the high part is hi + lo rounded to nearest (fast two-sum normalisation)
+  return exp_dd(x).h;
 }

@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
           const double aR = abs3(vR);
           free_gf(P.k, vR, aR, 1.0 / aR, gf.g, gf.dg);
         } else {
-          int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+          int cnt = qpgf_eval_g_dg(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
           if (cnt < 0) err = cnt;
           evals++;
         }
@@ -718,7 +718,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
           const double aR = abs3(vR);
           free_gf(P.k, vR, aR, 1.0 / aR, gf.g, gf.dg);
         } else {
-          int cnt = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
+          int cnt = qpgf_eval_g_dg(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
           if (cnt < 0) err = cnt;
           evals++;
         }

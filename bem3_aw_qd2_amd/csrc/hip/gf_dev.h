@@ -255,6 +255,12 @@ __device__ __forceinline__ void spatial_term(const GfParams& P, double rx, doubl
 
 // spectral order (n1, n2): d3_qpgf_d2.c:1965-2009 (v1) and :2349-2398 (v2).
 // returns 0 propagating, 1 evanescent, -3 Wood anomaly.
+// The z-derivative of an order is what the stopping rules read.  At z == 0 it
+// is formed exactly as the reference forms it (the evanescent difference is 0
+// and the propagating one the rounding residue of two correctly rounded
+// exponentials); at 0 < |z| < BQD2_ZFAST E, where it is cancellation noise, it
+// is taken in exact arithmetic to first order in z, so the sum is continuous in
+// z down to z -> 0.
 // fast: ce is the planar phase exp(i a_n . r) carried along the row by the
 // caller, and the evanescent pair e^{+-b z} erfc(b E +- z/2E) is formed as
 // exp(-(b^2 E^2 + z^2/4E^2)) erfcx(b E +- z/2E) (the same value, one exp and two
@@ -284,6 +290,13 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
     t.g = ce * F * ib;
     if (M != GF_QG) {
       cd Gz = cd{2.0 * (__dmul_rn(ef, w.x) - ff.x), -2.0 * ff.y};
+      if (!fast && rz != 0.0) {
+        // level to within rounding, not exactly: ef Re w - cos(bn z) is pure
+        // cancellation in floating point (the reference's stops there follow
+        // its libm); taken in exact arithmetic to first order in z:
+        // -y e^{x^2} (2/sqrt(pi) - 2 x Im w(x)), x = bn E, y = z/2E
+        Gz.x = -2.0 * azve * ef * (2.0 * isp - 2.0 * (bn * E) * w.y);
+      }
       t.dg[0] = t.g * an0;
       t.dg[1] = t.g * an1;
       t.dg[2] = ce * Gz;
@@ -318,6 +331,13 @@ __device__ __forceinline__ int spectral_term(const GfParams& P, double rx, doubl
       t.dg[0] = t.g * an0;
       t.dg[1] = t.g * an1;
       t.dg[2] = ce * (f1 - f2);
+      if (!fast && rz != 0.0) {
+        // level to within rounding, not exactly: f1 - f2 is pure cancellation;
+        // in exact arithmetic to first order in z, with g(d) = e^{2xd} erfc(x + d),
+        // x = bn E, d = z/2E: f1 - f2 = 2 d g'(0) = 2 d (2 x erfc(x) - (2/sqrt(pi)) e^{-x^2})
+        const double x = bn * E;
+        t.dg[2] = ce * (2.0 * azve * (2.0 * x * erfc(x) - 2.0 * isp * exp(-x * x)));
+      }
       if (M == GF_V2) {
         double ef = fast ? efs : exp(bn2 * E * E - azve * azve);
         t.d2g[0] = t.dg[0] * an0;
@@ -692,6 +712,27 @@ __device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, dou
   }
   if (err < 0) err -= use_fd ? 10 : 20;
   return err;
+}
+
+// Value and gradient as the reference's singular (polar) integrands take them:
+// the value from the value-only sum and the gradient from the gradient-only sum
+// (d3hm_qpgf_d2_qG / _dqG, d3_qpgf_d2.c:10-122; called by q0_bil_sqG_r /
+// q0_bil_sqH_r, q0_bil_coef.c:428-497, and q0_lit_sqG_r / _sqH_r).  Their
+// stopping rules read different partial sums, which matters only where the
+// offset is level with the source (z == 0: the Ewald spectral stop reads a
+// rounding residue, ewald_cell's non-fast range); elsewhere one value-and-
+// gradient sum gives both to rounding and is taken instead.
+__device__ __forceinline__ int qpgf_eval_g_dg(const GfParams& P, double rx, double ry, double rz, double eps,
+                                              Sum10& out, GfTally& tl) {
+  if (fabs(rz) >= BQD2_ZFAST * P.E0) return qpgf_eval<GF_V1>(P, rx, ry, rz, eps, out, tl);
+  const int c0 = qpgf_eval<GF_QG>(P, rx, ry, rz, eps, out, tl);
+  if (c0 < 0) return c0;
+  Sum10 d;
+  const int c1 = qpgf_eval<GF_DQG>(P, rx, ry, rz, eps, d, tl);
+  if (c1 < 0) return c1;
+#pragma unroll
+  for (int i = 0; i < 3; i++) out.dg[i] = d.dg[i];
+  return c0 + c1;
 }
 
 template <int M>

@@ -220,6 +220,27 @@ extern "C" int bqd2_download_domain(bqd2_problem *p, int did, double *G, double 
   return BQD2_OK;
 }
 
+extern "C" int bqd2_download_rows(bqd2_problem *p, int did, int nrows, const int *rows, double *G, double *H,
+                                  double *dG, double *dH, double *dF) {
+  BQD2_CHECK(p && did >= 0 && did <= p->m.MN && nrows >= 0 && (nrows == 0 || rows), BQD2_E_ARG,
+             "bqd2_download_rows: bad arguments");
+  DevDomain &D = p->dom[did];
+  BQD2_CHECK(D.assembled, BQD2_E_STATE, "bqd2_download_rows: domain %d not assembled", did);
+  const size_t N = D.N;
+  for (int i = 0; i < nrows; i++)
+    BQD2_CHECK(rows[i] >= 0 && (size_t)rows[i] < N, BQD2_E_ARG, "bqd2_download_rows: row %d out of range", rows[i]);
+  BQD2_HIP(hipStreamSynchronize(p->stream));
+  for (int i = 0; i < nrows; i++) {
+    const size_t r = (size_t)rows[i];
+    if (G) BQD2_HIP(hipMemcpy(G + 2 * N * i, D.G + 2 * N * r, N * 16, hipMemcpyDeviceToHost));
+    if (H) BQD2_HIP(hipMemcpy(H + 2 * N * i, D.H + 2 * N * r, N * 16, hipMemcpyDeviceToHost));
+    if (dG && D.dG) BQD2_HIP(hipMemcpy(dG + 4 * N * i, D.dG + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost));
+    if (dH && D.dH) BQD2_HIP(hipMemcpy(dH + 4 * N * i, D.dH + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost));
+    if (dF) BQD2_HIP(hipMemcpy(dF + 3 * i, D.dF + 3 * r, 3 * 8, hipMemcpyDeviceToHost));
+  }
+  return BQD2_OK;
+}
+
 extern "C" int bqd2_download_solution(bqd2_problem *p, int did, double *P, double *dP, double *pv, double *dpv) {
   BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_download_solution: bad domain");
   DevDomain &D = p->dom[did];

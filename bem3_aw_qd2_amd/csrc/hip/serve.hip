@@ -16,18 +16,25 @@
 // HIP stream, so the kernels of concurrent requests overlap on the device.
 //
 // Resident operators: a request with a nonzero operator key promises that
-// requests with the same key share the mesh, media, frequency, lattice and
-// in-plane wave vector (modulo the reciprocal lattice), so only the incident
-// wave differs; the assembled matrices (and, for the mixed-precision solver,
-// the complex float factors) of the first are kept and the next ones only form
-// and solve their right-hand side.  The promise is checked (element count,
-// frequency, lattice and in-plane wave vector); a mismatch fails the request.
+// requests with the same key share the mesh, media, frequency and lattice, so
+// that only the incident wave differs; the assembled matrices (and, for the
+// mixed-precision solver, the complex float factors) of the first are kept and
+// the next ones only form and solve their right-hand side.  A resident operator
+// serves a request only when the request's in-plane wave vector (kx, ky), which
+// the lattice sums take as their Bloch phase, has the operator's exact bits:
+// where a quadrature point is level with a source (z == 0) the lattice sums
+// stop on a rounding residue and so follow the last bits of (kx, ky), and a
+// wave vector equal only up to rounding (sin(pi - theta) against sin(theta))
+// would move the result.  Such a request is a miss and gets an operator of its
+// own.  The rest of the promise is checked (element count, frequency,
+// lattice); a mismatch fails the request.
 //
 // Every kernel sums in a fixed order, so a request's results do not depend on
 // what runs beside it, nor on whether its operator was resident.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -71,10 +78,22 @@ struct ModelCopy {
   }
 };
 
+// a resident operator: the caller's key and the bits of (kx, ky)
+struct OpKey {
+  unsigned long long key;
+  uint64_t kx, ky;
+  bool operator<(const OpKey &o) const {
+    if (key != o.key) return key < o.key;
+    if (kx != o.kx) return kx < o.kx;
+    return ky < o.ky;
+  }
+};
+
 struct Req {
   long long id;
   ModelCopy model;
-  unsigned long long key;
+  unsigned long long key;  // 0: no reuse
+  OpKey okey;
   int solver, npts, force_type;
   double rc[3];
   std::vector<double> pts;
@@ -105,8 +124,8 @@ struct bqd2_server {
   std::condition_variable cv_work, cv_done;
   std::deque<Req *> pending;
   std::map<long long, Req *> reqs;
-  std::map<unsigned long long, Op> cache;
-  std::map<unsigned long long, int> building;  // keys whose first request is assembling the operator
+  std::map<OpKey, Op> cache;
+  std::map<OpKey, int> building;  // operators whose first request is assembling them
   std::vector<std::thread> workers;
   bool stop = false;
   long long next_id = 1;
@@ -182,8 +201,8 @@ bool bqd2_server::pick(Req **r, Op **op) {
   bool miss_seen = false;
   for (auto it = pending.begin(); it != pending.end();) {
     Req *q = *it;
-    if (q->key && building.count(q->key)) { ++it; continue; }  // wait for the operator being assembled
-    auto c = q->key ? cache.find(q->key) : cache.end();
+    if (q->key && building.count(q->okey)) { ++it; continue; }  // wait for the operator being assembled
+    auto c = q->key ? cache.find(q->okey) : cache.end();
     if (c != cache.end()) {
       if (c->second.busy) { ++it; continue; }
       c->second.busy = true;
@@ -197,7 +216,7 @@ bool bqd2_server::pick(Req **r, Op **op) {
     if (used + q->need > budget) evict_idle(q->need);
     if (used + q->need <= budget) {
       used += q->need;
-      if (q->key) building[q->key] = 1;
+      if (q->key) building[q->okey] = 1;
       *r = q;
       *op = nullptr;
       pending.erase(it);
@@ -228,9 +247,11 @@ static int run_request(Req *q, bqd2_problem *p, bool fresh) {
     for (int d = 0; d <= m.MN; d++)
       if ((rc = bqd2_assemble(p, d, nullptr))) return rc;
   } else {
-    // the key's promise: same mesh, frequency and lattice; the incidence is set anew
+    // the key's promise: same mesh, frequency and lattice; (kx, ky) has the
+    // operator's bits by construction of the cache key; the incidence is set anew
     BQD2_CHECK(p->m.Ne == m.Ne && p->m.MN == m.MN && p->pw_k0 == m.pw_k0 &&
-                   !memcmp(p->qd8 + 4, m.qd + 4, 4 * sizeof(double)),
+                   !memcmp(p->qd8 + 4, m.qd + 4, 4 * sizeof(double)) &&
+                   !memcmp(p->qd8 + 1, m.qd + 1, 2 * sizeof(double)),
                BQD2_E_ARG, "bqd2_server: request %lld does not match the resident operator of its key", q->id);
     if ((rc = bqd2_set_incidence_R(p, m.pw_p, m.pw_tv, m.pw_R))) return rc;
   }
@@ -288,14 +309,14 @@ void bqd2_server::work() {
       if (actual > op->bytes) { used += actual - op->bytes; op->bytes = actual; }
     } else {
       n_misses++;
-      if (q->key) building.erase(q->key);
-      if (rc || q->key == 0 || cache.count(q->key)) {
+      if (q->key) building.erase(q->okey);
+      if (rc || q->key == 0 || cache.count(q->okey)) {
         if (p) bqd2_destroy(p);
         used -= q->need;
       } else {
         const size_t bytes = std::max(q->need, actual);
         used += bytes - q->need;
-        cache[q->key] = Op{p, bytes, false, t1};
+        cache[q->okey] = Op{p, bytes, false, t1};
       }
     }
     if (rc) n_failed++;
@@ -324,6 +345,9 @@ extern "C" int bqd2_server_submit(bqd2_server *s, const bqd2_request *rq, double
   Req *q = new Req();
   q->model.copy(*rq->model);
   q->key = rq->op_key;
+  q->okey.key = rq->op_key;
+  memcpy(&q->okey.kx, &rq->model->qd[1], sizeof(uint64_t));
+  memcpy(&q->okey.ky, &rq->model->qd[2], sizeof(uint64_t));
   q->solver = rq->solver;
   q->npts = rq->npts;
   q->force_type = rq->force_type;

@@ -13,6 +13,9 @@ void aqd2_rw(const BOUD *bd, int s, double z, double e, double *r, double *w);
 struct bqd2_problem;
 struct bqd2_problem *aqd2_problem(AQD2 *qd, int *has_solution);
 void aqd2_push_solution(AQD2 *qd, struct bqd2_problem *p);
+/* the device solution of every domain into the AQD2 arrays (exported for tools
+ * that drive the device problem directly and then read the solution) */
+void aqd2_pull_solution(AQD2 *qd, struct bqd2_problem *p);
 void aqd2_mark_solution(AQD2 *qd, int on);
 void aqd2_release_device(AQD2 *qd);
 

@@ -224,7 +224,7 @@ void aqd2_release_device(AQD2 *qd)
 }
 
 /* copy the device solution of every domain into the AQD2 arrays */
-static void pull_solution(AQD2 *qd, bqd2_problem *p)
+void aqd2_pull_solution(AQD2 *qd, bqd2_problem *p)
 {
   for (int d = 0; d <= qd->MN; d++) {
     SUBD *s = &qd->bd.sb[d];
@@ -302,7 +302,7 @@ void solve_bieq_aqd2(AQD2 *qd)
   ts = wall();
   rc = bqd2_solve_velocity(p);
   if (rc) fail("bqd2_solve_velocity", rc);
-  pull_solution(qd, p);
+  aqd2_pull_solution(qd, p);
   aqd2_mark_solution(qd, 1);
   printf("finished. Elapsed time : %5g (sec)\n", wall() - ts);
   printf("Total elapsed time : %g (sec)\n", wall() - t0);

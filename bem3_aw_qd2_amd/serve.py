@@ -64,6 +64,12 @@ def _quiet():
         os.close(devnull)
 
 
+class _ModelHead(C.Structure):
+    """the leading fields of bqd2_model (include/bqd2.h) up to its QPD2"""
+    _fields_ = [("Ne", C.c_int), ("Nn", C.c_int), ("NN", C.c_int), ("MN", C.c_int), ("ps", C.c_int),
+                ("arrays", C.c_void_p * 16), ("qd", C.c_double * 8)]
+
+
 class PackedModel:
     """The bqd2_model of a workload written by mesh.write_config (keyword
     arguments replace workload entries: n, radius, d1, d2, f, theta, psi, host,
@@ -81,6 +87,12 @@ class PackedModel:
                 self.ne, self.nn, self.mn = m.ne(), m.nn(), m.mn()
                 self.ptr = host.aqd2_model_new(m.buf)
                 m.finalize()
+
+    def kxky(self):
+        """the in-plane wave vector (kx, ky) of the host medium's QPD2, as the
+        lattice sums take it (bqd2_model.qd[1:3])"""
+        head = _ModelHead.from_address(self.ptr)
+        return head.qd[1], head.qd[2]
 
     def footprint(self, solver, npts=0, force_type=-1):
         return int(hip_lib().bqd2_footprint(self.ptr, solver, npts, force_type))

@@ -184,7 +184,9 @@ def cpu_baseline(model, req, npairs, nself, seed=0):
     orc = C.CDLL(orc_path)
     orc.orc_pair_sample.restype = C.c_long
     orc.orc_pair_sample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    orc.orc_threads.restype = C.c_int
+    want = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = int(orc.orc_threads(want))
     rng = np.random.default_rng(seed)
     total = 0.0
     parts = {}
@@ -244,7 +246,75 @@ def cpu_baseline(model, req, npairs, nself, seed=0):
               "domain, Green's function rates for the field and force evaluation counts, "
               "numpy LAPACK solve at n=%d scaled by n^3 to n=%d; extrapolated to one request: %s s"
               % (npairs, nself, ns, na, ", ".join("%s %.1f" % (k, v) for k, v in parts.items())))
-    return {"value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port", "sample": sample}
+    return {"value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port", "sample": sample,
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def force_vs_oracle(host, model, prob, req, npts, seed=0):
+    """The benched request's force against the oracle on the same surface
+    solution (copied from the device into the model's arrays):
+    type 0 (node values and the incident wave) in full; type 1 through the
+    boundary field it integrates (the total field at the elements' 9 points),
+    recomputed by the oracle at a sample of those points (each point sums every
+    element's 9-point rule and the own element's polar rule; correctly rounded
+    z == 0 exponentials, as on the GPU).  Relative errors: F normwise, field
+    points max over p and |v|."""
+    from concurrent.futures import ThreadPoolExecutor
+    orc_path = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(orc_path):
+        return None
+    orc = C.CDLL(orc_path)
+    host.aqd2_pull_solution(model.buf, prob)
+    F, N = np.zeros(3), np.zeros(3)
+    rc = np.zeros(3)
+    t0 = time.perf_counter()
+    err = orc.orc_force(model.buf, rc.ctypes.data_as(C.c_void_p), 0, F.ctypes.data_as(C.c_void_p),
+                        N.ctypes.data_as(C.c_void_p))
+    if err:
+        return None
+    e0 = float(np.linalg.norm(req.F[0] - F) / np.linalg.norm(F))
+    eN0 = float(np.linalg.norm(req.N[0] - N) / max(np.linalg.norm(N), np.linalg.norm(F) * 0.5e-3))
+    rng = np.random.default_rng(seed)
+    ne = model.ne()
+    b = model.lib
+    # the 9-point rule's (zeta, eta): BOUD.zt_49 and .et_49 at offsets 280 and
+    # 352 of BOUD, itself at 736 in AQD2 (include/bem3_aw_qd2.h)
+    zt = np.array(C.cast(C.addressof(model.buf) + 736 + 280, C.POINTER(C.c_double * 9)).contents[:])
+    et = np.array(C.cast(C.addressof(model.buf) + 736 + 352, C.POINTER(C.c_double * 9)).contents[:])
+    picks = [(int(rng.integers(1, ne + 1)), int(rng.integers(0, 9))) for _ in range(npts)]
+
+    def one(tq):
+        t, q = tq
+        p = (C.c_double * 2)()
+        v = (C.c_double * 6)()
+        b.pv_t_bd_aqd2(p, v, 0, t, C.c_double(zt[q]), C.c_double(et[q]), 1, model.buf)
+        po, vo = (C.c_double * 2)(), (C.c_double * 6)()
+        orc.orc_set_cr_exp(1)
+        r = orc.orc_field_bd_total(model.buf, t, C.c_double(zt[q]), C.c_double(et[q]), 1, po, vo)
+        if r:
+            return None
+        g = np.array(p[:] + v[:]).view(complex)
+        o = np.array(po[:] + vo[:]).view(complex)
+        return max(abs(g[0] - o[0]) / abs(o[0]), np.linalg.norm(g[1:] - o[1:]) / np.linalg.norm(o[1:]))
+
+    with ThreadPoolExecutor(max_workers=min(npts, os.cpu_count() or 1, 16)) as ex:
+        errs = list(ex.map(one, picks))
+    orc.orc_set_cr_exp(0)
+    if any(e is None for e in errs):
+        return None
+    return {"type0_F_vs_oracle": e0, "type0_N_vs_oracle": eN0,
+            "type1_boundary_field_vs_oracle": float(max(errs)), "type1_points_checked": npts,
+            "seconds": time.perf_counter() - t0}
 
 
 def lu32_leg(hip, prob, na):
@@ -513,6 +583,9 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=384)
     ap.add_argument("--cpu-self", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-force-check", action="store_true", help="skip the oracle check of the force")
+    ap.add_argument("--check-points", type=int, default=16,
+                    help="boundary points of the type-1 force whose field the oracle recomputes")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream measurement")
     ap.add_argument("--no-lu32", action="store_true", help="skip the mixed-precision solve of the request's system")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
@@ -600,12 +673,17 @@ def main():
     F_last = req.F.copy()
     if F_ref is None:
         F_ref = F_last
-    force_err = float(max(np.linalg.norm(F_last[k] - F_ref[k]) / np.linalg.norm(F_ref[k]) for k in (0, 1)))
+    force_rep = float(max(np.linalg.norm(F_last[k] - F_ref[k]) / np.linalg.norm(F_ref[k]) for k in (0, 1)))
+    force_check = None
+    if not args.no_force_check:
+        log("force check: the oracle on the request's surface solution")
+        force_check = force_vs_oracle(host, model, prob, req, args.check_points)
+        force_check["repeatability"] = force_rep
 
     achieved = float(np.sum(flops) / (np.sum(ms_pairs) * 1e-3) / 1e12)
     st0 = req.stats[0]
     traffic, traffic_src = measured_traffic(args.workload)
-    roof = {"bound": "mfma", "pipe": "FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+    roof = {"bound": "valu", "pipe": "FP64 vector ALU (MI355X FP64 vector peak 78.6 TFLOP/s)",
             "kernel": "k_q0_pairs", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
             "ms_per_launch": float(np.mean(ms_pairs)),
@@ -646,7 +724,7 @@ def main():
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "gf_evals_per_s": evals / elapsed,
-        "force_rel_err": force_err,
+        "force_check": force_check,
         "phases_ms": ph,
         "solver": req.solver,
         "force": {"type0": req.F[0].tolist(), "type1": req.F[1].tolist()},

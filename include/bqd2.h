@@ -140,6 +140,10 @@ int bqd2_counters(bqd2_problem *p, long long *out, int n);
 
 /* copy domain did's matrices to host buffers (any may be NULL) */
 int bqd2_download_domain(bqd2_problem *p, int did, double *G, double *H, double *dG, double *dH, double *dF);
+/* the same for nrows rows (indices (t-1)*4 + tn): G, H nrows x N complex, dG,
+ * dH nrows x 2N (the row's two tangential derivatives), dF nrows x 3 */
+int bqd2_download_rows(bqd2_problem *p, int did, int nrows, const int *rows, double *G, double *H, double *dG,
+                       double *dH, double *dF);
 
 /* Build the dense 2NN system (create_matrix_csr_dac, :244-345) and solve it; the
  * reference uses MKL PARDISO (d3qd2_mkl_solver.c:18-148).  Default: GMRES with
@@ -190,6 +194,7 @@ int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double
  * reference returns.  gf_evals (may be NULL): Green's function evaluations. */
 int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, double *out, int *did,
                long long *gf_evals);
+
 
 /* Domain of each point (domain_id / domain_id_aqd2, bem3_aw_qd2.c:111-156, :1148-1194):
  * solid-angle classification of the point reduced to the unit cell. */

@@ -18,6 +18,7 @@
  *   dF     [N x 3]    (F or 1+F, dF/dtz, dF/dte)
  */
 #include <math.h>
+#include <omp.h>
 #include <complex.h>
 #include <stdlib.h>
 #include <string.h>
@@ -111,12 +112,20 @@ long orc_assemble_row(const AQD2 *qd, int did, int t, int tn, double complex *G,
   double vtz[3], vte[3];
   orc_node_tangents(bd, td, tn, vtz, vte);
   double F = 0.0, Fz = 0.0, Fe = 0.0;
-  long terms = 0;
+  long terms = 0, err = 0;
+  /* the elements in parallel (inside an enclosing parallel region this runs
+   * on one thread); F and its derivatives summed afterwards in element order */
+  double *fs = malloc(sizeof(double) * 3 * (Ne + 1));
+#pragma omp parallel for schedule(dynamic) reduction(+ : terms)
   for (int s = 1; s <= Ne; s++) {
     int sd = bd->sb[did].sid[s];
     double complex CC[9], dCz[9], dCe[9];
     long c = orc_coef_block(qd, did, td, tn, sd, vtz, vte, CC, dCz, dCe);
-    if (c < 0) return c;
+    if (c < 0) {
+#pragma omp critical
+      err = c;
+      continue;
+    }
     terms += c;
     for (int i = 0; i < 4; i++) {
       G[(s - 1) * 4 + i] = CC[i];
@@ -130,10 +139,17 @@ long orc_assemble_row(const AQD2 *qd, int did, int t, int tn, double complex *G,
         dH[N + (s - 1) * 4 + i] = dCe[i + 4];
       }
     }
-    F += creal(CC[8]);
-    Fz += creal(dCz[8]);
-    Fe += creal(dCe[8]);
+    fs[3 * s] = creal(CC[8]);
+    fs[3 * s + 1] = creal(dCz[8]);
+    fs[3 * s + 2] = creal(dCe[8]);
   }
+  for (int s = 1; s <= Ne && !err; s++) {
+    F += fs[3 * s];
+    Fz += fs[3 * s + 1];
+    Fe += fs[3 * s + 2];
+  }
+  free(fs);
+  if (err) return err;
   double diag = (did == 0) ? 1.0 + F : F;
   H[(t - 1) * 4 + tn] += diag;
   dF[0] = diag;
@@ -410,4 +426,12 @@ int orc_row_fparts(const AQD2 *qd, int did, int t, int tn, double *out)
     out[3 * (s - 1) + 2] = creal(dCe[8]);
   }
   return 0;
+}
+
+/* OpenMP team of the oracle's parallel loops: set it (n > 0) and return the
+ * team size the next parallel region will use */
+int orc_threads(int n)
+{
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
 }

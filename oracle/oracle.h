@@ -22,10 +22,13 @@ double complex orc_faddeeva_w(double complex z);
 double complex orc_scaled_w(double x, double y);
 void orc_faddeeva_w_batch(int n, const double *z, double *w);
 
-/* exp of the two factors whose rounding residue the z == 0 stopping rule of the
- * Ewald spectral sum reads: libm's (mode 0, the default) or correctly rounded
- * (mode 1) -- qpgf_cpu.c. */
+/* The Ewald spectral sum's stopping rules where the offset is level with the
+ * source: mode 0 (default) evaluates them as the reference does with this
+ * libm; mode 1 with correctly rounded exponentials at z == 0 and exact
+ * first-order differences at 0 < |z| < 2e-8 E (qpgf_cpu.c).  orc_exp_z0: the
+ * exponential of the z == 0 residue in the current mode. */
 void orc_set_cr_exp(int on);
+int orc_cr_mode(void);
 double orc_exp_z0(double x);
 
 /* Quasi-periodic Green's function (qpgf_cpu.c).  mode: 0 value only

@@ -47,6 +47,7 @@ long orc_assemble_row(const AQD2 *qd, int did, int t, int tn, double complex *G,
 long orc_assemble_domain(const AQD2 *qd, int did, double complex *G, double complex *H, double complex *dG,
                          double complex *dH, double *dF);
 long orc_pair_sample(const AQD2 *qd, int did, int n, const int *row, const int *src);
+int orc_threads(int n);
 void orc_system_row(const AQD2 *qd, int did, const double complex *Grow, const double complex *Hrow,
                     double complex *A, double complex *b);
 void orc_solution_vector(const AQD2 *qd, double complex *x);

@@ -25,16 +25,30 @@
 #include <quadmath.h>
 #include "oracle.h"
 
-/* The exponentials whose product's rounding residue the z == 0 stopping rule
- * reads (the propagating Ewald spectral order, d3_qpgf_d2.c:1976-1989, and
- * Faddeeva's real axis, Faddeeva.cc:684-686).  Mode 0: this libm's exp, as the
- * reference compiled here evaluates them.  Mode 1: correctly rounded, from
- * libquadmath's 113-bit expq rounded once to double, the platform-independent
- * statement of the same rule (glibc 2.35's exp misrounds about 0.08 % of
- * arguments; the GPU evaluates these two exponentials correctly rounded). */
+/* Where the offset lies level with the source the stopping rules of the Ewald
+ * spectral sum read cancellation: the z-derivative of a propagating order,
+ * exp(bn^2 E^2) Re w(bn E + i z/2E) - cos(bn z) (d3_qpgf_d2.c:1976-1989,
+ * Faddeeva.cc:684-686 on the real axis), and of an evanescent one,
+ * e^{bz} erfc(bE + z/2E) - e^{-bz} erfc(bE - z/2E) (:1991-2004).
+ *   z == 0 exactly: the evanescent difference is exactly 0 on every platform
+ *     and the propagating one is the rounding residue of the two exponentials.
+ *   0 < |z| < 2e-8 E: both are floating-point noise, so where the reference
+ *     stops follows its libm's rounding (glibc's values there are up to 1e-2
+ *     off the limit z -> 0 of its own values).
+ * Mode 0: this libm's exp and erfc throughout, as the reference compiled here.
+ * Mode 1, the platform-independent statement that the GPU evaluates: at z == 0
+ * the two exponentials correctly rounded (libquadmath's 113-bit expq rounded
+ * once to double; glibc 2.35's exp misrounds about 0.08 % of arguments); at
+ * 0 < |z| < 2e-8 E the two differences in exact arithmetic, to first order in
+ * z (relative error below 1e-15):
+ *   propagating  -y e^{x^2} (2/sqrt(pi) - 2 x Im w(x)),
+ *   evanescent   2 d (2 x erfc(x) - (2/sqrt(pi)) e^{-x^2}),
+ * x = bn E, y = d = z/2E.  Mode 1 is continuous in z down to z -> 0. */
 static int cr_exp_mode = 0;
 void orc_set_cr_exp(int on) { cr_exp_mode = on; }
+int orc_cr_mode(void) { return cr_exp_mode; }
 double orc_exp_z0(double x) { return cr_exp_mode ? (double)expq((__float128)x) : exp(x); }
+#define ZLEVEL 2.0e-8
 
 enum { M_QG = 0, M_DQG = 1, M_V1 = 2, M_V2 = 3 };
 
@@ -164,10 +178,18 @@ static int spectral_term(int mode, const QPD2 *qd, const double *r, double E,
     /* exp(bn^2 E^2 - azve^2) and w(bn E + i azve) kept apart as in :1976-1985:
      * at z == 0 the difference ef*Re(w) - ff is pure rounding and the row
      * stopping rule depends on it */
-    double ef = orc_exp_z0(bn2 * E * E - azve * azve);
+    const int level = fabs(r[2]) < ZLEVEL * E;
+    double ef = r[2] == 0.0 ? orc_exp_z0(bn2 * E * E - azve * azve) : exp(bn2 * E * E - azve * azve);
     double complex w = orc_faddeeva_w(bn * E + I * azve);
     double complex F = 2.0 * (I * ef * cimag(w) + ff);
     double complex Gz = 2.0 * (ef * creal(w) - ff);
+    if (cr_exp_mode && level && r[2] != 0.0) {
+      /* level but not exactly (0 < |z| < 2e-8 E): ef Re w - cos(bn z) in
+       * exact arithmetic, -y e^{x^2} (2/sqrt(pi) - 2 x Im w(x)) + O(y^3) with
+       * x = bn E, y = z/2E, instead of the rounding residue it is in floating
+       * point (whose stopping decisions follow the libm) */
+      Gz = 2.0 * (-azve * ef * (1.1283791670955126 - 2.0 * (bn * E) * cimag(w))) - 2.0 * I * cimag(ff);
+    }
     t->g = ce / bn * F;
     t->dg[0] = t->g * an0;
     t->dg[1] = t->g * an1;
@@ -183,6 +205,7 @@ static int spectral_term(int mode, const QPD2 *qd, const double *r, double E,
     return 0;
   } else if (bn2 < 0.0) {
     double bn = sqrt(-bn2);
+    const int level = fabs(r[2]) < ZLEVEL * E;
     double ebz = exp(bn * r[2]);
     double f1 = ebz * erfc(bn * E + azve);
     double f2 = 1.0 / ebz * erfc(bn * E - azve);
@@ -190,6 +213,13 @@ static int spectral_term(int mode, const QPD2 *qd, const double *r, double E,
     t->dg[0] = t->g * an0;
     t->dg[1] = t->g * an1;
     t->dg[2] = ce * (f1 - f2);
+    if (cr_exp_mode && level && r[2] != 0.0) {
+      /* level but not exactly: f1 - f2 = g(d) - g(-d), g(d) = e^{2xd} erfc(x + d),
+       * x = bn E, d = z/2E, in exact arithmetic 2 d g'(0) + O(d^3) with
+       * g'(0) = 2 x erfc(x) - (2/sqrt(pi)) e^{-x^2} */
+      const double x = bn * E;
+      t->dg[2] = ce * (2.0 * azve * (2.0 * x * erfc(x) - 1.1283791670955126 * exp(-x * x)));
+    }
     if (mode == M_V2) {
       double ef = exp(bn2 * E * E - azve * azve);
       t->d2g[0] = t->dg[0] * an0;

@@ -124,3 +124,40 @@ def test_gf_z0(case, mode):
         assert relerr(Gg[same], Gr[same]).max() < 1e-12
     else:
         assert relerr(dGg[same], dGr[same]).max() < 1e-12
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_gf_tiny_z(case, mode):
+    """Offsets level with the source to within rounding (|z| from 1e-20 to 1e-8
+    of the cell size), as the quadrature points of mirrored mesh elements give.
+    There the z-derivatives that the stopping rules read are cancellation noise
+    in the reference's floating point (its values follow its libm: the
+    reference built here strays up to 1.4e-2 from the limit z -> 0 of its own
+    values); the GPU takes them in exact arithmetic to first order in z, as the
+    oracle's correctly rounded mode does.  Bars: values within 1e-12 of that
+    oracle, and within 1e-12 of the GPU's own values at the same (x, y) with
+    z = 1e-9 E (continuity)."""
+    from bem3_aw_qd2_amd import gf_params
+    q = qpd2(case)
+    k, th, ph, d1, d2, rm = CASES[case]
+    E = gf_params(q).E0
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-rm, rm, (600, 3))
+    pts[:, 2] = rng.choice([-1.0, 1.0], 600) * 10.0 ** rng.uniform(-20, -8, 600) * rm
+    Gg, dGg, d2Gg, tg = gpu_eval(mode, q, pts)
+    Go, dGo, d2Go, to = oracle_eval(mode, q, pts, cr_exp=True)
+    assert (tg > 0).all()
+    errs = [relerr(Gg, Go)] if mode != 1 else []
+    if mode >= 1:
+        errs.append(relerr(dGg, dGo))
+    if mode == 3:
+        errs.append(relerr(d2Gg, d2Go))
+    e = np.max(errs, axis=0)
+    far = pts.copy()
+    far[:, 2] = np.sign(pts[:, 2]) * 1e-9 * E
+    Gf, dGf, _, _ = gpu_eval(mode, q, far)
+    ec = relerr(Gg, Gf) if mode != 1 else relerr(dGg[:, :2], dGf[:, :2])
+    print(case, mode, "tiny z: max err %.2e vs oracle, %.2e vs z = 1e-9 E; term counts differing at %d of %d"
+          % (e.max(), ec.max(), (tg != to).sum(), len(pts)))
+    assert e.max() < 1e-12 and ec.max() < 1e-12

@@ -125,3 +125,50 @@ def test_z0_correctly_rounded_mode(case):
         same = t1 == tr
         assert same.mean() >= 0.995, (case, mode, int((~same).sum()))
         assert rel(G1[same], Gr[same]).max() < 1e-13
+
+
+def test_level_residues_first_order():
+    """The exact first-order forms that the correctly rounded mode takes at
+    0 < |z| < 2e-8 E, against 50-digit arithmetic: for g(d) = e^{2xd} erfc(x + d),
+    g(d) - g(-d) = 2 d (2 x erfc(x) - (2/sqrt(pi)) e^{-x^2}) + O(d^3), and
+    e^{x^2 - y^2} Re w(x + i y) - cos(2 x y) = -y e^{x^2} (2/sqrt(pi) - 2 x Im w(x)) + O(y^3)."""
+    mpmath.mp.dps = 50
+    for x in (0.2, 1.3, 3.1, 5.5):
+        x = mpmath.mpf(x)
+        for d in (mpmath.mpf("1e-9"), mpmath.mpf("-3e-12")):
+            exact = mpmath.exp(2 * x * d) * mpmath.erfc(x + d) - mpmath.exp(-2 * x * d) * mpmath.erfc(x - d)
+            first = 2 * d * (2 * x * mpmath.erfc(x) - 2 / mpmath.sqrt(mpmath.pi) * mpmath.exp(-x * x))
+            assert abs(exact - first) <= 1e-15 * abs(exact)
+            z = mpmath.mpc(x, d)
+            w = mpmath.exp(-z ** 2) * mpmath.erfc(-1j * z)
+            wx = mpmath.exp(-x ** 2) * mpmath.erfc(-1j * x)
+            exact = mpmath.exp(x * x - d * d) * mpmath.re(w) - mpmath.cos(2 * x * d)
+            first = -d * mpmath.exp(x * x) * (2 / mpmath.sqrt(mpmath.pi) - 2 * x * mpmath.im(wx))
+            assert abs(exact - first) <= 1e-15 * abs(exact)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_level_continuity(case):
+    """Mode 1 is continuous in z down to z -> 0: at |z| from 1e-20 E to 1e-10 E
+    its values equal those at z = 1e-9 E (where every mode and the reference
+    agree to 1e-14) within 1e-13.  The reference compiled here is not (printed:
+    up to 1.4e-2 on the 50 mm lattice), its stops following glibc's rounding."""
+    from bem3_aw_qd2_amd import gf_params
+    q = qpd2(case)
+    k, th, ph, d1, d2, rm = CASES[case]
+    E = gf_params(q).E0
+    rng = np.random.default_rng(3)
+    xy = rng.uniform(-rm, rm, (150, 2))
+    base = np.concatenate([xy, np.full((150, 1), 1e-9 * E)], axis=1)
+    Gb, dGb, _, _ = oracle_eval(2, q, base, cr_exp=True)
+    Gr0, _, _, _ = ref_eval(2, q, base)
+    assert rel(Gb, Gr0).max() < 1e-13
+    worst_ref = 0.0
+    for zr in (1e-20, 1e-17, 1e-15, 1e-13, 1e-10):
+        pts = base.copy()
+        pts[:, 2] = zr * E
+        G, dG, _, t = oracle_eval(2, q, pts, cr_exp=True)
+        Gr, _, _, _ = ref_eval(2, q, pts)
+        worst_ref = max(worst_ref, rel(Gr, Gb).max())
+        assert (t > 0).all() and rel(G, Gb).max() < 1e-13, (zr, rel(G, Gb).max())
+    print(case, "reference built here off its z -> 0 limit by up to %.2e" % worst_ref)
