@@ -59,6 +59,7 @@ def hip_lib():
         lib.bqd2_last_error.restype = C.c_char_p
         lib.bqd2_assemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(AssemblyStats)]
         lib.bqd2_download_domain.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
+        lib.bqd2_download_rows.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6
         lib.bqd2_solve_surface.argtypes = [C.c_void_p, C.c_void_p]
         lib.bqd2_solve_velocity.argtypes = [C.c_void_p]
         lib.bqd2_set_solver.argtypes = [C.c_void_p, C.c_int]
@@ -101,6 +102,8 @@ def host_lib():
         lib = _load("libbem3_aw_qd2.so")
         lib.aqd2_problem.restype = C.c_void_p
         lib.aqd2_problem.argtypes = [C.c_void_p, C.c_void_p]
+        lib.aqd2_pull_solution.argtypes = [C.c_void_p, C.c_void_p]
+        lib.aqd2_pull_solution.restype = None
         lib.pv_batch_aqd2.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]
         lib.pv_batch_aqd2.restype = None
