@@ -60,6 +60,8 @@ def hip_lib():
         lib.bqd2_assemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(AssemblyStats)]
         lib.bqd2_download_domain.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
         lib.bqd2_download_rows.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6
+        lib.bqd2_field_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
         lib.bqd2_solve_surface.argtypes = [C.c_void_p, C.c_void_p]
         lib.bqd2_solve_velocity.argtypes = [C.c_void_p]
         lib.bqd2_set_solver.argtypes = [C.c_void_p, C.c_int]

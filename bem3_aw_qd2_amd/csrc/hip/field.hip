@@ -20,6 +20,7 @@
 //                 then an ordered sum
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include "elem_dev.h"
 
 #define CBD_CDF_D 10.0
@@ -1022,6 +1023,80 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   BQD2_HIP(hipStreamSynchronize(st));
   if (gf_evals) *gf_evals = cnt[CNT_FIELD + 4];
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_field: Green's function error at %lld points", cnt[CNT_ERR]);
+  return BQD2_OK;
+}
+
+// ------------------------------------------------- field without allocation
+// bqd2_field_noalloc's device memory: one file-scope array per GPU, held until
+// the process ends.  Host code never passes the array's name to a kernel (on
+// the host it is only a stand-in for the symbol): its device address is looked
+// up once per device with hipGetSymbolAddress and kept in noalloc_base[].
+static __device__ __align__(256) char scratch[BQD2_NOALLOC_SCRATCH_BYTES];
+static constexpr int NOALLOC_MAX_DEV = 64;
+static char *noalloc_base[NOALLOC_MAX_DEV];
+static std::mutex noalloc_mu;
+
+// the current device's copy of the scratch array (set up on the first call on
+// that device, with the device current)
+static int noalloc_scratch(char **base) {
+  int dev = 0;
+  BQD2_HIP(hipGetDevice(&dev));
+  BQD2_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEV, BQD2_E_ARG, "bqd2_field_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> g(noalloc_mu);
+  if (!noalloc_base[dev]) {
+    char *ptr = nullptr;
+    BQD2_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    noalloc_base[dev] = ptr;
+  }
+  *base = noalloc_base[dev];
+  return BQD2_OK;
+}
+
+__global__ void k_zero_counters(long long *c) {
+  if (threadIdx.x < CNT_N) c[threadIdx.x] = 0;
+}
+
+__global__ void k_copy_errors(const long long *c, long long *errors) {
+  if (threadIdx.x == 0) *errors = c[CNT_ERR];
+}
+
+extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, int type, int total, double *d_out,
+                                  int *d_did, long long *d_errors, void *stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  BQD2_CHECK(p && n >= 0 && (n == 0 || (d_pts && d_out && d_did)), BQD2_E_ARG, "bqd2_field_noalloc: bad arguments");
+  BQD2_CHECK(type >= 0 && type <= 3, BQD2_E_ARG, "bqd2_field_noalloc: type must be 0..3 (4-point, 9-point, GLN, GHN)");
+  if (n == 0) return BQD2_OK;
+  const size_t nn = (size_t)n;
+  const int npt = rule_points(type, p->any_tri);
+  int nchunk = 1;
+  for (int d = 0; d <= p->m.MN; d++) {
+    const long long items = (long long)p->dom[d].Ne * npt, chunk = field_chunk_items(items);
+    nchunk = std::max(nchunk, (int)((items + chunk - 1) / chunk));
+  }
+  // scratch layout: counters, r0 (n x 3), lattice cell (n x 2 int), chunk partials
+  const size_t o_r0 = 256, o_l = o_r0 + pad256(nn * 24), o_part = o_l + pad256(nn * 8);
+  const size_t need = o_part + nn * nchunk * FP_N * 8;
+  BQD2_CHECK(need <= (size_t)BQD2_NOALLOC_SCRATCH_BYTES, BQD2_E_ARG,
+             "bqd2_field_noalloc: %d points of type %d need %zu bytes of scratch, more than its %zu bytes "
+             "(at most %zu points per call on this problem)",
+             n, type, need, (size_t)BQD2_NOALLOC_SCRATCH_BYTES,
+             (size_t)((BQD2_NOALLOC_SCRATCH_BYTES - 256 - 512) / (24 + 8 + (size_t)nchunk * FP_N * 8)));
+  char *base = nullptr;
+  int rc;
+  if ((rc = noalloc_scratch(&base))) return rc;
+  long long *ctr = (long long *)base;
+  double *d_r0 = (double *)(base + o_r0), *d_part = (double *)(base + o_part);
+  int *d_l = (int *)(base + o_l);
+  FieldCtx X = make_ctx(p);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, ctr);
+  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, stream, p->m, X, p->rn, p->ed, n, d_pts,
+                     d_did, d_r0, d_l);
+  hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
+                     d_did, d_part, ctr);
+  hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, stream, X, total, n, nchunk, d_part, d_l,
+                     d_pts, d_did, d_out);
+  if (d_errors) hipLaunchKernelGGL(k_copy_errors, dim3(1), dim3(64), 0, stream, ctr, d_errors);
+  BQD2_HIP(hipGetLastError());
   return BQD2_OK;
 }
 

@@ -195,6 +195,24 @@ int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double
 int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, double *out, int *did,
                long long *gf_evals);
 
+/* bqd2_field for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It calls no hipMalloc, hipFree or allocating
+ * library routine and does not synchronise: the kernels of bqd2_field run on
+ * `stream`, with their partial sums in a fixed device array of
+ * BQD2_NOALLOC_SCRATCH_BYTES per GPU (set up at the first call on a device and
+ * held until the process ends).  Inputs and outputs are the caller's device
+ * buffers: d_pts (n x 3), d_out (n x 8 as in bqd2_field), d_did (n), d_errors
+ * (may be NULL: receives the number of points whose Green's function
+ * evaluation failed).  Results are the bits bqd2_field returns.  A call needs
+ * at most 800 + n (32 + 144 c) bytes, c <= 64 the element chunks per point of
+ * the problem and rule (32 MiB: 3600 points at c = 64, 105 000 at c = 2); a
+ * larger n is rejected with BQD2_E_ARG before anything runs.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads, two problems) race and must be ordered by the caller. */
+#define BQD2_NOALLOC_SCRATCH_BYTES (32u << 20)
+int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, int type, int total, double *d_out,
+                       int *d_did, long long *d_errors, void *stream);
 
 /* Domain of each point (domain_id / domain_id_aqd2, bem3_aw_qd2.c:111-156, :1148-1194):
  * solid-angle classification of the point reduced to the unit cell. */
