@@ -134,9 +134,13 @@ FIX = os.path.join(G, "force_oracle.json")
 @pytest.mark.parametrize("name", sorted(SAMPLES))
 def test_type1_force_fixture(name):
     """The oracle's type-1 force of the samples (computed once, committed)
-    against the log: the log's own type-0/type-1 difference is ~1e-3, so a
-    restatement error in the 9-point field shows there; the two exponential
-    modes of the z == 0 stopping rule give nearly the same force."""
+    against the log.  Mode 0 (the reference's algorithm with this machine's
+    libm) reproduces the root sample's log to 1.3e-10 and the verification
+    sample's to 7.4e-8; the log's own type-0/type-1 difference is ~1e-3, so a
+    restatement error in the 9-point field would show far above that.  Mode 1
+    (exact stopping residues at level offsets, the GPU's) differs from mode 0
+    only through the reference's libm sensitivity: 4.6e-10 on the root sample,
+    2.2e-7 on the 50 mm lattice of the verification sample."""
     fix = json.load(open(FIX)) if os.path.exists(FIX) else {}
     if name not in fix:
         pytest.skip("no oracle type-1 force committed for %s" % name)
@@ -145,10 +149,14 @@ def test_type1_force_fixture(name):
     Fr, Nr = np.array(log["force"]["1"]["F"]), np.array(log["force"]["1"]["N"])
     F0 = np.array(log["force"]["0"]["F"])
     rule_gap = np.linalg.norm(Fr - F0) / np.linalg.norm(Fr)
-    for key in ("type1_libm", "type1_cr"):
-        eF, eN = force_errors(e[key]["F"], e[key]["N"], Fr, Nr, log["radius"])
-        print(name, key, "F %.2e N %.2e (log type 0 vs 1: %.2e)" % (eF, eN, rule_gap))
-        assert eF < 1e-7 and eN < 1e-7
+    eF, eN = force_errors(e["type1_libm"]["F"], e["type1_libm"]["N"], Fr, Nr, log["radius"])
+    print(name, "mode 0 vs log: F %.2e N %.2e (log type 0 vs 1: %.2e)" % (eF, eN, rule_gap))
+    assert eF < 1e-7 and eN < 1e-7
+    if name != "verification":
+        assert eF < 1e-8 and eN < 1e-8
     eF, eN = force_errors(e["type1_cr"]["F"], e["type1_cr"]["N"], np.array(e["type1_libm"]["F"]),
                           np.array(e["type1_libm"]["N"]), log["radius"])
-    assert eF < 1e-7 and eN < 1e-7
+    print(name, "mode 1 vs mode 0: F %.2e N %.2e" % (eF, eN))
+    assert eF < 5e-7 and eN < 5e-7
+    if name != "verification":
+        assert eF < 1e-8 and eN < 1e-8
