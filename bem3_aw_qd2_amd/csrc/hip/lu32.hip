@@ -9,24 +9,34 @@
 // system (gmres.hip), which restores the FP64 result.
 //
 // Layout: column-major, leading dimension n, complex numbers as float2 (re, im).
-// Right-looking blocked algorithm with panel width NB:
-//   panel     one launch per column (k_cstep): every workgroup owns a slice of
+// Right-looking blocked algorithm with panel width NB.  Panels:
+//   block-local (default)  the 128 x 128 diagonal block factored by one
+//             workgroup with pivots from its own rows (k_cpanel_local), which
+//             also forms L11^-1 and U11^-1; L21 = A21 U11^-1 and
+//             U12 = L11^-1 A12 are MFMA GEMMs.  Checked (the C ABI): a panel
+//             whose max |L21| exceeds the growth bound is restored and redone
+//             over all rows.  Unchecked (the preconditioners, whose FP64
+//             refinement validates the factors): no host decision per panel,
+//             and the next panel runs on its own CUs beside the trailing update.
+//   all rows  one launch per column (k_cstep): every workgroup owns a slice of
 //             the panel's rows; the launch takes the pivot from the candidates
 //             the previous launch left (one per workgroup), interchanges, scales
 //             and applies the rank-1 update to its rows, then leaves its
 //             candidate for the next column.  Launch boundaries are the only
-//             grid-wide synchronisation.
+//             grid-wide synchronisation.  U12 by k_ctrsm_l.
 //   swaps     the panel's interchanges on the other columns (k_cswap_cols)
-//   U12       = L11^-1 A12 (k_ctrsm_l)
-//   A22      -= L21 U12, the O(n^3) part: k_cgemm on v_mfma_f32_32x32x2_f32
-//             with FP32 accumulation
+//   A22      -= L21 U12, the O(n^3) part: k_cgemm on v_mfma_f32_32x32x2_f32,
+//             three real products per complex product, FP32 accumulation
 // After the last panel: inverses of the diagonal blocks' unit-lower (k_clinv)
-// and upper (k_ctrtri_u) factors, kept for the solves.  A look-ahead (the next
-// panel on a second stream during the update) gains nothing here: the update
-// occupies every CU and the panel's launches then wait for free slots.
+// and upper (k_ctrtri_u) factors, kept for the solves.
 // Solves (bqd2_cgetrs): permutation, then per diagonal block x_k = T_k b_k and
 // the update of the remaining rows, forward with L and backward with U; GEMV
 // kernels for up to 4 right-hand sides, the MFMA GEMM beyond.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
 #include <vector>
 #include "common.h"
 #include "tune.h"
@@ -52,6 +62,7 @@ __device__ __forceinline__ float2 crecipf(float2 a) {
 //   cand  2 x GMAXW candidate rows of NB entries
 //   rowj  2 x NB entries: row j of the panel before the interchange
 //   flag  first zero pivot (n when none)
+//   U, stat, bk  the block-local panel's inverses, status and backup
 struct LuWs {
   float2 *T;
   int *perm;
@@ -59,6 +70,9 @@ struct LuWs {
   float2 *cand;
   float2 *rowj;
   int *flag;
+  float2 *U;      // 2 x NB x NB: L11^-1 and U11^-1 of the current panel (block-local panel)
+  float2 *bk;     // n x NB: the panel before its block-local factorisation
+  int *stat;      // [0] zero pivot in the block, [1] max |L21| as float bits
 };
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -72,7 +86,10 @@ static LuWs lu_ws(int n, void *base) {
   w.slot = (float2 *)q; q += align256(2 * GMAXW * sizeof(float2));
   w.cand = (float2 *)q; q += align256((size_t)2 * GMAXW * NB * sizeof(float2));
   w.rowj = (float2 *)q; q += align256(2 * NB * sizeof(float2));
-  w.flag = (int *)q;
+  w.flag = (int *)q; q += 256;
+  w.U = (float2 *)q; q += align256((size_t)2 * NB * NB * sizeof(float2));
+  w.stat = (int *)q; q += 256;
+  w.bk = (float2 *)q;
   return w;
 }
 
@@ -81,7 +98,8 @@ extern "C" size_t bqd2_cgetrf_wsize(int n) {
   const size_t nbk = (size_t)(n + NB - 1) / NB;
   return align256(nbk * 2 * NB * NB * sizeof(float2)) + align256((size_t)n * sizeof(int)) +
          align256(2 * GMAXW * sizeof(float2)) + align256((size_t)2 * GMAXW * NB * sizeof(float2)) +
-         align256(2 * NB * sizeof(float2)) + 256;
+         align256(2 * NB * sizeof(float2)) + 256 + align256((size_t)2 * NB * NB * sizeof(float2)) + 256 +
+         align256((size_t)n * NB * sizeof(float2));
 }
 
 // ---------------------------------------------------------------- panel
@@ -220,6 +238,251 @@ __global__ void __launch_bounds__(256) k_cstep(int n, float2 *__restrict__ A, in
     for (int c = tid; c < kb; c += 256) cd[c] = A[(size_t)(k0 + c) * ld + br];
   if (jn >= r0 && jn < r1)
     for (int c = tid; c < kb; c += 256) W.rowj[parn * NB + c] = A[(size_t)(k0 + c) * ld + jn];
+}
+
+// ---------------------------------------------------------------- block-local panel
+// The panel with its pivots restricted to the rows of its diagonal block: the
+// kb x kb block is factored by one workgroup (partial pivoting within the
+// block, max |Re|+|Im|, ties to the lowest row) with the block in registers,
+// and the inverses L11^-1 (unit lower) and U11^-1 are formed for
+// L21 = A21 U11^-1 and U12 = L11^-1 A12 (MFMA GEMMs); the largest |L21| bounds
+// the growth.  Within the growth bound (16 unless set) the panel is kept (one launch and two GEMMs instead
+// of one launch per column and a triangular solve); otherwise, or at a zero
+// pivot in the block, the panel is restored and factored with pivoting over
+// all its rows (k_cstep).  Either way P A = L U holds: the factors
+// precondition GMRES on the FP64 system, which restores the FP64 result
+// whatever the pivots.
+
+// Thread t holds row r = t % 128 of the block in the columns c = g + 8 u,
+// u < 16, g = t / 128, as a 32-float vector (re, im of column g + 8 u at
+// 2u, 2u + 1): the step's column is read with a uniform index, which stays
+// in registers (indexed moves), and the loop is not unrolled, so that the
+// code of one column fits the instruction cache.  Column j's pivot search
+// runs in the two waves of g = j % 8: the largest |Re|+|Im| (as float bits,
+// which order as integers for non-negative floats) by a butterfly, then the
+// lowest row holding it by a ballot; rows j and p are exchanged through LDS,
+// the multipliers of column j and the pivot row meet in LDS, and every thread
+// updates its own entries: two barriers per column.  The inverses then take
+// thread t as column c = t % 128 of X, rows 8 u + g, with the factors in LDS:
+// X = U^-1 by back substitution (row q of X final at step q, the rows above
+// updated), X = L^-1 by forward substitution, one barrier per step.
+// stat[0] = 1 on a zero (or NaN) pivot in the block; stat[1] (the growth
+// maximum of k_cmaxabs) is cleared here.  With sticky set, stat[0] only
+// accumulates the zero pivots of all panels and stat[1] is left alone.  Linv, Uinv: NB x NB column-major.
+typedef float floatx32 __attribute__((ext_vector_type(32)));
+
+__device__ __forceinline__ float2 vget(const floatx32 &v, int u) { return make_float2(v[2 * u], v[2 * u + 1]); }
+__device__ __forceinline__ void vset(floatx32 &v, int u, float2 x) {
+  v[2 * u] = x.x;
+  v[2 * u + 1] = x.y;
+}
+
+__global__ void __launch_bounds__(1024) k_cpanel_local(int n, float2 *__restrict__ A, int k0, int kb,
+                                                       int *__restrict__ ipiv, float2 *__restrict__ Linv,
+                                                       float2 *__restrict__ Uinv, int *__restrict__ stat,
+                                                       int sticky) {
+  __shared__ float2 sD[NB * NB];  // after the factorisation: L\U column-major, (r, c) at c * NB + r
+  __shared__ float2 sP[NB], sJ[NB], sl[NB], sx[2][NB];
+  __shared__ unsigned wk[2];
+  __shared__ int wr[2];
+  __shared__ float2 wz[2], sjj;
+  const int tid = threadIdx.x, r = tid & (NB - 1), g = tid >> 7, lane = tid & 63;
+  const size_t ld = (size_t)n;
+  floatx32 a;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int c = g + 8 * u;
+    vset(a, u, (r < kb && c < kb) ? A[(size_t)(k0 + c) * ld + k0 + r] : make_float2(0.0f, 0.0f));
+  }
+  bool bad = false;
+  for (int j = 0; j < kb; j++) {
+    const int gj = j & 7, uj = j >> 3;
+    if (g == gj) {
+      const float2 x = vget(a, uj);
+      const float v = fabsf(x.x) + fabsf(x.y);
+      const bool cand = r >= j && r < kb && v == v;
+      const unsigned key = cand ? __float_as_uint(v) : 0u;
+      unsigned mx = key;
+      for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off, 64));
+      const unsigned long long hit = __ballot(cand && key == mx);
+      if (hit && lane == __ffsll((long long)hit) - 1) {
+        wk[(tid >> 6) & 1] = mx;
+        wr[(tid >> 6) & 1] = r;
+        wz[(tid >> 6) & 1] = x;
+      } else if (!hit && lane == 0) {
+        wk[(tid >> 6) & 1] = 0u;
+        wr[(tid >> 6) & 1] = -1;
+      }
+      if (r == j) sjj = x;
+    }
+    __syncthreads();
+    // wave 0 holds the lower rows: it wins ties
+    const int w = (wr[1] >= 0 && (wr[0] < 0 || wk[1] > wk[0])) ? 1 : 0;
+    int p = wr[w];
+    float2 piv = wz[w];
+    const bool zero = p < 0 || wk[w] == 0u;  // no finite candidate, or a zero column
+    if (zero) {
+      p = j;
+      piv = make_float2(0.0f, 0.0f);
+      bad = true;
+    }
+    const float2 ip = zero ? make_float2(0.0f, 0.0f) : crecipf(piv);
+    if (tid == 0) ipiv[k0 + j] = k0 + p;
+    if (r == p) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) sP[g + 8 * u] = vget(a, u);
+    }
+    if (r == j && p != j) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) sJ[g + 8 * u] = vget(a, u);
+    }
+    if (g == gj && r > j) sl[r] = cmulf((r == p) ? sjj : vget(a, uj), ip);
+    __syncthreads();
+    if (p != j) {
+      if (r == j) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) vset(a, u, sP[g + 8 * u]);
+      } else if (r == p) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) vset(a, u, sJ[g + 8 * u]);
+      }
+    }
+    if (r > j && r < kb) {
+      const float2 l = sl[r];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int c = g + 8 * u;
+        if (c == j) {
+          vset(a, u, l);
+        } else if (c > j) {
+          const float2 uc = sP[c];
+          float2 e = vget(a, u);
+          e.x -= fmaf(l.x, uc.x, -l.y * uc.y);
+          e.y -= fmaf(l.x, uc.y, l.y * uc.x);
+          vset(a, u, e);
+        }
+      }
+    }
+  }
+  if (tid == 0) {
+    if (!sticky) {
+      stat[0] = bad ? 1 : 0;
+      stat[1] = 0;
+    } else if (bad) {
+      stat[0] = 1;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int c = g + 8 * u;
+    sD[c * NB + r] = vget(a, u);
+    if (r < kb && c < kb) A[(size_t)(k0 + c) * ld + k0 + r] = vget(a, u);
+  }
+  __syncthreads();
+  // inverses: thread (column c, rows 8 u + g); rows and columns beyond kb are the identity's
+  const int c = r;
+  floatx32 x;
+#pragma unroll
+  for (int u = 0; u < 16; u++) vset(x, u, make_float2((8 * u + g == c) ? 1.0f : 0.0f, 0.0f));
+  for (int q = kb - 1; q >= 0; q--) {
+    if ((q & 7) == g) {
+      const float2 v = cmulf(vget(x, q >> 3), crecipf(sD[q * NB + q]));
+      vset(x, q >> 3, v);
+      sx[q & 1][c] = v;
+    }
+    __syncthreads();
+    const float2 xq = sx[q & 1][c];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int rr = 8 * u + g;
+      if (rr < q) {
+        const float2 uq = sD[q * NB + rr];
+        float2 e = vget(x, u);
+        e.x -= fmaf(uq.x, xq.x, -uq.y * xq.y);
+        e.y -= fmaf(uq.x, xq.y, uq.y * xq.x);
+        vset(x, u, e);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    Uinv[(size_t)c * NB + 8 * u + g] = vget(x, u);
+    vset(x, u, make_float2((8 * u + g == c) ? 1.0f : 0.0f, 0.0f));
+  }
+  __syncthreads();  // the last step's sx reads are done
+  for (int q = 0; q < kb; q++) {
+    if ((q & 7) == g) sx[q & 1][c] = vget(x, q >> 3);
+    __syncthreads();
+    const float2 xq = sx[q & 1][c];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int rr = 8 * u + g;
+      if (rr > q && rr < kb) {
+        const float2 lq = sD[q * NB + rr];
+        float2 e = vget(x, u);
+        e.x -= fmaf(lq.x, xq.x, -lq.y * xq.y);
+        e.y -= fmaf(lq.x, xq.y, lq.y * xq.x);
+        vset(x, u, e);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; u++) Linv[(size_t)c * NB + 8 * u + g] = vget(x, u);
+}
+
+// max |Re| + |Im| over rows x cols of A (leading dimension lda) into stat[1] as
+// float bits (non-negative floats order as integers; NaN above every finite):
+// blockIdx.y is the column, the x blocks stride its rows
+__global__ void __launch_bounds__(256) k_cmaxabs(int rows, const float2 *__restrict__ A, long long lda,
+                                                 int *__restrict__ stat) {
+  const float2 *col = A + (size_t)blockIdx.y * lda;
+  float mx = 0.0f;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < rows; i += gridDim.x * 256) {
+    const float2 v = col[i];
+    const float a = fabsf(v.x) + fabsf(v.y);
+    mx = (a > mx || a != a) ? a : mx;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float o = __shfl_xor(mx, off, 64);
+    mx = (o > mx || o != o) ? o : mx;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(stat + 1, __float_as_int(mx));
+}
+
+// a growth bound at or above this keeps every block-local panel unchecked
+#define BQD2_LU32_UNCHECKED 1.0e30f
+static std::atomic<int> g_local_panel{1};
+static std::atomic<float> g_growth_bound{16.0f};
+static std::atomic<long long> g_panels_local{0}, g_panels_full{0};
+// growth of each block-local panel of the most recent factorisation (any thread)
+static std::mutex g_growth_mu;
+static std::vector<float> g_growth_log;
+
+extern "C" int bqd2_set_lu32_pivoting(int mode) {
+  BQD2_CHECK(mode == 0 || mode == 1, BQD2_E_ARG, "bqd2_set_lu32_pivoting: mode is 0 (all rows) or 1 (block-local)");
+  g_local_panel = mode;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_set_lu32_growth_bound(double bound) {
+  BQD2_CHECK(bound >= 1.0, BQD2_E_ARG, "bqd2_set_lu32_growth_bound: the bound is at least 1");
+  g_growth_bound = (float)bound;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_lu32_panel_growth(float *out, int cap) {
+  BQD2_CHECK(out || cap == 0, BQD2_E_ARG, "bqd2_lu32_panel_growth: null argument");
+  std::lock_guard<std::mutex> lk(g_growth_mu);
+  const int cnt = (int)g_growth_log.size();
+  for (int i = 0; i < cnt && i < cap; i++) out[i] = g_growth_log[i];
+  return cnt;
+}
+
+extern "C" int bqd2_lu32_panel_counts(long long *out) {
+  BQD2_CHECK(out, BQD2_E_ARG, "bqd2_lu32_panel_counts: null argument");
+  out[0] = g_panels_local.load();
+  out[1] = g_panels_full.load();
+  return BQD2_OK;
 }
 
 // the interchanges of the panel [k0, k0 + kb) on the columns outside it
@@ -382,40 +645,50 @@ __global__ void __launch_bounds__(256) k_ceye(int nbk, float2 *__restrict__ T) {
 
 // ---------------------------------------------------------------- MFMA GEMM
 // C[m x nc] -= A[m x kk] B[kk x nc] (SUB) or C = A B (SET), column-major complex
-// float.  Workgroup tile 128 x 128 of C, four waves of 64 x 64; K staged in LDS
-// in slices of 32 complex columns.  v_mfma_f32_32x32x2_f32 computes
-// D[i][j] += sum_{k<2} X[i][k] Y[k][j], with lane l holding X[l%32][l/32] and
-// Y[l/32][l%32] and D[i][j] in register r of lane l for j = l%32,
-// i = 8 (r/4) + 4 (l/32) + r%4.  Each complex k is one instruction per real and
-// imaginary part of C^T = B^T A^T: the two k slots carry the real and imaginary
-// parts, X = (Br, Bi) for both, Y = (Ar, -Ai) for Re C and (Ai, Ar) for Im C.
-// With C^T as the product, j runs along the rows of C, so the 32 lanes of a
-// register store 32 consecutive rows of one column.  B may be C when m <= 128,
-// or A may be C when nc <= 128: a workgroup then reads only the columns (rows)
-// it writes, and writes after its last read.
-// WT x WT blocks of 32 x 32 per wave, four waves (2 x 2): a 64 WT square tile
-// per workgroup.  Every tile computes each element of C from the same MFMA
-// chain over k, so the tile size changes no bit of the result.
-#define GT 128
+// float.  Four waves (2 x 2) of WA x WB blocks of 32 x 32: a workgroup tile of
+// 64 WA rows x 64 WB columns; K staged in LDS in slices of 16 complex columns.
+// v_mfma_f32_32x32x2_f32 computes D[i][j] += sum_{k<2} X[i][k] Y[k][j], with
+// lane l holding X[l%32][l/32] and Y[l/32][l%32] and D[i][j] in register r of
+// lane l for j = l%32, i = 8 (r/4) + 4 (l/32) + r%4.  The product is formed as
+// C^T = B^T A^T, so j runs along the rows of C and the 32 lanes of a register
+// store 32 consecutive rows of one column.
+// Complex products, two forms:
+//   4M  one instruction per complex k for each of Re C and Im C: the two k
+//       slots carry the real and imaginary parts, X = (Br, Bi) for both,
+//       Y = (Ar, -Ai) for Re C and (Ai, Ar) for Im C;
+//   3M  (the trailing update) the two k slots carry two consecutive complex k
+//       and three real products are accumulated, P1 = Ar Br, P2 = Ai Bi and
+//       P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2 at the end:
+//       three instructions per two complex k instead of four (the 3M method,
+//       A. Karatsuba's identity applied to matrices; N.J. Higham, "Stability
+//       of a method for multiplying complex matrices with three real matrix
+//       multiplications", SIAM J. Matrix Anal. Appl. 13 (1992) 681-687: the
+//       error bound of Im C grows by a small constant factor).
+// In-place forms: B may be C when one workgroup covers all m rows, or A may be
+// C when one covers all nc columns: a workgroup then reads only the columns
+// (rows) it writes, and writes after its last read.
+// Every tile shape computes each element of C from the same MFMA chain over k,
+// so the tile shape changes no bit of the result (the form does).
 #define GK 16
 #define GPAD 1
-template <int SET, int WT>
+template <int SET, int WA, int WB, int M3>
 __global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const float2 *A, long long lda,
                                                   const float2 *B, long long ldb, float2 *C, long long ldc) {
-  constexpr int T = 64 * WT;     // tile edge
-  constexpr int AKL = 256 / T;   // lanes along k when staging A
-  __shared__ float2 sA[GK][T + GPAD];  // (k, row)
-  __shared__ float2 sB[GK][T + GPAD];  // (k, col)
+  constexpr int TR = 64 * WA, TC = 64 * WB;  // tile rows, columns
+  constexpr int AKL = 256 / TR;              // lanes along k when staging A
+  constexpr int NACC = M3 ? 3 : 2;
+  __shared__ float2 sA[GK][TR + GPAD];  // (k, row)
+  __shared__ float2 sB[GK][TC + GPAD];  // (k, col)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave & 1, wc = wave >> 1;
-  const int row0 = blockIdx.x * T, col0 = blockIdx.y * T;
+  const int row0 = blockIdx.x * TR, col0 = blockIdx.y * TC;
   const int l32 = lane & 31;
-  const bool hi = lane >= 32;
-  // global -> register staging of one K slice: A T rows x 16 k (consecutive
-  // lanes on consecutive rows), B 16 k x T columns (16 lanes per column)
-  const int ar = tid % T, ak = tid / T;
+  const int hi = lane >= 32 ? 1 : 0;
+  // global -> register staging of one K slice: A TR rows x 16 k (consecutive
+  // lanes on consecutive rows), B 16 k x TC columns (16 lanes per column)
+  const int ar = tid % TR, ak = tid / TR;
   const int bk = tid & 15, bc = tid >> 4;
-  float2 ra[GK / AKL], rb[T / 16];
+  float2 ra[GK / AKL], rb[TC / 16];
   auto load = [&](int k0) {
     const int gr = row0 + ar;
 #pragma unroll
@@ -425,89 +698,136 @@ __global__ void __launch_bounds__(256, 2) k_cgemm(int m, int nc, int kk, const f
     }
     const int gk = k0 + bk;
 #pragma unroll
-    for (int i = 0; i < T / 16; i++) {
+    for (int i = 0; i < TC / 16; i++) {
       const int gc = col0 + bc + 16 * i;
       rb[i] = (gc < nc && gk < kk) ? B[(size_t)gc * ldb + gk] : make_float2(0.0f, 0.0f);
     }
   };
-  floatx16 accR[WT][WT], accI[WT][WT];
+  floatx16 acc[NACC][WB][WA];
 #pragma unroll
-  for (int a = 0; a < WT; a++)
+  for (int t = 0; t < NACC; t++)
 #pragma unroll
-    for (int b = 0; b < WT; b++)
+    for (int b = 0; b < WB; b++)
 #pragma unroll
-      for (int r = 0; r < 16; r++) { accR[a][b][r] = 0.0f; accI[a][b][r] = 0.0f; }
+      for (int a = 0; a < WA; a++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[t][b][a][r] = 0.0f;
   load(0);
   for (int k0 = 0; k0 < kk; k0 += GK) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < GK / AKL; i++) sA[ak + AKL * i][ar] = ra[i];
 #pragma unroll
-    for (int i = 0; i < T / 16; i++) sB[bk][bc + 16 * i] = rb[i];
+    for (int i = 0; i < TC / 16; i++) sB[bk][bc + 16 * i] = rb[i];
     __syncthreads();
     if (k0 + GK < kk) load(k0 + GK);  // next slice in flight during this one's MFMAs
+    if (M3) {
 #pragma unroll 4
-    for (int q = 0; q < GK; q++) {
-      float xb[WT], yr[WT], yi[WT];
+      for (int q = 0; q < GK; q += 2) {
+        float xr[WB], xi[WB], xs[WB], yr[WA], yi[WA], ys[WA];
 #pragma unroll
-      for (int b = 0; b < WT; b++) {
-        const float2 v = sB[q][wc * 32 * WT + b * 32 + l32];
-        xb[b] = hi ? v.y : v.x;
-      }
-#pragma unroll
-      for (int a = 0; a < WT; a++) {
-        const float2 v = sA[q][wr * 32 * WT + a * 32 + l32];
-        yr[a] = hi ? -v.y : v.x;
-        yi[a] = hi ? v.x : v.y;
-      }
-#pragma unroll
-      for (int b = 0; b < WT; b++)
-#pragma unroll
-        for (int a = 0; a < WT; a++) {
-          accR[b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yr[a], accR[b][a], 0, 0, 0);
-          accI[b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yi[a], accI[b][a], 0, 0, 0);
+        for (int b = 0; b < WB; b++) {
+          const float2 v = sB[q + hi][wc * 32 * WB + b * 32 + l32];
+          xr[b] = v.x;
+          xi[b] = v.y;
+          xs[b] = v.x + v.y;
         }
+#pragma unroll
+        for (int a = 0; a < WA; a++) {
+          const float2 v = sA[q + hi][wr * 32 * WA + a * 32 + l32];
+          yr[a] = v.x;
+          yi[a] = v.y;
+          ys[a] = v.x + v.y;
+        }
+#pragma unroll
+        for (int b = 0; b < WB; b++)
+#pragma unroll
+          for (int a = 0; a < WA; a++) {
+            acc[0][b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[b], yr[a], acc[0][b][a], 0, 0, 0);
+            acc[1][b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xi[b], yi[a], acc[1][b][a], 0, 0, 0);
+            acc[NACC - 1][b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xs[b], ys[a], acc[NACC - 1][b][a], 0, 0, 0);
+          }
+      }
+    } else {
+#pragma unroll 4
+      for (int q = 0; q < GK; q++) {
+        float xb[WB], yr[WA], yi[WA];
+#pragma unroll
+        for (int b = 0; b < WB; b++) {
+          const float2 v = sB[q][wc * 32 * WB + b * 32 + l32];
+          xb[b] = hi ? v.y : v.x;
+        }
+#pragma unroll
+        for (int a = 0; a < WA; a++) {
+          const float2 v = sA[q][wr * 32 * WA + a * 32 + l32];
+          yr[a] = hi ? -v.y : v.x;
+          yi[a] = hi ? v.x : v.y;
+        }
+#pragma unroll
+        for (int b = 0; b < WB; b++)
+#pragma unroll
+          for (int a = 0; a < WA; a++) {
+            acc[0][b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yr[a], acc[0][b][a], 0, 0, 0);
+            acc[1][b][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(xb[b], yi[a], acc[1][b][a], 0, 0, 0);
+          }
+      }
     }
   }
-  // accR[b][a] register r: row l32 of row block a, column 8 (r/4) + 4 hi + r%4 of column block b
+  // acc[.][b][a] register r: row l32 of row block a, column 8 (r/4) + 4 hi + r%4 of column block b
 #pragma unroll
-  for (int b = 0; b < WT; b++)
+  for (int b = 0; b < WB; b++)
 #pragma unroll
-    for (int a = 0; a < WT; a++) {
-      const int gr = row0 + wr * 32 * WT + a * 32 + l32;
+    for (int a = 0; a < WA; a++) {
+      const int gr = row0 + wr * 32 * WA + a * 32 + l32;
       if (gr >= m) continue;
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const int gc = col0 + wc * 32 * WT + b * 32 + 8 * (r >> 2) + 4 * (hi ? 1 : 0) + (r & 3);
+        const int gc = col0 + wc * 32 * WB + b * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
         if (gc >= nc) continue;
+        float re = acc[0][b][a][r], im = acc[1][b][a][r];
+        if (M3) {
+          const float p1 = re, p2 = im;
+          re = p1 - p2;
+          im = acc[NACC - 1][b][a][r] - p1 - p2;
+        }
         float2 *p = C + (size_t)gc * ldc + gr;
         if (SET) {
-          *p = make_float2(accR[b][a][r], accI[b][a][r]);
+          *p = make_float2(re, im);
         } else {
           float2 o = *p;
-          o.x -= accR[b][a][r];
-          o.y -= accI[b][a][r];
+          o.x -= re;
+          o.y -= im;
           *p = o;
         }
       }
     }
 }
 
-// wt: 2 for 128 x 128 tiles (the in-place forms need them: a workgroup must
-// cover all of C's rows or columns, as the aliasing rule above states), 1 for
-// 64 x 64 tiles on distinct operands
+// Launch shapes.  GEMM_SQ128: 128 x 128 tiles, 4M (the in-place forms need a
+// workgroup to cover all of C's rows or columns, as stated above);
+// GEMM_SQ64: 64 x 64 tiles, 4M; GEMM_3M64 / GEMM_3M128X64: the trailing
+// update, 3M, on 64 x 64 or 128 x 64 tiles (distinct operands only).
+enum GemmShape { GEMM_SQ128 = 0, GEMM_SQ64 = 1, GEMM_3M64 = 2, GEMM_3M128X64 = 3 };
 static int cgemm(bool set, int m, int nc, int kk, const float2 *A, long long lda, const float2 *B, long long ldb,
-                 float2 *C, long long ldc, hipStream_t st, int wt = 2) {
+                 float2 *C, long long ldc, hipStream_t st, GemmShape shape = GEMM_SQ128) {
   if (m <= 0 || nc <= 0 || kk <= 0) return BQD2_OK;
-  const int T = 64 * wt;
-  dim3 grid(bqd2_blocks(m, T), bqd2_blocks(nc, T));
-  if (wt == 1) {
-    if (set) hipLaunchKernelGGL((k_cgemm<1, 1>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
-    else hipLaunchKernelGGL((k_cgemm<0, 1>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
-  } else {
-    if (set) hipLaunchKernelGGL((k_cgemm<1, 2>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
-    else hipLaunchKernelGGL((k_cgemm<0, 2>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc);
+  if (set && (shape == GEMM_3M64 || shape == GEMM_3M128X64)) shape = GEMM_SQ64;
+  const int tr = (shape == GEMM_SQ128 || shape == GEMM_3M128X64) ? 128 : 64;
+  const int tc = shape == GEMM_SQ128 ? 128 : 64;
+  dim3 grid(bqd2_blocks(m, tr), bqd2_blocks(nc, tc));
+#define BQD2_GEMM_LAUNCH(S, WA, WB, M3) \
+  hipLaunchKernelGGL((k_cgemm<S, WA, WB, M3>), grid, dim3(256), 0, st, m, nc, kk, A, lda, B, ldb, C, ldc)
+  switch (shape) {
+    case GEMM_SQ128:
+      if (set) BQD2_GEMM_LAUNCH(1, 2, 2, 0); else BQD2_GEMM_LAUNCH(0, 2, 2, 0);
+      break;
+    case GEMM_SQ64:
+      if (set) BQD2_GEMM_LAUNCH(1, 1, 1, 0); else BQD2_GEMM_LAUNCH(0, 1, 1, 0);
+      break;
+    case GEMM_3M64: BQD2_GEMM_LAUNCH(0, 1, 1, 1); break;
+    case GEMM_3M128X64: BQD2_GEMM_LAUNCH(0, 2, 1, 1); break;
   }
+#undef BQD2_GEMM_LAUNCH
   BQD2_HIP(hipGetLastError());
   return BQD2_OK;
 }
@@ -570,60 +890,44 @@ static int small_update(bool set, int m, int kk, int nrhs, const float2 *A, long
 
 // perm from the interchanges: row i of P A is row perm[i] of A (one lane: the
 // interchanges are sequential)
-__global__ void k_perm(int n, const int *__restrict__ ipiv, int *__restrict__ perm) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int i = 0; i < n; i++) perm[i] = i;
-  for (int i = 0; i < n; i++) {
-    const int p = ipiv[i];
-    const int t = perm[i];
-    perm[i] = perm[p];
-    perm[p] = t;
-  }
-}
-
-__global__ void k_init_flag(int *flag, int n) {
-  if (threadIdx.x == 0) *flag = n;
-}
-
-extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream) {
-  BQD2_CHECK(n > 0 && d_A && d_ipiv && d_ws, BQD2_E_ARG, "bqd2_cgetrf: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
-  float2 *A = (float2 *)d_A;
-  LuWs W = lu_ws(n, d_ws);
-  hipLaunchKernelGGL(k_init_flag, dim3(1), dim3(64), 0, st, W.flag, n);
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int kb = (n - k0 < NB) ? n - k0 : NB;
-    const int rows = n - k0;
-    int nw = bqd2_blocks(rows, PROWS);
-    if (nw > GMAXW) nw = GMAXW;
-    const int chunk = bqd2_blocks(rows, nw);
-    nw = bqd2_blocks(rows, chunk);
-    for (int j = k0 - 1; j < k0 + kb; j++)
-      hipLaunchKernelGGL(k_cstep, dim3(nw), dim3(256), 0, st, n, A, k0, kb, j, chunk, nw, d_ipiv, W);
-    if (n > kb)
-      hipLaunchKernelGGL(k_cswap_cols, dim3(bqd2_blocks(n - kb, 256)), dim3(256), 0, st, n, A, k0, kb, d_ipiv);
-    BQD2_HIP(hipGetLastError());
-    const int rest = n - k0 - kb;
-    if (rest > 0) {
-      // U12 = L11^-1 A12 in place, then A22 -= L21 U12 on the MFMA units
-      float2 *A12 = A + (size_t)(k0 + kb) * n + k0;
-      hipLaunchKernelGGL(k_ctrsm_l, dim3(bqd2_blocks(rest, 256)), dim3(256), 0, st, n, A + (size_t)k0 * n + k0, kb,
-                         A12, (long long)n, rest);
-      BQD2_HIP(hipGetLastError());
-      // operands distinct from A22: either tile size, as the tuner picks
-      const TuneTicket tk = tune_begin(TUNE_CGEMM, tune_class(rest), 8.0 * rest * rest * kb, st);
-      int rc = cgemm(false, rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n, A12, n,
-                     A + (size_t)(k0 + kb) * n + k0 + kb, n, st, tk.cand == 1 ? 2 : 1);
-      tune_end(tk, st);
-      if (rc) return rc;
+// the row permutation from the interchanges: a sequential chain, run in LDS
+// when the permutation fits (PERM_LDS ints), else in memory
+#define PERM_LDS 32768
+__global__ void __launch_bounds__(256) k_perm(int n, const int *__restrict__ ipiv, int *__restrict__ perm,
+                                              int in_lds) {
+  __shared__ int sperm[PERM_LDS];
+  int *w = in_lds ? sperm : perm;
+  for (int i = threadIdx.x; i < n; i += 256) w[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < n; i++) {
+      const int p = ipiv[i];
+      const int t = w[i];
+      w[i] = w[p];
+      w[p] = t;
     }
-  }
-  // inverses of the diagonal blocks' factors for the solves, and the row permutation
+  __syncthreads();
+  if (in_lds)
+    for (int i = threadIdx.x; i < n; i += 256) perm[i] = w[i];
+}
+
+__global__ void k_init_flag(int *flag, int n, int *stat) {
+  if (threadIdx.x == 0) *flag = n;
+  if (threadIdx.x < 2) stat[threadIdx.x] = 0;
+}
+
+// After the panels: inverses of the diagonal blocks' factors for the solves,
+// the row permutation, the solves' form of the off-diagonal blocks, and the
+// zero-pivot check.
+static int cgetrf_finish(int n, float2 *A, int *d_ipiv, const LuWs &W, hipStream_t st) {
   const int nbk = bqd2_blocks(n, NB);
   hipLaunchKernelGGL(k_ceye, dim3(bqd2_blocks((long long)nbk * NB * NB, 256)), dim3(256), 0, st, nbk, W.T);
   hipLaunchKernelGGL(k_clinv, dim3(nbk), dim3(256), 0, st, n, A, W.T);
   hipLaunchKernelGGL(k_ctrtri_u, dim3(nbk), dim3(NB), 0, st, n, A, W.T);
-  hipLaunchKernelGGL(k_perm, dim3(1), dim3(64), 0, st, n, d_ipiv, W.perm);
+  {
+    const bool in_lds = n <= PERM_LDS;
+    hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, st, n, d_ipiv, W.perm, in_lds ? 1 : 0);
+  }
   BQD2_HIP(hipGetLastError());
   // the solves' form of the factors: each off-diagonal block times the inverse of
   // its column's diagonal block (L_rk L_kk^-1 below, U_rk U_kk^-1 above), in place
@@ -641,6 +945,271 @@ extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *str
   BQD2_HIP(hipStreamSynchronize(st));
   BQD2_CHECK(h_flag == n, BQD2_E_SINGULAR, "bqd2_cgetrf: zero pivot at column %d", h_flag);
   return BQD2_OK;
+}
+
+// Streams of the relaxed factorisation: the trailing work and the panels on
+// disjoint sets of CUs (LA_PANEL_CUS for the panels), so that a panel (one
+// workgroup of 1024 threads and 136 KB of LDS) starts beside the running
+// update instead of waiting for a CU to drain.  Created once per host thread
+// and device, default priority, kept for the process.
+#define LA_PANEL_CUS 8
+static bool lookahead_streams(hipStream_t *m, hipStream_t *pn) {
+  struct Pair {
+    int dev = -1;
+    hipStream_t main = nullptr, panel = nullptr;
+  };
+  static thread_local Pair S;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (S.main && S.dev == dev) {
+    *m = S.main;
+    *pn = S.panel;
+    return true;
+  }
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 8 * LA_PANEL_CUS)
+    return false;
+  const int words = (ncu + 31) / 32;
+  std::vector<uint32_t> mm(words, 0u), pm(words, 0u);
+  for (int c = 0; c < ncu; c++) (c < LA_PANEL_CUS ? pm : mm)[c / 32] |= 1u << (c % 32);
+  hipStream_t a = nullptr, b = nullptr;
+  if (hipExtStreamCreateWithCUMask(&a, (uint32_t)words, mm.data()) != hipSuccess) return false;
+  if (hipExtStreamCreateWithCUMask(&b, (uint32_t)words, pm.data()) != hipSuccess) {
+    (void)hipStreamDestroy(a);
+    return false;
+  }
+  S.dev = dev;
+  S.main = a;
+  S.panel = b;
+  *m = a;
+  *pn = b;
+  return true;
+}
+
+// The relaxed factorisation (block-local pivots whatever the growth; the FP64
+// refinement that these factors precondition validates them): no host
+// decision per panel, so every launch is queued at once.  Order per panel k
+// on the main stream: L21 = A21 U11^-1, swaps, U12 = L11^-1 A12, A22 -= L21 U12
+// on the next panel's columns, then on the rest; the next panel's kernel runs
+// on the panel stream between the two updates (events order it after the
+// first and the next L21 after it; the rest of the update touches none of its
+// columns).  Returns BQD2_E_SINGULAR when a diagonal block met a zero pivot.
+static int cgetrf_relaxed(int n, float2 *A, int *d_ipiv, const LuWs &W, hipStream_t st) {
+  hipStream_t M = st, P = st;
+  const bool two = lookahead_streams(&M, &P);
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // start/end, next panel's columns updated, panel done
+  struct EvGuard {
+    hipEvent_t *e;
+    ~EvGuard() {
+      for (int i = 0; i < 3; i++)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  if (two) {
+    for (int i = 0; i < 3; i++) BQD2_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    BQD2_HIP(hipEventRecord(ev[0], st));
+    BQD2_HIP(hipStreamWaitEvent(M, ev[0], 0));
+    BQD2_HIP(hipStreamWaitEvent(P, ev[0], 0));
+  }
+  const size_t ld = (size_t)n;
+  hipLaunchKernelGGL(k_init_flag, dim3(1), dim3(64), 0, M, W.flag, n, W.stat);
+  auto panel = [&](hipStream_t s, int k0) {
+    hipLaunchKernelGGL(k_cpanel_local, dim3(1), dim3(1024), 0, s, n, A, k0, std::min(NB, n - k0), d_ipiv, W.U,
+                       W.U + NB * NB, W.stat, 1);
+  };
+  panel(M, 0);
+  BQD2_HIP(hipGetLastError());
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int kb = std::min(NB, n - k0);
+    const int rest = n - k0 - kb;
+    if (k0 > 0 && two) BQD2_HIP(hipStreamWaitEvent(M, ev[2], 0));
+    float2 *A21 = A + (size_t)k0 * ld + k0 + kb;
+    if (rest > 0) {
+      int rc = cgemm(true, rest, kb, kb, A21, n, W.U + NB * NB, NB, A21, n, M, GEMM_SQ128);
+      if (rc) return rc;
+    }
+    if (n > kb)
+      hipLaunchKernelGGL(k_cswap_cols, dim3(bqd2_blocks(n - kb, 256)), dim3(256), 0, M, n, A, k0, kb, d_ipiv);
+    BQD2_HIP(hipGetLastError());
+    if (rest <= 0) continue;
+    float2 *A12 = A + (size_t)(k0 + kb) * ld + k0;
+    float2 *A22 = A + (size_t)(k0 + kb) * ld + k0 + kb;
+    int rc = cgemm(true, kb, rest, kb, W.U, NB, A12, n, A12, n, M, GEMM_SQ128);
+    const int c0 = std::min(NB, rest);
+    if (!rc) rc = cgemm(false, rest, c0, kb, A21, n, A12, n, A22, n, M, GEMM_3M64);
+    if (rc) return rc;
+    if (two) {
+      BQD2_HIP(hipEventRecord(ev[1], M));
+      BQD2_HIP(hipStreamWaitEvent(P, ev[1], 0));
+      panel(P, k0 + kb);
+      BQD2_HIP(hipEventRecord(ev[2], P));
+    } else {
+      panel(M, k0 + kb);
+    }
+    BQD2_HIP(hipGetLastError());
+    if (rest > c0) {
+      const int nc = rest - c0;
+      const TuneTicket tk = tune_begin(TUNE_CGEMM, tune_class(rest), 8.0 * rest * nc * kb, M);
+      rc = cgemm(false, rest, nc, kb, A21, n, A12 + (size_t)c0 * ld, n, A22 + (size_t)c0 * ld, n, M,
+                 tk.cand == 1 ? GEMM_3M128X64 : GEMM_3M64);
+      tune_end(tk, M);
+      if (rc) return rc;
+    }
+  }
+  int rc = cgetrf_finish(n, A, d_ipiv, W, M);
+  int zero = 0;
+  if (!rc) {
+    BQD2_HIP(hipMemcpyAsync(&zero, W.stat, sizeof(int), hipMemcpyDeviceToHost, M));
+    BQD2_HIP(hipStreamSynchronize(M));
+  }
+  if (two) {
+    BQD2_HIP(hipEventRecord(ev[0], M));
+    BQD2_HIP(hipStreamWaitEvent(st, ev[0], 0));
+  }
+  if (rc) return rc;
+  g_panels_local += bqd2_blocks(n, NB);
+  BQD2_CHECK(!zero, BQD2_E_SINGULAR, "bqd2_cgetrf: zero pivot in a diagonal block (block-local pivots)");
+  return BQD2_OK;
+}
+
+// Host-visible status of the block-local panel: pinned, one per host thread
+// (a factorisation waits for its own panels), allocated on first use and kept.
+static int *panel_status_buffer() {
+  static thread_local int *h = nullptr;
+  if (!h && hipHostMalloc((void **)&h, 64, hipHostMallocDefault) != hipSuccess) h = nullptr;
+  return h;
+}
+
+// Order of the work per panel k with block-local pivoting (one stream):
+//   swaps_k, U12_k = L11^-1 A12 (GEMM), A22 -= L21 U12 on the next panel's
+//   columns, the next panel's attempt (backup, k_cpanel_local, L21 GEMM,
+//   growth), then A22 -= L21 U12 on the remaining columns.
+// The host waits for the attempt's status (an event) while the GPU runs the
+// big update behind it, so the decision costs no idle time; a rejected panel
+// is restored and factored with k_cstep launches queued after that update
+// (which does not touch the panel's columns).
+int bqd2_cgetrf_bound(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream, float bound, float *max_growth) {
+  BQD2_CHECK(n > 0 && d_A && d_ipiv && d_ws, BQD2_E_ARG, "bqd2_cgetrf: bad arguments");
+  if (max_growth) *max_growth = 0.0f;
+  hipStream_t st = (hipStream_t)stream;
+  float2 *A = (float2 *)d_A;
+  LuWs W = lu_ws(n, d_ws);
+  if (g_local_panel.load() && bound >= BQD2_LU32_UNCHECKED) {
+    if (max_growth) *max_growth = INFINITY;  // not measured
+    return cgetrf_relaxed(n, A, d_ipiv, W, st);
+  }
+  const size_t ld = (size_t)n;
+  int *h_stat = nullptr;
+  hipEvent_t ev = nullptr;
+  bool local_on = g_local_panel.load() != 0;
+  std::vector<float> growths;
+  if (local_on) {
+    h_stat = panel_status_buffer();
+    BQD2_CHECK(h_stat, BQD2_E_HIP, "bqd2_cgetrf: pinned status buffer");
+    BQD2_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  struct EvGuard {
+    hipEvent_t e;
+    ~EvGuard() { if (e) (void)hipEventDestroy(e); }
+  } guard{ev};
+  hipLaunchKernelGGL(k_init_flag, dim3(1), dim3(64), 0, st, W.flag, n, W.stat);
+  // attempt of the block-local panel at k0: backup, factor, L21, growth, status to the host
+  auto attempt = [&](int k0) -> int {
+    const int kb = std::min(NB, n - k0), rows = n - k0, rest = rows - kb;
+    BQD2_HIP(hipMemcpy2DAsync(W.bk, (size_t)rows * sizeof(float2), A + (size_t)k0 * ld + k0, ld * sizeof(float2),
+                              (size_t)rows * sizeof(float2), kb, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_cpanel_local, dim3(1), dim3(1024), 0, st, n, A, k0, kb, d_ipiv, W.U, W.U + NB * NB, W.stat,
+                       0);
+    BQD2_HIP(hipGetLastError());
+    if (rest > 0) {
+      float2 *A21 = A + (size_t)k0 * ld + k0 + kb;
+      int rc = cgemm(true, rest, kb, kb, A21, n, W.U + NB * NB, NB, A21, n, st, GEMM_SQ128);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_cmaxabs, dim3(std::min(bqd2_blocks(rest, 256), 8), kb), dim3(256), 0, st, rest, A21,
+                         (long long)n, W.stat);
+      BQD2_HIP(hipGetLastError());
+    }
+    BQD2_HIP(hipMemcpyAsync(h_stat, W.stat, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    BQD2_HIP(hipEventRecord(ev, st));
+    return BQD2_OK;
+  };
+  if (local_on) {
+    int rc = attempt(0);
+    if (rc) return rc;
+  }
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int kb = std::min(NB, n - k0);
+    const int rows = n - k0;
+    const int rest = rows - kb;
+    bool local = false;
+    if (local_on) {
+      BQD2_HIP(hipEventSynchronize(ev));
+      float growth;
+      memcpy(&growth, &h_stat[1], sizeof(float));
+      if (h_stat[0]) growth = INFINITY;
+      growths.push_back(growth);
+      local = growth <= bound;
+      if (local && max_growth && growth > *max_growth) *max_growth = growth;
+      if (!local)
+        BQD2_HIP(hipMemcpy2DAsync(A + (size_t)k0 * ld + k0, ld * sizeof(float2), W.bk, (size_t)rows * sizeof(float2),
+                                  (size_t)rows * sizeof(float2), kb, hipMemcpyDeviceToDevice, st));
+    }
+    if (local) {
+      g_panels_local++;
+    } else {
+      g_panels_full++;
+      int nw = bqd2_blocks(rows, PROWS);
+      if (nw > GMAXW) nw = GMAXW;
+      const int chunk = bqd2_blocks(rows, nw);
+      nw = bqd2_blocks(rows, chunk);
+      for (int j = k0 - 1; j < k0 + kb; j++)
+        hipLaunchKernelGGL(k_cstep, dim3(nw), dim3(256), 0, st, n, A, k0, kb, j, chunk, nw, d_ipiv, W);
+    }
+    if (n > kb)
+      hipLaunchKernelGGL(k_cswap_cols, dim3(bqd2_blocks(n - kb, 256)), dim3(256), 0, st, n, A, k0, kb, d_ipiv);
+    BQD2_HIP(hipGetLastError());
+    if (rest > 0) {
+      // U12 = L11^-1 A12 in place
+      float2 *A12 = A + (size_t)(k0 + kb) * ld + k0;
+      if (local) {
+        int rc = cgemm(true, kb, rest, kb, W.U, NB, A12, n, A12, n, st, GEMM_SQ128);
+        if (rc) return rc;
+      } else {
+        hipLaunchKernelGGL(k_ctrsm_l, dim3(bqd2_blocks(rest, 256)), dim3(256), 0, st, n, A + (size_t)k0 * ld + k0,
+                           kb, A12, (long long)n, rest);
+        BQD2_HIP(hipGetLastError());
+      }
+      // A22 -= L21 U12 on the MFMA units (3M form); with block-local panels the
+      // next panel's columns first, then its attempt, then the other columns
+      const float2 *L21 = A + (size_t)k0 * ld + k0 + kb;
+      float2 *A22 = A + (size_t)(k0 + kb) * ld + k0 + kb;
+      int c0 = 0;
+      if (local_on) {
+        c0 = std::min(NB, rest);
+        int rc = cgemm(false, rest, c0, kb, L21, n, A12, n, A22, n, st, GEMM_3M64);
+        if (!rc) rc = attempt(k0 + kb);
+        if (rc) return rc;
+      }
+      if (rest > c0) {
+        // operands distinct from A22: either tile shape, as the tuner picks
+        const int nc = rest - c0;
+        const TuneTicket tk = tune_begin(TUNE_CGEMM, tune_class(rest), 8.0 * rest * nc * kb, st);
+        int rc = cgemm(false, rest, nc, kb, L21, n, A12 + (size_t)c0 * ld, n, A22 + (size_t)c0 * ld, n, st,
+                       tk.cand == 1 ? GEMM_3M128X64 : GEMM_3M64);
+        tune_end(tk, st);
+        if (rc) return rc;
+      }
+    }
+  }
+  if (local_on) {
+    std::lock_guard<std::mutex> lk(g_growth_mu);
+    g_growth_log.swap(growths);
+  }
+  return cgetrf_finish(n, A, d_ipiv, W, st);
+}
+
+extern "C" int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream) {
+  return bqd2_cgetrf_bound(n, d_A, d_ipiv, d_ws, stream, g_growth_bound.load(), nullptr);
 }
 
 // One block step of a triangular sweep.  The factorisation leaves the
@@ -916,6 +1485,7 @@ int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const doub
 extern "C" int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, const float *d_B,
                           long long ldb, float *d_C, long long ldc, void *stream) {
   BQD2_CHECK(m >= 0 && nc >= 0 && kk >= 0 && d_A && d_B && d_C, BQD2_E_ARG, "bqd2_cgemm: bad arguments");
+  // C = A B: 128 x 128 tiles (the in-place forms); C -= A B: the trailing update's 3M form
   return cgemm(set != 0, m, nc, kk, (const float2 *)d_A, lda, (const float2 *)d_B, ldb, (float2 *)d_C, ldc,
-               (hipStream_t)stream);
+               (hipStream_t)stream, set ? GEMM_SQ128 : GEMM_3M64);
 }

@@ -54,6 +54,7 @@ struct DevDomain {
   double *gdinv;       // N complex
   int *gipiv;          // N
   bool g_lu_valid;
+  float g_growth;  // largest block-local panel growth kept in its factors
   hipGraphExec_t g_graph;  // its triangular sweeps (3 right-hand sides) as a graph
 };
 
@@ -100,6 +101,7 @@ struct bqd2_problem {
   double *dinv;            // na complex: inverse diagonal of the system
   hipGraphExec_t lu_graph; // the triangular sweeps of its factors as a graph
   bool lu32_valid;
+  float lu32_growth;       // largest block-local panel growth kept in the factors
   double ms_factor;        // time of the last factorisation (conversion + LU)
   double *hk2d;            // k2[m] / k2[s] table (16 x 16 complex), set once
   double *xg;              // GMRES solution of the system (na)
@@ -136,6 +138,17 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
                const Precond *pc = nullptr);
 // lu32.hip: the row-scaled row-major complex double matrix rounded to column-major
 // complex float; out = M^-1 in with the factors (in may be out)
+// bqd2_cgetrf with the growth bound of block-local panels given (the C ABI
+// uses the process-wide one); *max_growth (may be null) receives the largest
+// growth of a kept block-local panel, 0 when none was kept.
+int bqd2_cgetrf_bound(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream, float bound, float *max_growth);
+// Growth bounds: the stable one of the C ABI's default (threshold pivoting
+// with |L21| <= 16), and the relaxed one of the preconditioners (block-local
+// pivots unchecked: refinement on the FP64 system restores the accuracy, and
+// the factors are redone with the stable bound when refinement does not
+// converge or a diagonal block meets a zero pivot, BQD2_E_SINGULAR).
+#define BQD2_LU32_STABLE_GROWTH 16.0f
+#define BQD2_LU32_RELAXED_GROWTH 3.0e38f
 int bqd2_lu32_convert(int n, const double *A, long long lda, const int *map, const double *dinv, float *Af,
                       hipStream_t st);
 int bqd2_lu32_apply(int n, int nrhs, const float *Af, const void *ws, const double *in, long long ldi, double *out,

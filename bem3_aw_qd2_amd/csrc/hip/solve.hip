@@ -428,7 +428,7 @@ extern "C" int bqd2_set_solver(bqd2_problem *p, int mode) {
 
 // Complex float LU of the row-scaled row-major system in p->A, once per
 // assembly; the factors stay on the device for every later right-hand side.
-static int factor_lu32(bqd2_problem *p, hipStream_t st) {
+static int factor_lu32(bqd2_problem *p, hipStream_t st, float bound) {
   const int na = p->na;
   int rc;
   if (!p->Af) {
@@ -443,7 +443,7 @@ static int factor_lu32(bqd2_problem *p, hipStream_t st) {
   BQD2_HIP(hipEventRecord(e0, st));
   if ((rc = bqd2_jacobi(na, p->A, na, nullptr, p->dinv, st))) return rc;
   if ((rc = bqd2_lu32_convert(na, p->A, na, nullptr, p->dinv, p->Af, st))) return rc;
-  if ((rc = bqd2_cgetrf(na, p->Af, p->ipiv, p->luws, st))) return rc;
+  if ((rc = bqd2_cgetrf_bound(na, p->Af, p->ipiv, p->luws, st, bound, &p->lu32_growth))) return rc;
   BQD2_HIP(hipEventRecord(e1, st));
   BQD2_HIP(hipEventSynchronize(e1));
   float ms = 0.0f;
@@ -492,10 +492,21 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
     // complex float LU of the scaled system, then GMRES on the FP64 system
     // right-preconditioned by it; plain GMRES and the FP64 LU remain as fallbacks
     if ((rc = build_system(p, 1, st))) return rc;
-    if (!p->lu32_valid && (rc = factor_lu32(p, st))) return rc;
+    if (!p->lu32_valid) {
+      // block-local pivots unchecked; a zero pivot in a diagonal block: the stable bound
+      rc = factor_lu32(p, st, BQD2_LU32_RELAXED_GROWTH);
+      if (rc == BQD2_E_SINGULAR) rc = factor_lu32(p, st, BQD2_LU32_STABLE_GROWTH);
+      if (rc) return rc;
+    }
     Precond pc{p->Af, p->luws, p->lutmp, &p->lu_graph};
     rc = bqd2_gmres(&p->main, na, 1, p->A, na, nullptr, p->b, na, p->xg, na, GM_TOL, IR_RESTART, IR_MAXIT,
                     &p->info_its[0], &p->info_res[0], &pc);
+    if (rc == 1 && p->lu32_growth > BQD2_LU32_STABLE_GROWTH) {
+      // the relaxed pivots made too weak a preconditioner: factor again with the stable bound
+      if ((rc = factor_lu32(p, st, BQD2_LU32_STABLE_GROWTH))) return rc;
+      rc = bqd2_gmres(&p->main, na, 1, p->A, na, nullptr, p->b, na, p->xg, na, GM_TOL, IR_RESTART, IR_MAXIT,
+                      &p->info_its[0], &p->info_res[0], &pc);
+    }
     if (rc < 0) return rc;
     if (rc == 0) x = p->xg;
   }
@@ -529,7 +540,7 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
 
 // complex float factors of domain D's row-scaled G (identity on unused rows),
 // once per assembly: the preconditioner of its dpv systems in the mixed solver
-static int factor_g32(bqd2_problem *p, DevDomain &D, hipStream_t st) {
+static int factor_g32(bqd2_problem *p, DevDomain &D, hipStream_t st, float bound) {
   const size_t N = D.N;
   int rc;
   if (!D.gAf) {
@@ -541,7 +552,7 @@ static int factor_g32(bqd2_problem *p, DevDomain &D, hipStream_t st) {
   }
   if ((rc = bqd2_jacobi(D.N, D.G, D.N, D.rowmap, D.gdinv, st))) return rc;
   if ((rc = bqd2_lu32_convert(D.N, D.G, D.N, D.rowmap, D.gdinv, D.gAf, st))) return rc;
-  if ((rc = bqd2_cgetrf(D.N, D.gAf, D.gipiv, D.gws, st))) return rc;
+  if ((rc = bqd2_cgetrf_bound(D.N, D.gAf, D.gipiv, D.gws, st, bound, &D.g_growth))) return rc;
   D.g_lu_valid = true;
   return BQD2_OK;
 }
@@ -615,10 +626,19 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
       rc = 1;
       if (p->solver == 2) {
         // complex float factors of G (once per assembly) as the preconditioner
-        if (!D.g_lu_valid && (rc = factor_g32(p, D, st))) return rc;
+        if (!D.g_lu_valid) {
+          rc = factor_g32(p, D, st, BQD2_LU32_RELAXED_GROWTH);
+          if (rc == BQD2_E_SINGULAR) rc = factor_g32(p, D, st, BQD2_LU32_STABLE_GROWTH);
+          if (rc) return rc;
+        }
         Precond pc{D.gAf, D.gws, D.gtmp, &D.g_graph};
         rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
                         &p->info_its[1 + d], &p->info_res[1 + d], &pc);
+        if (rc == 1 && D.g_growth > BQD2_LU32_STABLE_GROWTH) {
+          if ((rc = factor_g32(p, D, st, BQD2_LU32_STABLE_GROWTH))) return rc;
+          rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
+                          &p->info_its[1 + d], &p->info_res[1 + d], &pc);
+        }
         if (rc < 0) return rc;
       }
       if (rc == 1)

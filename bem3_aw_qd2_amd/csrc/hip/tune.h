@@ -11,7 +11,7 @@
 #include <hip/hip_runtime.h>
 
 enum TuneSite {
-  TUNE_CGEMM = 0,  // trailing update of the complex float LU: tile 64 (0) or 128 (1)
+  TUNE_CGEMM = 0,  // trailing update of the complex float LU: 3M tiles 64 x 64 (0) or 128 x 64 (1)
   TUNE_GMV = 1,    // GMRES matrix-vector product: 4 (0), 2 (1), 8 (2) or 1 (3) rows per workgroup
   TUNE_NSITES = 2
 };

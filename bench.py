@@ -55,6 +55,15 @@ METRIC = "solves/sec + p50/p99 latency at <=1e-8 rel force error; GF evals/sec"
 # formulas of d3_qpgf_d2.c (:1814-1838 spatial, :1965-2009 spectral) with add/mul 1,
 # FMA 2, sqrt/div 8, exp 20, sincos 30, erfc 40, w(z) 120
 F_TERM = {"spatial": 256, "propagating": 288, "evanescent": 212, "pure_spectral": 96}
+# the same for a term of the Hessian evaluation of the field (:2131-2170,
+# :2349-2398): the value-and-gradient count plus the Hessian's own operations in
+# the same convention -- spatial: the second radial factor (16), its combination
+# with the first (6), six components (28), six more accumulations (12), 62;
+# propagating: d2g_zz (12), five products with the order's wave vector (10),
+# accumulations (12), 34; evanescent: the exp of d2g_zz (24), its combination
+# (8), five products (10), accumulations (12), 54; pure spectral: components and
+# accumulations, 32
+F_TERM_V2 = {"spatial": 318, "propagating": 322, "evanescent": 266, "pure_spectral": 128}
 # flops per GH64 point of a split pair: free-space kernel and gradient (sincos 30,
 # 18 mul/add/FMA) and its accumulation into the four shape functions (42)
 F_FREE = 90
@@ -350,6 +359,11 @@ def lu32_leg(hip, prob, na):
     flops = 8.0 / 3.0 * na ** 3
     x0, x2 = out[0]["x"], out[2]["x"]
     return {"unknowns": na, "factor_ms": ms.value, "factor_tflops_8n3_3": flops / (ms.value * 1e-3) / 1e12,
+            "factor_mfma_tflops_6n3_3": 0.75 * flops / (ms.value * 1e-3) / 1e12,
+            "trailing_update": "3 real products per complex product (3M): the nominal 8n^3/3 count is 4/3 of the "
+                               "MFMA work",
+            "panels": "block-local pivots, unchecked (the refinement validates the factors), next panel on 8 "
+                      "CUs of its own beside the update",
             "peak_fp32_mfma_tflops": 157.3, "refinement_steps": out[2]["its"], "scaled_residual": out[2]["res"],
             "solve_ms_with_factor": out[2]["t"] * 1e3, "solve_ms_factor_reused": t_reuse * 1e3,
             "fp64_gmres_steps": out[0]["its"], "fp64_gmres_solve_ms": out[0]["t"] * 1e3,
@@ -357,7 +371,7 @@ def lu32_leg(hip, prob, na):
             "force_rel_diff_vs_fp64": float(np.linalg.norm(out[2]["F"] - out[0]["F"]) / np.linalg.norm(out[0]["F"]))}
 
 
-AUTOTUNE_SITES = {0: ("lu32_trailing_update", ["tile64", "tile128"]),
+AUTOTUNE_SITES = {0: ("lu32_trailing_update", ["3m_64x64", "3m_128x64"]),
                   1: ("gmres_matvec", ["rows4", "rows2", "rows8", "rows1"])}
 
 
@@ -693,6 +707,19 @@ def main():
             "evals_per_launch": int(st0.pair_evals), "split_pairs": int(st0.pairs_split),
             "gh_pairs": int(st0.pairs_gh), "flops_per_term": F_TERM, "flops_per_free_point": F_FREE}
 
+    # the field phase: Hessian evaluations of the field points (k_field_part, the
+    # phase's dominant kernel; the phase time bounds its duration from above)
+    fc = req.field_counts
+    kinds = ("spatial", "propagating", "evanescent", "pure_spectral")
+    field_flops = float(sum(int(fc[11 + i]) * F_TERM_V2[k] for i, k in enumerate(kinds)))
+    field_s = float(np.mean([p["field"] for p in phases]))
+    roof_field = {"bound": "valu", "kernel": "k_field_part (whole field phase timed)",
+                  "achieved": field_flops / field_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                  "frac": field_flops / field_s / 1e12 / PEAK_FP64_TFLOPS, "ms_per_launch": field_s * 1e3,
+                  "flops_per_launch": field_flops,
+                  "terms_per_launch": {k: int(fc[11 + i]) for i, k in enumerate(kinds)},
+                  "evals_per_launch": int(fc[15]), "flops_per_term": F_TERM_V2}
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: sampling the oracle")
@@ -730,6 +757,7 @@ def main():
         "force": {"type0": req.F[0].tolist(), "type1": req.F[1].tolist()},
         "torque": {"type0": req.N[0].tolist(), "type1": req.N[1].tolist()},
         "roofline": roof,
+        "roofline_field": roof_field,
         "cpu_baseline": cpu,
     }
     if not args.no_lu32:

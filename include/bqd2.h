@@ -244,10 +244,28 @@ int bqd2_zgetrs(int n, int nrhs, const double *d_A, const int *d_ipiv, double *d
  * n x nrhs complex floats. */
 size_t bqd2_cgetrf_wsize(int n);
 int bqd2_cgetrf(int n, float *d_A, int *d_ipiv, void *d_ws, void *stream);
+/* Pivoting of bqd2_cgetrf's panels (process-wide; no counterpart in the
+ * reference): 1 (default) restricts a panel's pivots to the rows of its
+ * 128 x 128 diagonal block and keeps that when max |L21| is within the growth
+ * bound, else redoes the panel with pivots over all its rows; 0 always pivots
+ * over all rows.  Counts of panels factored each way since the process
+ * started: out[0] block-local, out[1] all rows. */
+int bqd2_set_lu32_pivoting(int mode);
+int bqd2_lu32_panel_counts(long long *out);
+/* No counterpart in the reference.  The growth bound of block-local panels
+ * (max |L21|, default 16, at least 1; process-wide; 1e30 or more: unchecked,
+ * the form the mixed-precision solver uses, which reports a zero pivot inside
+ * a diagonal block as BQD2_E_SINGULAR), and the growth of each checked panel
+ * of the most recent checked factorisation (inf for a zero pivot in the
+ * block): up to cap values into out, returns their number. */
+int bqd2_set_lu32_growth_bound(double bound);
+int bqd2_lu32_panel_growth(float *out, int cap);
 int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B, float *d_tmp, void *stream);
 /* The trailing-update kernel of bqd2_cgetrf, exposed for tests: C -= A B
- * (set = 0) or C = A B (set = 1), column-major complex float, A m x kk,
- * B kk x nc, C m x nc. */
+ * (set = 0, three real products per complex product, as the trailing update)
+ * or C = A B (set = 1, four real products; B may be C when m <= 128, A may be
+ * C when nc <= 128), column-major complex float, A m x kk, B kk x nc,
+ * C m x nc. */
 int bqd2_cgemm(int set, int m, int nc, int kk, const float *d_A, long long lda, const float *d_B, long long ldb,
                float *d_C, long long ldc, void *stream);
 /* No counterpart in the reference.  The mixed-precision solves replay their

@@ -12,7 +12,9 @@ numbers below ~1e3), sizes inside one panel (n = 50), a multiple of the
 (n = 3072), 1 and 3 right-hand sides, plus 16 right-hand sides (MFMA solves).
 Bars: backward error |A x - b| / (|A| |x|) below 2e-6 (complex float, n <= 3072)
 and the solution within 1e-4 of LAPACK's FP64 solution; a matrix with a zero
-column is reported singular."""
+column is reported singular.  Both panel pivotings: block-local (the default,
+kept while max |L21| <= 16) and over all rows; the matrix that needs a pivot
+from outside the first diagonal block makes the block-local panel fall back."""
 import ctypes as C
 
 import numpy as np
@@ -132,3 +134,84 @@ def test_lu32_reports_singular():
     msg = hip_lib().bqd2_last_error()
     print(msg)
     assert b"zero pivot" in msg
+
+
+def panel_counts():
+    c = np.zeros(2, np.int64)
+    assert hip_lib().bqd2_lu32_panel_counts(c.ctypes.data) == 0
+    return c
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_lu32_pivoting_modes(mode):
+    hip = hip_lib()
+    n, nrhs = 1000, 3
+    A = matrix(n, 21)
+    rng = np.random.default_rng(8)
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    c0 = panel_counts()
+    assert hip.bqd2_set_lu32_pivoting(mode) == 0
+    try:
+        rc, X = solve_gpu(A, B)
+    finally:
+        hip.bqd2_set_lu32_pivoting(1)
+    c1 = panel_counts()
+    assert rc == 0
+    res = np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X))
+    print("mode", mode, "panels (block-local, all rows):", c1 - c0, "res %.2e" % res)
+    assert res < 2e-6
+    npan = (n + 127) // 128
+    assert (c1 - c0).tolist() == ([0, npan] if mode == 0 else [npan, 0])
+
+
+def test_lu32_local_panel_falls_back():
+    """the pivot of column 0 lies outside the first diagonal block: that panel's
+    block-local factorisation grows beyond the bound and is redone over all rows"""
+    n = 300
+    A = matrix(n, 11)
+    A[:, 0] *= 1e-6
+    A[137, 0] = 40.0
+    c0 = panel_counts()
+    rc, X = solve_gpu(A, np.ones((n, 1), complex))
+    c1 = panel_counts()
+    assert rc == 0
+    assert (c1 - c0)[1] >= 1
+    res = np.linalg.norm(A @ X - 1.0) / (np.linalg.norm(A) * np.linalg.norm(X))
+    assert res < 2e-6
+
+
+def test_lu32_unchecked_block_local_matches_checked():
+    """growth bound >= 1e30: the block-local panels go unchecked and the panels
+    run beside the trailing updates on their own CUs; on a matrix whose panels
+    all stay block-local under the default bound, the factors (and solution)
+    are bit-identical to the checked path's"""
+    hip = hip_lib()
+    n, nrhs = 1000, 3
+    A = matrix(n, 21)
+    rng = np.random.default_rng(8)
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    c0 = panel_counts()
+    rc, X1 = solve_gpu(A, B)
+    assert rc == 0
+    assert (panel_counts() - c0)[1] == 0
+    assert hip.bqd2_set_lu32_growth_bound(C.c_double(1e30)) == 0
+    try:
+        rc, X2 = solve_gpu(A, B)
+    finally:
+        hip.bqd2_set_lu32_growth_bound(C.c_double(16.0))
+    assert rc == 0
+    assert np.array_equal(X1, X2)
+
+
+def test_lu32_unchecked_reports_block_zero_pivot():
+    hip = hip_lib()
+    A = matrix(200, 3)
+    A[:, 77] = 0.0
+    assert hip.bqd2_set_lu32_growth_bound(C.c_double(1e30)) == 0
+    try:
+        rc, _ = solve_gpu(A, np.ones((200, 1), complex))
+    finally:
+        hip.bqd2_set_lu32_growth_bound(C.c_double(16.0))
+    assert rc != 0
+    assert b"diagonal block" in hip_lib().bqd2_last_error()
+    assert hip.bqd2_set_lu32_growth_bound(C.c_double(0.5)) != 0
