@@ -349,6 +349,17 @@ def lu32_leg(hip, prob, na):
         lu = C.c_int()
         check(hip.bqd2_solver_info(prob, its, res, C.byref(lu)), "bqd2_solver_info")
         out[mode] = dict(x=xs, F=np.array(F[:]), t=t1 - t0, its=its[0], res=res[0])
+    # the FP64 direct solve (the dense LU on the MFMA units replacing the MKL
+    # solve): system build, factorisation and the two triangular solves
+    check(hip.bqd2_set_solver(prob, 1), "bqd2_set_solver")
+    x1 = np.zeros(na, np.complex128)
+    t0 = time.perf_counter()
+    check(hip.bqd2_solve_surface(prob, x1.ctypes.data), "bqd2_solve_surface")
+    t_lu64 = time.perf_counter() - t0
+    lu64 = {"solve_ms": t_lu64 * 1e3, "tflops_8n3_3_over_solve": 8.0 / 3.0 * na ** 3 / t_lu64 / 1e12,
+            "peak_fp64_mfma_tflops": 78.6, "pivoting": "partial over all rows (LAPACK's interchanges)",
+            "solution_rel_diff_vs_gmres": float(np.linalg.norm(x1 - out[0]["x"]) / np.linalg.norm(out[0]["x"]))}
+    check(hip.bqd2_set_solver(prob, 2), "bqd2_set_solver")
     # a second incidence on the same operator: the factors are reused
     t0 = time.perf_counter()
     check(hip.bqd2_solve_surface(prob, None), "bqd2_solve_surface")
@@ -368,6 +379,7 @@ def lu32_leg(hip, prob, na):
             "solve_ms_with_factor": out[2]["t"] * 1e3, "solve_ms_factor_reused": t_reuse * 1e3,
             "fp64_gmres_steps": out[0]["its"], "fp64_gmres_solve_ms": out[0]["t"] * 1e3,
             "solution_rel_diff_vs_fp64": float(np.linalg.norm(x2 - x0) / np.linalg.norm(x0)),
+            "fp64_lu": lu64,
             "force_rel_diff_vs_fp64": float(np.linalg.norm(out[2]["F"] - out[0]["F"]) / np.linalg.norm(out[0]["F"]))}
 
 

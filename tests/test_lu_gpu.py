@@ -3,7 +3,8 @@ buffers, against numpy's LAPACK on the same seeded matrices.
 
 Sizes cover a single partial block (n = 50), a multiple of the 64-column panel
 (n = 512), a ragged last panel (n = 1000) and the system size of the sphere384
-workload (n = 3072), with 1 and 3 right-hand sides.  Bar: the solution within
+workload (n = 3072), with 1 and 3 right-hand sides; on an unweighted random
+matrix the interchanges equal LAPACK's and the factors scipy's.  Bar: the solution within
 1e-11 normwise of LAPACK's (both are backward stable; the matrices are
 diagonally weighted so that their condition numbers stay below ~1e3), and the
 residual |A x - b| / (|A| |x|) below 1e-14.  A matrix with a zero column is
@@ -65,3 +66,24 @@ def test_lu_reports_singular():
     msg = hip_lib().bqd2_last_error()
     print(msg)
     assert b"zero pivot" in msg
+
+
+def test_lu_interchanges_match_lapack():
+    """a matrix without diagonal weight, so that nearly every column pivots:
+    the interchanges equal LAPACK's zgetrf's (max |Re|+|Im|, first of ties) and
+    the factors agree with scipy's to rounding"""
+    import scipy.linalg
+    import torch
+    n = 700
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    dA = torch.tensor(np.ascontiguousarray(A.T).view(np.float64), device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    assert hip_lib().bqd2_zgetrf(n, C.c_void_p(dA.data_ptr()), C.c_void_p(ipiv.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    LU = dA.cpu().numpy().view(np.complex128).reshape(n, n).T
+    lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+    assert np.array_equal(ipiv.cpu().numpy(), piv_ref)
+    err = np.abs(LU - lu_ref).max() / np.abs(lu_ref).max()
+    print("interchanges equal; max factor difference %.2e" % err)
+    assert err < 1e-12
