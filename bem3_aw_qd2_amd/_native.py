@@ -116,6 +116,8 @@ def host_lib():
         lib.pv_t_bd_aqd2.restype = None
         lib.force_FN_aqd2.restype = None
         lib.surface_area_aqd2.restype = None
+        lib.aqd2_point_batches.argtypes = [C.c_void_p, C.c_void_p]
+        lib.aqd2_point_batches.restype = None
         _host = lib
     return _host
 

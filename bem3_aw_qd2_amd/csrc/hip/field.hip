@@ -2,8 +2,8 @@
 // radiation force and torque (bilinear and linear-triangle elements).
 //
 //   k_domain_id   point classification by signed solid angles (domain_id,
-//                 bem3_aw_qd2.c:1148-1257), one lane per point, the reference's
-//                 sequential order and early exits
+//                 bem3_aw_qd2.c:1148-1257), one wavefront per point, the
+//                 reference's domain order and early exit
 //   k_field_part  partial sums of the representation formula and its gradient
 //                 (pv_s_dbieq_aqd2, bem3_aw_qd2_field.c:227-297 with
 //                 q0_bil_dcoef_grad_* / bil_dcoef_grad_*) over chunks of a
@@ -103,12 +103,20 @@ __device__ double solid_angle(bool quad, double r[4][3], int &flag) {
   return om;
 }
 
-// corner positions come from the mesh nodes: rn (Nn+1) x 3 and ed (Ne+1) x 4
+// corner positions come from the mesh nodes: rn (Nn+1) x 3 and ed (Ne+1) x 4.
+// One wavefront per point (four per workgroup): the lanes take a domain's
+// elements in turn and keep their shares of the signed solid angles in LDS
+// (the element's domain and the domain on its other side, any of MN + 1);
+// the domain's total is the wavefront's sum.  A point on an element (solid
+// angle undefined) belongs to the domain being summed, as in the reference's
+// loop, which stops at the first such element: which one does not matter.
 __global__ void __launch_bounds__(256) k_domain_id(DevMesh M, FieldCtx X, const double *__restrict__ rn,
                                                    const int *__restrict__ ed, int n, const double *__restrict__ pts,
                                                    int *__restrict__ did_out, double *__restrict__ r0_out,
                                                    int *__restrict__ l_out) {
-  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sOg[4][16][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ip = blockIdx.x * 4 + wv;
   if (ip >= n) return;
   double rt[3] = {pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2]};
   int l1 = 0, l2 = 0;
@@ -118,15 +126,17 @@ __global__ void __launch_bounds__(256) k_domain_id(DevMesh M, FieldCtx X, const 
     rt[0] = rt[0] - (double)l1 * X.d1x - (double)l2 * X.d2x;
     rt[1] = rt[1] - (double)l1 * X.d1y - (double)l2 * X.d2y;
   }
-  r0_out[3 * ip] = rt[0]; r0_out[3 * ip + 1] = rt[1]; r0_out[3 * ip + 2] = rt[2];
-  l_out[2 * ip] = l1; l_out[2 * ip + 1] = l2;
-  double Og[16];
-  for (int d = 0; d <= X.MN; d++) Og[d] = 0.0;
+  if (lane == 0) {
+    r0_out[3 * ip] = rt[0]; r0_out[3 * ip + 1] = rt[1]; r0_out[3 * ip + 2] = rt[2];
+    l_out[2 * ip] = l1; l_out[2 * ip + 1] = l2;
+  }
+  double *Og = &sOg[wv][0][lane];  // Og[d] at Og[64 d]
+  for (int d = 0; d <= X.MN; d++) Og[64 * d] = 0.0;
   int result = -1;
   for (int d = 0; d <= X.MN && result < 0; d++) {
     const DomView &V = X.dom[d];
     bool hit = false;
-    for (int i = 1; i <= V.Ne; i++) {
+    for (int i = 1 + lane; i <= V.Ne && !hit; i += 64) {
       int s = V.sid[i];
       if (s <= 0) continue;
       double rv[4][3];
@@ -135,14 +145,15 @@ __global__ void __launch_bounds__(256) k_domain_id(DevMesh M, FieldCtx X, const 
       int flag;
       double om = solid_angle(!M.tri[s], rv, flag);
       if (flag < 0) { hit = true; break; }
-      Og[d] += om;
-      Og[M.sd[s]] -= om;
+      Og[64 * d] += om;
+      Og[64 * M.sd[s]] -= om;
     }
-    if (hit) { result = d; break; }
-    if (d == 0 && fabs(Og[d]) < 2.0 * M_PI) result = d;
-    else if (Og[d] > 2.0 * M_PI) result = d;
+    if (__any(hit)) { result = d; break; }
+    const double tot = wave_sum(Og[64 * d]);
+    if (d == 0 && fabs(tot) < 2.0 * M_PI) result = d;
+    else if (tot > 2.0 * M_PI) result = d;
   }
-  did_out[ip] = result;
+  if (lane == 0) did_out[ip] = result;
 }
 
 // ------------------------------------------------------------ source points
@@ -212,7 +223,10 @@ __host__ __device__ __forceinline__ long long field_chunk_items(long long items)
   return 64 * (per > 0 ? per : 1);
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_field_part(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
+#ifndef BQD2_FIELD_WPE
+#define BQD2_FIELD_WPE 2
+#endif
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FIELD_WPE, BQD2_FIELD_WPE))) k_field_part(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
                                                    int type, int n, int nchunk, const double *__restrict__ r0s,
                                                    const int *__restrict__ did_io,
                                                    double *__restrict__ part, long long *__restrict__ counters) {
@@ -306,7 +320,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         asm volatile("" ::: "memory");
         const double x0 = scf[19][lane], x1 = scf[20][lane], x2 = scf[21][lane];
         Sum10 gf;
-        int c = qpgf_eval<GF_V2>(P, x0, x1, x2, 1.0e-15, gf, tl, part);
+        // part 1: the image share or the whole pure spectral sum; part 2: the spectral share
+        int c = part == 1 ? qpgf_eval<GF_V2, 5>(P, x0, x1, x2, 1.0e-15, gf, tl)
+                          : qpgf_eval<GF_V2, 2>(P, x0, x1, x2, 1.0e-15, gf, tl);
         if (c < 0) err = c;
         asm volatile("" ::: "memory");
         const double W0 = scf[16][lane], W1 = scf[17][lane], W2 = scf[18][lane];
@@ -1009,7 +1025,7 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
   BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
-  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
+  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
@@ -1089,7 +1105,7 @@ extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, i
   int *d_l = (int *)(base + o_l);
   FieldCtx X = make_ctx(p);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, ctr);
-  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, stream, p->m, X, p->rn, p->ed, n, d_pts,
+  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, stream, p->m, X, p->rn, p->ed, n, d_pts,
                      d_did, d_r0, d_l);
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
@@ -1224,7 +1240,7 @@ extern "C" int bqd2_domain_id(bqd2_problem *p, int n, const double *pts, int *di
     return rc;
   BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
-  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
+  hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipMemcpyAsync(did, d_did, (size_t)n * 4, hipMemcpyDeviceToHost, st));

@@ -540,11 +540,10 @@ __device__ __forceinline__ void sum_zero(Sum10& s) {
 
 // Ewald evaluation in the unit cell (d3_qpgf_d2.c:1671-1712, :2011-2052).
 // parts selects the shares summed (1: images, 2: spectral orders, 3: both):
-// the result is linear in the two, so a caller may take them in two calls and
-// hold neither across the other's sum.
-template <int M>
-__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl,
-                          int parts = 3) {
+// the result is linear in the two, so a caller may take them in two calls (two
+// kernels) and hold neither across the other's sum.
+template <int M, int parts = 3>
+__device__ int ewald_cell(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
   double E = P.E0;
   double r2 = rx * rx + ry * ry + rz * rz;
   double k2 = P.k * P.k;
@@ -680,25 +679,36 @@ __device__ int fd_cell(const GfParams& P, double rx, double ry, double rz, doubl
   return cc;
 }
 
+// whether the evaluation at offset z from the lattice plane takes the pure
+// spectral sum (its truncation error estimate below eps; d3_qpgf_d2.c's choice)
+__device__ __forceinline__ bool qpgf_uses_fd(const GfParams& P, double rz, double eps) {
+  if (P.ewald_only) return false;
+  const double zs = fabs(rz) / P.fd_lift;
+  return (P.bN / P.bNp * exp(-P.bNp * zs) < eps) && (P.bN / P.bNm * exp(-P.bNm * zs) < eps);
+}
+
 // full evaluation at offset r (source minus target); returns the term count or
-// the reference's negative code (-1x pure spectral, -2x Ewald).  parts: the
-// shares of the Ewald split to sum (ewald_cell); a pure spectral evaluation is
-// all taken with the image share (parts & 1) and is zero for parts == 2.
-template <int M>
-__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl,
-                         int parts = 3) {
-  bool use_fd = false;
-  if (!P.ewald_only) {
-    const double zs = fabs(rz) / P.fd_lift;
-    use_fd = (P.bN / P.bNp * exp(-P.bNp * zs) < eps) && (P.bN / P.bNm * exp(-P.bNm * zs) < eps);
-  }
+// the reference's negative code (-1x pure spectral, -2x Ewald).  PARTS: the
+// shares to sum, 1 Ewald images, 2 Ewald spectral orders, 4 pure spectral sum
+// (the other kind of evaluation then returns zero); the result is linear in
+// them, so a caller may take them in separate kernels, each of which holds
+// only its own sum's registers.
+template <int M, int PARTS = 7>
+__device__ int qpgf_eval(const GfParams& P, double rx, double ry, double rz, double eps, Sum10& out, GfTally& tl) {
+  const bool use_fd = qpgf_uses_fd(P, rz, eps);
   int l1 = (int)floor((rx * P.d2y - ry * P.d2x) * P.idd + 0.5);
   int l2 = (int)floor(-(rx * P.d1y - ry * P.d1x) * P.idd + 0.5);
   double tx = rx - l1 * P.d1x - l2 * P.d2x;
   double ty = ry - l1 * P.d1y - l2 * P.d2y;
   sum_zero(out);
-  if (use_fd && !(parts & 1)) return 0;
-  int err = use_fd ? fd_cell<M>(P, tx, ty, rz, eps, out, tl) : ewald_cell<M>(P, tx, ty, rz, eps, out, tl, parts);
+  int err;
+  if (use_fd) {
+    if (!(PARTS & 4)) return 0;
+    err = fd_cell<M>(P, tx, ty, rz, eps, out, tl);
+  } else {
+    if (!(PARTS & 3)) return 0;
+    err = ewald_cell<M, PARTS & 3>(P, tx, ty, rz, eps, out, tl);
+  }
   if (l1 != 0 || l2 != 0) {
     double arg = -(P.vkx * (l1 * P.d1x + l2 * P.d2x) + P.vky * (l1 * P.d1y + l2 * P.d2y));
     cd cf = cis(arg);

@@ -2,13 +2,18 @@
  * field.c -- the reference's field API (bem3_aw_qd2.h:49-96) on the GPU.
  *
  * Each call evaluates through libbqd2_hip.so (bqd2_field, bqd2_field_bd,
- * bqd2_domain_id); pv_batch_aqd2 is the batched entry point for grids.  Calls
- * from several host threads (example2.c drives pv_t_aqd2 from an OpenMP loop)
- * run concurrently: the device library gives each calling thread its own
- * stream and workspace.  One lock only guards the lazy upload of a solution
- * read from ex.dat.
+ * bqd2_domain_id); pv_batch_aqd2 is the batched entry point for grids.  The
+ * single-point calls (pv_t_aqd2 and the like, which example2.c:28-31 drives
+ * from an OpenMP loop) are coalesced: concurrent callers' points join one
+ * batch, evaluated by one bqd2_field call (a point's result does not depend
+ * on the batch it travels in).  Batches of different threads may run at once:
+ * the device library gives each calling thread its own stream and workspace.
+ * One lock guards the lazy upload of a solution read from ex.dat.
  */
+#include <errno.h>
 #include <pthread.h>
+#include <string.h>
+#include <time.h>
 #include "host_internal.h"
 #include "bqd2.h"
 
@@ -49,15 +54,106 @@ void pv_batch_aqd2(int n, const double *rt, int type, int total, double complex 
   free(out);
 }
 
+/* ---------------------------------------------------------------- coalescing
+ * The first caller of a batch leads it: it waits until as many points have
+ * joined as there were callers inside at the recent closes (the threads
+ * calling in a loop), PB_CAP points or PB_WAIT_US microseconds, closes the
+ * batch, evaluates it and wakes the others, who copy their results.  A lone
+ * caller leads a batch of one without waiting. */
+#define PB_CAP 512
+#define PB_WAIT_US 2000
+typedef struct PtBatch {
+  AQD2 *qd;
+  int type, total;
+  int n, done, users;
+  double rt[3 * PB_CAP];
+  double out[8 * PB_CAP];
+  int did[PB_CAP];
+} PtBatch;
+
+static pthread_mutex_t g_pb_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t g_pb_cv = PTHREAD_COND_INITIALIZER;
+static PtBatch *g_open;  /* the batch accepting points */
+static int g_inside;     /* callers inside one_point */
+static int g_expect = 1; /* points the leader waits for: the most callers inside at the last closes */
+static int g_recent[4];
+static long long g_batches, g_points;
+
+/* at a close: the callers inside then (this batch's and those of batches in
+ * flight, who come back for their next point) set the next leaders' wait */
+static void pb_closed(int n)
+{
+  memmove(g_recent + 1, g_recent, 3 * sizeof(int));
+  g_recent[0] = g_inside;
+  int m = 1;
+  for (int i = 0; i < 4; i++) m = g_recent[i] > m ? g_recent[i] : m;
+  g_expect = m < PB_CAP ? m : PB_CAP;
+  g_batches++;
+  g_points += n;
+}
+
 static int one_point(double complex *p, double complex *pv, double *rt, int type, int total, AQD2 *qd)
 {
-  double complex tp, tv[3];
-  int did;
-  pv_batch_aqd2(1, rt, type, total, &tp, tv, &did, qd);
-  if (p) *p = tp;
+  pthread_mutex_lock(&g_pb_mu);
+  g_inside++;
+  PtBatch *b = g_open;
+  int lead = 0;
+  if (!b || b->qd != qd || b->type != type || b->total != total || b->n == PB_CAP) {
+    b = calloc(1, sizeof(*b));
+    if (!b) {
+      pthread_mutex_unlock(&g_pb_mu);
+      fail("point batch allocation", -1);
+    }
+    b->qd = qd;
+    b->type = type;
+    b->total = total;
+    g_open = b;
+    lead = 1;
+  }
+  const int i = b->n++;
+  b->users++;
+  memcpy(b->rt + 3 * i, rt, 3 * sizeof(double));
+  if (lead) {
+    struct timespec dl;
+    clock_gettime(CLOCK_REALTIME, &dl);
+    dl.tv_nsec += PB_WAIT_US * 1000L;
+    dl.tv_sec += dl.tv_nsec / 1000000000L;
+    dl.tv_nsec %= 1000000000L;
+    while (b->n < g_expect && b->n < PB_CAP)
+      if (pthread_cond_timedwait(&g_pb_cv, &g_pb_mu, &dl) == ETIMEDOUT) break;
+    if (g_open == b) g_open = NULL;
+    const int n = b->n;
+    pb_closed(n);
+    pthread_mutex_unlock(&g_pb_mu);
+    struct bqd2_problem *pr = ready(qd);
+    int rc = bqd2_field(pr, n, b->rt, type, total, b->out, b->did, NULL);
+    if (rc) fail("bqd2_field", rc);
+    pthread_mutex_lock(&g_pb_mu);
+    b->done = 1;
+    pthread_cond_broadcast(&g_pb_cv);
+  } else {
+    pthread_cond_broadcast(&g_pb_cv); /* the leader counts the points */
+    while (!b->done) pthread_cond_wait(&g_pb_cv, &g_pb_mu);
+  }
+  const double *o = b->out + 8 * i;
+  if (p) *p = o[0] + I * o[1];
   if (pv)
-    for (int c = 0; c < 3; c++) pv[c] = tv[c];
+    for (int c = 0; c < 3; c++) pv[c] = o[2 + 2 * c] + I * o[3 + 2 * c];
+  const int did = b->did[i];
+  if (--b->users == 0) free(b);
+  g_inside--;
+  pthread_mutex_unlock(&g_pb_mu);
   return did;
+}
+
+/* No counterpart in the reference: batches and points of the coalesced
+ * single-point calls since the library was loaded. */
+void aqd2_point_batches(long long *batches, long long *points)
+{
+  pthread_mutex_lock(&g_pb_mu);
+  *batches = g_batches;
+  *points = g_points;
+  pthread_mutex_unlock(&g_pb_mu);
 }
 
 int pv_s_aqd2(double complex *p, double complex *pv, double *rt, int type, AQD2 *qd)

@@ -16,6 +16,8 @@ void aqd2_push_solution(AQD2 *qd, struct bqd2_problem *p);
 /* the device solution of every domain into the AQD2 arrays (exported for tools
  * that drive the device problem directly and then read the solution) */
 void aqd2_pull_solution(AQD2 *qd, struct bqd2_problem *p);
+/* batches and points of the coalesced single-point field calls (field.c) */
+void aqd2_point_batches(long long *batches, long long *points);
 void aqd2_mark_solution(AQD2 *qd, int on);
 void aqd2_release_device(AQD2 *qd);
 

@@ -84,7 +84,10 @@ def test_solver_driver_writes_the_library_solution(tmp_path):
 
 
 def test_example2_threads_match_batched_field(tmp_path):
+    import time
+    t0 = time.perf_counter()
     run([driver("example2"), SAMPLE], str(tmp_path), timeout=170, threads=8)
+    print("example2, 40 000 points from 8 threads: %.2f s" % (time.perf_counter() - t0))
     rows = [l.split() for l in open(str(tmp_path / "Ip_xz.txt")) if l.strip() and not l.startswith("#")]
     data = np.array([[float(x) for x in r] for r in rows])
     assert data.shape == (200 * 200, 3)
@@ -103,3 +106,40 @@ def test_example2_threads_match_batched_field(tmp_path):
     m.finalize()
     # intensities printed to 15 digits
     assert np.allclose(data[idx, 2], np.abs(p) ** 2, rtol=1e-13, atol=0)
+
+
+def test_single_point_calls_are_coalesced():
+    """pv_t_aqd2 from 8 host threads at once: the points travel in shared
+    bqd2_field batches (fewer batches than points) and each result equals the
+    batched call's bit for bit"""
+    import ctypes as C
+    import threading
+    host = host_lib()
+    m = exdat.Model(host).dat_read(SAMPLE)
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-0.03, 0.03, (8, 30, 3))
+    b0, p0 = C.c_longlong(), C.c_longlong()
+    host.aqd2_point_batches(C.byref(b0), C.byref(p0))
+    res = [[None] * 30 for _ in range(8)]
+
+    def work(t):
+        for i in range(30):
+            res[t][i] = m.pv_t(pts[t, i], 0)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    b1, p1 = C.c_longlong(), C.c_longlong()
+    host.aqd2_point_batches(C.byref(b1), C.byref(p1))
+    p, v, did = m.pv_batch(pts.reshape(-1, 3), 0, 1)
+    m.finalize()
+    nb, npnt = b1.value - b0.value, p1.value - p0.value
+    print("%d points in %d batches" % (npnt, nb))
+    assert npnt == 240 and nb < 240
+    for t in range(8):
+        for i in range(30):
+            d, pp, vv = res[t][i]
+            k = 30 * t + i
+            assert d == did[k] and pp == p[k] and np.array_equal(np.asarray(vv), v[k])
