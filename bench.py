@@ -482,15 +482,21 @@ def batch_leg(args, hip, host):
     return out
 
 
-def serving_stream(nreq, rate, repeat, seed=0, args_mirror=0.0):
-    """A seeded stream of solve requests of mixed size (a scaled-down form of
-    the 384-6144-element serving workload): Poisson arrivals at `rate` per
-    second; each request is, with probability `repeat`, another incidence on an
-    operator already requested (a new amplitude and phase of the same plane
-    wave, or with probability `args_mirror` the mirrored direction pi - theta),
-    else a new operator: quad or
-    triangle cube-sphere of 6 n^2 quads (n in 4..8), square or triangular lattice
-    with period / radius in [2.5, 6], ka in [0.3, 4], sphere density ratio in
+# mesh sizes of the request streams: "small" 6 n^2 quads for n in 4..8 (or
+# their triangle splits); "config" 384-6144 elements: 6 n^2 quads for
+# n in {8, 12, 16, 20, 24, 28, 32}, or 12 n^2 triangles for n in {6, 9, 12, 15, 18, 22}
+STREAM_SIZES = {"small": {False: list(range(4, 9)), True: list(range(4, 9))},
+                "config": {False: [8, 12, 16, 20, 24, 28, 32], True: [6, 9, 12, 15, 18, 22]}}
+
+
+def serving_stream(nreq, rate, repeat, seed=0, args_mirror=0.0, shape="small"):
+    """A seeded stream of solve requests of mixed size: Poisson arrivals at
+    `rate` per second; each request is, with probability `repeat`, another
+    incidence on an operator already requested (a new amplitude and phase of
+    the same plane wave, or with probability `args_mirror` the mirrored
+    direction pi - theta), else a new operator: quad or triangle cube-sphere
+    (sizes of STREAM_SIZES[shape]), square or triangular lattice with
+    period / radius in [2.5, 6], ka in [0.3, 4], sphere density ratio in
     [0.8, 3], sound-speed ratio in [0.5, 2], random incidence.  Lattices with a
     diffraction order within 1e-3 of grazing (a Wood anomaly) are redrawn."""
     rng = np.random.default_rng(seed)
@@ -525,7 +531,8 @@ def serving_stream(nreq, rate, repeat, seed=0, args_mirror=0.0):
                        for i in range(-12, 13) for j in range(-12, 13))
             if near > 1e-3:
                 break
-        over = dict(n=int(rng.integers(4, 9)), d1=d1, d2=d2, f=f, theta=theta, psi=psi,
+        sizes = STREAM_SIZES[shape][tri]
+        over = dict(n=int(sizes[int(rng.integers(len(sizes)))]), d1=d1, d2=d2, f=f, theta=theta, psi=psi,
                     sphere=(rho * rng.uniform(0.8, 3.0), c * rng.uniform(0.5, 2.0)))
         ops.append(dict(over=over, tri=tri))
         reqs.append(dict(t=t, key=len(ops), over=over, tri=tri, mirrored=False))
@@ -544,7 +551,11 @@ def run_stream(reqs, models, slots, solver, keyed, budget, pts):
             if dt > 0:
                 time.sleep(dt)
             tickets.append(srv.submit(m, key=r["key"] if keyed else 0, solver=solver, pts=pts, force_type=0))
-        res = [srv.wait(t) for t in tickets]
+        res = []
+        for i, t in enumerate(tickets):
+            res.append(srv.wait(t))
+            log("  request %d of %d done at %.1f s (%d elements)" % (i + 1, len(tickets), time.perf_counter() - t0,
+                                                                     models[i].ne))
         wall = time.perf_counter() - t0 - reqs[0]["t"]
         st = srv.stats()
     for r in res:
@@ -558,7 +569,8 @@ def serving_bench(args):
     FP64 GMRES) against the serving form (resident operators, `slots` concurrent
     requests, the mixed-precision solver) on the same request stream."""
     from bem3_aw_qd2_amd import serve
-    reqs = serving_stream(args.serve_requests, args.serve_rate, args.serve_repeat, args_mirror=args.serve_mirror)
+    reqs = serving_stream(args.serve_requests, args.serve_rate, args.serve_repeat, args_mirror=args.serve_mirror,
+                          shape=args.serve_shape)
     models = [serve.PackedModel("sphere384", triangles=r["tri"], **r["over"]) for r in reqs]
     pts = grid_points(8, 1.5e-3)
     budget = int(args.serve_budget_gib * 2 ** 30)
@@ -575,7 +587,7 @@ def serving_bench(args):
     ferr_mirr = max([e for e, a, r in zip(errs, fast, reqs) if a["reused"] and r["mirrored"]] or [0.0])
     elems = [m.ne for m in models]
     return {
-        "requests": len(reqs), "operators": len(set(r["key"] for r in reqs)),
+        "requests": len(reqs), "operators": len(set(r["key"] for r in reqs)), "mesh_sizes": args.serve_shape,
         "elements_min_max": [int(min(elems)), int(max(elems))], "arrival_rate_per_s": args.serve_rate,
         "repeat_fraction": args.serve_repeat,
         "plain_fp64": {"solves_per_s": len(reqs) / wall_p, "p50_ms": float(np.percentile(lat_p, 50)),
@@ -628,6 +640,8 @@ def main():
                          "in-plane wave vector then differs from the resident operator's in the last bits, which "
                          "moves the lattice sums at points level with a source element by up to 1e-6 in the force)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-shape", default="small", choices=sorted(STREAM_SIZES),
+                    help="mesh sizes of the stream: small (96-768 elements) or config (384-6144 elements)")
     ap.add_argument("--serve-solver", type=int, default=2)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--copy-mib", type=int, default=0,
