@@ -6,7 +6,7 @@ libbem3_aw_qd2.so (the reference's bem3_aw_qd2.h C ABI on the host, calling the
 HIP library).  This package only locates, builds and binds them.
 """
 from ._native import (LIB_DIR, build, hip_lib, host_lib, QPD2, GfParams,
-                      gf_params, BQD2Error, AssemblyStats, check)
+                      gf_params, BQD2Error, AssemblyStats, LargeStats, check)
 
 __all__ = ["LIB_DIR", "build", "hip_lib", "host_lib", "QPD2", "GfParams",
-           "gf_params", "BQD2Error", "AssemblyStats", "check"]
+           "gf_params", "BQD2Error", "AssemblyStats", "LargeStats", "check"]

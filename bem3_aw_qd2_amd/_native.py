@@ -43,6 +43,16 @@ class GfParams(C.Structure):
                [("nc", C.c_int), ("ewald_only", C.c_int), ("fd_lift", C.c_double)]
 
 
+class LargeStats(C.Structure):
+    """bqd2_large_stats (include/bqd2.h)"""
+    _fields_ = [("windows", C.c_int), ("resident", C.c_int), ("recomputed", C.c_int), ("pad_", C.c_int),
+                ("ms_rows", C.c_double), ("ms_build", C.c_double), ("ms_factor", C.c_double),
+                ("ms_solve", C.c_double), ("ms_velocity_rows", C.c_double), ("ms_reassemble", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
 class AssemblyStats(C.Structure):
     _fields_ = [("pairs_gl", C.c_longlong), ("pairs_gh", C.c_longlong), ("self_rows", C.c_longlong),
                 ("gf_evals", C.c_longlong), ("gf_terms", C.c_longlong),
@@ -94,6 +104,11 @@ def hip_lib():
                                       C.c_void_p]
         lib.bqd2_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         lib.bqd2_domain_id.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        lib.bqd2_create_large.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.bqd2_large_config.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.bqd2_large_info.argtypes = [C.c_void_p, C.POINTER(LargeStats)]
+        lib.bqd2_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.bqd2_destroy.argtypes = [C.c_void_p]
         lib.bqd2_gf_params.argtypes = [C.POINTER(C.c_double), C.POINTER(GfParams)]
         lib.bqd2_gf_batch.argtypes = [C.POINTER(GfParams), C.c_int, C.c_int, C.c_void_p, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -118,6 +133,10 @@ def host_lib():
         lib.surface_area_aqd2.restype = None
         lib.aqd2_point_batches.argtypes = [C.c_void_p, C.c_void_p]
         lib.aqd2_point_batches.restype = None
+        lib.aqd2_model_new.restype = C.c_void_p
+        lib.aqd2_model_new.argtypes = [C.c_void_p]
+        lib.aqd2_model_free.argtypes = [C.c_void_p]
+        lib.aqd2_model_free.restype = None
         _host = lib
     return _host
 

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     }
   }
   D.rule[gid] = rule;
-  D.pkey[gid] = key;
+  if (D.pkey) D.pkey[gid] = key;
 }
 
 // ------------------------------------------------------------------ periodic pairs
@@ -273,13 +273,13 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
   if (on && sl == 0) {
     // the triangle rules carry 1/24 of the parts' area (q0_lit_dcoef_v2_cc.c:231-237)
     const double sc = tri ? 1.0 / 24.0 : 1.0;
-    const size_t base = ((size_t)row * D.N + (size_t)(s - 1) * 4) * 2;
+    const size_t base = dom_gh(D, row, (size_t)(s - 1) * 4) * 2;
 #pragma unroll
     for (int ep = 0; ep < 4; ep++) {
       D.G[base + 2 * ep] = g[ep].x * sc; D.G[base + 2 * ep + 1] = g[ep].y * sc;
       D.H[base + 2 * ep] = h[ep].x * sc; D.H[base + 2 * ep + 1] = h[ep].y * sc;
     }
-    double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
+    double *fp = D.Fpair + dom_fp(D, row, s);
     fp[0] = F * (BQD2_IFP * sc); fp[1] = F0 * (BQD2_IFP * sc); fp[2] = F1 * (BQD2_IFP * sc);
   }
   if (lane == 0) {
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
 
 // ------------------------------------------------------------------ free-space pairs
 __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  const int s = blockIdx.x + 1, row = blockIdx.y, lane = threadIdx.x;
+  const int s = blockIdx.x + 1, row = D.r0 + blockIdx.y, lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td) || s == t) return;
   const Tab &T = *tab;
@@ -387,19 +387,24 @@ __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restr
   F0 = wave_sum(F0) * ifp * sg;
   F1 = wave_sum(F1) * ifp * sg;
   if (lane == 0) {
-    const size_t N = D.N, col = (size_t)(s - 1) * 4;
-    const size_t base = ((size_t)row * N + col) * 2;
-    const size_t b0 = ((size_t)(2 * row) * N + col) * 2, b1 = ((size_t)(2 * row + 1) * N + col) * 2;
+    const size_t col = (size_t)(s - 1) * 4;
+    const size_t base = dom_gh(D, row, col) * 2;
+    const size_t b0 = dom_dgh(D, row, 0, col) * 2, b1 = dom_dgh(D, row, 1, col) * 2;
 #pragma unroll
     for (int ep = 0; ep < 4; ep++) {
       D.G[base + 2 * ep] = g[ep].x; D.G[base + 2 * ep + 1] = g[ep].y;
       D.H[base + 2 * ep] = h[ep].x; D.H[base + 2 * ep + 1] = h[ep].y;
-      D.dG[b0 + 2 * ep] = dg0[ep].x; D.dG[b0 + 2 * ep + 1] = dg0[ep].y;
-      D.dG[b1 + 2 * ep] = dg1[ep].x; D.dG[b1 + 2 * ep + 1] = dg1[ep].y;
-      D.dH[b0 + 2 * ep] = dh0[ep].x; D.dH[b0 + 2 * ep + 1] = dh0[ep].y;
-      D.dH[b1 + 2 * ep] = dh1[ep].x; D.dH[b1 + 2 * ep + 1] = dh1[ep].y;
     }
-    double *fp = D.Fpair + ((size_t)row * D.Ne + (s - 1)) * 3;
+    if (D.dG) {
+#pragma unroll
+      for (int ep = 0; ep < 4; ep++) {
+        D.dG[b0 + 2 * ep] = dg0[ep].x; D.dG[b0 + 2 * ep + 1] = dg0[ep].y;
+        D.dG[b1 + 2 * ep] = dg1[ep].x; D.dG[b1 + 2 * ep + 1] = dg1[ep].y;
+        D.dH[b0 + 2 * ep] = dh0[ep].x; D.dH[b0 + 2 * ep + 1] = dh0[ep].y;
+        D.dH[b1 + 2 * ep] = dh1[ep].x; D.dH[b1 + 2 * ep + 1] = dh1[ep].y;
+      }
+    }
+    double *fp = D.Fpair + dom_fp(D, row, s);
     fp[0] = F; fp[1] = F0; fp[2] = F1;
   }
 }
@@ -430,7 +435,7 @@ __device__ double self_F_wave(const ElemC &E, const Tab &T, double zt, double et
 
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
                                                   long long *__restrict__ counters) {
-  const int row = blockIdx.x, lane = threadIdx.x;
+  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const Tab &T = *tab;
@@ -538,13 +543,13 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   double Fzp = self_F_wave(E, T, zt + h_, et, lane), Fzm = self_F_wave(E, T, zt - h_, et, lane);
   double Fep = self_F_wave(E, T, zt, et + h_, lane), Fem = self_F_wave(E, T, zt, et - h_, lane);
   if (lane == 0) {
-    const size_t base = ((size_t)row * D.N + (size_t)(t - 1) * 4) * 2;
+    const size_t base = dom_gh(D, row, (size_t)(t - 1) * 4) * 2;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
       D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
       D.H[base + 2 * n] = h[n].x; D.H[base + 2 * n + 1] = h[n].y;
     }
-    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    double *fp = D.Fpair + dom_fp(D, row, t);
     fp[0] = F;
     fp[1] = 1.0 / abs3(Tz) * (Fzp - Fzm) / (2.0 * h_);
     fp[2] = 1.0 / abs3(Te) * (Fep - Fem) / (2.0 * h_);
@@ -556,7 +561,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
 // free-space self element: G, H, F (bil_coef_bd) and the principal-value
 // tangential derivatives dG, dH, dF along t_zeta and t_eta (bil_dcoef.c:369-740)
 __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  const int row = blockIdx.x, lane = threadIdx.x;
+  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const Tab &T = *tab;
@@ -734,19 +739,24 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
     dFv[dir] = plane ? 0.0 : wave_sum(dFv[dir]) * (bdC * BQD2_IFP * iT[dir] * sg);
   }
   if (lane == 0) {
-    const size_t N = D.N, col = (size_t)(t - 1) * 4;
-    const size_t base = ((size_t)row * N + col) * 2;
-    const size_t b0 = ((size_t)(2 * row) * N + col) * 2, b1 = ((size_t)(2 * row + 1) * N + col) * 2;
+    const size_t col = (size_t)(t - 1) * 4;
+    const size_t base = dom_gh(D, row, col) * 2;
+    const size_t b0 = dom_dgh(D, row, 0, col) * 2, b1 = dom_dgh(D, row, 1, col) * 2;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
       D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
       D.H[base + 2 * n] = h[n].x; D.H[base + 2 * n + 1] = h[n].y;
-      D.dG[b0 + 2 * n] = dG[0][n].x; D.dG[b0 + 2 * n + 1] = dG[0][n].y;
-      D.dG[b1 + 2 * n] = dG[1][n].x; D.dG[b1 + 2 * n + 1] = dG[1][n].y;
-      D.dH[b0 + 2 * n] = dH[0][n].x; D.dH[b0 + 2 * n + 1] = dH[0][n].y;
-      D.dH[b1 + 2 * n] = dH[1][n].x; D.dH[b1 + 2 * n + 1] = dH[1][n].y;
     }
-    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    if (D.dG) {
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        D.dG[b0 + 2 * n] = dG[0][n].x; D.dG[b0 + 2 * n + 1] = dG[0][n].y;
+        D.dG[b1 + 2 * n] = dG[1][n].x; D.dG[b1 + 2 * n + 1] = dG[1][n].y;
+        D.dH[b0 + 2 * n] = dH[0][n].x; D.dH[b0 + 2 * n + 1] = dH[0][n].y;
+        D.dH[b1 + 2 * n] = dH[1][n].x; D.dH[b1 + 2 * n + 1] = dH[1][n].y;
+      }
+    }
+    double *fp = D.Fpair + dom_fp(D, row, t);
     fp[0] = F * sg; fp[1] = dFv[0]; fp[2] = dFv[1];
   }
 }
@@ -762,7 +772,7 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
 // quadrilateral self element
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
                                                       long long *__restrict__ counters) {
-  const int row = blockIdx.x, lane = threadIdx.x;
+  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const int s = abs(td);
@@ -857,13 +867,13 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   int anyerr = __any(err != 0);
   if (lane == 0) {
-    const size_t base = ((size_t)row * D.N + (size_t)(t - 1) * 4) * 2;
+    const size_t base = dom_gh(D, row, (size_t)(t - 1) * 4) * 2;
 #pragma unroll
     for (int n = 0; n < 3; n++) {
       D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
       D.H[base + 2 * n] = h[n].x; D.H[base + 2 * n + 1] = h[n].y;
     }
-    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    double *fp = D.Fpair + dom_fp(D, row, t);
     fp[0] = 0.0; fp[1] = 0.0; fp[2] = 0.0;
     tally_add(counters + CNT_SELF, tsp, tpr, tev, tfd, split ? 3 * 144 : npts);
     if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
@@ -874,7 +884,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
 // and the principal-value dG/dv along both element parameters
 // (lit_dcoef_bd_tz0_PV / _te0_PV, lit_dcoef.c:461-637, :845-952)
 __global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  const int row = blockIdx.x, lane = threadIdx.x;
+  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const int s = abs(td);
@@ -971,41 +981,47 @@ __global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__re
     for (int n = 0; n < 3; n++)
       dG[dir][n] = cd{wave_sum(dG[dir][n].x), wave_sum(dG[dir][n].y)} * (-BQD2_IFP * iT[dir]);
   if (lane == 0) {
-    const size_t N = D.N, col = (size_t)(t - 1) * 4;
-    const size_t base = ((size_t)row * N + col) * 2;
-    const size_t b0 = ((size_t)(2 * row) * N + col) * 2, b1 = ((size_t)(2 * row + 1) * N + col) * 2;
+    const size_t col = (size_t)(t - 1) * 4;
+    const size_t base = dom_gh(D, row, col) * 2;
+    const size_t b0 = dom_dgh(D, row, 0, col) * 2, b1 = dom_dgh(D, row, 1, col) * 2;
 #pragma unroll
     for (int n = 0; n < 3; n++) {
       D.G[base + 2 * n] = g[n].x; D.G[base + 2 * n + 1] = g[n].y;
       D.H[base + 2 * n] = 0.0; D.H[base + 2 * n + 1] = 0.0;
-      D.dG[b0 + 2 * n] = dG[0][n].x; D.dG[b0 + 2 * n + 1] = dG[0][n].y;
-      D.dG[b1 + 2 * n] = dG[1][n].x; D.dG[b1 + 2 * n + 1] = dG[1][n].y;
-      D.dH[b0 + 2 * n] = 0.0; D.dH[b0 + 2 * n + 1] = 0.0;
-      D.dH[b1 + 2 * n] = 0.0; D.dH[b1 + 2 * n + 1] = 0.0;
     }
-    double *fp = D.Fpair + ((size_t)row * D.Ne + (t - 1)) * 3;
+    if (D.dG) {
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        D.dG[b0 + 2 * n] = dG[0][n].x; D.dG[b0 + 2 * n + 1] = dG[0][n].y;
+        D.dG[b1 + 2 * n] = dG[1][n].x; D.dG[b1 + 2 * n + 1] = dG[1][n].y;
+        D.dH[b0 + 2 * n] = 0.0; D.dH[b0 + 2 * n + 1] = 0.0;
+        D.dH[b1 + 2 * n] = 0.0; D.dH[b1 + 2 * n + 1] = 0.0;
+      }
+    }
+    double *fp = D.Fpair + dom_fp(D, row, t);
     fp[0] = 0.0; fp[1] = 0.0; fp[2] = 0.0;
   }
 }
 
 // per-row F sums in source order and the diagonal term (bem3_aw_qd2_solve_bieq.c:185-198)
 __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= D.N) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.nr) return;
+  const int row = D.r0 + i;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) {
     D.dF[3 * row] = D.dF[3 * row + 1] = D.dF[3 * row + 2] = 0.0;
     return;
   }
   double F = 0.0, Fz = 0.0, Fe = 0.0;
-  const double *fp = D.Fpair + (size_t)row * D.Ne * 3;
+  const double *fp = D.Fpair + dom_fp(D, row, 1);
   for (int s = 0; s < D.Ne; s++) {
     F += fp[3 * s];
     Fz += fp[3 * s + 1];
     Fe += fp[3 * s + 2];
   }
   double diag = D.did == 0 ? 1.0 + F : F;
-  D.H[((size_t)row * D.N + row) * 2] += diag;
+  D.H[dom_gh(D, row, row) * 2] += diag;
   D.dF[3 * row] = diag;
   D.dF[3 * row + 1] = Fz;
   D.dF[3 * row + 2] = Fe;
@@ -1064,23 +1080,54 @@ struct AsmPhase {
   hipEvent_t e0, e1, e1b, e2, e3;
 };
 
+// zero the rows [r0, r0 + nr) of G, H, dG, dH and Fpair
+int bqd2_dom_clear(const DevDomain &D, hipStream_t st) {
+  const size_t nr = (size_t)D.nr, rowb = (size_t)D.N * 16;
+  if (D.ldm == D.N) {
+    BQD2_HIP(hipMemsetAsync(D.G, 0, nr * rowb, st));
+    BQD2_HIP(hipMemsetAsync(D.H, 0, nr * rowb, st));
+  } else if (D.ldm == 2 * (long long)D.N && D.G == D.H + 2 * (size_t)D.N) {
+    // system layout: the rows of H and G are whole rows of one array
+    BQD2_HIP(hipMemsetAsync(D.H, 0, 2 * nr * rowb, st));
+  } else {
+    BQD2_HIP(hipMemset2DAsync(D.G, (size_t)D.ldm * 16, 0, rowb, nr, st));
+    BQD2_HIP(hipMemset2DAsync(D.H, (size_t)D.ldm * 16, 0, rowb, nr, st));
+  }
+  if (D.dG) {
+    BQD2_HIP(hipMemsetAsync(D.dG, 0, 2 * nr * rowb, st));
+    BQD2_HIP(hipMemsetAsync(D.dH, 0, 2 * nr * rowb, st));
+  }
+  BQD2_HIP(hipMemsetAsync(D.Fpair, 0, nr * (size_t)D.Ne * 3 * 8, st));
+  return BQD2_OK;
+}
+
+// the free-space rows [D.r0, D.r0 + D.nr) of domain D (rules decided): pairs,
+// self blocks and row sums
+int bqd2_fs_rows(bqd2_problem *p, const DevDomain &D, hipStream_t st) {
+  if (D.nr <= 0) return BQD2_OK;
+  int rc = bqd2_dom_clear(D, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
+  hipLaunchKernelGGL(k_fs_self, dim3(D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
+  if (D.has_tri) hipLaunchKernelGGL(k_fs_self_tri, dim3(D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
+  hipLaunchKernelGGL(k_rows, dim3(bqd2_blocks(D.nr, 256)), dim3(256), 0, st, p->m, D);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
 static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
   DevDomain &D = p->dom[did];
   const bool periodic = (did == 0 && p->m.ps == 1);
   hipStream_t st_ = p->stream;
+  BQD2_CHECK(D.r0 == 0 && D.nr == D.N, BQD2_E_STATE, "bqd2_assemble: domain %d holds a window of its rows", did);
   BQD2_HIP(hipEventCreate(&ev.e0));
   BQD2_HIP(hipEventCreate(&ev.e1));
   BQD2_HIP(hipEventCreate(&ev.e1b));
   BQD2_HIP(hipEventCreate(&ev.e2));
   BQD2_HIP(hipEventCreate(&ev.e3));
   BQD2_HIP(hipMemsetAsync(p->d_counters, 0, CNT_N * sizeof(long long), st_));
-  BQD2_HIP(hipMemsetAsync(D.G, 0, (size_t)D.N * D.N * 16, st_));
-  BQD2_HIP(hipMemsetAsync(D.H, 0, (size_t)D.N * D.N * 16, st_));
-  if (D.dG) {
-    BQD2_HIP(hipMemsetAsync(D.dG, 0, (size_t)2 * D.N * D.N * 16, st_));
-    BQD2_HIP(hipMemsetAsync(D.dH, 0, (size_t)2 * D.N * D.N * 16, st_));
-  }
-  BQD2_HIP(hipMemsetAsync(D.Fpair, 0, (size_t)D.N * D.Ne * 3 * 8, st_));
+  int rc = bqd2_dom_clear(D, st_);
+  if (rc) return rc;
   BQD2_HIP(hipEventRecord(ev.e0, st_));
   long long npairs = (long long)D.N * D.Ne;
   // the rules and the pair list depend on the geometry and the lattice only:
@@ -1109,8 +1156,10 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
     std::vector<int> plist;
     if (periodic) {
       pair_list(D, rule, key, p->tripos[did], plist);
-      if (!plist.empty())
-        BQD2_HIP(hipMemcpy(D.plist, plist.data(), plist.size() * sizeof(int), hipMemcpyHostToDevice));
+      if (!plist.empty()) {
+        BQD2_HIP(hipMemcpyAsync(D.plist, plist.data(), plist.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+        BQD2_HIP(hipStreamSynchronize(st_));  // the host list is released on return
+      }
     }
     D.plist_n = periodic ? (int)plist.size() : 1;
   }
@@ -1136,7 +1185,8 @@ static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *
   BQD2_HIP(hipEventRecord(ev.e3, st_));
   BQD2_HIP(hipEventSynchronize(ev.e3));
   long long cnt[CNT_N];
-  BQD2_HIP(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  BQD2_HIP(hipMemcpyAsync(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st_));
+  BQD2_HIP(hipStreamSynchronize(st_));
   const long long ngh = D.n_gh, nsplit = D.n_split;
   if (st) {
     memset(st, 0, sizeof(*st));
@@ -1171,8 +1221,44 @@ static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *
   return BQD2_OK;
 }
 
+// the rules of domain did's pairs (first time only) and, for the periodic
+// domain, its pair list, without the pairs themselves (the streamed interior
+// domain of the system layout assembles its rows later, window by window)
+int bqd2_decide_only(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
+  DevDomain &D = p->dom[did];
+  AsmPhase ev;
+  BQD2_HIP(hipEventCreate(&ev.e0));
+  BQD2_HIP(hipEventCreate(&ev.e1));
+  BQD2_HIP(hipEventRecord(ev.e0, p->stream));
+  if (D.plist_n == 0) {
+    const long long npairs = (long long)D.N * D.Ne;
+    hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, p->stream, p->m, p->tab, D, p->gf, 0);
+    BQD2_HIP(hipGetLastError());
+    D.plist_n = 1;
+  }
+  BQD2_HIP(hipEventRecord(ev.e1, p->stream));
+  BQD2_HIP(hipEventSynchronize(ev.e1));
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    float a = 0.0f;
+    BQD2_HIP(hipEventElapsedTime(&a, ev.e0, ev.e1));
+    st->ms_decide = a;
+  }
+  (void)hipEventDestroy(ev.e0);
+  (void)hipEventDestroy(ev.e1);
+  return BQD2_OK;
+}
+
+int bqd2_lg_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st);
+int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st);
+
 extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_assemble: bad domain %d", did);
+  if (p->large) return bqd2_lg_assemble(p, did, st);
+  return bqd2_assemble_full(p, did, st);
+}
+
+int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   DevDomain &D = p->dom[did];
   if (D.Ne == 0) return BQD2_OK;
   const bool periodic = (did == 0 && p->m.ps == 1);
@@ -1197,8 +1283,10 @@ extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) 
 // launch on the first problem's stream (other domains as bqd2_assemble).
 extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_assembly_stats *st) {
   BQD2_CHECK(ps && count > 0, BQD2_E_ARG, "bqd2_assemble_batch: no problems");
-  for (int i = 0; i < count; i++)
+  for (int i = 0; i < count; i++) {
     BQD2_CHECK(ps[i] && did >= 0 && did <= ps[i]->m.MN, BQD2_E_ARG, "bqd2_assemble_batch: bad problem %d or domain", i);
+    BQD2_CHECK(!ps[i]->large, BQD2_E_ARG, "bqd2_assemble_batch: problem %d is in the system layout", i);
+  }
   std::vector<AsmPhase> ev(count);
   std::vector<int> batched;
   for (int i = 0; i < count; i++) {
@@ -1227,8 +1315,11 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
       hb[j].wg_end = tasks;
     }
     hipStream_t bs = ps[batched[0]]->stream;
-    PairBatch *d_inst = nullptr;
-    BQD2_HIP(hipMalloc(&d_inst, hb.size() * sizeof(PairBatch)));
+    // the instance table in the first problem's workspace (its stream runs the launch)
+    char *wsb = nullptr;
+    int wrc = bqd2_workspace(&ps[batched[0]]->main, hb.size() * sizeof(PairBatch), &wsb);
+    if (wrc) return wrc;
+    PairBatch *d_inst = (PairBatch *)wsb;
     BQD2_HIP(hipMemcpyAsync(d_inst, hb.data(), hb.size() * sizeof(PairBatch), hipMemcpyHostToDevice, bs));
     // the launch waits for every problem's set-up, every problem waits for it
     for (int i : batched) {
@@ -1246,7 +1337,6 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
       BQD2_HIP(hipStreamWaitEvent(ps[i]->stream, ev[i].e2, 0));
     }
     BQD2_HIP(hipStreamSynchronize(bs));
-    BQD2_HIP(hipFree(d_inst));
   }
   for (int i = 0; i < count; i++) {
     bqd2_problem *p = ps[i];

@@ -211,8 +211,8 @@ int bqd2_jacobi(int n, const double *A, long long lda, const int *map, double *d
 
 int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, const int *map, const double *B,
                long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres,
-               const Precond *pc) {
-  BQD2_CHECK(n > 0 && nrhs >= 1 && nrhs <= 3 && m >= 2, BQD2_E_ARG, "bqd2_gmres: bad arguments");
+               const Precond *pc, const GmOp *op) {
+  BQD2_CHECK(n > 0 && nrhs >= 1 && nrhs <= 3 && m >= 2 && (!op || nrhs == 1), BQD2_E_ARG, "bqd2_gmres: bad arguments");
   hipStream_t st = cx->st;
   // workspace: V (nrhs x (m+1) x n), W, R, Db (nrhs x n), dinv (n), h (nrhs x (2(m+1)+1)), y (nrhs x m)
   const long long vstride = (long long)(m + 1) * n;
@@ -227,8 +227,15 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
   double *V = take(bV), *W = take(bN), *R = take(bN), *Db = take(bN), *dinv = take(bD), *H = take(bH), *Y = take(bY);
   double *Z = take(bN);  // M^-1 of the basis vector / of the update (preconditioned runs)
   const dim3 gv(bqd2_blocks(n, 256), nrhs), b256(256);
+  // Y = dinv A X: the dense row-major matrix, or the caller's operator
+  auto mv = [&](const double *Xv, long long ldxv, double *Yv) -> int {
+    if (op) return op->mv(op->ctx, Xv, Yv, st);
+    launch_mv(nrhs, n, A, lda, map, dinv, Xv, ldxv, Yv, (long long)n, st);
+    return BQD2_OK;
+  };
 
-  hipLaunchKernelGGL(k_gm_dinv, dim3(bqd2_blocks(n, 256)), b256, 0, st, n, A, lda, map, dinv);
+  if (op) BQD2_HIP(hipMemcpyAsync(dinv, op->dinv, bD, hipMemcpyDeviceToDevice, st));
+  else hipLaunchKernelGGL(k_gm_dinv, dim3(bqd2_blocks(n, 256)), b256, 0, st, n, A, lda, map, dinv);
   hipLaunchKernelGGL(k_gm_scale_rhs, gv, b256, 0, st, n, dinv, B, ldb, Db, (long long)n);
   for (int s = 0; s < nrhs; s++) BQD2_HIP(hipMemsetAsync(X + 2 * (size_t)s * ldx, 0, (size_t)n * 16, st));
   BQD2_HIP(hipGetLastError());
@@ -250,7 +257,7 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
     if (first) {
       BQD2_HIP(hipMemcpyAsync(R, Db, bN, hipMemcpyDeviceToDevice, st));
     } else {
-      launch_mv(nrhs, n, A, lda, map, dinv, X, ldx, W, (long long)n, st);
+      if ((rc = mv(X, ldx, W))) return rc;
       hipLaunchKernelGGL(k_gm_sub, gv, b256, 0, st, n, Db, W, R, (long long)n);
     }
     hipLaunchKernelGGL(k_gm_norm, dim3(nrhs), b256, 0, st, n, R, (long long)n, H);
@@ -273,9 +280,9 @@ int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, con
         int rc2 = bqd2_lu32_apply(n, nrhs, pc->Af, pc->ws, V + 2 * (size_t)j * n, vstride, Z, (long long)n, pc->tmp,
                                   st, pc->graph);
         if (rc2) return rc2;
-        launch_mv(nrhs, n, A, lda, map, dinv, Z, (long long)n, W, (long long)n, st);
+        if ((rc = mv(Z, (long long)n, W))) return rc;
       } else {
-        launch_mv(nrhs, n, A, lda, map, dinv, V + 2 * (size_t)j * n, vstride, W, (long long)n, st);
+        if ((rc = mv(V + 2 * (size_t)j * n, vstride, W))) return rc;
       }
       // classical Gram-Schmidt, twice; h of pass p at H[s][p (m+1) + i], the norm at H[s][2 (m+1)]
       for (int pass = 0; pass < 2; pass++) {

@@ -35,10 +35,12 @@ struct DevDomain {
   int *rowmap;         // N: compressed node index of row (t-1)*4+tn, -1 for unused triangle slot
   int *fperm;          // Ne: element positions in the field kernels' order (latitude bands, then azimuth)
   double k;            // real wave number of the domain
-  double *G, *H;       // N x N complex
-  double *dG, *dH;     // 2N x N complex (interior / isolated domains)
+  double *G, *H;       // N x N complex: row `row` at (row - r0) ldm
+  double *dG, *dH;     // 2N x N complex (interior / isolated domains): rows 2 (row - r0) + j, leading dimension N
+  long long ldm;       // row stride of G and H in complex entries: N, or 2N in the system layout
+  int r0, nr;          // rows held by G, H, dG, dH and Fpair: [r0, r0 + nr) (all N rows unless streamed in windows)
   double *dF;          // N x 3
-  double *Fpair;       // N x Ne x 3 : per pair F, dF/dtz, dF/dte
+  double *Fpair;       // nr x Ne x 3 : per pair F, dF/dtz, dF/dte
   unsigned char *rule; // N x Ne : 1 = GH64 rule
   int *plist;          // N x Ne + 32 : (row, s) pairs sorted by rule and element kind
   float *pkey;         // N x Ne : |z| offset of the element centre from the target (pair order)
@@ -57,6 +59,18 @@ struct DevDomain {
   float g_growth;  // largest block-local panel growth kept in its factors
   hipGraphExec_t g_graph;  // its triangular sweeps (3 right-hand sides) as a graph
 };
+
+// complex-entry offsets of G/H entry (row, col), of dG/dH row 2 row + j, column
+// col, and the double offset of the Fpair triple (row, source position s)
+__host__ __device__ __forceinline__ size_t dom_gh(const DevDomain &D, int row, size_t col) {
+  return (size_t)(row - D.r0) * (size_t)D.ldm + col;
+}
+__host__ __device__ __forceinline__ size_t dom_dgh(const DevDomain &D, int row, int j, size_t col) {
+  return (size_t)(2 * (row - D.r0) + j) * (size_t)D.N + col;
+}
+__host__ __device__ __forceinline__ size_t dom_fp(const DevDomain &D, int row, int s) {
+  return ((size_t)(row - D.r0) * (size_t)D.Ne + (size_t)(s - 1)) * 3;
+}
 
 // execution context of one calling host thread for the field and force entry
 // points: its own stream, grow-only workspace and counters, so that callers
@@ -116,9 +130,27 @@ struct bqd2_problem {
   std::thread::id owner;
   std::mutex ctx_mu;
   std::vector<std::pair<std::thread::id, CallCtx *>> ctxs;
+  // system layout (bqd2_create_large): both domains' matrices in one row-major
+  // na x na array S = [H0 G0; H1 G1] (na = 2N, the system's own column order);
+  // the interior domain's rows are assembled in windows of lwin rows when needed
+  int large;
+  double *S;
+  int s_lower;               // what rows [N, na) of S hold: LG_EMPTY, LG_G1H1 or LG_AF
+  int lwin;                  // rows per window of the interior domain
+  int lmax_resident;         // windows kept resident by the mixed solver (-1: as many as fit)
+  std::vector<double *> lpool;  // resident windows of [H1 G1] rows, windows 0, 1, ...
+  double *lscr;              // one window of [H1 G1] rows for the others
+  double *lFp;               // Fpair of one window
+  double *ldgw, *ldhw;       // dG, dH of one window (velocity recovery)
+  double *ldinv;             // na complex: inverse diagonal of the system
+  double *lvec;              // 3 na complex: the operator's two column vectors, the right-hand side's
+  bqd2_large_stats lst;      // timings of the last solve and velocity recovery
 };
+enum { LG_EMPTY = 0, LG_G1H1 = 1, LG_AF = 2 };
 
 int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes);
+// release one allocation of bqd2_alloc before the problem ends (*ptr = nullptr)
+void bqd2_release(bqd2_problem *p, void **ptr, size_t bytes);
 
 // Right preconditioner of GMRES: the complex float LU factors (lu32.hip) of the
 // row-scaled matrix D A (D = the inverse diagonal), so that D A M^-1 is close to
@@ -129,13 +161,20 @@ struct Precond {
   float *tmp;
   hipGraphExec_t *graph;  // the sweeps, captured at the first application (owned by the factors' holder)
 };
+// A matrix given as an operator: mv(ctx, X, Y, st) sets Y = dinv A X for one
+// complex vector of n entries (the caller's row scaling dinv, n complex)
+struct GmOp {
+  const double *dinv;
+  int (*mv)(void *ctx, const double *X, double *Y, hipStream_t st);
+  void *ctx;
+};
 // restarted GMRES on nrhs <= 3 right-hand sides sharing the row-major matrix A
 // (gmres.hip), right-preconditioned by pc when it is not null; returns 0 when
 // the scaled residual is below tol, 1 when not (the caller falls back to
 // another solver), < 0 on errors
 int bqd2_gmres(CallCtx *cx, int n, int nrhs, const double *A, long long lda, const int *map, const double *B,
                long long ldb, double *X, long long ldx, double tol, int m, int maxit, int *its, double *relres,
-               const Precond *pc = nullptr);
+               const Precond *pc = nullptr, const GmOp *op = nullptr);
 // lu32.hip: the row-scaled row-major complex double matrix rounded to column-major
 // complex float; out = M^-1 in with the factors (in may be out)
 // bqd2_cgetrf with the growth bound of block-local panels given (the C ABI

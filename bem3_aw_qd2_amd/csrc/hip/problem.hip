@@ -18,6 +18,18 @@ int bqd2_alloc(bqd2_problem *p, void **ptr, size_t bytes) {
   return BQD2_OK;
 }
 
+void bqd2_release(bqd2_problem *p, void **ptr, size_t bytes) {
+  if (!*ptr) return;
+  for (auto it = p->owned.begin(); it != p->owned.end(); ++it)
+    if (*it == *ptr) {
+      p->owned.erase(it);
+      p->alloc_bytes -= bytes ? bytes : 16;
+      break;
+    }
+  (void)hipFree(*ptr);
+  *ptr = nullptr;
+}
+
 void bqd2_rebind(bqd2_problem *p) { p->owner = std::this_thread::get_id(); }
 
 size_t bqd2_bytes(bqd2_problem *p) {
@@ -50,11 +62,45 @@ static int upload(bqd2_problem *p, T **dst, const T *src, size_t n) {
     }                         \
   } while (0)
 
-extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
+// the system layout needs one sphere of quadrilaterals in a periodic lattice,
+// each element with its own four nodes in element order, domain 0 the lattice
+// side of every element and domain 1 the sphere side
+static int large_layout_ok(const bqd2_model *m) {
+  BQD2_CHECK(m->MN == 1 && m->ps == 1, BQD2_E_ARG,
+             "bqd2_create_large: one sphere (MN = 1) in a periodic lattice (ps = 1) required, got MN=%d ps=%d", m->MN,
+             m->ps);
+  BQD2_CHECK(m->NN == 4 * m->Ne && m->dom_ne[0] == m->Ne && m->dom_ne[1] == m->Ne, BQD2_E_ARG,
+             "bqd2_create_large: every element must bound both domains and carry four nodes");
+  const int *sid0 = m->dom_sid, *sid1 = m->dom_sid + m->Ne + 1;
+  for (int s = 1; s <= m->Ne; s++) {
+    BQD2_CHECK(m->ed[4 * s + 3] != 0, BQD2_E_ARG, "bqd2_create_large: element %d is a triangle", s);
+    BQD2_CHECK(m->md[s] == 0 && m->sd[s] == 1, BQD2_E_ARG, "bqd2_create_large: element %d does not face domain 0", s);
+    BQD2_CHECK(sid0[s] == s && sid1[s] == -s, BQD2_E_ARG, "bqd2_create_large: domain element lists out of order");
+    for (int l = 0; l < 4; l++)
+      BQD2_CHECK(m->eni[4 * s + l] == 4 * (s - 1) + l, BQD2_E_ARG, "bqd2_create_large: node %d of element %d", l, s);
+  }
+  return BQD2_OK;
+}
+
+static int create_impl(const bqd2_model *m, int large, bqd2_problem **out);
+
+extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) { return create_impl(m, 0, out); }
+
+extern "C" int bqd2_create_large(const bqd2_model *m, bqd2_problem **out) {
+  BQD2_CHECK(m && out, BQD2_E_ARG, "bqd2_create_large: null argument");
+  int rc = large_layout_ok(m);
+  if (rc) return rc;
+  return create_impl(m, 1, out);
+}
+
+static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
   BQD2_CHECK(m && out, BQD2_E_ARG, "bqd2_create: null argument");
   BQD2_CHECK(m->Ne > 0 && m->MN >= 0 && m->MN < 16, BQD2_E_ARG, "bqd2_create: bad sizes Ne=%d MN=%d", m->Ne, m->MN);
   bqd2_problem *p = new bqd2_problem();
   *out = nullptr;
+  p->large = large;
+  p->lwin = 2048;
+  p->lmax_resident = -1;
   p->m.Ne = m->Ne;
   p->m.NN = m->NN;
   p->m.MN = m->MN;
@@ -152,17 +198,22 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     }
     const size_t N = D.N;
     const bool periodic0 = (d == 0 && m->ps == 1);
-    TRY(bqd2_alloc(p, (void **)&D.G, N * N * 16));
-    TRY(bqd2_alloc(p, (void **)&D.H, N * N * 16));
-    if (!periodic0) {
-      TRY(bqd2_alloc(p, (void **)&D.dG, 2 * N * N * 16));
-      TRY(bqd2_alloc(p, (void **)&D.dH, 2 * N * N * 16));
+    D.ldm = D.N;
+    D.r0 = 0;
+    D.nr = D.N;
+    if (!large) {
+      TRY(bqd2_alloc(p, (void **)&D.G, N * N * 16));
+      TRY(bqd2_alloc(p, (void **)&D.H, N * N * 16));
+      if (!periodic0) {
+        TRY(bqd2_alloc(p, (void **)&D.dG, 2 * N * N * 16));
+        TRY(bqd2_alloc(p, (void **)&D.dH, 2 * N * N * 16));
+      }
+      TRY(bqd2_alloc(p, (void **)&D.Fpair, N * D.Ne * 3 * 8));
+      TRY(bqd2_alloc(p, (void **)&D.pkey, N * D.Ne * sizeof(float)));
     }
     TRY(bqd2_alloc(p, (void **)&D.dF, N * 3 * 8));
-    TRY(bqd2_alloc(p, (void **)&D.Fpair, N * D.Ne * 3 * 8));
     TRY(bqd2_alloc(p, (void **)&D.rule, N * D.Ne));
-    TRY(bqd2_alloc(p, (void **)&D.plist, (N * D.Ne + 32) * sizeof(int)));
-    TRY(bqd2_alloc(p, (void **)&D.pkey, N * D.Ne * sizeof(float)));
+    if (!large || periodic0) TRY(bqd2_alloc(p, (void **)&D.plist, (N * D.Ne + 32) * sizeof(int)));
     TRY(bqd2_alloc(p, (void **)&D.P, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.dP, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.pv, ((size_t)D.Ne + 1) * 12 * 16));
@@ -173,6 +224,19 @@ extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) {
     BQD2_HIP(hipMemset(D.dpv, 0, ((size_t)D.Ne + 1) * 12 * 16));
   }
   p->na = 2 * m->NN;
+  if (large) {
+    // S = [H0 G0; H1 G1], row-major, na = 2N columns per row; the pair kernels
+    // write straight into it (domain matrices with row stride na)
+    const size_t na = (size_t)p->na, N = (size_t)p->dom[0].N;
+    TRY(bqd2_alloc(p, (void **)&p->S, na * na * 16));
+    for (int d = 0; d < 2; d++) {
+      DevDomain &D = p->dom[d];
+      D.H = p->S + 2 * (size_t)d * N * na;
+      D.G = D.H + 2 * N;
+      D.ldm = (long long)na;
+    }
+    p->s_lower = LG_EMPTY;
+  }
   p->A = p->b = nullptr;
   p->ipiv = nullptr;
   p->solved = false;
@@ -212,8 +276,10 @@ extern "C" int bqd2_download_domain(bqd2_problem *p, int did, double *G, double 
   BQD2_CHECK(D.assembled, BQD2_E_STATE, "bqd2_download_domain: domain %d not assembled", did);
   const size_t N = D.N;
   BQD2_HIP(hipStreamSynchronize(p->stream));
-  if (G) BQD2_HIP(hipMemcpy(G, D.G, N * N * 16, hipMemcpyDeviceToHost));
-  if (H) BQD2_HIP(hipMemcpy(H, D.H, N * N * 16, hipMemcpyDeviceToHost));
+  BQD2_CHECK(D.r0 == 0 && D.nr == D.N && !(p->large && did == 1 && p->s_lower != LG_G1H1), BQD2_E_STATE,
+             "bqd2_download_domain: domain %d's rows are not held", did);
+  if (G) BQD2_HIP(hipMemcpy2D(G, N * 16, D.G, (size_t)D.ldm * 16, N * 16, N, hipMemcpyDeviceToHost));
+  if (H) BQD2_HIP(hipMemcpy2D(H, N * 16, D.H, (size_t)D.ldm * 16, N * 16, N, hipMemcpyDeviceToHost));
   if (dG && D.dG) BQD2_HIP(hipMemcpy(dG, D.dG, 2 * N * N * 16, hipMemcpyDeviceToHost));
   if (dH && D.dH) BQD2_HIP(hipMemcpy(dH, D.dH, 2 * N * N * 16, hipMemcpyDeviceToHost));
   if (dF) BQD2_HIP(hipMemcpy(dF, D.dF, N * 3 * 8, hipMemcpyDeviceToHost));
@@ -227,13 +293,15 @@ extern "C" int bqd2_download_rows(bqd2_problem *p, int did, int nrows, const int
   DevDomain &D = p->dom[did];
   BQD2_CHECK(D.assembled, BQD2_E_STATE, "bqd2_download_rows: domain %d not assembled", did);
   const size_t N = D.N;
+  BQD2_CHECK(D.r0 == 0 && D.nr == D.N && !(p->large && did == 1 && p->s_lower != LG_G1H1), BQD2_E_STATE,
+             "bqd2_download_rows: domain %d's rows are not held", did);
   for (int i = 0; i < nrows; i++)
     BQD2_CHECK(rows[i] >= 0 && (size_t)rows[i] < N, BQD2_E_ARG, "bqd2_download_rows: row %d out of range", rows[i]);
   BQD2_HIP(hipStreamSynchronize(p->stream));
   for (int i = 0; i < nrows; i++) {
     const size_t r = (size_t)rows[i];
-    if (G) BQD2_HIP(hipMemcpy(G + 2 * N * i, D.G + 2 * N * r, N * 16, hipMemcpyDeviceToHost));
-    if (H) BQD2_HIP(hipMemcpy(H + 2 * N * i, D.H + 2 * N * r, N * 16, hipMemcpyDeviceToHost));
+    if (G) BQD2_HIP(hipMemcpy(G + 2 * N * i, D.G + 2 * dom_gh(D, (int)r, 0), N * 16, hipMemcpyDeviceToHost));
+    if (H) BQD2_HIP(hipMemcpy(H + 2 * N * i, D.H + 2 * dom_gh(D, (int)r, 0), N * 16, hipMemcpyDeviceToHost));
     if (dG && D.dG) BQD2_HIP(hipMemcpy(dG + 4 * N * i, D.dG + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost));
     if (dH && D.dH) BQD2_HIP(hipMemcpy(dH + 4 * N * i, D.dH + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost));
     if (dF) BQD2_HIP(hipMemcpy(dF + 3 * i, D.dF + 3 * r, 3 * 8, hipMemcpyDeviceToHost));

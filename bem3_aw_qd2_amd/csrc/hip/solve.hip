@@ -59,8 +59,8 @@ __global__ void __launch_bounds__(256) k_system_A(DevMesh M, DevDomain D, int ro
   const int nl = M.tri[asd] ? 3 : 4;
   const bool main_side = (D.did == M.md[asd]);
   const int nn = M.NN;
-  const double *Gr = D.G + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
-  const double *Hr = D.H + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
+  const double *Gr = D.G + 2 * dom_gh(D, row, (size_t)(s - 1) * 4);
+  const double *Hr = D.H + 2 * dom_gh(D, row, (size_t)(s - 1) * 4);
   for (int l = 0; l < nl; l++) {
     const int col = M.eni[4 * asd + l];
     double *b = A + 2 * sys_at(rm, arow, (size_t)nn + col, na);
@@ -93,8 +93,8 @@ __global__ void __launch_bounds__(64) k_system_b(DevMesh M, DevDomain D, int row
     if (D.did == mdid) continue;
     const int nl = M.tri[asd] ? 3 : 4;
     cd hk2 = cd{hk2tab[2 * (mdid * 16 + sdid)], hk2tab[2 * (mdid * 16 + sdid) + 1]};
-    const double *Gr = D.G + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
-    const double *Hr = D.H + 2 * ((size_t)row * D.N + (size_t)(s - 1) * 4);
+    const double *Gr = D.G + 2 * dom_gh(D, row, (size_t)(s - 1) * 4);
+    const double *Hr = D.H + 2 * dom_gh(D, row, (size_t)(s - 1) * 4);
     for (int l = 0; l < nl; l++) {
       const int col = M.eni[4 * asd + l];
       cd h = hk2 * cd{Hr[2 * l], Hr[2 * l + 1]};
@@ -149,13 +149,13 @@ __global__ void __launch_bounds__(256) k_store_solution(DevMesh M, DevDomain D, 
 // open_out != NULL also the open-domain velocity from the interface conditions
 __global__ void __launch_bounds__(64) k_velocity_rows(DevMesh M, DevDomain D, const Tab *__restrict__ tab, PwDev W,
                                                       double rho_d, double rho_0, double *__restrict__ open_out) {
-  const int row = blockIdx.x, lane = threadIdx.x;
+  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
   int t = row / 4 + 1, tn = row % 4;
   const int at = D.sid[t], atd = abs(at);
   if (M.tri[atd] && tn == 3) return;
   const size_t N = D.N;
-  const double *g0 = D.dG + 2 * (size_t)(2 * row) * N, *g1 = D.dG + 2 * (size_t)(2 * row + 1) * N;
-  const double *h0 = D.dH + 2 * (size_t)(2 * row) * N, *h1 = D.dH + 2 * (size_t)(2 * row + 1) * N;
+  const double *g0 = D.dG + 2 * dom_dgh(D, row, 0, 0), *g1 = D.dG + 2 * dom_dgh(D, row, 1, 0);
+  const double *h0 = D.dH + 2 * dom_dgh(D, row, 0, 0), *h1 = D.dH + 2 * dom_dgh(D, row, 1, 0);
   cd z = cd{0, 0}, e = cd{0, 0};
   for (size_t c = lane; c < N; c += 64) {
     // column c = (s-1)*4 + sn -> P[s][sn] at index 4 s + sn = c + 4
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(256) k_dpv_A(DevMesh M, DevDomain D, double *_
   int row = (int)(gid / D.N), col = (int)(gid % D.N);
   int r = D.rowmap[row], c = D.rowmap[col];
   if (r < 0 || c < 0) return;
-  const double *g = D.G + 2 * ((size_t)row * D.N + col);
+  const double *g = D.G + 2 * dom_gh(D, row, col);
   double *a = A + 2 * ((size_t)c * D.nn + r);
   a[0] = g[0];
   a[1] = g[1];
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(64) k_dpv_B(DevMesh M, DevDomain D, double *__
   cd acc[3] = {};
   for (int c = lane; c < D.N; c += 64) {
     if (D.rowmap[c] < 0) continue;
-    cd h = cd{D.H[2 * ((size_t)row * D.N + c)], D.H[2 * ((size_t)row * D.N + c) + 1]};
+    cd h = cd{D.H[2 * dom_gh(D, row, c)], D.H[2 * dom_gh(D, row, c) + 1]};
     for (int i = 0; i < 3; i++) acc[i] += h * cd{D.pv[2 * (12 + 3 * c + i)], D.pv[2 * (12 + 3 * c + i) + 1]};
   }
   for (int i = 0; i < 3; i++) {
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(64) k_dpv_B_slots(DevDomain D, double *__restr
   if (used)
     for (int c = lane; c < D.N; c += 64) {
       if (D.rowmap[c] < 0) continue;
-      cd h = cd{D.H[2 * ((size_t)row * D.N + c)], D.H[2 * ((size_t)row * D.N + c) + 1]};
+      cd h = cd{D.H[2 * dom_gh(D, row, c)], D.H[2 * dom_gh(D, row, c) + 1]};
       for (int i = 0; i < 3; i++) acc[i] += h * cd{D.pv[2 * (12 + 3 * c + i)], D.pv[2 * (12 + 3 * c + i) + 1]};
     }
   for (int i = 0; i < 3; i++) {
@@ -368,7 +368,7 @@ int bqd2_set_incidence_R(bqd2_problem *p, const double *pw_p, const double *tv, 
   return BQD2_OK;
 }
 
-static void hk2_table(const bqd2_problem *p, double *tab) {
+void bqd2_hk2_table(const bqd2_problem *p, double *tab) {
   // tab[(m*16 + s)] = k2[m] / k2[s]
   for (int m = 0; m <= p->m.MN; m++)
     for (int s = 0; s <= p->m.MN; s++) {
@@ -388,7 +388,7 @@ static int ensure_hk2(bqd2_problem *p) {
   if (p->hk2d) return BQD2_OK;
   double tab[16 * 16 * 2];
   memset(tab, 0, sizeof(tab));
-  hk2_table(p, tab);
+  bqd2_hk2_table(p, tab);
   int rc = bqd2_alloc(p, (void **)&p->hk2d, sizeof(tab));
   if (rc) return rc;
   BQD2_HIP(hipMemcpy(p->hk2d, tab, sizeof(tab), hipMemcpyHostToDevice));
@@ -471,6 +471,19 @@ extern "C" int bqd2_solver_info(bqd2_problem *p, int *iters, double *relres, int
   return BQD2_OK;
 }
 
+int bqd2_lg_solve(bqd2_problem *p, const double **x);
+static int store_solution(bqd2_problem *p, const double *x, double *x_host);
+int bqd2_lg_velocity(bqd2_problem *p, const void *W, double *open_out);
+
+// the velocity rows of domain D's rows [D.r0, D.r0 + D.nr) (its dG, dH held for them)
+int bqd2_velocity_rows(bqd2_problem *p, const DevDomain &D, const void *W, double *open_out, hipStream_t st) {
+  const int d = D.did;
+  hipLaunchKernelGGL(k_velocity_rows, dim3(D.nr), dim3(64), 0, st, p->m, D, p->tab, *(const PwDev *)W, p->rho0[d],
+                     p->rho0[0], open_out);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
 extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
   BQD2_CHECK(p, BQD2_E_ARG, "bqd2_solve_surface: null problem");
   for (int d = 0; d <= p->m.MN; d++)
@@ -478,6 +491,14 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
   const int na = p->na;
   hipStream_t st = p->stream;
   int rc;
+  if (p->large) {
+    if ((rc = ensure_hk2(p))) return rc;
+    p->info_lu = 0;
+    for (int i = 0; i < 17; i++) { p->info_its[i] = 0; p->info_res[i] = 0.0; }
+    const double *xl = nullptr;
+    if ((rc = bqd2_lg_solve(p, &xl))) return rc;
+    return store_solution(p, xl, x_host);
+  }
   if (!p->A) {
     if ((rc = bqd2_alloc(p, (void **)&p->A, (size_t)na * na * 16))) return rc;
     if ((rc = bqd2_alloc(p, (void **)&p->b, (size_t)na * 16))) return rc;
@@ -526,6 +547,13 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
     if ((rc = bqd2_zgetrs(na, 1, p->A, p->ipiv, p->b, st))) return rc;
     x = p->b;
   }
+  return store_solution(p, x, x_host);
+}
+
+// the system's solution x (device) to P, dP of every domain; x_host may be null
+static int store_solution(bqd2_problem *p, const double *x, double *x_host) {
+  hipStream_t st = p->stream;
+  const int na = p->na;
   for (int d = 0; d <= p->m.MN; d++) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
@@ -550,8 +578,8 @@ static int factor_g32(bqd2_problem *p, DevDomain &D, hipStream_t st, float bound
     if ((rc = bqd2_alloc(p, (void **)&D.gdinv, N * 16))) return rc;
     if ((rc = bqd2_alloc(p, (void **)&D.gipiv, N * sizeof(int)))) return rc;
   }
-  if ((rc = bqd2_jacobi(D.N, D.G, D.N, D.rowmap, D.gdinv, st))) return rc;
-  if ((rc = bqd2_lu32_convert(D.N, D.G, D.N, D.rowmap, D.gdinv, D.gAf, st))) return rc;
+  if ((rc = bqd2_jacobi(D.N, D.G, D.ldm, D.rowmap, D.gdinv, st))) return rc;
+  if ((rc = bqd2_lu32_convert(D.N, D.G, D.ldm, D.rowmap, D.gdinv, D.gAf, st))) return rc;
   if ((rc = bqd2_cgetrf_bound(D.N, D.gAf, D.gipiv, D.gws, st, bound, &D.g_growth))) return rc;
   D.g_lu_valid = true;
   return BQD2_OK;
@@ -599,11 +627,15 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
     if (!p->tpv && (rc = bqd2_alloc(p, (void **)&p->tpv, (size_t)p->m.NN * 3 * 16))) return rc;
     BQD2_HIP(hipMemsetAsync(p->tpv, 0, (size_t)p->m.NN * 3 * 16, st));
   }
-  for (int d = periodic ? 1 : 0; d <= p->m.MN; d++) {
-    DevDomain &D = p->dom[d];
-    if (D.Ne == 0) continue;
-    hipLaunchKernelGGL(k_velocity_rows, dim3(D.N), dim3(64), 0, st, p->m, D, p->tab, W, p->rho0[d], p->rho0[0],
-                       periodic ? p->tpv : (double *)nullptr);
+  if (p->large) {
+    // the sphere's rows window by window (domain 0 assembled again if an FP64 LU consumed it)
+    if ((rc = bqd2_lg_velocity(p, &W, p->tpv))) return rc;
+  } else {
+    for (int d = periodic ? 1 : 0; d <= p->m.MN; d++) {
+      DevDomain &D = p->dom[d];
+      if (D.Ne == 0) continue;
+      if ((rc = bqd2_velocity_rows(p, D, &W, periodic ? p->tpv : (double *)nullptr, st))) return rc;
+    }
   }
   if (periodic && p->dom[0].Ne > 0)
     hipLaunchKernelGGL(k_pv_open_domain, dim3(bqd2_blocks(p->dom[0].Ne, 256)), dim3(256), 0, st, p->m, p->dom[0],
@@ -619,12 +651,14 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
     DevDomain &D = p->dom[d];
     if (D.Ne == 0) continue;
     bool done = false;
-    if (p->solver != 1) {
+    // the system layout solves the dpv systems by FP64 GMRES whatever the surface solver
+    const int solver = p->large ? 0 : p->solver;
+    if (solver != 1) {
       double *B = p->dpvw, *X = p->dpvw + 2 * (size_t)3 * D.N;
       hipLaunchKernelGGL(k_dpv_B_slots, dim3(D.N), dim3(64), 0, st, D, B);
       BQD2_HIP(hipGetLastError());
       rc = 1;
-      if (p->solver == 2) {
+      if (solver == 2) {
         // complex float factors of G (once per assembly) as the preconditioner
         if (!D.g_lu_valid) {
           rc = factor_g32(p, D, st, BQD2_LU32_RELAXED_GROWTH);
@@ -632,17 +666,17 @@ extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
           if (rc) return rc;
         }
         Precond pc{D.gAf, D.gws, D.gtmp, &D.g_graph};
-        rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
+        rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.ldm, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
                         &p->info_its[1 + d], &p->info_res[1 + d], &pc);
         if (rc == 1 && D.g_growth > BQD2_LU32_STABLE_GROWTH) {
           if ((rc = factor_g32(p, D, st, BQD2_LU32_STABLE_GROWTH))) return rc;
-          rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
+          rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.ldm, D.rowmap, B, D.N, X, D.N, GM_TOL, IR_RESTART, IR_MAXIT,
                           &p->info_its[1 + d], &p->info_res[1 + d], &pc);
         }
         if (rc < 0) return rc;
       }
       if (rc == 1)
-        rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.N, D.rowmap, B, D.N, X, D.N, GM_TOL, GM_RESTART, GM_MAXIT,
+        rc = bqd2_gmres(&p->main, D.N, 3, D.G, D.ldm, D.rowmap, B, D.N, X, D.N, GM_TOL, GM_RESTART, GM_MAXIT,
                         &p->info_its[1 + d], &p->info_res[1 + d]);
       if (rc < 0) return rc;
       if (rc == 0) {

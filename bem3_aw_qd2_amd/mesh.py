@@ -127,6 +127,14 @@ CONFIGS = {
     # same sphere on a triangular 1.8 mm lattice, theta 30 deg, phi 15 deg, 6 x 24 x 24 quads
     "sphere3456": dict(n=24, radius=0.5e-3, d1=(1.8e-3, 0.0), d2=(0.9e-3, 0.9e-3 * math.sqrt(3.0)), f=1.0e6,
                        theta=math.pi / 6, psi=math.pi / 12, host=(1000.0, 1500.0), sphere=(1200.0, 2400.0)),
+    # the mixed-precision instance: 6 x 48 x 48 quads, ka = 6 in the host medium
+    # (k = 12000 / m), square lattice with a period of 2.2 radii, incidence
+    # tilted by 1e-6 rad at azimuth pi/4 as sphere384
+    "sphere13824": dict(n=48, radius=0.5e-3, d1=(1.1e-3, 0.0), d2=(0.0, 1.1e-3), f=6.0 * 1500.0 / (2.0 * math.pi * 0.5e-3),
+                        theta=1.0e-6, psi=math.pi / 4, host=(1000.0, 1500.0), sphere=(1200.0, 2400.0)),
+    # the same physics on the 3456-quad mesh (a layout test that also fits the default layout)
+    "sphere3456k6": dict(n=24, radius=0.5e-3, d1=(1.1e-3, 0.0), d2=(0.0, 1.1e-3), f=6.0 * 1500.0 / (2.0 * math.pi * 0.5e-3),
+                         theta=1.0e-6, psi=math.pi / 4, host=(1000.0, 1500.0), sphere=(1200.0, 2400.0)),
     # small case for tests: 6 x 2 x 2 quads, same physics as sphere384
     "sphere24": dict(n=2, radius=0.5e-3, d1=(2.0e-3, 0.0), d2=(0.0, 2.0e-3), f=1.0e6,
                      theta=1.0e-6, psi=math.pi / 4, host=(1000.0, 1500.0), sphere=(1200.0, 2400.0)),

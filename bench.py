@@ -607,6 +607,108 @@ def serving_bench(args):
     }
 
 
+def large_main(args):
+    """The mixed-precision instance in the system layout (bqd2_create_large): one
+    request with the complex float LU and GMRES refinement (assembly of both
+    domains, surface solve, velocity recovery, force and torque of both rules),
+    then the FP64 LU of the same system as the reference solution (its velocity
+    recovery assembles domain 0 again), and the differences between the two."""
+    from bem3_aw_qd2_amd import AssemblyStats, LargeStats, check, exdat, hip_lib, host_lib, mesh
+    hip, host = hip_lib(), host_lib()
+    check(hip.bqd2_set_autotune(args.autotune), "bqd2_set_autotune")
+    cfg = mesh.CONFIGS[args.workload]
+    tmp = tempfile.mkdtemp(prefix="bqd2_bench_")
+    paths = mesh.write_config(args.workload, tmp)
+    with quiet_stdout():
+        model = exdat.Model(host).read_inputs(*paths, out=os.path.join(tmp, "ex.dat")).initialize()
+    ne, nn = model.ne(), model.nn()
+    mp = host.aqd2_model_new(model.buf)
+    p = C.c_void_p()
+    check(hip.bqd2_create_large(mp, C.byref(p)), "bqd2_create_large")
+    host.aqd2_model_free(mp)
+    check(hip.bqd2_large_config(p, args.large_rows, args.large_resident), "bqd2_large_config")
+    log("bench: workload %s in the system layout, %d elements, %d unknowns" % (args.workload, ne, 2 * nn))
+    rc3 = (C.c_double * 3)(0.0, 0.0, 0.0)
+
+    def request(solver, assemble):
+        t, st = {}, [AssemblyStats(), AssemblyStats()]
+        t0 = time.perf_counter()
+        if assemble:
+            for d in (0, 1):
+                check(hip.bqd2_assemble(p, d, C.byref(st[d])), "bqd2_assemble")
+                log("  domain %d assembled: %.1f s" % (d, time.perf_counter() - t0))
+        t1 = time.perf_counter()
+        check(hip.bqd2_set_solver(p, solver), "bqd2_set_solver")
+        x = np.zeros(2 * nn, complex)
+        check(hip.bqd2_solve_surface(p, x.ctypes.data), "bqd2_solve_surface")
+        t2 = time.perf_counter()
+        log("  surface solve (solver %d): %.1f s" % (solver, t2 - t1))
+        check(hip.bqd2_solve_velocity(p), "bqd2_solve_velocity")
+        t3 = time.perf_counter()
+        log("  velocity recovery: %.1f s" % (t3 - t2))
+        F, N = np.zeros((2, 3)), np.zeros((2, 3))
+        for typ in (0, 1):
+            f, n = (C.c_double * 3)(), (C.c_double * 3)()
+            check(hip.bqd2_force(p, rc3, typ, f, n), "bqd2_force")
+            F[typ], N[typ] = f[:], n[:]
+        t4 = time.perf_counter()
+        log("  force and torque, both rules: %.1f s" % (t4 - t3))
+        its, res, lu = (C.c_int * 17)(), (C.c_double * 17)(), C.c_int()
+        check(hip.bqd2_solver_info(p, its, res, C.byref(lu)), "bqd2_solver_info")
+        ls = LargeStats()
+        check(hip.bqd2_large_info(p, C.byref(ls)), "bqd2_large_info")
+        t.update(assemble=t1 - t0, solve=t2 - t1, velocity=t3 - t2, force=t4 - t3, total=t4 - t0)
+        return {"x": x, "F": F, "N": N, "t": t, "st": st, "krylov_steps": list(its[:3]),
+                "scaled_residual": list(res[:3]), "lu": lu.value, "layout": ls.as_dict()}
+
+    log("mixed-precision request")
+    mixed = request(2, True)
+    st0 = mixed["st"][0]
+    flops = pair_flops(st0)
+    achieved = flops / (st0.ms_pairs * 1e-3) / 1e12
+    n3 = 8.0 / 3.0 * (2 * nn) ** 3
+    line = {
+        "metric": METRIC, "value": 1.0 / mixed["t"]["total"], "unit": "solves/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": mixed["t"]["total"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 LU + f64 refinement",
+        "data": "synthetic (generated cube-sphere mesh, plane-wave incidence)",
+        "config": {"workload": "%s: %d-quad cube-sphere a=%.2g m, square lattice %.3g m (2.2 radii), ka=6 "
+                               "(f=%.6g Hz), theta=%.3g phi=%.4g; system layout, complex float LU + FP64 GMRES "
+                               "refinement, velocity recovery, force/torque type 0 and 1"
+                               % (args.workload, ne, cfg["radius"], cfg["d1"][0], cfg["f"], cfg["theta"],
+                                  cfg["psi"]),
+                   "elements": ne, "unknowns": 2 * nn, "parallelism": "1 GPU"},
+        "phases_ms": {k: v * 1e3 for k, v in mixed["t"].items()},
+        "layout": mixed["layout"],
+        "solver": {"krylov_steps": mixed["krylov_steps"], "scaled_residual": mixed["scaled_residual"],
+                   "lu_fallbacks": mixed["lu"],
+                   "lu32_tflops_nominal": n3 / (mixed["layout"]["ms_factor"] * 1e-3) / 1e12
+                   if mixed["layout"]["ms_factor"] > 0 else None},
+        "force": {"type0": mixed["F"][0].tolist(), "type1": mixed["F"][1].tolist()},
+        "torque": {"type0": mixed["N"][0].tolist(), "type1": mixed["N"][1].tolist()},
+        "roofline": {"bound": "valu", "pipe": "FP64 vector ALU (MI355X FP64 vector peak 78.6 TFLOP/s)",
+                     "kernel": "k_q0_pairs", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "ms_per_launch": st0.ms_pairs,
+                     "flops_per_launch": flops},
+        "cpu_baseline": None,
+    }
+    if not args.no_lu_reference:
+        log("FP64 LU of the same system (reference solution)")
+        ref = request(1, False)
+        ex = float(np.linalg.norm(mixed["x"] - ref["x"]) / np.linalg.norm(ref["x"]))
+        ef = {t: float(np.linalg.norm(mixed["F"][t] - ref["F"][t]) / np.linalg.norm(ref["F"][t])) for t in (0, 1)}
+        line["vs_fp64_lu"] = {
+            "surface_solution_rel_err": ex, "force_rel_err_type0": ef[0], "force_rel_err_type1": ef[1],
+            "fp64_lu_request_ms": {k: v * 1e3 for k, v in ref["t"].items()}, "fp64_lu_layout": ref["layout"],
+            "fp64_lu_tflops": n3 / (ref["layout"]["ms_factor"] * 1e-3) / 1e12 if ref["layout"]["ms_factor"] else None,
+            "force_fp64_lu": {"type0": ref["F"][0].tolist(), "type1": ref["F"][1].tolist()}}
+        log("mixed vs FP64 LU: surface solution %.2e, force type 0 %.2e, type 1 %.2e" % (ex, ef[0], ef[1]))
+    hip.bqd2_destroy(p)
+    with quiet_stdout():
+        model.finalize()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -644,6 +746,13 @@ def main():
                     help="mesh sizes of the stream: small (96-768 elements) or config (384-6144 elements)")
     ap.add_argument("--serve-solver", type=int, default=2)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--layout", default="", choices=("", "default", "system"),
+                    help="system: the system layout of bqd2_create_large (default for sphere13824)")
+    ap.add_argument("--large-rows", type=int, default=2048, help="system layout: rows per window of the sphere")
+    ap.add_argument("--large-resident", type=int, default=-1,
+                    help="system layout: windows the mixed solver keeps resident (-1: as memory allows)")
+    ap.add_argument("--no-lu-reference", action="store_true",
+                    help="system layout: skip the FP64 LU reference solution")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the steps, copy a device buffer of this size once (a known-traffic kernel that "
                          "calibrates profiler byte counters)")
@@ -657,6 +766,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.layout == "system" or (args.layout == "" and args.workload == "sphere13824"):
+        large_main(args)
+        return
     if args.serving_only:
         sv = serving_bench(args)
         print(json.dumps({"metric": METRIC, "value": sv["serving"]["solves_per_s"], "unit": "solves/s",

@@ -299,6 +299,47 @@ int bqd2_autotune_reset(void);
 int bqd2_lu32_info(bqd2_problem *p, double *ms_factor);
 
 /* ------------------------------------------------------------------------
+ * The system layout (no counterpart in the reference, which writes every
+ * domain matrix to temporary files and reads it back to build the system,
+ * bem3_aw_qd2_solve_bieq.c:55-86, :202-217, then hands it to PARDISO)
+ * ------------------------------------------------------------------------ */
+
+/* A problem whose matrices do not fit one GPU in the default layout: one sphere
+ * (MN = 1) of quadrilaterals, each with its own four nodes, in a periodic lattice
+ * (BQD2_E_ARG otherwise).  Both domains' matrices live in one row-major
+ * 2N x 2N complex double array in the system's own column order, [H0 G0; H1 G1];
+ * the sphere's rows are assembled in windows whenever they are needed, and dG,
+ * dH of the sphere never exist whole.  The calls are those of the default
+ * layout (bqd2_assemble of both domains, bqd2_solve_surface, bqd2_solve_velocity,
+ * bqd2_field, bqd2_force) with these differences:
+ *   bqd2_assemble(p, 1) only decides the sphere's quadrature rules;
+ *   solver 2 keeps its complex float factors in the memory of the sphere's half
+ *     of the array, and the sphere's rows in window buffers as far as device
+ *     memory allows (the others are assembled again in every product of the
+ *     refinement);
+ *   solver 1 (FP64 LU) factors the array in place: the next bqd2_solve_velocity
+ *     assembles domain 0 again;
+ *   the dpv systems of bqd2_solve_velocity use FP64 GMRES whatever the solver. */
+int bqd2_create_large(const bqd2_model *m, bqd2_problem **out);
+/* rows per window of the sphere's rows (default 2048) and the most windows the
+ * mixed solver keeps resident (-1: as many as device memory allows, keeping
+ * 6 GiB free); takes effect at the next bqd2_assemble of domain 1 */
+int bqd2_large_config(bqd2_problem *p, int rows_per_window, int max_resident);
+typedef struct {
+  int windows;             /* windows of the sphere's rows */
+  int resident;            /* of them kept resident by the last mixed solve */
+  int recomputed;          /* window assemblies inside the last solve's products */
+  int pad_;
+  double ms_rows;          /* last solve: assembling the sphere's rows before the solve proper */
+  double ms_build;         /* ... the row scaling, right-hand side and float rounding of the system */
+  double ms_factor;        /* ... the factorisation (complex float, or FP64 LU with its transposition) */
+  double ms_solve;         /* ... GMRES (refinement or FP64) or the FP64 triangular solves */
+  double ms_velocity_rows; /* last velocity recovery: the sphere's rows with dG, dH and the velocity rows */
+  double ms_reassemble;    /* ... assembling domain 0 again after an FP64 LU */
+} bqd2_large_stats;
+int bqd2_large_info(bqd2_problem *p, bqd2_large_stats *st);
+
+/* ------------------------------------------------------------------------
  * Serving (no counterpart in the reference, which solves one model per process:
  * aw_d3qd2_bv_solver.c:5-11, then example1.c:10-48 for fields and forces)
  * ------------------------------------------------------------------------ */
