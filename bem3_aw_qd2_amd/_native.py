@@ -43,6 +43,10 @@ class GfParams(C.Structure):
                [("nc", C.c_int), ("ewald_only", C.c_int), ("fd_lift", C.c_double)]
 
 
+# bqd2_progress_fn: (stage, done, total, ctx)
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_longlong, C.c_longlong, C.c_void_p)
+
+
 class LargeStats(C.Structure):
     """bqd2_large_stats (include/bqd2.h)"""
     _fields_ = [("windows", C.c_int), ("resident", C.c_int), ("recomputed", C.c_int), ("pad_", C.c_int),
@@ -108,6 +112,7 @@ def hip_lib():
         lib.bqd2_large_config.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.bqd2_large_info.argtypes = [C.c_void_p, C.POINTER(LargeStats)]
         lib.bqd2_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.bqd2_set_progress.argtypes = [PROGRESS_FN, C.c_void_p]
         lib.bqd2_destroy.argtypes = [C.c_void_p]
         lib.bqd2_gf_params.argtypes = [C.POINTER(C.c_double), C.POINTER(GfParams)]
         lib.bqd2_gf_batch.argtypes = [C.POINTER(GfParams), C.c_int, C.c_int, C.c_void_p, C.c_double,

@@ -290,38 +290,13 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
 
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
                                                    const int *__restrict__ list, long long nlist,
-                                                   long long *__restrict__ counters) {
-  q0_pairs_wave(M, tab, D, P, list, nlist, counters, blockIdx.x);
-}
-
-// One instance of a batched assembly: the periodic domain of one problem.
-struct PairBatch {
-  DevMesh M;
-  DevDomain D;
-  GfParams P;
-  const Tab *tab;
-  long long *counters;
-  long long wg_end;  // tasks of this and the earlier instances
-};
-
-// The pair tasks of several problems in one persistent launch: workgroup b takes
-// the tasks b, b + gridDim.x, ... of the concatenated task lists (every
-// workgroup stops at the total; no workgroup waits for another).  A pair is
-// computed the same way whichever workgroup takes it.
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_pairs_batch(const PairBatch *__restrict__ inst, int ninst) {
-  const long long total = inst[ninst - 1].wg_end;
-  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-    int i = 0;
-    while (t >= inst[i].wg_end) i++;
-    const PairBatch &B = inst[i];
-    const long long t0 = i ? inst[i - 1].wg_end : 0;
-    q0_pairs_wave(B.M, B.tab, B.D, B.P, B.D.plist, (long long)B.D.plist_n, B.counters, t - t0);
-  }
+                                                   long long *__restrict__ counters, long long wg0) {
+  q0_pairs_wave(M, tab, D, P, list, nlist, counters, wg0 + blockIdx.x);
 }
 
 // ------------------------------------------------------------------ free-space pairs
-__global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  const int s = blockIdx.x + 1, row = D.r0 + blockIdx.y, lane = threadIdx.x;
+__device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D, int s, int row) {
+  const int lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td) || s == t) return;
   const Tab &T = *tab;
@@ -409,6 +384,10 @@ __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restr
   }
 }
 
+__global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  fs_pairs_body(M, tab, D, (int)blockIdx.x + 1, D.r0 + (int)blockIdx.y);
+}
+
 // ------------------------------------------------------------------ self elements
 // static F of the self element at (zt, et): polar rule, wave-parallel over the
 // 4 x 128 x 12 points (bil_coef.c:525-536 integrand, q0_bil_coef.c:315-337 rule)
@@ -433,9 +412,8 @@ __device__ double self_F_wave(const ElemC &E, const Tab &T, double zt, double et
   return wave_sum(acc) * BQD2_IFP * bdC;
 }
 
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
-                                                  long long *__restrict__ counters) {
-  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ void q0_self_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D, const GfParams &P, long long *__restrict__ counters, int row) {
+  const int lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const Tab &T = *tab;
@@ -558,10 +536,15 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   }
 }
 
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                  long long *__restrict__ counters) {
+  q0_self_body(M, tab, D, P, counters, D.r0 + (int)blockIdx.x);
+}
+
 // free-space self element: G, H, F (bil_coef_bd) and the principal-value
 // tangential derivatives dG, dH, dF along t_zeta and t_eta (bil_dcoef.c:369-740)
-__global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ void fs_self_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D, int row) {
+  const int lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const Tab &T = *tab;
@@ -761,6 +744,10 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
   }
 }
 
+__global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  fs_self_body(M, tab, D, D.r0 + (int)blockIdx.x);
+}
+
 // ------------------------------------------------------------------ triangle self elements
 // Polar rules about the node over the three sub-triangles to the edges, two
 // GH64 halves in theta and GL12 in r.  The element is flat: F, dF and the
@@ -770,9 +757,8 @@ __global__ void __launch_bounds__(WAVE) k_fs_self(DevMesh M, const Tab *__restri
 // periodic self element (q0_lit_coef_bd, q0_lit_coef.c:297-372; dC = 0,
 // q0_lit_dcoef_v2_cc.c:331-343), with the same near/smooth split as the
 // quadrilateral self element
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
-                                                      long long *__restrict__ counters) {
-  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ void q0_self_tri_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D, const GfParams &P, long long *__restrict__ counters, int row) {
+  const int lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const int s = abs(td);
@@ -880,11 +866,16 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   }
 }
 
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_q0_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
+                                                      long long *__restrict__ counters) {
+  q0_self_tri_body(M, tab, D, P, counters, D.r0 + (int)blockIdx.x);
+}
+
 // free-space self element: G on the polar rule (lit_coef_bd, lit_coef.c:290-350)
 // and the principal-value dG/dv along both element parameters
 // (lit_dcoef_bd_tz0_PV / _te0_PV, lit_dcoef.c:461-637, :845-952)
-__global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  const int row = D.r0 + blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ void fs_self_tri_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D, int row) {
+  const int lane = threadIdx.x;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
   const int s = abs(td);
@@ -1000,6 +991,57 @@ __global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__re
     }
     double *fp = D.Fpair + dom_fp(D, row, t);
     fp[0] = 0.0; fp[1] = 0.0; fp[2] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
+  fs_self_tri_body(M, tab, D, D.r0 + (int)blockIdx.x);
+}
+
+// ------------------------------------------------------------------ batched assembly
+// One domain of one problem in a batched assembly, with its tasks in order:
+// the pair tasks (periodic domain: the wavefronts of its pair list; free-space
+// domain: one (element, row) pair each, row-major), then one self-row task per
+// row (the quadrilateral rule, then, with triangles, the triangle rule).
+struct AsmBatch {
+  DevMesh M;
+  DevDomain D;
+  GfParams P;
+  const Tab *tab;
+  long long *counters;
+  int periodic;
+  long long n_pair;
+  long long t_end;  // tasks of this and the earlier instances
+};
+
+// Every pair and self-block task of several problems' domains in one
+// persistent launch: workgroup b takes the tasks b, b + gridDim.x, ... of the
+// concatenated task lists (every workgroup stops at the total; none waits for
+// another).  A task is computed exactly as the per-domain kernels compute it.
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_asm_batch(const AsmBatch *__restrict__ inst, int ninst) {
+  const long long total = inst[ninst - 1].t_end;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    int i = 0;
+    while (t >= inst[i].t_end) i++;
+    const AsmBatch &B = inst[i];
+    long long u = t - (i ? inst[i - 1].t_end : 0);
+    if (u < B.n_pair) {
+      if (B.periodic)
+        q0_pairs_wave(B.M, B.tab, B.D, B.P, B.D.plist, (long long)B.D.plist_n, B.counters, u);
+      else
+        fs_pairs_body(B.M, B.tab, B.D, (int)(u % B.D.Ne) + 1, (int)(u / B.D.Ne));
+      continue;
+    }
+    u -= B.n_pair;
+    const int row = (int)(u % B.D.N);
+    const bool tri = u >= B.D.N;
+    if (B.periodic) {
+      if (tri) q0_self_tri_body(B.M, B.tab, B.D, B.P, B.counters, row);
+      else q0_self_body(B.M, B.tab, B.D, B.P, B.counters, row);
+    } else {
+      if (tri) fs_self_tri_body(B.M, B.tab, B.D, row);
+      else fs_self_body(B.M, B.tab, B.D, row);
+    }
   }
 }
 
@@ -1167,11 +1209,13 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
   return BQD2_OK;
 }
 
-static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *st) {
+static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *st, bool self_done = false,
+                   bool own_counters = true) {
   DevDomain &D = p->dom[did];
   const bool periodic = (did == 0 && p->m.ps == 1);
   hipStream_t st_ = p->stream;
-  if (periodic) {
+  if (self_done) {
+  } else if (periodic) {
     hipLaunchKernelGGL(k_q0_self, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
     if (D.has_tri)
       hipLaunchKernelGGL(k_q0_self_tri, dim3(D.N), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, p->d_counters);
@@ -1195,6 +1239,7 @@ static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *
     BQD2_HIP(hipEventElapsedTime(&b, ev.e1b, ev.e2));
     BQD2_HIP(hipEventElapsedTime(&c, ev.e2, ev.e3));
     st->ms_decide = a; st->ms_pairs = b; st->ms_self = c;
+    if (!own_counters && !periodic) memset(cnt + CNT_PAIRS, 0, (CNT_N - CNT_PAIRS) * sizeof(long long));
     for (int i = 0; i < 4; i++) {
       st->pair_terms[i] = cnt[CNT_PAIRS + i];
       st->self_terms[i] = cnt[CNT_SELF + i];
@@ -1266,84 +1311,121 @@ int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   AsmPhase ev;
   int rc = asm_begin(p, did, ev);
   if (rc) return rc;
+  // with a progress callback the pairs go in 16 launches, reported as each ends
+  std::vector<hipEvent_t> parts;
   if (periodic) {
-    if (D.plist_n > 0)
-      hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(D.plist_n / PPW)), dim3(WAVE), 0, st_, p->m, p->tab, D,
-                         p->gf, D.plist, (long long)D.plist_n, p->d_counters);
+    const long long nwg = D.plist_n / PPW;
+    const int np = (bqd2_progress_on() && nwg >= 16) ? 16 : 1;
+    for (int k = 0; k < np; k++) {
+      const long long a = nwg * k / np, b = nwg * (k + 1) / np;
+      if (b > a)
+        hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(b - a)), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, D.plist,
+                           (long long)D.plist_n, p->d_counters, a);
+      if (np > 1) {
+        hipEvent_t e;
+        BQD2_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BQD2_HIP(hipEventRecord(e, st_));
+        parts.push_back(e);
+      }
+    }
   } else {
     hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
   }
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipEventRecord(ev.e2, st_));
+  for (size_t k = 0; k < parts.size(); k++) {
+    BQD2_HIP(hipEventSynchronize(parts[k]));
+    (void)hipEventDestroy(parts[k]);
+    bqd2_progress("periodic pairs", (long long)k + 1, (long long)parts.size());
+  }
   return asm_end(p, did, ev, st);
 }
 
-// No counterpart in the reference.  Domain did of every problem; the
-// off-diagonal pairs of the periodic domains of all of them in one persistent
-// launch on the first problem's stream (other domains as bqd2_assemble).
+// No counterpart in the reference.  Domain did of every problem (did = -1:
+// every domain of every problem): the set-up of each (rules and pair list at
+// the first assembly) on its problem's stream, then every pair and self block
+// of all of them in one persistent launch on the first problem's stream, then
+// the row sums of each.
 extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_assembly_stats *st) {
   BQD2_CHECK(ps && count > 0, BQD2_E_ARG, "bqd2_assemble_batch: no problems");
   for (int i = 0; i < count; i++) {
-    BQD2_CHECK(ps[i] && did >= 0 && did <= ps[i]->m.MN, BQD2_E_ARG, "bqd2_assemble_batch: bad problem %d or domain", i);
+    BQD2_CHECK(ps[i] && did >= -1 && did <= ps[i]->m.MN, BQD2_E_ARG, "bqd2_assemble_batch: bad problem %d or domain",
+               i);
     BQD2_CHECK(!ps[i]->large, BQD2_E_ARG, "bqd2_assemble_batch: problem %d is in the system layout", i);
   }
-  std::vector<AsmPhase> ev(count);
-  std::vector<int> batched;
+  struct Job {
+    bqd2_problem *p;
+    int did;
+    bqd2_assembly_stats *st;
+    AsmPhase ev;
+  };
+  std::vector<Job> jobs;
+  size_t si = 0;
   for (int i = 0; i < count; i++) {
-    bqd2_problem *p = ps[i];
-    if (p->dom[did].Ne == 0) continue;
-    if (!(did == 0 && p->m.ps == 1)) {
-      int rc = bqd2_assemble(p, did, st ? st + i : nullptr);
-      if (rc) return rc;
-      continue;
+    const int d0 = did < 0 ? 0 : did, d1 = did < 0 ? ps[i]->m.MN : did;
+    for (int d = d0; d <= d1; d++, si++) {
+      bqd2_assembly_stats *sp = st ? st + si : nullptr;
+      if (ps[i]->dom[d].Ne == 0) {
+        if (sp) memset(sp, 0, sizeof(*sp));
+        continue;
+      }
+      jobs.push_back(Job{ps[i], d, sp, AsmPhase{}});
     }
-    int rc = asm_begin(p, did, ev[i]);
-    if (rc) return rc;
-    if (p->dom[did].plist_n > 0) batched.push_back(i);
   }
-  if (!batched.empty()) {
-    std::vector<PairBatch> hb(batched.size());
-    long long tasks = 0;
-    for (size_t j = 0; j < batched.size(); j++) {
-      bqd2_problem *p = ps[batched[j]];
-      hb[j].M = p->m;
-      hb[j].D = p->dom[did];
-      hb[j].P = p->gf;
-      hb[j].tab = p->tab;
-      hb[j].counters = p->d_counters;
-      tasks += p->dom[did].plist_n / PPW;
-      hb[j].wg_end = tasks;
+  if (jobs.empty()) return BQD2_OK;
+  // the counters of a problem are cleared once, before its first domain's set-up
+  for (size_t j = 0; j < jobs.size(); j++) {
+    const bool first_of_problem = j == 0 || jobs[j - 1].p != jobs[j].p;
+    if (!first_of_problem) {
+      // asm_begin clears the counters; the problem's earlier domain has not run yet
     }
-    hipStream_t bs = ps[batched[0]]->stream;
-    // the instance table in the first problem's workspace (its stream runs the launch)
-    char *wsb = nullptr;
-    int wrc = bqd2_workspace(&ps[batched[0]]->main, hb.size() * sizeof(PairBatch), &wsb);
-    if (wrc) return wrc;
-    PairBatch *d_inst = (PairBatch *)wsb;
-    BQD2_HIP(hipMemcpyAsync(d_inst, hb.data(), hb.size() * sizeof(PairBatch), hipMemcpyHostToDevice, bs));
-    // the launch waits for every problem's set-up, every problem waits for it
-    for (int i : batched) {
-      BQD2_HIP(hipStreamWaitEvent(bs, ev[i].e1b, 0));
-      BQD2_HIP(hipEventRecord(ev[i].e1b, bs));  // the pair time is the batch's
-    }
-    int dev = 0, ncu = 0;
-    BQD2_HIP(hipGetDevice(&dev));
-    BQD2_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const long long grid = std::min<long long>(tasks, (long long)ncu * 8);
-    hipLaunchKernelGGL(k_q0_pairs_batch, dim3((unsigned)grid), dim3(WAVE), 0, bs, d_inst, (int)hb.size());
-    BQD2_HIP(hipGetLastError());
-    for (int i : batched) {
-      BQD2_HIP(hipEventRecord(ev[i].e2, bs));
-      BQD2_HIP(hipStreamWaitEvent(ps[i]->stream, ev[i].e2, 0));
-    }
-    BQD2_HIP(hipStreamSynchronize(bs));
-  }
-  for (int i = 0; i < count; i++) {
-    bqd2_problem *p = ps[i];
-    if (p->dom[did].Ne == 0 || !(did == 0 && p->m.ps == 1)) continue;
-    if (p->dom[did].plist_n == 0) BQD2_HIP(hipEventRecord(ev[i].e2, p->stream));
-    int rc = asm_end(p, did, ev[i], st ? st + i : nullptr);
+    int rc = asm_begin(jobs[j].p, jobs[j].did, jobs[j].ev);
     if (rc) return rc;
+  }
+  std::vector<AsmBatch> hb(jobs.size());
+  long long tasks = 0;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    bqd2_problem *p = jobs[j].p;
+    const DevDomain &D = p->dom[jobs[j].did];
+    const bool periodic = (jobs[j].did == 0 && p->m.ps == 1);
+    hb[j].M = p->m;
+    hb[j].D = D;
+    hb[j].P = p->gf;
+    hb[j].tab = p->tab;
+    hb[j].counters = p->d_counters;
+    hb[j].periodic = periodic ? 1 : 0;
+    hb[j].n_pair = periodic ? (long long)(D.plist_n / PPW) : (long long)D.Ne * D.N;
+    tasks += hb[j].n_pair + (long long)D.N * (D.has_tri ? 2 : 1);
+    hb[j].t_end = tasks;
+  }
+  hipStream_t bs = jobs[0].p->stream;
+  // the instance table in the first problem's workspace (its stream runs the launch)
+  char *wsb = nullptr;
+  int rc = bqd2_workspace(&jobs[0].p->main, hb.size() * sizeof(AsmBatch), &wsb);
+  if (rc) return rc;
+  AsmBatch *d_inst = (AsmBatch *)wsb;
+  BQD2_HIP(hipMemcpyAsync(d_inst, hb.data(), hb.size() * sizeof(AsmBatch), hipMemcpyHostToDevice, bs));
+  // the launch waits for every set-up; every problem waits for the launch
+  for (auto &j : jobs) {
+    BQD2_HIP(hipStreamWaitEvent(bs, j.ev.e1b, 0));
+    BQD2_HIP(hipEventRecord(j.ev.e1b, bs));  // the pair time is the batch's
+  }
+  int dev = 0, ncu = 0;
+  BQD2_HIP(hipGetDevice(&dev));
+  BQD2_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const long long grid = std::min<long long>(tasks, (long long)ncu * 8);
+  hipLaunchKernelGGL(k_asm_batch, dim3((unsigned)grid), dim3(WAVE), 0, bs, d_inst, (int)hb.size());
+  BQD2_HIP(hipGetLastError());
+  for (auto &j : jobs) {
+    BQD2_HIP(hipEventRecord(j.ev.e2, bs));
+    BQD2_HIP(hipStreamWaitEvent(j.p->stream, j.ev.e2, 0));
+  }
+  BQD2_HIP(hipStreamSynchronize(bs));
+  for (auto &j : jobs) {
+    // several domains of one problem share its counters: each domain's
+    // statistics keep the periodic domain's terms only
+    const bool shared = did < 0 && j.p->m.MN > 0;
+    if ((rc = asm_end(j.p, j.did, j.ev, j.st, true, !shared))) return rc;
   }
   return BQD2_OK;
 }

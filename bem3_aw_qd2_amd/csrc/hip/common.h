@@ -24,3 +24,8 @@ void bqd2_set_error(const char *fmt, ...);
   } while (0)
 
 static inline int bqd2_blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// the process-wide progress callback (bqd2_set_progress): whether one is set,
+// and a report from the calling thread
+bool bqd2_progress_on();
+void bqd2_progress(const char *stage, long long done, long long total);

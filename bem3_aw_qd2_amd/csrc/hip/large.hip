@@ -220,6 +220,15 @@ static int lg_rows(bqd2_problem *p, int w, double **rows, int *recomputed, hipSt
   return BQD2_OK;
 }
 
+// a progress report every 8 windows of a pass over the sphere's rows
+static int lg_report(bqd2_problem *p, const char *stage, int done, hipStream_t st) {
+  const int nw = lg_nwin(p);
+  if (!bqd2_progress_on() || (done % 8 != 0 && done != nw)) return BQD2_OK;
+  BQD2_HIP(hipStreamSynchronize(st));
+  bqd2_progress(stage, done, nw);
+  return BQD2_OK;
+}
+
 // every window of the sphere's rows assembled into S's lower half
 static int lg_fill_lower(bqd2_problem *p, hipStream_t st) {
   if (p->s_lower == LG_G1H1) return BQD2_OK;
@@ -231,6 +240,7 @@ static int lg_fill_lower(bqd2_problem *p, hipStream_t st) {
   for (int w0 = 0; w0 < N; w0 += p->lwin) {
     const int nr = std::min(p->lwin, N - w0);
     if ((rc = bqd2_fs_rows(p, lg_win(p, w0, nr, lg_lower(p, w0), nullptr, nullptr), st))) return rc;
+    lg_report(p, "sphere rows", w0 / p->lwin + 1, st);
   }
   p->s_lower = LG_G1H1;
   return BQD2_OK;
@@ -343,6 +353,7 @@ static int lg_mixed_build(bqd2_problem *p, cd hk2, bool factor, float bound, hip
     }
     BQD2_HIP(hipStreamSynchronize(st));
     ms_rows += lg_now() - tr;
+    lg_report(p, "float system rows", w + 1, st);
     double *bw = p->b + 2 * ((size_t)N + w0);
     lg_mv_launch<4>(nr, na, rows, na, ub, nullptr, cd{-1.0, 0.0}, bw, st);
     if (factor) {
@@ -528,6 +539,7 @@ int bqd2_lg_velocity(bqd2_problem *p, const void *W, double *open_out) {
     const DevDomain Dw = lg_win(p, w0, nr, lg_lower(p, w0), p->ldgw, p->ldhw);
     if ((rc = bqd2_fs_rows(p, Dw, st))) return rc;
     if ((rc = bqd2_velocity_rows(p, Dw, W, open_out, st))) return rc;
+    lg_report(p, "sphere rows with dG, dH", w0 / p->lwin + 1, st);
   }
   p->s_lower = LG_G1H1;
   lg_free_velocity(p);

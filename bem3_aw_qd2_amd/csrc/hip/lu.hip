@@ -374,6 +374,12 @@ extern "C" int bqd2_zgetrf(int n, double *d_A, int *ipiv, void *stream) {
   int rc = zws_get(&W);
   if (rc) return rc;
   hipLaunchKernelGGL(k_zinit_flag, dim3(1), dim3(64), 0, st, W.flag, n);
+  // with a progress callback: an event every 64 panels, the one before waited
+  // for and reported (two parts in flight)
+  const int npan = (n + NB - 1) / NB, every = 64;
+  const bool prog = bqd2_progress_on() && npan >= 2 * every;
+  hipEvent_t pe[2] = {nullptr, nullptr};
+  int pk = 0;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = std::min(NB, n - k0);
     const int rows = n - k0, rest = rows - kb;
@@ -394,7 +400,20 @@ extern "C" int bqd2_zgetrf(int n, double *d_A, int *ipiv, void *stream) {
       if (rc) return rc;
     }
     BQD2_HIP(hipGetLastError());
+    const int pan = k0 / NB + 1;
+    if (prog && pan % every == 0) {
+      hipEvent_t &e = pe[pk & 1];
+      if (!e) BQD2_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      BQD2_HIP(hipEventRecord(e, st));
+      pk++;
+      if (pk >= 2) {
+        BQD2_HIP(hipEventSynchronize(pe[pk & 1]));
+        bqd2_progress("FP64 LU panels", (long long)(pan - every), npan);
+      }
+    }
   }
+  for (hipEvent_t e : pe)
+    if (e) (void)hipEventDestroy(e);
   int h_flag = n;
   BQD2_HIP(hipMemcpyAsync(&h_flag, W.flag, sizeof(int), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));

@@ -30,6 +30,19 @@ void bqd2_release(bqd2_problem *p, void **ptr, size_t bytes) {
   *ptr = nullptr;
 }
 
+static bqd2_progress_fn g_progress = nullptr;
+static void *g_progress_ctx = nullptr;
+extern "C" int bqd2_set_progress(bqd2_progress_fn fn, void *ctx) {
+  g_progress = fn;
+  g_progress_ctx = ctx;
+  return BQD2_OK;
+}
+bool bqd2_progress_on() { return g_progress != nullptr; }
+void bqd2_progress(const char *stage, long long done, long long total) {
+  bqd2_progress_fn fn = g_progress;
+  if (fn) fn(stage, done, total, g_progress_ctx);
+}
+
 void bqd2_rebind(bqd2_problem *p) { p->owner = std::this_thread::get_id(); }
 
 size_t bqd2_bytes(bqd2_problem *p) {

@@ -613,8 +613,10 @@ def large_main(args):
     domains, surface solve, velocity recovery, force and torque of both rules),
     then the FP64 LU of the same system as the reference solution (its velocity
     recovery assembles domain 0 again), and the differences between the two."""
+    import torch
     from bem3_aw_qd2_amd import AssemblyStats, LargeStats, check, exdat, hip_lib, host_lib, mesh
     hip, host = hip_lib(), host_lib()
+    hip.bqd2_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     check(hip.bqd2_set_autotune(args.autotune), "bqd2_set_autotune")
     cfg = mesh.CONFIGS[args.workload]
     tmp = tempfile.mkdtemp(prefix="bqd2_bench_")
@@ -628,6 +630,13 @@ def large_main(args):
     host.aqd2_model_free(mp)
     check(hip.bqd2_large_config(p, args.large_rows, args.large_resident), "bqd2_large_config")
     log("bench: workload %s in the system layout, %d elements, %d unknowns" % (args.workload, ne, 2 * nn))
+    from bem3_aw_qd2_amd._native import PROGRESS_FN
+    t_start = time.perf_counter()
+
+    def progress(stage, done, total, _ctx):
+        log("    %s: %d/%d (%.0f s)" % (stage.decode(), done, total, time.perf_counter() - t_start))
+    progress_cb = PROGRESS_FN(progress)  # kept alive until the problem is destroyed
+    check(hip.bqd2_set_progress(progress_cb, None), "bqd2_set_progress")
     rc3 = (C.c_double * 3)(0.0, 0.0, 0.0)
 
     def request(solver, assemble):
@@ -657,9 +666,14 @@ def large_main(args):
         check(hip.bqd2_solver_info(p, its, res, C.byref(lu)), "bqd2_solver_info")
         ls = LargeStats()
         check(hip.bqd2_large_info(p, C.byref(ls)), "bqd2_large_info")
+        nb = C.c_size_t()
+        check(hip.bqd2_device_bytes(p, C.byref(nb)), "bqd2_device_bytes")
+        free_b, total_b = torch.cuda.mem_get_info()
         t.update(assemble=t1 - t0, solve=t2 - t1, velocity=t3 - t2, force=t4 - t3, total=t4 - t0)
         return {"x": x, "F": F, "N": N, "t": t, "st": st, "krylov_steps": list(its[:3]),
-                "scaled_residual": list(res[:3]), "lu": lu.value, "layout": ls.as_dict()}
+                "scaled_residual": list(res[:3]), "lu": lu.value,
+                "layout": dict(ls.as_dict(), problem_device_bytes=nb.value, device_free_bytes=free_b,
+                               device_total_bytes=total_b)}
 
     log("mixed-precision request")
     mixed = request(2, True)
@@ -704,6 +718,7 @@ def large_main(args):
             "force_fp64_lu": {"type0": ref["F"][0].tolist(), "type1": ref["F"][1].tolist()}}
         log("mixed vs FP64 LU: surface solution %.2e, force type 0 %.2e, type 1 %.2e" % (ex, ef[0], ef[1]))
     hip.bqd2_destroy(p)
+    check(hip.bqd2_set_progress(PROGRESS_FN(), None), "bqd2_set_progress")
     with quiet_stdout():
         model.finalize()
     print(json.dumps(line), flush=True)

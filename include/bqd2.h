@@ -339,6 +339,14 @@ typedef struct {
 } bqd2_large_stats;
 int bqd2_large_info(bqd2_problem *p, bqd2_large_stats *st);
 
+/* No counterpart in the reference.  A process-wide progress callback for calls
+ * that run for minutes at large sizes: fn(stage, done, total, ctx) is called on
+ * the calling thread as parts of the periodic pair assembly, of the FP64 LU and
+ * of the system layout's window passes complete.  NULL (the default): none, and
+ * the work is launched without the waits that the reports need. */
+typedef void (*bqd2_progress_fn)(const char *stage, long long done, long long total, void *ctx);
+int bqd2_set_progress(bqd2_progress_fn fn, void *ctx);
+
 /* ------------------------------------------------------------------------
  * Serving (no counterpart in the reference, which solves one model per process:
  * aw_d3qd2_bv_solver.c:5-11, then example1.c:10-48 for fields and forces)

@@ -17,6 +17,7 @@ def declared(header):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
     text = re.sub(r"typedef struct[^;]*?\}[^;]*;", "", text, flags=re.S)
+    text = re.sub(r"typedef[^{;]*;", "", text)  # function-pointer and other brace-less typedefs
     names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "while", "return")))
 
