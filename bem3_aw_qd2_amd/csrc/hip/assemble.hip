@@ -1373,13 +1373,9 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
     }
   }
   if (jobs.empty()) return BQD2_OK;
-  // the counters of a problem are cleared once, before its first domain's set-up
-  for (size_t j = 0; j < jobs.size(); j++) {
-    const bool first_of_problem = j == 0 || jobs[j - 1].p != jobs[j].p;
-    if (!first_of_problem) {
-      // asm_begin clears the counters; the problem's earlier domain has not run yet
-    }
-    int rc = asm_begin(jobs[j].p, jobs[j].did, jobs[j].ev);
+  // every set-up clears its problem's counters; all of them precede the launch
+  for (auto &j : jobs) {
+    int rc = asm_begin(j.p, j.did, j.ev);
     if (rc) return rc;
   }
   std::vector<AsmBatch> hb(jobs.size());

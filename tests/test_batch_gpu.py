@@ -1,5 +1,8 @@
-"""Assembly of several problems' periodic domains in one persistent launch
-(bqd2_assemble_batch) against assembly one problem at a time (bqd2_assemble).
+"""Assembly of several problems in one persistent launch (bqd2_assemble_batch)
+against assembly one problem at a time (bqd2_assemble): the periodic domains
+alone (did = 0), and every domain of every problem (did = -1: the periodic
+pairs and self blocks, the free-space pairs with their tangential derivatives
+and self blocks, of quadrilateral and triangle meshes, all in one launch).
 
 Three uneven instances (24 quads on the 2 mm square lattice, 96 quads on the
 1.8 mm triangular lattice at theta = 30 deg, 108 triangles on a 2.4 mm square
@@ -33,12 +36,16 @@ def problem(tmp, name, over, tri, periodic):
     return m, host_lib().aqd2_problem(m.buf, None)
 
 
-def matrices(prob, n):
+def matrices(prob, n, did=0, derivs=False):
     G = np.zeros(n * n * 2)
     H = np.zeros(n * n * 2)
     dF = np.zeros(n * 3)
-    assert hip_lib().bqd2_download_domain(prob, 0, G.ctypes.data, H.ctypes.data, None, None, dF.ctypes.data) == 0
-    return G, H, dF
+    dG = np.zeros(4 * n * n) if derivs else None
+    dH = np.zeros(4 * n * n) if derivs else None
+    assert hip_lib().bqd2_download_domain(prob, did, G.ctypes.data, H.ctypes.data,
+                                          dG.ctypes.data if derivs else None, dH.ctypes.data if derivs else None,
+                                          dF.ctypes.data) == 0
+    return (G, H, dF) + ((dG, dH) if derivs else ())
 
 
 def test_batch_equals_one_at_a_time():
@@ -68,3 +75,41 @@ def test_batch_equals_one_at_a_time():
             assert bst[i].pair_evals == sstats[i].pair_evals
         for m in models:
             m.finalize()
+
+
+def test_batch_all_domains_equals_one_at_a_time():
+    hip = hip_lib()
+    with tempfile.TemporaryDirectory() as tmp:
+        models, probs = [], []
+        for i, (name, over, tri, per) in enumerate(INSTANCES):
+            m, p = problem(os.path.join(tmp, str(i)), name, over, tri, per)
+            models.append(m)
+            probs.append(p)
+        single, sstats = [], []
+        for i, (m, p) in enumerate(zip(models, probs)):
+            per_dom = []
+            for d in range(m.mn() + 1):
+                st = AssemblyStats()
+                assert hip.bqd2_assemble(p, d, C.byref(st)) == 0
+                sstats.append(st)
+            for d in range(m.mn() + 1):
+                per_dom.append(matrices(p, 4 * m.ne(), d, derivs=(d == 1 or not INSTANCES[i][3])))
+            single.append(per_dom)
+        arr = (C.c_void_p * len(probs))(*probs)
+        bst = (AssemblyStats * len(sstats))()
+        assert hip.bqd2_assemble_batch(arr, len(probs), -1, bst) == 0
+        k = 0
+        for i, (m, p) in enumerate(zip(models, probs)):
+            for d in range(m.mn() + 1):
+                got = matrices(p, 4 * m.ne(), d, derivs=(d == 1 or not INSTANCES[i][3]))
+                same = [np.array_equal(a, b) for a, b in zip(got, single[i][d])]
+                print("instance %d domain %d (%d elements): G, H, dF(, dG, dH) identical %s; self terms %s / %s"
+                      % (i, d, m.ne(), same, list(bst[k].self_terms), list(sstats[k].self_terms)))
+                assert all(same)
+                if d == 0:
+                    assert list(bst[k].pair_terms) == list(sstats[k].pair_terms)
+                    assert list(bst[k].self_terms) == list(sstats[k].self_terms)
+                k += 1
+        for m in models:
+            m.finalize()
+
