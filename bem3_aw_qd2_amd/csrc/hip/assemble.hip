@@ -999,50 +999,62 @@ __global__ void __launch_bounds__(WAVE) k_fs_self_tri(DevMesh M, const Tab *__re
 }
 
 // ------------------------------------------------------------------ batched assembly
-// One domain of one problem in a batched assembly, with its tasks in order:
-// the pair tasks (periodic domain: the wavefronts of its pair list; free-space
-// domain: one (element, row) pair each, row-major), then one self-row task per
-// row (the quadrilateral rule, then, with triangles, the triangle rule).
+// One domain of one problem in a batched assembly.  Its tasks fall in six
+// kinds, each run for all instances by one persistent launch: periodic pairs
+// (the wavefronts of its pair list), periodic self rows (quadrilateral rule,
+// triangle rule), free-space pairs (one (element, row) pair each, row-major) and
+// free-space self rows (quadrilateral rule, triangle rule).
+enum { BK_Q0_PAIRS, BK_Q0_SELF, BK_Q0_SELF_TRI, BK_FS_PAIRS, BK_FS_SELF, BK_FS_SELF_TRI, BK_N };
 struct AsmBatch {
   DevMesh M;
   DevDomain D;
   GfParams P;
   const Tab *tab;
   long long *counters;
-  int periodic;
-  long long n_pair;
-  long long t_end;  // tasks of this and the earlier instances
+  long long t_end[BK_N];  // tasks of each kind in this and the earlier instances
 };
 
-// Every pair and self-block task of several problems' domains in one
-// persistent launch: workgroup b takes the tasks b, b + gridDim.x, ... of the
-// concatenated task lists (every workgroup stops at the total; none waits for
-// another).  A task is computed exactly as the per-domain kernels compute it.
+// Task t of one kind among the concatenated task lists of the instances; a task
+// is computed exactly as the per-domain kernels compute it.
+template <int KIND>
+__device__ __forceinline__ void batch_task(const AsmBatch *__restrict__ inst, long long t) {
+  int i = 0;
+  while (t >= inst[i].t_end[KIND]) i++;
+  // one instance per wavefront: a uniform index keeps the instance's fields in
+  // scalar registers, as kernel arguments would be
+  i = __builtin_amdgcn_readfirstlane(i);
+  const long long u = t - (i ? inst[i - 1].t_end[KIND] : 0);
+  const DevMesh &M = inst[i].M;
+  const DevDomain &D = inst[i].D;
+  const GfParams &P = inst[i].P;
+  const Tab *tab = inst[i].tab;
+  long long *counters = inst[i].counters;
+  // row and element indices uniform too, as blockIdx is in the per-domain kernels
+  const int row = __builtin_amdgcn_readfirstlane((int)(KIND == BK_FS_PAIRS ? u / D.Ne : u));
+  if (KIND == BK_Q0_PAIRS) q0_pairs_wave(M, tab, D, P, D.plist, (long long)D.plist_n, counters, u);
+  if (KIND == BK_Q0_SELF) q0_self_body(M, tab, D, P, counters, row);
+  if (KIND == BK_Q0_SELF_TRI) q0_self_tri_body(M, tab, D, P, counters, row);
+  if (KIND == BK_FS_PAIRS) fs_pairs_body(M, tab, D, __builtin_amdgcn_readfirstlane((int)(u % D.Ne)) + 1, row);
+  if (KIND == BK_FS_SELF) fs_self_body(M, tab, D, row);
+  if (KIND == BK_FS_SELF_TRI) fs_self_tri_body(M, tab, D, row);
+}
+
+// The tasks of one kind for several problems' domains in one launch, one
+// workgroup per task over the concatenated task lists.  (A persistent form, a
+// grid of 8 workgroups per CU striding over the tasks, measured 11-15 % slower
+// on the pair kernels than this grid, which the hardware dispatches as
+// workgroups finish.)
+template <int KIND>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_asm_batch(const AsmBatch *__restrict__ inst, int ninst) {
-  const long long total = inst[ninst - 1].t_end;
-  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-    int i = 0;
-    while (t >= inst[i].t_end) i++;
-    const AsmBatch &B = inst[i];
-    long long u = t - (i ? inst[i - 1].t_end : 0);
-    if (u < B.n_pair) {
-      if (B.periodic)
-        q0_pairs_wave(B.M, B.tab, B.D, B.P, B.D.plist, (long long)B.D.plist_n, B.counters, u);
-      else
-        fs_pairs_body(B.M, B.tab, B.D, (int)(u % B.D.Ne) + 1, (int)(u / B.D.Ne));
-      continue;
-    }
-    u -= B.n_pair;
-    const int row = (int)(u % B.D.N);
-    const bool tri = u >= B.D.N;
-    if (B.periodic) {
-      if (tri) q0_self_tri_body(B.M, B.tab, B.D, B.P, B.counters, row);
-      else q0_self_body(B.M, B.tab, B.D, B.P, B.counters, row);
-    } else {
-      if (tri) fs_self_tri_body(B.M, B.tab, B.D, row);
-      else fs_self_body(B.M, B.tab, B.D, row);
-    }
-  }
+  const long long t = blockIdx.x;
+  if (t < inst[ninst - 1].t_end[KIND]) batch_task<KIND>(inst, t);
+}
+// the free-space kinds with the per-domain kernels' register bound (no wave
+// count fixed: the self rows then keep their registers, 0 B spilled)
+template <int KIND>
+__global__ void __launch_bounds__(WAVE) k_asm_batch_fs(const AsmBatch *__restrict__ inst, int ninst) {
+  const long long t = blockIdx.x;
+  if (t < inst[ninst - 1].t_end[KIND]) batch_task<KIND>(inst, t);
 }
 
 // per-row F sums in source order and the diagonal term (bem3_aw_qd2_solve_bieq.c:185-198)
@@ -1263,6 +1275,7 @@ static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *
   D.assembled = true;
   D.g_lu_valid = false;
   p->lu32_valid = false;
+  p->lu64_valid = false;
   return BQD2_OK;
 }
 
@@ -1343,9 +1356,9 @@ int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
 
 // No counterpart in the reference.  Domain did of every problem (did = -1:
 // every domain of every problem): the set-up of each (rules and pair list at
-// the first assembly) on its problem's stream, then every pair and self block
-// of all of them in one persistent launch on the first problem's stream, then
-// the row sums of each.
+// the first assembly) on its problem's stream, then the pairs and self rows of
+// all of them, one launch per task kind on the first problem's stream, then the
+// row sums of each.
 extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_assembly_stats *st) {
   BQD2_CHECK(ps && count > 0, BQD2_E_ARG, "bqd2_assemble_batch: no problems");
   for (int i = 0; i < count; i++) {
@@ -1379,7 +1392,7 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
     if (rc) return rc;
   }
   std::vector<AsmBatch> hb(jobs.size());
-  long long tasks = 0;
+  long long tasks[BK_N] = {0, 0, 0, 0, 0, 0};
   for (size_t j = 0; j < jobs.size(); j++) {
     bqd2_problem *p = jobs[j].p;
     const DevDomain &D = p->dom[jobs[j].did];
@@ -1389,10 +1402,17 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
     hb[j].P = p->gf;
     hb[j].tab = p->tab;
     hb[j].counters = p->d_counters;
-    hb[j].periodic = periodic ? 1 : 0;
-    hb[j].n_pair = periodic ? (long long)(D.plist_n / PPW) : (long long)D.Ne * D.N;
-    tasks += hb[j].n_pair + (long long)D.N * (D.has_tri ? 2 : 1);
-    hb[j].t_end = tasks;
+    const long long rows = D.N, rows_tri = D.has_tri ? D.N : 0;
+    if (periodic) {
+      tasks[BK_Q0_PAIRS] += D.plist_n / PPW;
+      tasks[BK_Q0_SELF] += rows;
+      tasks[BK_Q0_SELF_TRI] += rows_tri;
+    } else {
+      tasks[BK_FS_PAIRS] += (long long)D.Ne * D.N;
+      tasks[BK_FS_SELF] += rows;
+      tasks[BK_FS_SELF_TRI] += rows_tri;
+    }
+    for (int k = 0; k < BK_N; k++) hb[j].t_end[k] = tasks[k];
   }
   hipStream_t bs = jobs[0].p->stream;
   // the instance table in the first problem's workspace (its stream runs the launch)
@@ -1406,11 +1426,17 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
     BQD2_HIP(hipStreamWaitEvent(bs, j.ev.e1b, 0));
     BQD2_HIP(hipEventRecord(j.ev.e1b, bs));  // the pair time is the batch's
   }
-  int dev = 0, ncu = 0;
-  BQD2_HIP(hipGetDevice(&dev));
-  BQD2_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const long long grid = std::min<long long>(tasks, (long long)ncu * 8);
-  hipLaunchKernelGGL(k_asm_batch, dim3((unsigned)grid), dim3(WAVE), 0, bs, d_inst, (int)hb.size());
+  // one launch per task kind over every instance's tasks
+  auto launch = [&](int kind, void (*kern)(const AsmBatch *, int)) {
+    if (tasks[kind] > 0)
+      hipLaunchKernelGGL(kern, dim3((unsigned)tasks[kind]), dim3(WAVE), 0, bs, d_inst, (int)hb.size());
+  };
+  launch(BK_Q0_PAIRS, k_asm_batch<BK_Q0_PAIRS>);
+  launch(BK_FS_PAIRS, k_asm_batch_fs<BK_FS_PAIRS>);
+  launch(BK_Q0_SELF, k_asm_batch<BK_Q0_SELF>);
+  launch(BK_Q0_SELF_TRI, k_asm_batch<BK_Q0_SELF_TRI>);
+  launch(BK_FS_SELF, k_asm_batch_fs<BK_FS_SELF>);
+  launch(BK_FS_SELF_TRI, k_asm_batch_fs<BK_FS_SELF_TRI>);
   BQD2_HIP(hipGetLastError());
   for (auto &j : jobs) {
     BQD2_HIP(hipEventRecord(j.ev.e2, bs));

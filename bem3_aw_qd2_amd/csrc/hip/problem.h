@@ -115,6 +115,7 @@ struct bqd2_problem {
   double *dinv;            // na complex: inverse diagonal of the system
   hipGraphExec_t lu_graph; // the triangular sweeps of its factors as a graph
   bool lu32_valid;
+  bool lu64_valid;         // p->A holds the FP64 LU factors of the current assembly (with ipiv)
   float lu32_growth;       // largest block-local panel growth kept in the factors
   double ms_factor;        // time of the last factorisation (conversion + LU)
   double *hk2d;            // k2[m] / k2[s] table (16 x 16 complex), set once

@@ -98,8 +98,10 @@ __global__ void __launch_bounds__(64) k_system_b(DevMesh M, DevDomain D, int row
     for (int l = 0; l < nl; l++) {
       const int col = M.eni[4 * asd + l];
       cd h = hk2 * cd{Hr[2 * l], Hr[2 * l + 1]};
-      double *a = A + 2 * sys_at(rm, arow, col, na);
-      a[0] = h.x; a[1] = h.y;
+      if (A) {
+        double *a = A + 2 * sys_at(rm, arow, col, na);
+        a[0] = h.x; a[1] = h.y;
+      }
       if (mdid == 0) {
         cd pi = cd{M.Pi[2 * (4 * asd + l)], M.Pi[2 * (4 * asd + l) + 1]};
         cd dpi = cd{M.dPi[2 * (4 * asd + l)], M.dPi[2 * (4 * asd + l) + 1]};
@@ -395,9 +397,25 @@ static int ensure_hk2(bqd2_problem *p) {
   return BQD2_OK;
 }
 
+// the right-hand side of the system alone (the matrix is kept: its LU factors)
+static int build_rhs(bqd2_problem *p, hipStream_t st) {
+  BQD2_HIP(hipMemsetAsync(p->b, 0, (size_t)p->na * 16, st));
+  int rowoff = 0;
+  for (int d = 0; d <= p->m.MN; d++) {
+    DevDomain &D = p->dom[d];
+    if (D.Ne == 0) continue;
+    hipLaunchKernelGGL(k_system_b, dim3(D.N), dim3(64), 0, st, p->m, D, rowoff, (double *)nullptr, p->na, 0, p->hk2d,
+                       p->b);
+    BQD2_HIP(hipGetLastError());
+    rowoff += D.nn;
+  }
+  return BQD2_OK;
+}
+
 // the dense system of all domains, column-major (rm = 0) or row-major (rm = 1)
 static int build_system(bqd2_problem *p, int rm, hipStream_t st) {
   const int na = p->na;
+  p->lu64_valid = false;
   BQD2_HIP(hipMemsetAsync(p->A, 0, (size_t)na * na * 16, st));
   BQD2_HIP(hipMemsetAsync(p->b, 0, (size_t)na * 16, st));
   int rowoff = 0;
@@ -540,10 +558,16 @@ extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
     if (rc == 0) x = p->xg;
   }
   if (!x) {
-    // dense LU with partial pivoting on the column-major system
+    // dense LU with partial pivoting on the column-major system; the factors
+    // stay valid for later right-hand sides on the same assembly
     p->info_lu++;
-    if ((rc = build_system(p, 0, st))) return rc;
-    if ((rc = bqd2_zgetrf(na, p->A, p->ipiv, st))) return rc;
+    if (p->lu64_valid) {
+      if ((rc = build_rhs(p, st))) return rc;
+    } else {
+      if ((rc = build_system(p, 0, st))) return rc;
+      if ((rc = bqd2_zgetrf(na, p->A, p->ipiv, st))) return rc;
+      p->lu64_valid = true;
+    }
     if ((rc = bqd2_zgetrs(na, 1, p->A, p->ipiv, p->b, st))) return rc;
     x = p->b;
   }

@@ -482,6 +482,159 @@ def batch_leg(args, hip, host):
     return out
 
 
+def diffraction_incidences(k, d1, d2, theta0, psi0, count=16, max_theta=math.pi / 3):
+    """`count` incidences that share one quasi-periodic operator: the plane wave
+    (theta0, psi0) and the directions of its propagating diffraction orders with
+    theta <= max_theta (in-plane wave vectors k_par + m1 b1 + m2 b2, b the
+    reciprocal lattice), nearest orders first; amplitudes and phases vary, and
+    directions repeat when there are fewer than `count` orders."""
+    det = d1[0] * d2[1] - d1[1] * d2[0]
+    b1 = np.array([d2[1], -d2[0]]) * (2 * math.pi / det)
+    b2 = np.array([-d1[1], d1[0]]) * (2 * math.pi / det)
+    kp = k * math.sin(theta0) * np.array([math.cos(psi0), math.sin(psi0)])
+    dirs = []
+    for m1 in range(-12, 13):
+        for m2 in range(-12, 13):
+            q = kp + m1 * b1 + m2 * b2
+            sq = float(np.hypot(q[0], q[1])) / k
+            if 1e-6 < sq <= math.sin(max_theta):
+                dirs.append((m1 * m1 + m2 * m2, m1, m2, math.asin(sq), math.atan2(q[1], q[0])))
+    dirs.sort()
+    out = []
+    for j in range(count):
+        _, _, _, th, ps = dirs[j % len(dirs)]
+        amp = complex(np.exp(2j * math.pi * j / count) * (0.5 + j / count))
+        out.append((amp, th, ps))
+    return out, len(dirs)
+
+
+def batch_config_leg(args):
+    """The batched workload at its full size: 64 instances, 6 n^2 quads with n
+    drawn from {8, 12, 16, 20, 24} (384-3456 quads), ka uniform in [0.5, 3],
+    square or triangular lattice with period / radius uniform in [3, 6], seeded.
+    Instances are created in waves that fit the device-memory budget; per wave:
+    assembly one instance (and domain) at a time, then every pair and self block
+    of every instance in one launch per task kind (bqd2_assemble_batch, did = -1),
+    with the GPU time of each from HIP events; then each instance's 16
+    incidences (its plane wave and the directions of its propagating
+    diffraction orders with theta <= 60 deg, which share its operator) solved
+    with FP64 GMRES each (plain) and with the complex float factors reused by
+    all 16 and FP64 refinement (reuse): surface solve, velocity recovery and
+    type-0 force per incidence."""
+    from bem3_aw_qd2_amd import AssemblyStats, check, exdat, hip_lib, host_lib, mesh, serve
+    hip, host = serve._bind()
+    rng = np.random.default_rng(args.batch_seed)
+    a, c = 0.5e-3, 1500.0
+    tmp = tempfile.mkdtemp(prefix="bqd2_batchcfg_")
+    specs = []
+    for i in range(args.batch_instances):
+        n = int(rng.choice([8, 12, 16, 20, 24]))
+        period = a * rng.uniform(3.0, 6.0)
+        tri_lat = bool(rng.random() < 0.5)
+        d2 = (0.5 * period, 0.5 * math.sqrt(3.0) * period) if tri_lat else (0.0, period)
+        ka = float(rng.uniform(0.5, 3.0))
+        theta, psi = float(rng.uniform(0.05, math.pi / 3)), float(rng.uniform(0.0, 2 * math.pi))
+        over = dict(n=n, d1=(period, 0.0), d2=d2, f=ka * c / (2 * math.pi * a), theta=theta, psi=psi)
+        paths = mesh.write_config("sphere384", os.path.join(tmp, str(i)), **over)
+        with quiet_stdout():
+            m = exdat.Model(host).read_inputs(*paths, out=os.path.join(tmp, "ex%d.dat" % i)).initialize()
+        mp = host.aqd2_model_new(m.buf)
+        k = ka / a
+        inc, norders = diffraction_incidences(k, (period, 0.0), d2, theta, psi)
+        specs.append({"m": m, "mp": mp, "ne": m.ne(), "ka": ka, "ratio": period / a, "tri_lattice": tri_lat,
+                      "inc": inc, "orders": norders,
+                      "bytes": int(hip.bqd2_footprint(mp, 2, 0, 0))})
+    budget = int(args.batch_budget_gib * 2 ** 30)
+    waves, cur, used = [], [], 0
+    for sp in specs:
+        if cur and used + sp["bytes"] > budget:
+            waves.append(cur)
+            cur, used = [], 0
+        cur.append(sp)
+        used += sp["bytes"]
+    waves.append(cur)
+    log("batch (configs): %d instances, %d-%d quads, %d memory waves" %
+        (len(specs), min(sp["ne"] for sp in specs), max(sp["ne"] for sp in specs), len(waves)))
+    gpu_one = gpu_bat = wall_one = wall_bat = 0.0
+    t_plain = t_reuse = 0.0
+    nsol, errs, steps = 0, [], []
+    for wi, wave in enumerate(waves):
+        probs = []
+        for sp in wave:
+            p = C.c_void_p()
+            check(hip.bqd2_create(sp["mp"], C.byref(p)), "bqd2_create")
+            probs.append(p)
+        arr = (C.c_void_p * len(probs))(*probs)
+        # one at a time (first assembly: the rule decisions and pair lists are made here)
+        t0 = time.perf_counter()
+        for p, sp in zip(probs, wave):
+            for d in (0, 1):
+                st = AssemblyStats()
+                check(hip.bqd2_assemble(p, d, C.byref(st)), "bqd2_assemble")
+                gpu_one += (st.ms_pairs + st.ms_self) * 1e-3
+        wall_one += time.perf_counter() - t0
+        log("  wave %d: %d instances assembled one at a time (%.1f s)" % (wi, len(wave), time.perf_counter() - t0))
+        # the pairs and self blocks of the wave, one launch per task kind
+        bst = (AssemblyStats * (2 * len(probs)))()
+        t0 = time.perf_counter()
+        check(hip.bqd2_assemble_batch(arr, len(probs), -1, bst), "bqd2_assemble_batch")
+        wall_bat += time.perf_counter() - t0
+        gpu_bat += (bst[0].ms_pairs + sum(bst[j].ms_self for j in range(2 * len(probs)))) * 1e-3
+        log("  wave %d: batched assembly (%.1f s)" % (wi, time.perf_counter() - t0))
+        for ii, (p, sp) in enumerate(zip(probs, wave)):
+            forces = {}
+            for mode in (0, 2):
+                check(hip.bqd2_set_solver(p, mode), "bqd2_set_solver")
+                t0 = time.perf_counter()
+                for j, (amp, th, ps) in enumerate(sp["inc"]):
+                    pw7 = (C.c_double * 7)(amp.real, amp.imag, th, ps, 0.0, 0.0, 0.0)
+                    check(hip.bqd2_set_incidence(p, pw7), "bqd2_set_incidence")
+                    check(hip.bqd2_solve_surface(p, None), "bqd2_solve_surface")
+                    check(hip.bqd2_solve_velocity(p), "bqd2_solve_velocity")
+                    F, N = (C.c_double * 3)(), (C.c_double * 3)()
+                    check(hip.bqd2_force(p, (C.c_double * 3)(0.0, 0.0, 0.0), 0, F, N), "bqd2_force")
+                    forces[(mode, j)] = np.array(F[:])
+                    if mode == 2:
+                        its = (C.c_int * 17)()
+                        check(hip.bqd2_solver_info(p, its, None, None), "bqd2_solver_info")
+                        steps.append(its[0])
+                dt = time.perf_counter() - t0
+                if mode == 0:
+                    t_plain += dt
+                else:
+                    t_reuse += dt
+            for j in range(len(sp["inc"])):
+                f0, f2 = forces[(0, j)], forces[(2, j)]
+                errs.append(float(np.linalg.norm(f2 - f0) / np.linalg.norm(f0)))
+            nsol += len(sp["inc"])
+            log("    instance %d of wave %d (%d quads): 16 incidences solved twice" % (ii, wi, sp["ne"]))
+        for p in probs:
+            hip.bqd2_destroy(p)
+        log("  wave %d: %d instances done (assembly GPU one at a time %.1f s, batched %.1f s; solves so far "
+            "plain %.1f s, reuse %.1f s)" % (wi, len(wave), gpu_one, gpu_bat, t_plain, t_reuse))
+    for sp in specs:
+        host.aqd2_model_free(sp["mp"])
+        with quiet_stdout():
+            sp["m"].finalize()
+    return {
+        "instances": len(specs), "waves": len(waves), "budget_gib": args.batch_budget_gib,
+        "elements": [sp["ne"] for sp in specs], "ka": [round(sp["ka"], 4) for sp in specs],
+        "period_over_radius": [round(sp["ratio"], 4) for sp in specs],
+        "triangular_lattice": [sp["tri_lattice"] for sp in specs],
+        "diffraction_orders_within_60deg": [sp["orders"] for sp in specs],
+        "incidences_per_instance": 16,
+        "assembly_gpu_s_one_at_a_time": gpu_one, "assembly_gpu_s_batched": gpu_bat,
+        "assembly_wall_s_one_at_a_time_incl_rules": wall_one, "assembly_wall_s_batched": wall_bat,
+        "assembly_speedup_gpu": gpu_one / gpu_bat,
+        "plain_fp64_solves_per_s": nsol / t_plain, "factor_reuse_solves_per_s": nsol / t_reuse,
+        "solve_speedup": t_plain / t_reuse,
+        "refinement_steps_mean": float(np.mean(steps)), "refinement_steps_max": int(np.max(steps)),
+        "force_rel_diff_max": max(errs),
+        "solves_per_s_end_to_end_batched_reuse": nsol / (gpu_bat + t_reuse),
+        "solves_per_s_end_to_end_plain": nsol / (gpu_one + t_plain),
+    }
+
+
 # mesh sizes of the request streams: "small" 6 n^2 quads for n in 4..8 (or
 # their triangle splits); "config" 384-6144 elements: 6 n^2 quads for
 # n in {8, 12, 16, 20, 24, 28, 32}, or 12 n^2 triangles for n in {6, 9, 12, 15, 18, 22}
@@ -746,7 +899,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
     ap.add_argument("--autotune", type=int, default=1, choices=(0, 1),
                     help="online autotuning of launch shapes (0: default shapes everywhere)")
-    ap.add_argument("--batch-instances", type=int, default=6)
+    ap.add_argument("--batch-instances", type=int, default=None,
+                    help="instances of the batch leg (default 6; 64 with --batch-config)")
+    ap.add_argument("--batch-config", action="store_true",
+                    help="only the batched workload at its full size (64 instances of 384-3456 quads)")
+    ap.add_argument("--batch-seed", type=int, default=2)
+    ap.add_argument("--batch-budget-gib", type=float, default=240.0)
     ap.add_argument("--serving-only", action="store_true", help="only the request-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests per second")
@@ -783,6 +941,15 @@ def main():
         sys.exit(1)
     if args.layout == "system" or (args.layout == "" and args.workload == "sphere13824"):
         large_main(args)
+        return
+    if args.batch_instances is None:
+        args.batch_instances = 64 if args.batch_config else 6
+    if args.batch_config:
+        bc = batch_config_leg(args)
+        print(json.dumps({"metric": METRIC, "value": bc["solves_per_s_end_to_end_batched_reuse"], "unit": "solves/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64 (c64 factors, f64 refinement)",
+                          "config": {"workload": "batched instances (64, 384-3456 quads, 16 incidences each)"},
+                          "batch": bc}), flush=True)
         return
     if args.serving_only:
         sv = serving_bench(args)

@@ -8,8 +8,9 @@ diffraction order, whose in-plane wave vector is the base one plus g1
 (|k_par + g1| = 3142 < k = 4189, a propagating order).  For each, the solution
 on the base problem after bqd2_set_incidence must equal the solution of a
 problem created and assembled with that incidence: surface P and dP within
-1e-10 normwise, force within 1e-9, for the FP64 GMRES solver and for the
-mixed-precision solver (whose factors are reused, not recomputed).  An
+1e-10 normwise, force within 1e-9, for the FP64 GMRES solver, the FP64 LU
+(whose factors are kept: a later incidence rebuilds the right-hand side only)
+and the mixed-precision solver (whose factors are reused, not recomputed).  An
 incidence with another in-plane wave vector is rejected."""
 import ctypes as C
 import math
@@ -65,7 +66,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_incidences_share_the_operator(mode):
     hip = hip_lib()
     with tempfile.TemporaryDirectory() as tmp:
