@@ -91,6 +91,7 @@ def hip_lib():
                                    C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p]
         lib.bqd2_lu32_info.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.bqd2_set_lu32_pivoting.argtypes = [C.c_int]
+        lib.bqd2_set_lu32_lookahead.argtypes = [C.c_int]
         lib.bqd2_lu32_panel_counts.argtypes = [C.c_void_p]
         lib.bqd2_set_lu32_growth_bound.argtypes = [C.c_double]
         lib.bqd2_lu32_panel_growth.argtypes = [C.c_void_p, C.c_int]

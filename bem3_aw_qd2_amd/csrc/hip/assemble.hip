@@ -110,6 +110,9 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
 // per wavefront leaves a quarter of the lanes idle in its last pass.
 #define PPW 4
 #define LPP 16
+// single-wave workgroups per launch at most: the dispatch packet holds the grid
+// size in work-items as a 32-bit number (2^25 x 64 = 2^31)
+#define BQD2_MAX_WG (1LL << 25)
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
   for (int off = LPP / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1045,15 +1048,15 @@ __device__ __forceinline__ void batch_task(const AsmBatch *__restrict__ inst, lo
 // on the pair kernels than this grid, which the hardware dispatches as
 // workgroups finish.)
 template <int KIND>
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_asm_batch(const AsmBatch *__restrict__ inst, int ninst) {
-  const long long t = blockIdx.x;
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_asm_batch(const AsmBatch *__restrict__ inst, int ninst, long long t0) {
+  const long long t = t0 + blockIdx.x;
   if (t < inst[ninst - 1].t_end[KIND]) batch_task<KIND>(inst, t);
 }
 // the free-space kinds with the per-domain kernels' register bound (no wave
 // count fixed: the self rows then keep their registers, 0 B spilled)
 template <int KIND>
-__global__ void __launch_bounds__(WAVE) k_asm_batch_fs(const AsmBatch *__restrict__ inst, int ninst) {
-  const long long t = blockIdx.x;
+__global__ void __launch_bounds__(WAVE) k_asm_batch_fs(const AsmBatch *__restrict__ inst, int ninst, long long t0) {
+  const long long t = t0 + blockIdx.x;
   if (t < inst[ninst - 1].t_end[KIND]) batch_task<KIND>(inst, t);
 }
 
@@ -1324,17 +1327,20 @@ int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   AsmPhase ev;
   int rc = asm_begin(p, did, ev);
   if (rc) return rc;
-  // with a progress callback the pairs go in 16 launches, reported as each ends
+  // with a progress callback the pairs go in 16 launches, reported as each ends;
+  // in any case in parts of at most BQD2_MAX_WG workgroups (the dispatch holds
+  // the grid's size in work-items as 32 bits: 13 824 quads need 1.9e8)
   std::vector<hipEvent_t> parts;
   if (periodic) {
     const long long nwg = D.plist_n / PPW;
-    const int np = (bqd2_progress_on() && nwg >= 16) ? 16 : 1;
+    const int np = std::max<int>((bqd2_progress_on() && nwg >= 16) ? 16 : 1,
+                                 (int)((nwg + BQD2_MAX_WG - 1) / BQD2_MAX_WG));
     for (int k = 0; k < np; k++) {
       const long long a = nwg * k / np, b = nwg * (k + 1) / np;
       if (b > a)
         hipLaunchKernelGGL(k_q0_pairs, dim3((unsigned)(b - a)), dim3(WAVE), 0, st_, p->m, p->tab, D, p->gf, D.plist,
                            (long long)D.plist_n, p->d_counters, a);
-      if (np > 1) {
+      if (np > 1 && bqd2_progress_on()) {
         hipEvent_t e;
         BQD2_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         BQD2_HIP(hipEventRecord(e, st_));
@@ -1426,10 +1432,13 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
     BQD2_HIP(hipStreamWaitEvent(bs, j.ev.e1b, 0));
     BQD2_HIP(hipEventRecord(j.ev.e1b, bs));  // the pair time is the batch's
   }
-  // one launch per task kind over every instance's tasks
-  auto launch = [&](int kind, void (*kern)(const AsmBatch *, int)) {
-    if (tasks[kind] > 0)
-      hipLaunchKernelGGL(kern, dim3((unsigned)tasks[kind]), dim3(WAVE), 0, bs, d_inst, (int)hb.size());
+  // one launch per task kind over every instance's tasks (in parts of at most
+  // BQD2_MAX_WG single-wave workgroups: the dispatch holds the grid's size in
+  // work-items as 32 bits)
+  auto launch = [&](int kind, void (*kern)(const AsmBatch *, int, long long)) {
+    for (long long t0 = 0; t0 < tasks[kind]; t0 += BQD2_MAX_WG)
+      hipLaunchKernelGGL(kern, dim3((unsigned)std::min<long long>(BQD2_MAX_WG, tasks[kind] - t0)), dim3(WAVE), 0, bs,
+                         d_inst, (int)hb.size(), t0);
   };
   launch(BK_Q0_PAIRS, k_asm_batch<BK_Q0_PAIRS>);
   launch(BK_FS_PAIRS, k_asm_batch_fs<BK_FS_PAIRS>);

@@ -464,6 +464,12 @@ extern "C" int bqd2_set_lu32_pivoting(int mode) {
   return BQD2_OK;
 }
 
+static bool g_lookahead = true;
+extern "C" int bqd2_set_lu32_lookahead(int on) {
+  g_lookahead = on != 0;
+  return BQD2_OK;
+}
+
 extern "C" int bqd2_set_lu32_growth_bound(double bound) {
   BQD2_CHECK(bound >= 1.0, BQD2_E_ARG, "bqd2_set_lu32_growth_bound: the bound is at least 1");
   g_growth_bound = (float)bound;
@@ -996,7 +1002,7 @@ static bool lookahead_streams(hipStream_t *m, hipStream_t *pn) {
 // columns).  Returns BQD2_E_SINGULAR when a diagonal block met a zero pivot.
 static int cgetrf_relaxed(int n, float2 *A, int *d_ipiv, const LuWs &W, hipStream_t st) {
   hipStream_t M = st, P = st;
-  const bool two = lookahead_streams(&M, &P);
+  const bool two = g_lookahead && lookahead_streams(&M, &P);
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // start/end, next panel's columns updated, panel done
   struct EvGuard {
     hipEvent_t *e;

@@ -188,6 +188,12 @@ bqd2_problem *aqd2_problem(AQD2 *qd, int *has_solution)
   pack_model(qd, &m, &pk);
   bqd2_problem *p = NULL;
   int rc = bqd2_create(&m, &p);
+  if (rc == BQD2_E_NOMEM) {
+    /* the default layout does not fit the device: the system layout, for the
+     * meshes it holds (one sphere of quadrilaterals in a periodic lattice) */
+    int rc2 = bqd2_create_large(&m, &p);
+    if (rc2 == 0) rc = 0;
+  }
   free_packed(&pk);
   if (rc) {
     pthread_mutex_unlock(&g_side_lock);

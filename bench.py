@@ -601,13 +601,16 @@ def batch_config_leg(args):
                 dt = time.perf_counter() - t0
                 if mode == 0:
                     t_plain += dt
+                    dt_plain = dt
                 else:
                     t_reuse += dt
+                    dt_reuse = dt
             for j in range(len(sp["inc"])):
                 f0, f2 = forces[(0, j)], forces[(2, j)]
                 errs.append(float(np.linalg.norm(f2 - f0) / np.linalg.norm(f0)))
             nsol += len(sp["inc"])
-            log("    instance %d of wave %d (%d quads): 16 incidences solved twice" % (ii, wi, sp["ne"]))
+            log("    instance %d of wave %d (%d quads): 16 incidences, FP64 GMRES %.2f s, factors reused %.2f s"
+                % (ii, wi, sp["ne"], dt_plain, dt_reuse))
         for p in probs:
             hip.bqd2_destroy(p)
         log("  wave %d: %d instances done (assembly GPU one at a time %.1f s, batched %.1f s; solves so far "
@@ -905,6 +908,8 @@ def main():
                     help="only the batched workload at its full size (64 instances of 384-3456 quads)")
     ap.add_argument("--batch-seed", type=int, default=2)
     ap.add_argument("--batch-budget-gib", type=float, default=240.0)
+    ap.add_argument("--lu32-lookahead", type=int, default=1, choices=(0, 1),
+                    help="the mixed-precision factorisation's next panel on 8 CUs of its own (1) or not (0)")
     ap.add_argument("--serving-only", action="store_true", help="only the request-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests per second")
@@ -942,6 +947,8 @@ def main():
     if args.layout == "system" or (args.layout == "" and args.workload == "sphere13824"):
         large_main(args)
         return
+    from bem3_aw_qd2_amd import hip_lib as _hip_lib
+    _hip_lib().bqd2_set_lu32_lookahead(args.lu32_lookahead)
     if args.batch_instances is None:
         args.batch_instances = 64 if args.batch_config else 6
     if args.batch_config:

@@ -261,6 +261,11 @@ int bqd2_lu32_panel_counts(long long *out);
 int bqd2_set_lu32_growth_bound(double bound);
 int bqd2_lu32_panel_growth(float *out, int cap);
 int bqd2_cgetrs(int n, int nrhs, const float *d_A, const void *d_ws, float *d_B, float *d_tmp, void *stream);
+/* No counterpart in the reference.  Whether the unchecked factorisation of the
+ * mixed-precision solver runs each next panel on a stream of its own restricted
+ * to 8 CUs beside the trailing update (1, default) or everything on the
+ * caller's stream (0); process-wide. */
+int bqd2_set_lu32_lookahead(int on);
 /* The trailing-update kernel of bqd2_cgetrf, exposed for tests: C -= A B
  * (set = 0, three real products per complex product, as the trailing update)
  * or C = A B (set = 1, four real products; B may be C when m <= 128, A may be

@@ -113,3 +113,46 @@ def test_batch_all_domains_equals_one_at_a_time():
         for m in models:
             m.finalize()
 
+
+
+def test_batch_beyond_one_dispatch():
+    """Two 3456-quad instances: their free-space pairs (9.6e7 single-wave
+    workgroups) exceed what one dispatch can address (the grid's size in
+    work-items is a 32-bit number), so the batched launches go in parts; sampled
+    rows of every matrix must still equal one instance at a time."""
+    hip = hip_lib()
+    over2 = dict(d1=(2.2e-3, 0.0), d2=(0.0, 2.2e-3), theta=0.3, psi=0.4)
+    with tempfile.TemporaryDirectory() as tmp:
+        models, probs = [], []
+        for i, over in enumerate(({}, over2)):
+            m, p = problem(os.path.join(tmp, str(i)), "sphere3456", over, False, True)
+            models.append(m)
+            probs.append(p)
+        N = 4 * models[0].ne()
+        rows = np.array([0, 1, 5, 4000, 9999, N - 1], np.int32)
+
+        def sample(p, d):
+            G = np.zeros((len(rows), N), complex)
+            H = np.zeros((len(rows), N), complex)
+            dF = np.zeros((len(rows), 3))
+            dG = np.zeros((len(rows), 2 * N), complex) if d == 1 else None
+            dH = np.zeros((len(rows), 2 * N), complex) if d == 1 else None
+            assert hip.bqd2_download_rows(p, d, len(rows), rows.ctypes.data, G.ctypes.data, H.ctypes.data,
+                                          dG.ctypes.data if d == 1 else None, dH.ctypes.data if d == 1 else None,
+                                          dF.ctypes.data) == 0
+            return (G, H, dF) + ((dG, dH) if d == 1 else ())
+
+        single = []
+        for p in probs:
+            for d in (0, 1):
+                assert hip.bqd2_assemble(p, d, None) == 0
+            single.append([sample(p, d) for d in (0, 1)])
+        arr = (C.c_void_p * len(probs))(*probs)
+        assert hip.bqd2_assemble_batch(arr, len(probs), -1, None) == 0
+        for i, p in enumerate(probs):
+            for d in (0, 1):
+                same = [np.array_equal(a, b) for a, b in zip(sample(p, d), single[i][d])]
+                print("instance %d domain %d: sampled rows identical %s" % (i, d, same))
+                assert all(same)
+        for m in models:
+            m.finalize()
