@@ -62,7 +62,8 @@ template <class T>
 static int upload(bqd2_problem *p, T **dst, const T *src, size_t n) {
   int rc = bqd2_alloc(p, (void **)dst, n * sizeof(T));
   if (rc) return rc;
-  BQD2_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  BQD2_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, p->stream));
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // the host array may be released on return
   return BQD2_OK;
 }
 
@@ -114,6 +115,10 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
   p->large = large;
   p->lwin = 2048;
   p->lmax_resident = -1;
+  // the problem's stream does not wait for the null stream, nor it for the
+  // problem: requests of a serving loop on other problems never wait for it
+  // (every transfer of the problem is ordered on this stream)
+  BQD2_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   p->m.Ne = m->Ne;
   p->m.NN = m->NN;
   p->m.MN = m->MN;
@@ -160,7 +165,6 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
   else memset(&p->gf, 0, sizeof(p->gf));
 
   TRY(bqd2_alloc(p, (void **)&p->d_counters, CNT_N * sizeof(long long)));
-  BQD2_HIP(hipStreamCreate(&p->stream));
 
   size_t off = 0;
   p->dom.resize(m->MN + 1);
@@ -231,10 +235,10 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
     TRY(bqd2_alloc(p, (void **)&D.dP, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.pv, ((size_t)D.Ne + 1) * 12 * 16));
     TRY(bqd2_alloc(p, (void **)&D.dpv, ((size_t)D.Ne + 1) * 12 * 16));
-    BQD2_HIP(hipMemset(D.P, 0, ((size_t)D.Ne + 1) * 4 * 16));
-    BQD2_HIP(hipMemset(D.dP, 0, ((size_t)D.Ne + 1) * 4 * 16));
-    BQD2_HIP(hipMemset(D.pv, 0, ((size_t)D.Ne + 1) * 12 * 16));
-    BQD2_HIP(hipMemset(D.dpv, 0, ((size_t)D.Ne + 1) * 12 * 16));
+    BQD2_HIP(hipMemsetAsync(D.P, 0, ((size_t)D.Ne + 1) * 4 * 16, p->stream));
+    BQD2_HIP(hipMemsetAsync(D.dP, 0, ((size_t)D.Ne + 1) * 4 * 16, p->stream));
+    BQD2_HIP(hipMemsetAsync(D.pv, 0, ((size_t)D.Ne + 1) * 12 * 16, p->stream));
+    BQD2_HIP(hipMemsetAsync(D.dpv, 0, ((size_t)D.Ne + 1) * 12 * 16, p->stream));
   }
   p->na = 2 * m->NN;
   if (large) {
@@ -258,6 +262,8 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
   p->main.ws_bytes = 0;
   p->main.cnt = p->d_counters;
   p->owner = std::this_thread::get_id();
+  // the host arrays of the model are the caller's: every upload done on return
+  BQD2_HIP(hipStreamSynchronize(p->stream));
   *out = p;
   return BQD2_OK;
 }
@@ -291,11 +297,12 @@ extern "C" int bqd2_download_domain(bqd2_problem *p, int did, double *G, double 
   BQD2_HIP(hipStreamSynchronize(p->stream));
   BQD2_CHECK(D.r0 == 0 && D.nr == D.N && !(p->large && did == 1 && p->s_lower != LG_G1H1), BQD2_E_STATE,
              "bqd2_download_domain: domain %d's rows are not held", did);
-  if (G) BQD2_HIP(hipMemcpy2D(G, N * 16, D.G, (size_t)D.ldm * 16, N * 16, N, hipMemcpyDeviceToHost));
-  if (H) BQD2_HIP(hipMemcpy2D(H, N * 16, D.H, (size_t)D.ldm * 16, N * 16, N, hipMemcpyDeviceToHost));
-  if (dG && D.dG) BQD2_HIP(hipMemcpy(dG, D.dG, 2 * N * N * 16, hipMemcpyDeviceToHost));
-  if (dH && D.dH) BQD2_HIP(hipMemcpy(dH, D.dH, 2 * N * N * 16, hipMemcpyDeviceToHost));
-  if (dF) BQD2_HIP(hipMemcpy(dF, D.dF, N * 3 * 8, hipMemcpyDeviceToHost));
+  if (G) BQD2_HIP(hipMemcpy2DAsync(G, N * 16, D.G, (size_t)D.ldm * 16, N * 16, N, hipMemcpyDeviceToHost, p->stream));
+  if (H) BQD2_HIP(hipMemcpy2DAsync(H, N * 16, D.H, (size_t)D.ldm * 16, N * 16, N, hipMemcpyDeviceToHost, p->stream));
+  if (dG && D.dG) BQD2_HIP(hipMemcpyAsync(dG, D.dG, 2 * N * N * 16, hipMemcpyDeviceToHost, p->stream));
+  if (dH && D.dH) BQD2_HIP(hipMemcpyAsync(dH, D.dH, 2 * N * N * 16, hipMemcpyDeviceToHost, p->stream));
+  if (dF) BQD2_HIP(hipMemcpyAsync(dF, D.dF, N * 3 * 8, hipMemcpyDeviceToHost, p->stream));
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // the caller reads or reuses the host buffers
   return BQD2_OK;
 }
 
@@ -313,12 +320,13 @@ extern "C" int bqd2_download_rows(bqd2_problem *p, int did, int nrows, const int
   BQD2_HIP(hipStreamSynchronize(p->stream));
   for (int i = 0; i < nrows; i++) {
     const size_t r = (size_t)rows[i];
-    if (G) BQD2_HIP(hipMemcpy(G + 2 * N * i, D.G + 2 * dom_gh(D, (int)r, 0), N * 16, hipMemcpyDeviceToHost));
-    if (H) BQD2_HIP(hipMemcpy(H + 2 * N * i, D.H + 2 * dom_gh(D, (int)r, 0), N * 16, hipMemcpyDeviceToHost));
-    if (dG && D.dG) BQD2_HIP(hipMemcpy(dG + 4 * N * i, D.dG + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost));
-    if (dH && D.dH) BQD2_HIP(hipMemcpy(dH + 4 * N * i, D.dH + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost));
-    if (dF) BQD2_HIP(hipMemcpy(dF + 3 * i, D.dF + 3 * r, 3 * 8, hipMemcpyDeviceToHost));
+    if (G) BQD2_HIP(hipMemcpyAsync(G + 2 * N * i, D.G + 2 * dom_gh(D, (int)r, 0), N * 16, hipMemcpyDeviceToHost, p->stream));
+    if (H) BQD2_HIP(hipMemcpyAsync(H + 2 * N * i, D.H + 2 * dom_gh(D, (int)r, 0), N * 16, hipMemcpyDeviceToHost, p->stream));
+    if (dG && D.dG) BQD2_HIP(hipMemcpyAsync(dG + 4 * N * i, D.dG + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost, p->stream));
+    if (dH && D.dH) BQD2_HIP(hipMemcpyAsync(dH + 4 * N * i, D.dH + 4 * N * r, 2 * N * 16, hipMemcpyDeviceToHost, p->stream));
+    if (dF) BQD2_HIP(hipMemcpyAsync(dF + 3 * i, D.dF + 3 * r, 3 * 8, hipMemcpyDeviceToHost, p->stream));
   }
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // the caller reads or reuses the host buffers
   return BQD2_OK;
 }
 
@@ -327,10 +335,11 @@ extern "C" int bqd2_download_solution(bqd2_problem *p, int did, double *P, doubl
   DevDomain &D = p->dom[did];
   const size_t n1 = (size_t)D.Ne + 1;
   BQD2_HIP(hipStreamSynchronize(p->stream));
-  if (P) BQD2_HIP(hipMemcpy(P, D.P, n1 * 4 * 16, hipMemcpyDeviceToHost));
-  if (dP) BQD2_HIP(hipMemcpy(dP, D.dP, n1 * 4 * 16, hipMemcpyDeviceToHost));
-  if (pv) BQD2_HIP(hipMemcpy(pv, D.pv, n1 * 12 * 16, hipMemcpyDeviceToHost));
-  if (dpv) BQD2_HIP(hipMemcpy(dpv, D.dpv, n1 * 12 * 16, hipMemcpyDeviceToHost));
+  if (P) BQD2_HIP(hipMemcpyAsync(P, D.P, n1 * 4 * 16, hipMemcpyDeviceToHost, p->stream));
+  if (dP) BQD2_HIP(hipMemcpyAsync(dP, D.dP, n1 * 4 * 16, hipMemcpyDeviceToHost, p->stream));
+  if (pv) BQD2_HIP(hipMemcpyAsync(pv, D.pv, n1 * 12 * 16, hipMemcpyDeviceToHost, p->stream));
+  if (dpv) BQD2_HIP(hipMemcpyAsync(dpv, D.dpv, n1 * 12 * 16, hipMemcpyDeviceToHost, p->stream));
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // the caller reads or reuses the host buffers
   return BQD2_OK;
 }
 
@@ -340,17 +349,19 @@ extern "C" int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, c
   DevDomain &D = p->dom[did];
   const size_t n1 = (size_t)D.Ne + 1;
   BQD2_HIP(hipStreamSynchronize(p->stream));
-  if (P) BQD2_HIP(hipMemcpy(D.P, P, n1 * 4 * 16, hipMemcpyHostToDevice));
-  if (dP) BQD2_HIP(hipMemcpy(D.dP, dP, n1 * 4 * 16, hipMemcpyHostToDevice));
-  if (pv) BQD2_HIP(hipMemcpy(D.pv, pv, n1 * 12 * 16, hipMemcpyHostToDevice));
-  if (dpv) BQD2_HIP(hipMemcpy(D.dpv, dpv, n1 * 12 * 16, hipMemcpyHostToDevice));
+  if (P) BQD2_HIP(hipMemcpyAsync(D.P, P, n1 * 4 * 16, hipMemcpyHostToDevice, p->stream));
+  if (dP) BQD2_HIP(hipMemcpyAsync(D.dP, dP, n1 * 4 * 16, hipMemcpyHostToDevice, p->stream));
+  if (pv) BQD2_HIP(hipMemcpyAsync(D.pv, pv, n1 * 12 * 16, hipMemcpyHostToDevice, p->stream));
+  if (dpv) BQD2_HIP(hipMemcpyAsync(D.dpv, dpv, n1 * 12 * 16, hipMemcpyHostToDevice, p->stream));
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // the caller reads or reuses the host buffers
   return BQD2_OK;
 }
 
 extern "C" int bqd2_counters(bqd2_problem *p, long long *out, int n) {
   BQD2_CHECK(p && out && n > 0 && n <= CNT_N, BQD2_E_ARG, "bqd2_counters: bad arguments");
   BQD2_HIP(hipStreamSynchronize(p->stream));
-  BQD2_HIP(hipMemcpy(out, p->d_counters, (size_t)n * sizeof(long long), hipMemcpyDeviceToHost));
+  BQD2_HIP(hipMemcpyAsync(out, p->d_counters, (size_t)n * sizeof(long long), hipMemcpyDeviceToHost, p->stream));
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // the caller reads or reuses the host buffers
   return BQD2_OK;
 }
 
@@ -381,7 +392,7 @@ CallCtx *bqd2_call_ctx(bqd2_problem *p) {
   CallCtx *c = new CallCtx();
   c->ws = nullptr;
   c->ws_bytes = 0;
-  if (hipStreamCreate(&c->st) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     bqd2_set_error("hipStreamCreate failed for a calling thread");
     delete c;
     return nullptr;

@@ -393,7 +393,8 @@ static int ensure_hk2(bqd2_problem *p) {
   bqd2_hk2_table(p, tab);
   int rc = bqd2_alloc(p, (void **)&p->hk2d, sizeof(tab));
   if (rc) return rc;
-  BQD2_HIP(hipMemcpy(p->hk2d, tab, sizeof(tab), hipMemcpyHostToDevice));
+  BQD2_HIP(hipMemcpyAsync(p->hk2d, tab, sizeof(tab), hipMemcpyHostToDevice, p->stream));
+  BQD2_HIP(hipStreamSynchronize(p->stream));  // tab is released on return
   return BQD2_OK;
 }
 
