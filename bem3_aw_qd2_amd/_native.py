@@ -105,6 +105,8 @@ def hip_lib():
         lib.bqd2_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         lib.bqd2_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
+        lib.bqd2_set_field_far.argtypes = [C.c_void_p, C.c_int]
+        lib.bqd2_field_far_info.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         lib.bqd2_field_bd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                       C.c_void_p]
         lib.bqd2_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]

@@ -37,6 +37,9 @@ struct FieldCtx {
   DomView dom[16];
   int MN, ps;
   int any_tri;  // the mesh has linear triangles: rules are sized for both kinds
+  int far_on;   // points of the periodic domain beyond far_h of the boundary's z
+                // extent [far_zbot, far_ztop] take the plane-wave amplitudes
+  double far_ztop, far_zbot, far_h;
   double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
   double d1x, d1y, d2x, d2y, idd, vkx, vky;
 };
@@ -210,6 +213,15 @@ __host__ __device__ __forceinline__ int rule_points(int type, int any_tri) {
 
 __device__ __forceinline__ cd ld(const double *a, size_t i) { return cd{a[2 * i], a[2 * i + 1]}; }
 
+// side of a field point of domain did at height z: 0 above the periodic
+// boundary by far_h or more, 1 below it by far_h or more, -1 otherwise
+__device__ __forceinline__ int far_side(const FieldCtx &X, int did, double z) {
+  if (!X.far_on || did != 0) return -1;
+  if (z >= X.far_ztop + X.far_h) return 0;
+  if (z <= X.far_zbot - X.far_h) return 1;
+  return -1;
+}
+
 // ------------------------------------------------------------ off-boundary field
 // The (element, quadrature point) items of one field point are cut into chunks
 // of a fixed size that depends only on the mesh and the rule (not on how many
@@ -242,6 +254,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
   const DomView &V = X.dom[did];
   const bool periodic = (X.ps == 1 && did == 0);
   const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
+  // far points: F and grad F here, the lattice sums from k_far_eval
+  const bool far = far_side(X, did, r0[2]) >= 0;
   const int npt = rule_points(type, X.any_tri);
   const double k = V.k;
   // Accumulators and the per-item coefficients live in LDS (per lane), so that
@@ -271,6 +285,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
     double vR[3] = {r[0] - r0[0], r[1] - r0[1], r[2] - r0[2]};
     double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
     double rdW = dot3(vR, w) * iR;
+    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+    F += sg * wt * rdW * iR * iR;
+    for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
+    if (far) continue;
     cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
     for (int m = 0; m < 4; m++) {
       if (Nv[m] == 0.0) continue;
@@ -281,9 +299,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
         dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
       }
     }
-    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
-    F += sg * wt * rdW * iR * iR;
-    for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
     // per item: Pa += g alpha + (dg.W) beta, dPa_c += -dg_c alpha - (H W)_c beta,
     // PVa_c += g gamma_c + (dg.W) delta_c, with G = g aW and H = dg.W the kernel
     // values of the representation formula (free space or lattice sum)
@@ -445,6 +460,216 @@ __global__ void __launch_bounds__(256) k_field_fin(FieldCtx X, int total, int n,
   for (int c2 = 0; c2 < 3; c2++) {
     out[8 * ip + 2 + 2 * c2] = pv[c2].x;
     out[8 * ip + 3 + 2 * c2] = pv[c2].y;
+  }
+}
+
+// ------------------------------------------------------------ far field
+// A field point x of the periodic domain with x_z >= z_top + h (side 0) or
+// x_z <= z_bot - h (side 1), z_bot .. z_top the boundary's extent, has every
+// source point y at least h away in z.  There the lattice sum is the pure
+// spectral series (d3_qpgf_d2.c:484-533, :887-950, what fd_cell sums per pair)
+//   G(r) = (i/2A) sum_n exp(i a_n . r_t + i b_n |r_z|) / b_n,   r = y - x,
+//   a_n = -k_t + n1 g1 + n2 g2,  b_n = sqrt(k^2 - |a_n|^2), Im b_n >= 0,
+// whose gradient takes i kappa_n and Hessian -kappa_n kappa_n^T per term,
+// kappa_n = (a_n, b_n sgn r_z).  With r_z of one sign every term is a target
+// factor times a source factor, so the representation formula's sums over the
+// boundary items (k_field_part's alpha, beta, gamma_c, delta_c, weight vector w)
+// become per order four amplitudes, summed once for all points,
+//   A_n   = (i/2A)/b_n sum_items Y_n (alpha   + i (kappa_n . w) beta),
+//   B_n,c = (i/2A)/b_n sum_items Y_n (gamma_c + i (kappa_n . w) delta_c),
+// and per point a sum over the orders
+//   Pa = sum_n X_n A_n,  dPa_c = sum_n X_n (-i kappa_n,c) A_n,
+//   PVa_c = sum_n X_n B_n,c.
+// Side 0: Y_n = exp(i a_n . y_t + i b_n (z_top - y_z)),
+//         X_n = exp(-i a_n . x_t + i b_n (x_z - z_top)), kappa_z = -b_n;
+// side 1: Y_n = exp(i a_n . y_t + i b_n (y_z - z_bot)),
+//         X_n = exp(-i a_n . x_t + i b_n (z_bot - x_z)), kappa_z = b_n:
+// no factor exceeds 1 in size.  The orders are the grid of FarGeom, which keeps
+// every order that has not decayed by 1e-20 over the height h.
+
+// order j of the grid: a_n and b_n; false at a Wood anomaly (b_n = 0)
+__device__ __forceinline__ bool far_order(const GfParams &P, const FarGeom &F, int j, double &ax, double &ay, cd &bn) {
+  const int n1 = F.n1lo + j / F.n2n, n2 = F.n2lo + j % F.n2n;
+  ax = -P.vkx + n1 * P.g1x + n2 * P.g2x;
+  ay = -P.vky + n1 * P.g1y + n2 * P.g2y;
+  const double b2 = P.k * P.k - ax * ax - ay * ay;
+  bn = b2 > 0.0 ? cd{sqrt(b2), 0.0} : cd{0.0, sqrt(-b2)};
+  return b2 != 0.0;
+}
+
+// exp(i ph) exp(i bn d), d >= 0
+__device__ __forceinline__ cd far_phase(double ph, const cd &bn, double d) {
+  if (bn.y == 0.0) return cis(ph + bn.x * d);
+  return cis(ph) * exp(-bn.y * d);
+}
+
+// partial amplitudes: blockIdx.x blocks of 64 orders, blockIdx.y the side,
+// blockIdx.z one of FAR_NCH item chunks of domain 0; the items of a chunk pass
+// through LDS 64 at a time.  part: (chunk, side, order) x 8 doubles
+__global__ void __launch_bounds__(64) k_far_amp(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
+                                                FarGeom F, int type, double *__restrict__ part) {
+  const int lane = threadIdx.x, j = blockIdx.x * 64 + lane, side = blockIdx.y, ch = blockIdx.z;
+  const Tab &T = *tab;
+  const DomView &V = X.dom[0];
+  const int npt = rule_points(type, X.any_tri);
+  const long long items = (long long)V.Ne * npt;
+  const long long per = (items + FAR_NCH - 1) / FAR_NCH;
+  const long long i0 = (long long)ch * per, i1 = min(items, i0 + per);
+  double ax = 0.0, ay = 0.0;
+  cd bn = {0.0, 0.0};
+  const bool ok = j < F.G;
+  if (ok) far_order(P, F, j, ax, ay, bn);
+  const cd kz = side == 0 ? -bn : bn;
+  const double zref = side == 0 ? F.ztop : F.zbot;
+  // per item: y (3), w (3), alpha, beta, gamma[3], delta[3] (re, im)
+  __shared__ double sit[22][64];
+  cd A = {0.0, 0.0}, B[3] = {};
+  for (long long b0 = i0; b0 < i1; b0 += 64) {
+    const long long it = b0 + lane;
+    double y[3] = {0.0, 0.0, zref}, w[3] = {0.0, 0.0, 0.0};
+    cd al = {}, be = {}, gm[3] = {}, dl[3] = {};
+    if (it < i1) {
+      const int s = (int)(it / npt) + 1, q = (int)(it % npt);
+      const int as = abs(V.sid[s]);
+      ElemC E;
+      if (type != 0) load_elem(M, as, E);
+      double r[3], wv[3], wt, Nv[4];
+      if (source_point(M, T, E, as, type, q, r, wv, wt, Nv)) {
+        cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
+        for (int m = 0; m < 4; m++) {
+          if (Nv[m] == 0.0) continue;
+          Pi += ld(V.P, 4 * s + m) * Nv[m];
+          dPi += ld(V.dP, 4 * s + m) * Nv[m];
+          for (int c = 0; c < 3; c++) {
+            pvi[c] += ld(V.pv, 12 * s + 3 * m + c) * Nv[m];
+            dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
+          }
+        }
+        const double aW = abs3(wv);
+        al = dPi * (wt * aW);
+        be = Pi * (-wt);
+        for (int c = 0; c < 3; c++) {
+          gm[c] = dpvi[c] * (wt * aW);
+          dl[c] = pvi[c] * (-wt);
+          y[c] = r[c];
+          w[c] = wv[c];
+        }
+      }
+    }
+    for (int c = 0; c < 3; c++) {
+      sit[c][lane] = y[c];
+      sit[3 + c][lane] = w[c];
+      sit[10 + 2 * c][lane] = gm[c].x;
+      sit[11 + 2 * c][lane] = gm[c].y;
+      sit[16 + 2 * c][lane] = dl[c].x;
+      sit[17 + 2 * c][lane] = dl[c].y;
+    }
+    sit[6][lane] = al.x;
+    sit[7][lane] = al.y;
+    sit[8][lane] = be.x;
+    sit[9][lane] = be.y;
+    __syncthreads();
+    const int nb = (int)min((long long)64, i1 - b0);
+    if (ok) {
+      for (int i = 0; i < nb; i++) {
+        const double d = side == 0 ? zref - sit[2][i] : sit[2][i] - zref;
+        const cd Y = far_phase(ax * sit[0][i] + ay * sit[1][i], bn, d);
+        const cd ikw = mul_i(cd{ax * sit[3][i] + ay * sit[4][i], 0.0} + kz * sit[5][i]);
+        A += Y * (cd{sit[6][i], sit[7][i]} + ikw * cd{sit[8][i], sit[9][i]});
+        for (int c = 0; c < 3; c++)
+          B[c] += Y * (cd{sit[10 + 2 * c][i], sit[11 + 2 * c][i]} + ikw * cd{sit[16 + 2 * c][i], sit[17 + 2 * c][i]});
+      }
+    }
+    __syncthreads();
+  }
+  if (!ok) return;
+  double *dst = part + (((size_t)ch * 2 + side) * F.G + j) * 8;
+  dst[0] = A.x;
+  dst[1] = A.y;
+  for (int c = 0; c < 3; c++) {
+    dst[2 + 2 * c] = B[c].x;
+    dst[3 + 2 * c] = B[c].y;
+  }
+}
+
+// amplitudes: the chunks summed in order, times (i/2A)/b_n.  amp: (side, order)
+// x 8 doubles.  A Wood anomaly (b_n = 0) counts as an error, as in fd_cell.
+__global__ void __launch_bounds__(256) k_far_fin(GfParams P, FarGeom F, const double *__restrict__ part,
+                                                 double *__restrict__ amp, long long amp_terms,
+                                                 long long *__restrict__ counters) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx == 0) atomicAdd((unsigned long long *)&counters[CNT_FAR + 2], (unsigned long long)amp_terms);
+  if (idx >= 2 * F.G) return;
+  const int side = idx / F.G, j = idx % F.G;
+  double ax, ay;
+  cd bn;
+  double *dst = amp + (size_t)idx * 8;
+  if (!far_order(P, F, j, ax, ay, bn)) {
+    for (int i = 0; i < 8; i++) dst[i] = 0.0;
+    atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
+    return;
+  }
+  double acc[8];
+  for (int i = 0; i < 8; i++) acc[i] = 0.0;
+  for (int ch = 0; ch < FAR_NCH; ch++) {
+    const double *q = part + (((size_t)ch * 2 + side) * F.G + j) * 8;
+    for (int i = 0; i < 8; i++) acc[i] += q[i];
+  }
+  const cd cf = cd{0.0, 0.5 * P.aidd} * crecip(bn);
+  for (int i = 0; i < 4; i++) {
+    const cd v = cd{acc[2 * i], acc[2 * i + 1]} * cf;
+    dst[2 * i] = v.x;
+    dst[2 * i + 1] = v.y;
+  }
+}
+
+// far points: the sums over the orders, added to the point's first chunk of
+// k_field_part's partial sums (which hold F and grad F only for these points)
+__global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom F, int n, int nchunk,
+                                                 const double *__restrict__ r0s, const int *__restrict__ did_io,
+                                                 const double *__restrict__ amp, double *__restrict__ part,
+                                                 long long *__restrict__ counters) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  int side = -1;
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  if (ip < n) {
+    x0 = r0s[3 * ip];
+    x1 = r0s[3 * ip + 1];
+    x2 = r0s[3 * ip + 2];
+    const int did = did_io[ip];
+    if (did >= 0) side = far_side(X, did, x2);
+  }
+  const unsigned long long nfar = __popcll(__ballot(side >= 0));
+  if (threadIdx.x == 0 && nfar) {
+    atomicAdd((unsigned long long *)&counters[CNT_FAR], nfar);
+    atomicAdd((unsigned long long *)&counters[CNT_FAR + 1], nfar * (unsigned long long)F.G);
+  }
+  if (side < 0) return;
+  const double d = side == 0 ? x2 - F.ztop : F.zbot - x2;
+  const double *am = amp + (size_t)side * F.G * 8;
+  cd Pa = {}, dPa[3] = {}, PVa[3] = {};
+  for (int j = 0; j < F.G; j++) {
+    double ax, ay;
+    cd bn;
+    far_order(P, F, j, ax, ay, bn);
+    const cd Xn = far_phase(-(ax * x0 + ay * x1), bn, d);
+    const double *a = am + (size_t)j * 8;
+    const cd XA = Xn * cd{a[0], a[1]};
+    Pa += XA;
+    // -i kappa_c X_n A_n
+    dPa[0] += mul_i(XA) * (-ax);
+    dPa[1] += mul_i(XA) * (-ay);
+    dPa[2] += mul_i(XA * (side == 0 ? bn : -bn));
+    for (int c = 0; c < 3; c++) PVa[c] += Xn * cd{a[2 + 2 * c], a[3 + 2 * c]};
+  }
+  double *dst = part + (size_t)ip * nchunk * FP_N;
+  dst[0] += Pa.x;
+  dst[1] += Pa.y;
+  for (int c = 0; c < 3; c++) {
+    dst[2 + 2 * c] += dPa[c].x;
+    dst[3 + 2 * c] += dPa[c].y;
+    dst[8 + 2 * c] += PVa[c].x;
+    dst[9 + 2 * c] += PVa[c].y;
   }
 }
 
@@ -969,7 +1194,68 @@ static FieldCtx make_ctx(const bqd2_problem *p) {
   X.idd = 1.0 / (X.d1x * X.d2y - X.d2x * X.d1y);
   X.vkx = p->qd8[1];
   X.vky = p->qd8[2];
+  X.far_on = (p->m.ps == 1 && p->far.on && p->far_enabled) ? 1 : 0;
+  X.far_ztop = p->far.ztop;
+  X.far_zbot = p->far.zbot;
+  X.far_h = p->far.h;
   return X;
+}
+
+// device bytes of the far field's partial amplitudes and of its amplitudes
+static size_t far_part_bytes(const bqd2_problem *p) { return (size_t)FAR_NCH * 2 * p->far.G * 8 * sizeof(double); }
+static size_t far_amp_bytes(const bqd2_problem *p) { return (size_t)2 * p->far.G * 8 * sizeof(double); }
+
+// the far field's amplitudes of rule type `type` into amp, through the partial
+// buffer part, on stream st
+static int far_launch(bqd2_problem *p, const FieldCtx &X, int type, double *part, double *amp, long long *ctr,
+                      hipStream_t st) {
+  const FarGeom &F = p->far;
+  const long long items = (long long)p->dom[0].Ne * rule_points(type, X.any_tri);
+  hipLaunchKernelGGL(k_far_amp, dim3((F.G + 63) / 64, 2, FAR_NCH), dim3(64), 0, st, p->m, p->tab, X, p->gf, F, type,
+                     part);
+  hipLaunchKernelGGL(k_far_fin, dim3(bqd2_blocks(2 * F.G, 256)), dim3(256), 0, st, p->gf, F, part, amp,
+                     2 * (long long)F.G * items, ctr);
+  BQD2_HIP(hipGetLastError());
+  return BQD2_OK;
+}
+
+// the amplitudes of the current solution for rule type `type`: made on the
+// caller's stream when the solution changed since they were last made (under
+// the problem's far-field lock), else the caller's stream waits for the call
+// that made them
+static int far_amplitudes(bqd2_problem *p, CallCtx *cx, const FieldCtx &X, int type, double *part, long long *ctr,
+                          const double **amp) {
+  std::lock_guard<std::mutex> g(p->far_mu);
+  const unsigned long long want = p->sol_epoch.load() + 1;
+  if (p->far_amp_epoch[type] == want) {
+    BQD2_HIP(hipStreamWaitEvent(cx->st, p->far_ev[type], 0));
+    *amp = p->far_amp[type];
+    return BQD2_OK;
+  }
+  int rc;
+  if (!p->far_amp[type] && (rc = bqd2_alloc(p, (void **)&p->far_amp[type], far_amp_bytes(p)))) return rc;
+  if (!p->far_ev[type]) BQD2_HIP(hipEventCreateWithFlags(&p->far_ev[type], hipEventDisableTiming));
+  if ((rc = far_launch(p, X, type, part, p->far_amp[type], ctr, cx->st))) return rc;
+  BQD2_HIP(hipEventRecord(p->far_ev[type], cx->st));
+  p->far_amp_epoch[type] = want;
+  *amp = p->far_amp[type];
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_set_field_far(bqd2_problem *p, int on) {
+  BQD2_CHECK(p && (on == 0 || on == 1), BQD2_E_ARG, "bqd2_set_field_far: bad arguments");
+  p->far_enabled = on;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_field_far_info(bqd2_problem *p, int *on, int *orders, double *zbot, double *ztop, double *h) {
+  BQD2_CHECK(p, BQD2_E_ARG, "bqd2_field_far_info: null problem");
+  if (on) *on = (p->m.ps == 1 && p->far.on && p->far_enabled) ? 1 : 0;
+  if (orders) *orders = p->far.G;
+  if (zbot) *zbot = p->far.zbot;
+  if (ztop) *ztop = p->far.ztop;
+  if (h) *h = p->far.h;
+  return BQD2_OK;
 }
 
 // per-call device buffers carved from the problem's workspace, which grows
@@ -1015,20 +1301,34 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
     const long long items = (long long)p->dom[d].Ne * npt, chunk = field_chunk_items(items);
     nchunk = std::max(nchunk, (int)((items + chunk - 1) / chunk));
   }
+  FieldCtx X = make_ctx(p);
+  // far points in the call (a point's height is not changed by the reduction
+  // to the lattice cell)
+  bool any_far = false;
+  if (X.far_on)
+    for (int i = 0; i < n && !any_far; i++)
+      any_far = pts[3 * i + 2] >= X.far_ztop + X.far_h || pts[3 * i + 2] <= X.far_zbot - X.far_h;
+  if (!any_far) X.far_on = 0;
+  double *d_fpart = nullptr;
   if ((rc = S.init(cx, pad256(nn * 24) * 2 + pad256(nn * 64) + pad256(nn * 8) + pad256(nn * 4) +
-                          pad256(nn * nchunk * FP_N * 8))))
+                          pad256(nn * nchunk * FP_N * 8) + (any_far ? pad256(far_part_bytes(p)) : 0))))
     return rc;
   if ((rc = S.get((void **)&d_pts, (size_t)n * 24)) || (rc = S.get((void **)&d_r0, (size_t)n * 24)) ||
       (rc = S.get((void **)&d_out, (size_t)n * 64)) || (rc = S.get((void **)&d_l, (size_t)n * 8)) ||
       (rc = S.get((void **)&d_did, (size_t)n * 4)) || (rc = S.get((void **)&d_part, nn * nchunk * FP_N * 8)))
     return rc;
+  if (any_far && (rc = S.get((void **)&d_fpart, far_part_bytes(p)))) return rc;
   BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
-  FieldCtx X = make_ctx(p);
   BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
+  const double *amp = nullptr;
+  if (any_far && (rc = far_amplitudes(p, cx, X, type, d_fpart, ctr, &amp))) return rc;
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
+  if (any_far)
+    hipLaunchKernelGGL(k_far_eval, dim3(bqd2_blocks(n, 64)), dim3(64), 0, st, X, p->gf, p->far, n, nchunk, d_r0, d_did,
+                       amp, d_part, ctr);
   hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, X, total, n, nchunk, d_part, d_l, d_pts,
                      d_did, d_out);
   BQD2_HIP(hipGetLastError());
@@ -1089,26 +1389,37 @@ extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, i
     const long long items = (long long)p->dom[d].Ne * npt, chunk = field_chunk_items(items);
     nchunk = std::max(nchunk, (int)((items + chunk - 1) / chunk));
   }
-  // scratch layout: counters, r0 (n x 3), lattice cell (n x 2 int), chunk partials
+  FieldCtx X = make_ctx(p);
+  // scratch layout: counters, r0 (n x 3), lattice cell (n x 2 int), chunk
+  // partials, then the far field's partial amplitudes and amplitudes (made in
+  // every call: the same kernels on the same data as bqd2_field's, so the same
+  // bits)
   const size_t o_r0 = 256, o_l = o_r0 + pad256(nn * 24), o_part = o_l + pad256(nn * 8);
-  const size_t need = o_part + nn * nchunk * FP_N * 8;
+  const size_t o_fpart = o_part + pad256(nn * nchunk * FP_N * 8);
+  const size_t o_famp = o_fpart + (X.far_on ? pad256(far_part_bytes(p)) : 0);
+  const size_t need = o_famp + (X.far_on ? far_amp_bytes(p) : 0);
   BQD2_CHECK(need <= (size_t)BQD2_NOALLOC_SCRATCH_BYTES, BQD2_E_ARG,
              "bqd2_field_noalloc: %d points of type %d need %zu bytes of scratch, more than its %zu bytes "
              "(at most %zu points per call on this problem)",
              n, type, need, (size_t)BQD2_NOALLOC_SCRATCH_BYTES,
-             (size_t)((BQD2_NOALLOC_SCRATCH_BYTES - 256 - 512) / (24 + 8 + (size_t)nchunk * FP_N * 8)));
+             (size_t)((BQD2_NOALLOC_SCRATCH_BYTES - 256 - 768 - (o_famp - o_fpart) - (need - o_famp)) /
+                      (24 + 8 + (size_t)nchunk * FP_N * 8)));
   char *base = nullptr;
   int rc;
   if ((rc = noalloc_scratch(&base))) return rc;
   long long *ctr = (long long *)base;
   double *d_r0 = (double *)(base + o_r0), *d_part = (double *)(base + o_part);
   int *d_l = (int *)(base + o_l);
-  FieldCtx X = make_ctx(p);
+  double *d_fpart = (double *)(base + o_fpart), *d_famp = (double *)(base + o_famp);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, ctr);
+  if (X.far_on && (rc = far_launch(p, X, type, d_fpart, d_famp, ctr, stream))) return rc;
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, stream, p->m, X, p->rn, p->ed, n, d_pts,
                      d_did, d_r0, d_l);
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
+  if (X.far_on)
+    hipLaunchKernelGGL(k_far_eval, dim3(bqd2_blocks(n, 64)), dim3(64), 0, stream, X, p->gf, p->far, n, nchunk, d_r0,
+                       d_did, d_famp, d_part, ctr);
   hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, stream, X, total, n, nchunk, d_part, d_l,
                      d_pts, d_did, d_out);
   if (d_errors) hipLaunchKernelGGL(k_copy_errors, dim3(1), dim3(64), 0, stream, ctr, d_errors);

@@ -1,5 +1,6 @@
 // problem.h -- device-resident state of one boundary-value problem.
 #pragma once
+#include <atomic>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -82,6 +83,18 @@ struct CallCtx {
   long long *cnt;
 };
 
+// Field points of the periodic domain beyond the boundary's z extent by h or
+// more: there the lattice sum is the spectral series in which the source and
+// target factors separate (field.hip, k_far_*).  Orders (n1, n2) on the grid
+// n1lo .. n1lo + n1n - 1, n2lo .. n2lo + n2n - 1 (G = n1n n2n of them); set at
+// creation, on = 0 where the grid would exceed FAR_MAX_ORDERS.
+struct FarGeom {
+  int on;
+  int n1lo, n1n, n2lo, n2n, G;
+  double ztop, zbot, h;
+};
+enum { FAR_MAX_ORDERS = 8192, FAR_NCH = 8 };
+
 struct bqd2_problem {
   DevMesh m;
   Tab *tab;            // device
@@ -146,6 +159,16 @@ struct bqd2_problem {
   double *ldinv;             // na complex: inverse diagonal of the system
   double *lvec;              // 3 na complex: the operator's two column vectors, the right-hand side's
   bqd2_large_stats lst;      // timings of the last solve and velocity recovery
+  // far field: geometry, and the plane-wave amplitudes of the current solution
+  // per field rule type, made at the first field call that needs them after
+  // the solution changed (sol_epoch counts the changes)
+  FarGeom far;
+  int far_enabled;           // bqd2_set_field_far (1 by default)
+  std::atomic<unsigned long long> sol_epoch;
+  std::mutex far_mu;
+  double *far_amp[4];        // 2 sides x G orders x 8 doubles
+  unsigned long long far_amp_epoch[4];  // sol_epoch + 1 they were made for (0: none)
+  hipEvent_t far_ev[4];      // recorded after they were made
 };
 enum { LG_EMPTY = 0, LG_G1H1 = 1, LG_AF = 2 };
 
@@ -210,7 +233,9 @@ int bqd2_workspace(CallCtx *c, size_t bytes, char **base);
 // device counter slots: errors, then per kernel family the summed Green's
 // function terms by kind (spatial, propagating, evanescent, pure spectral) and
 // the number of evaluations
-enum { CNT_ERR = 0, CNT_PAIRS = 1, CNT_SELF = 6, CNT_FIELD = 11, CNT_N = 16 };
+// and, from CNT_FAR, the far field's points, their point-order products and
+// the item-order products of its amplitudes
+enum { CNT_ERR = 0, CNT_PAIRS = 1, CNT_SELF = 6, CNT_FIELD = 11, CNT_FAR = 16, CNT_N = 19 };
 
 __device__ __forceinline__ void tally_add(long long *c, long long sp, long long pr, long long ev, long long fd,
                                           long long evals) {

@@ -98,6 +98,49 @@ static int large_layout_ok(const bqd2_model *m) {
 
 static int create_impl(const bqd2_model *m, int large, bqd2_problem **out);
 
+// Far-field geometry of the periodic domain.  Above z_top + h (below z_bot - h)
+// every source point lies at least h lower (higher), so the lattice sum's
+// spectral series converges for every (point, source) pair of the field and
+// its order-n term is a product of a target factor and a source factor; the
+// orders kept are those with |a_n| <= sqrt(k^2 + (FAR_DECAY / h)^2): the ones
+// beyond decay by exp(-FAR_DECAY) = 1e-20 or more over the height h.  h is a
+// quarter of the shorter lattice vector, which keeps the order grid near 4000
+// orders on lattices of comparable sides.
+static void far_setup(bqd2_problem *p, const bqd2_model *m) {
+  FarGeom &F = p->far;
+  memset(&F, 0, sizeof(F));
+  const int ne0 = m->dom_ne[0];
+  const int *sid = m->dom_sid;  // domain 0's list comes first
+  double zt = -1e300, zb = 1e300;
+  for (int t = 1; t <= ne0; t++) {
+    const int s = abs(sid[t]);
+    for (int j = 0; j < 4; j++) {
+      const int nd = m->ed[4 * (size_t)s + j];
+      if (j == 3 && nd == 0) continue;  // linear triangle
+      const double z = m->rn[3 * (size_t)nd + 2];
+      zt = std::max(zt, z);
+      zb = std::min(zb, z);
+    }
+  }
+  const GfParams &P = p->gf;
+  const double l1 = hypot(P.d1x, P.d1y), l2 = hypot(P.d2x, P.d2y);
+  const double FAR_DECAY = 46.0, h = 0.25 * std::min(l1, l2);
+  const double R = sqrt(P.k * P.k + (FAR_DECAY / h) * (FAR_DECAY / h)) + hypot(P.vkx, P.vky);
+  // a_n + vk = n1 g1 + n2 g2 with g_i . d_j = 2 pi delta_ij: |n_i| <= R |d_i| / 2 pi
+  const int N1 = (int)ceil(R * l1 / (2.0 * M_PI)), N2 = (int)ceil(R * l2 / (2.0 * M_PI));
+  const long long G = (long long)(2 * N1 + 1) * (2 * N2 + 1);
+  if (ne0 <= 0 || !(zt >= zb) || G > FAR_MAX_ORDERS) return;
+  F.n1lo = -N1;
+  F.n1n = 2 * N1 + 1;
+  F.n2lo = -N2;
+  F.n2n = 2 * N2 + 1;
+  F.G = (int)G;
+  F.ztop = zt;
+  F.zbot = zb;
+  F.h = h;
+  F.on = 1;
+}
+
 extern "C" int bqd2_create(const bqd2_model *m, bqd2_problem **out) { return create_impl(m, 0, out); }
 
 extern "C" int bqd2_create_large(const bqd2_model *m, bqd2_problem **out) {
@@ -240,6 +283,8 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
     BQD2_HIP(hipMemsetAsync(D.pv, 0, ((size_t)D.Ne + 1) * 12 * 16, p->stream));
     BQD2_HIP(hipMemsetAsync(D.dpv, 0, ((size_t)D.Ne + 1) * 12 * 16, p->stream));
   }
+  p->far_enabled = 1;
+  if (m->ps == 1) far_setup(p, m);
   p->na = 2 * m->NN;
   if (large) {
     // S = [H0 G0; H1 G1], row-major, na = 2N columns per row; the pair kernels
@@ -280,6 +325,8 @@ extern "C" int bqd2_destroy(bqd2_problem *p) {
     delete c;
   }
   if (p->lu_graph) (void)hipGraphExecDestroy(p->lu_graph);
+  for (int t = 0; t < 4; t++)
+    if (p->far_ev[t]) (void)hipEventDestroy(p->far_ev[t]);
   for (auto &D : p->dom)
     if (D.g_graph) (void)hipGraphExecDestroy(D.g_graph);
   for (void *ptr : p->owned) (void)hipFree(ptr);
@@ -346,6 +393,7 @@ extern "C" int bqd2_download_solution(bqd2_problem *p, int did, double *P, doubl
 extern "C" int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double *dP, const double *pv,
                                     const double *dpv) {
   BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_upload_solution: bad domain");
+  p->sol_epoch++;
   DevDomain &D = p->dom[did];
   const size_t n1 = (size_t)D.Ne + 1;
   BQD2_HIP(hipStreamSynchronize(p->stream));

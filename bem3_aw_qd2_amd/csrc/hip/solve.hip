@@ -505,6 +505,7 @@ int bqd2_velocity_rows(bqd2_problem *p, const DevDomain &D, const void *W, doubl
 
 extern "C" int bqd2_solve_surface(bqd2_problem *p, double *x_host) {
   BQD2_CHECK(p, BQD2_E_ARG, "bqd2_solve_surface: null problem");
+  p->sol_epoch++;  // the field's plane-wave amplitudes are made again
   for (int d = 0; d <= p->m.MN; d++)
     BQD2_CHECK(p->dom[d].Ne == 0 || p->dom[d].assembled, BQD2_E_STATE, "bqd2_solve_surface: domain %d not assembled", d);
   const int na = p->na;
@@ -640,6 +641,7 @@ static int dpv_lu(bqd2_problem *p, DevDomain &D, hipStream_t st) {
 
 extern "C" int bqd2_solve_velocity(bqd2_problem *p) {
   BQD2_CHECK(p && p->solved, BQD2_E_STATE, "bqd2_solve_velocity: surface solution missing");
+  p->sol_epoch++;
   hipStream_t st = p->stream;
   PwDev W;
   memcpy(W.p, p->pw_p, sizeof(W.p));

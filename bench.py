@@ -67,6 +67,16 @@ F_TERM_V2 = {"spatial": 318, "propagating": 322, "evanescent": 266, "pure_spectr
 # flops per GH64 point of a split pair: free-space kernel and gradient (sincos 30,
 # 18 mul/add/FMA) and its accumulation into the four shape functions (42)
 F_FREE = 90
+# flops of the far field's plane-wave sums in the same convention.  Amplitude
+# term (one boundary item and one order, k_far_amp): phase argument and height 4;
+# exp(i a.y) exp(-b dz) (sincos 30, exp 20) 50; kappa . w 6; the four
+# coefficients alpha + i (kappa . w) beta (complex product and sum) 32; their
+# products with the source factor accumulated 32: 124.  Point-order term (one
+# point and one order, k_far_eval): the order's a_n, b_n (6 products, b_n^2 3,
+# sqrt 8) 19; phase argument 3; the target factor 50; its product with A_n 6;
+# pressure 2; three gradient components 16; three boundary-value components 24:
+# 120
+F_FAR = {"amplitude_term": 124, "point_order_term": 120}
 PEAK_FP64_TFLOPS = 78.6
 
 
@@ -158,8 +168,8 @@ class Request:
 
     def _counters(self):
         from bem3_aw_qd2_amd import check
-        c = np.zeros(16, np.int64)
-        check(self.hip.bqd2_counters(self.p, c.ctypes.data, 16), "bqd2_counters")
+        c = np.zeros(19, np.int64)
+        check(self.hip.bqd2_counters(self.p, c.ctypes.data, 19), "bqd2_counters")
         return c
 
     def gf_evals(self):
@@ -893,6 +903,9 @@ def main():
                     help="0: FP64 GMRES (default), 1: FP64 LU, 2: complex float LU on MFMA + FP64 GMRES refinement")
     ap.add_argument("--cpu-pairs", type=int, default=384)
     ap.add_argument("--cpu-self", type=int, default=16)
+    ap.add_argument("--field-far", type=int, default=1, choices=(0, 1),
+                    help="field points far above or below the lattice of spheres from plane-wave amplitudes (1) "
+                         "or from per-pair lattice sums (0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-force-check", action="store_true", help="skip the oracle check of the force")
     ap.add_argument("--check-points", type=int, default=16,
@@ -978,6 +991,7 @@ def main():
         model = exdat.Model(host).read_inputs(*paths, out=os.path.join(tmp, "ex.dat")).initialize()
     prob = host.aqd2_problem(model.buf, None)
     lam = cfg["host"][1] / cfg["f"]
+    check(hip.bqd2_set_field_far(prob, args.field_far), "bqd2_set_field_far")
     req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam, args.planes), args.solver)
     log("bench: workload %s, %d elements, %d unknowns" % (args.workload, model.ne(), 2 * model.nn()))
 
@@ -1038,14 +1052,20 @@ def main():
     # phase's dominant kernel; the phase time bounds its duration from above)
     fc = req.field_counts
     kinds = ("spatial", "propagating", "evanescent", "pure_spectral")
-    field_flops = float(sum(int(fc[11 + i]) * F_TERM_V2[k] for i, k in enumerate(kinds)))
+    far_flops = float(int(fc[18]) * F_FAR["amplitude_term"] + int(fc[17]) * F_FAR["point_order_term"])
+    field_flops = float(sum(int(fc[11 + i]) * F_TERM_V2[k] for i, k in enumerate(kinds))) + far_flops
     field_s = float(np.mean([p["field"] for p in phases]))
-    roof_field = {"bound": "valu", "kernel": "k_field_part (whole field phase timed)",
+    fon, forders = C.c_int(), C.c_int()
+    check(hip.bqd2_field_far_info(prob, C.byref(fon), C.byref(forders), None, None, None), "bqd2_field_far_info")
+    roof_field = {"bound": "valu", "kernel": "k_field_part + k_far_* (whole field phase timed)",
                   "achieved": field_flops / field_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                   "frac": field_flops / field_s / 1e12 / PEAK_FP64_TFLOPS, "ms_per_launch": field_s * 1e3,
                   "flops_per_launch": field_flops,
                   "terms_per_launch": {k: int(fc[11 + i]) for i, k in enumerate(kinds)},
-                  "evals_per_launch": int(fc[15]), "flops_per_term": F_TERM_V2}
+                  "evals_per_launch": int(fc[15]), "flops_per_term": F_TERM_V2,
+                  "far_field": {"on": fon.value, "orders": forders.value, "points": int(fc[16]),
+                                "point_order_terms": int(fc[17]), "amplitude_terms": int(fc[18]),
+                                "flops_per_term": F_FAR}}
 
     cpu = None
     if not args.no_cpu:
