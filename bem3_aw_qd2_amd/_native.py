@@ -63,7 +63,7 @@ class AssemblyStats(C.Structure):
                 ("ms_decide", C.c_double), ("ms_pairs", C.c_double), ("ms_self", C.c_double),
                 ("pair_terms", C.c_longlong * 4), ("pair_evals", C.c_longlong),
                 ("self_terms", C.c_longlong * 4), ("self_evals", C.c_longlong),
-                ("pairs_split", C.c_longlong)]
+                ("pairs_split", C.c_longlong), ("pairs_smooth", C.c_longlong), ("free_points", C.c_longlong)]
 
 
 def hip_lib():
@@ -105,6 +105,8 @@ def hip_lib():
         lib.bqd2_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         lib.bqd2_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
+        lib.bqd2_set_remainder_rules.argtypes = [C.c_void_p, C.c_int]
+        lib.bqd2_gauss_legendre.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         lib.bqd2_set_field_far.argtypes = [C.c_void_p, C.c_int]
         lib.bqd2_field_far_info.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         lib.bqd2_field_bd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,

@@ -64,9 +64,86 @@ __device__ __forceinline__ bool split_ok(const ElemC &E, const Tab &T, const dou
   return true;
 }
 
-// rule per pair: 0 GL12, 1 GH64, 2 GH64 split (periodic domain only)
+// Rules of a pair (D.rule):
+//   0  the lattice sum on GL12 (the reference's rule)
+//   1  the lattice sum on GH64 (the reference's rule for near pairs)
+//   2  free-space kernel on GH64, the remainder on GL12
+//   3, 4, 5  free-space kernel on GH64, the remainder on 4, 6, 8 points a side
+//   6, 7, 8  free-space kernel on GL12, the remainder on 4, 6, 8 points a side
+// (2-8 in the periodic domain only).  The remainder G - G0 of the lattice sum is
+// analytic over the element up to the other lattice images of the target, so a
+// Gauss rule of n points a side integrates it with an error that falls as
+// t^-(2n+2), t the distance of the nearest such image in element half-widths.
+__host__ __device__ __forceinline__ int rule_free_n(int r) { return r >= 6 ? 12 : (r >= 2 ? 64 : 0); }
+__host__ __device__ __forceinline__ int rule_lat_n(int r) {
+  if (r == 1) return 64;
+  if (r <= 2) return 12;
+  return 4 + 2 * ((r - 3) % 3);
+}
+__device__ __forceinline__ void rule_tab(const Tab &T, int n, const double *&x, const double *&w) {
+  if (n == 4) { x = T.xs4; w = T.ws4; }
+  else if (n == 6) { x = T.xs6; w = T.ws6; }
+  else if (n == 8) { x = T.xs8; w = T.ws8; }
+  else if (n == 12) { x = T.xli; w = T.wli; }
+  else { x = T.xhi; w = T.whi; }
+}
+
+// Gauss points a side for the remainder of the pair (element E, target rt):
+// 4, 6 or 8, the smallest whose error estimate 3 (q/6)^(2n) stays below 1e-14
+// of the row, q = h max(k, 1/E0, 1/(d - r)) the element's half-width h = r/sqrt 2
+// against the remainder's shortest length (wavelength / 2 pi, Ewald parameter,
+// distance to the nearest lattice image of the target other than l = 0; r the
+// element's radius, d that image's distance from the element's centre); 0 when
+// none does.  The estimate is fitted to the measured convergence of the three
+// rules to GL12 on the 96- and 384-quad spheres of the 2 mm lattice (q 0.41 and
+// 0.21: 5e-10 and 5e-12 with 4 points, 1e-15 with 6 and 8).
+__device__ __forceinline__ int smooth_order(const ElemC &E, const double *rt, const GfParams &P) {
+  double c[3], rad = 0.0;
+  bil_rw_r(E, 0.0, 0.0, c);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double q[3];
+    bil_rw_r(E, (k == 1 || k == 2) ? 1.0 : -1.0, (k >= 2) ? 1.0 : -1.0, q);
+    rad = fmax(rad, sqrt((q[0] - c[0]) * (q[0] - c[0]) + (q[1] - c[1]) * (q[1] - c[1]) + (q[2] - c[2]) * (q[2] - c[2])));
+  }
+  const double ox = c[0] - rt[0], oy = c[1] - rt[1], oz = c[2] - rt[2];
+  // the lattice point nearest the offset, then the images around it
+  const int k1 = (int)floor((ox * P.d2y - oy * P.d2x) * P.idd + 0.5);
+  const int k2 = (int)floor(-(ox * P.d1y - oy * P.d1x) * P.idd + 0.5);
+  double dmin = 1e300;
+  for (int m1 = -2; m1 <= 2; m1++) {
+    for (int m2 = -2; m2 <= 2; m2++) {
+      const int l1 = m1 - k1, l2 = m2 - k2;
+      if (l1 == 0 && l2 == 0) continue;
+      const double dx = ox + l1 * P.d1x + l2 * P.d2x, dy = oy + l1 * P.d1y + l2 * P.d2y;
+      dmin = fmin(dmin, sqrt(dx * dx + dy * dy + oz * oz));
+    }
+  }
+  if (!(dmin > 2.0 * rad) || rad <= 0.0) return 0;
+  const double q = 0.7071067811865476 * rad * fmax(fmax(P.k, 1.0 / P.E0), 1.0 / (dmin - rad));
+  if (q <= 0.088) return 4;
+  if (q <= 0.36) return 6;
+  if (q <= 0.73) return 8;
+  return 0;
+}
+
+// a point of rule n on the element with a z offset level with the target
+__device__ __forceinline__ bool any_degenerate(const ElemC &E, const Tab &T, const double *rt, bool tri, int n) {
+  const double *x, *w;
+  rule_tab(T, n, x, w);
+  double z, e, wt;
+  for (int p = 0; p < rule_npts(tri, n); p++) {
+    rule_point(tri, n, x, w, p, z, e, wt);
+    if (offset_degenerate(E, z, e, rt)) return true;
+  }
+  return false;
+}
+
+// rule per pair (above); smooth = 0 keeps the remainder on GL12 (rules 0-2 only),
+// 1 takes smooth_order's choice, 4, 6 or 8 that order wherever smooth_order
+// allows a remainder rule at all
 __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict__ tab, DevDomain D, GfParams P,
-                                                int periodic) {
+                                                int periodic, int smooth) {
   long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)D.N * D.Ne) return;
   int row = (int)(gid / D.Ne), s = (int)(gid % D.Ne) + 1;
@@ -84,6 +161,12 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     const bool tri = M.tri[as] != 0;
     rule = (tri ? needs_gh_tri(E, T, rt, vtz, vte) : needs_gh(E, T, rt, vtz, vte)) ? 1 : 0;
     if (rule && periodic && split_ok(E, T, rt, P, tri)) rule = 2;
+    if (periodic && smooth && rule != 1) {
+      int n = smooth_order(E, rt, P);
+      if (n && smooth > 1) n = smooth;
+      if (n && !any_degenerate(E, T, rt, tri, n) && (rule == 2 || !any_degenerate(E, T, rt, tri, 12)))
+        rule = (unsigned char)((rule == 2 ? 3 : 6) + (n - 4) / 2);
+    }
     key = (float)fabs(E.cr[2][0] - rt[2]);
     if (periodic) {
       // the low six mantissa bits of the key carry the in-plane offset of the
@@ -119,6 +202,34 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+// the free-space kernel e^{ikR}/(4 pi R) of a split pair on its NF x NF rule
+// (x, w), this lane's points of the pair's 16 lanes, added to g, h and the
+// solid-angle terms
+template <int NF>
+__device__ __forceinline__ void free_part(const ElemC &E, const Tab &T, const double *x, const double *w,
+                                          const double *rt, const double *vtz, const double *vte, double k, bool tri,
+                                          bool on, int sl, cd *g, cd *h, double &F, double &F0, double &F1) {
+  const int npf = on ? rule_npts(tri, NF) : 0;
+  for (int p = sl; p < npf; p += LPP) {
+    double z, e, wt;
+    rule_point(tri, NF, x, w, p, z, e, wt);
+    PtGeo q;
+    point_geo(E, z, e, rt, vtz, vte, q);
+    cd g0, dg0[3];
+    free_gf(k, q.vR, q.aR, q.iR, g0, dg0);
+    cd hn = dg0[0] * q.vW[0] + dg0[1] * q.vW[1] + dg0[2] * q.vW[2];
+#pragma unroll
+    for (int ep = 0; ep < 4; ep++) {
+      double Np = shape_N(tri, ep, z, e);
+      g[ep] += g0 * (wt * Np * q.aW);
+      h[ep] += hn * (wt * Np);
+    }
+    F += wt * q.rdW * q.iR * q.iR;
+    F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
+    F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
+  }
+}
+
 // The pairs PPW wg .. PPW wg + PPW - 1 of the list (one wavefront); wg is the
 // workgroup of a launch over one domain, or a task of the batched launch.
 __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
@@ -149,30 +260,14 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
   GfTally tl{0, 0, 0, 0};
   int err = 0;
   long long nev = 0;
-  if (rule == 2) {
-    // near-singular image on GH64 with the free-space kernel
-    const int np64 = on ? rule_npts(tri, 64) : 0;
-    for (int p = sl; p < np64; p += LPP) {
-      double z, e, wt;
-      rule_point(tri, 64, T.xhi, T.whi, p, z, e, wt);
-      PtGeo q;
-      point_geo(E, z, e, rt, vtz, vte, q);
-      cd g0, dg0[3];
-      free_gf(P.k, q.vR, q.aR, q.iR, g0, dg0);
-      cd hn = dg0[0] * q.vW[0] + dg0[1] * q.vW[1] + dg0[2] * q.vW[2];
-#pragma unroll
-      for (int ep = 0; ep < 4; ep++) {
-        double Np = shape_N(tri, ep, z, e);
-        g[ep] += g0 * (wt * Np * q.aW);
-        h[ep] += hn * (wt * Np);
-      }
-      F += wt * q.rdW * q.iR * q.iR;
-      F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
-      F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
-    }
-  }
+  const int nfree = rule_free_n(rule);
+  // the image l = 0 with the free-space kernel, on GH64 (near pairs) or GL12
+  if (nfree == 64)
+    free_part<64>(E, T, T.xhi, T.whi, rt, vtz, vte, P.k, tri, on, sl, g, h, F, F0, F1);
+  else if (nfree == 12)
+    free_part<12>(E, T, T.xli, T.wli, rt, vtz, vte, P.k, tri, on, sl, g, h, F, F0, F1);
   {
-    // lattice sum on the pair's rule (GL12 for the remainder of a split pair).
+    // lattice sum on the pair's rule (the remainder's rule for a split pair).
     // Only the offset and the accumulators stay in registers across the lattice
     // sum: the pair's element, target node and tangents sit in LDS per pair, and
     // the point's weights for the accumulation are formed before the sum and
@@ -201,10 +296,9 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
       }
     }
     __syncthreads();
-    const bool gh = rule == 1;
-    const int n = gh ? 64 : 12;
-    const double *x = gh ? T.xhi : T.xli;
-    const double *w = gh ? T.whi : T.wli;
+    const int n = rule_lat_n(rule);
+    const double *x, *w;
+    rule_tab(T, n, x, w);
     const int npts = on ? rule_npts(tri, n) : 0;
     nev = (sl == 0) ? npts : 0;
     for (int p = sl; p < npts; p += LPP) {
@@ -230,7 +324,7 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
       }
 #pragma unroll
       for (int d = 0; d < 3; d++) spt[8 + d][lane] = q.vW[d];
-      if (rule != 2) {
+      if (!nfree) {
         F += wt * q.rdW * q.iR * q.iR;
         F0 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv0 * q.rdW - q.Wdv0);
         F1 += wt * q.iR * q.iR * q.iR * (3.0 * q.rdv1 * q.rdW - q.Wdv1);
@@ -241,7 +335,7 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
       int c = qpgf_eval<GF_V1>(P, vR[0], vR[1], vR[2], 1.0e-15, gf, tl);
       if (c < 0) err = c;
       asm volatile("" ::: "memory");
-      if (rule == 2) {
+      if (nfree) {
         cd g0, dg0[3];
         const double aR = abs3(vR);
         free_gf(P.k, vR, aR, 1.0 / aR, g0, dg0);
@@ -1094,7 +1188,7 @@ __global__ void __launch_bounds__(256) k_rows(DevMesh M, DevDomain D) {
 // sum (pure spectral beyond a height, Ewald below) and about as many terms.
 static void pair_list(const DevDomain &D, const std::vector<unsigned char> &rule, const std::vector<float> &key,
                       const std::vector<char> &tripos, std::vector<int> &out) {
-  std::vector<int> cls[6];
+  std::vector<int> cls[18];
   for (int row = 0; row < D.N; row++) {
     const int t = row / 4 + 1, tn = row % 4;
     if (tripos[t] && tn == 3) continue;
@@ -1192,7 +1286,7 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
   const bool first = (D.plist_n == 0);
   if (first) {
     hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, st_, p->m, p->tab, D, p->gf,
-                       periodic ? 1 : 0);
+                       periodic ? 1 : 0, p->smooth_mode);
     BQD2_HIP(hipGetLastError());
   }
   BQD2_HIP(hipEventRecord(ev.e1, st_));
@@ -1205,10 +1299,16 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
       BQD2_HIP(hipMemcpyAsync(key.data(), D.pkey, (size_t)npairs * sizeof(float), hipMemcpyDeviceToHost, st_));
     }
     BQD2_HIP(hipStreamSynchronize(st_));
-    D.n_gh = D.n_split = 0;
-    for (unsigned char r : rule) {
-      D.n_gh += (r != 0);
-      D.n_split += (r == 2);
+    D.n_gh = D.n_split = D.n_smooth = D.n_free_pts = 0;
+    for (long long gid = 0; gid < npairs; gid++) {
+      const int r = rule[gid];
+      D.n_gh += (r >= 1 && r <= 5);
+      D.n_split += (r >= 2 && r <= 5);
+      D.n_smooth += (r >= 3);
+      if (r >= 2) {
+        const int nf = rule_free_n(r), s = (int)(gid % D.Ne) + 1;
+        D.n_free_pts += (p->tripos[did][s] ? 3 : 1) * (long long)nf * nf;
+      }
     }
     std::vector<int> plist;
     if (periodic) {
@@ -1266,6 +1366,8 @@ static int asm_end(bqd2_problem *p, int did, AsmPhase &ev, bqd2_assembly_stats *
     for (int i = 0; i < 4; i++) st->gf_terms += st->pair_terms[i] + st->self_terms[i];
     st->pairs_gh = ngh;
     st->pairs_split = nsplit;
+    st->pairs_smooth = D.n_smooth;
+    st->free_points = D.n_free_pts;
     (void)periodic;
     const long long valid_rows = D.nn;  // collocation nodes: 4 per quadrilateral, 3 per triangle
     st->self_rows = valid_rows;
@@ -1293,7 +1395,8 @@ int bqd2_decide_only(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   BQD2_HIP(hipEventRecord(ev.e0, p->stream));
   if (D.plist_n == 0) {
     const long long npairs = (long long)D.N * D.Ne;
-    hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, p->stream, p->m, p->tab, D, p->gf, 0);
+    hipLaunchKernelGGL(k_decide, dim3(bqd2_blocks(npairs, 256)), dim3(256), 0, p->stream, p->m, p->tab, D, p->gf, 0,
+                       0);
     BQD2_HIP(hipGetLastError());
     D.plist_n = 1;
   }
@@ -1312,6 +1415,18 @@ int bqd2_decide_only(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
 
 int bqd2_lg_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st);
 int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st);
+
+// The remainder rules 3-8 on (1), off (0: rules 0-2 only, the lattice sum on
+// every GL12 point as the reference sums it) or on with one order (4, 6, 8)
+// wherever a remainder rule is allowed; the rules are decided again at the
+// next assembly of each domain
+extern "C" int bqd2_set_remainder_rules(bqd2_problem *p, int on) {
+  BQD2_CHECK(p && (on == 0 || on == 1 || on == 4 || on == 6 || on == 8), BQD2_E_ARG,
+             "bqd2_set_remainder_rules: bad arguments");
+  p->smooth_mode = on;
+  for (auto &D : p->dom) D.plist_n = 0;
+  return BQD2_OK;
+}
 
 extern "C" int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
   BQD2_CHECK(p && did >= 0 && did <= p->m.MN, BQD2_E_ARG, "bqd2_assemble: bad domain %d", did);

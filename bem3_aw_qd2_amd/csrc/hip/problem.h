@@ -12,6 +12,9 @@ struct Tab {
   double xli[12], wli[12], xhi[64], whi[64];
   double zt_44[4], et_44[4], wt_44[4], zt_34[4], et_34[4], wt_34[4];
   double zt_49[9], et_49[9], wt_49[9], zt_37[7], et_37[7], wt_37[7];
+  // Gauss-Legendre rules of 4, 6 and 8 points for the smooth remainder of the
+  // lattice sum (bqd2_gauss_legendre)
+  double xs4[4], ws4[4], xs6[6], ws6[6], xs8[8], ws8[8];
 };
 
 // element data on the device, indexed by element id (row 0 unused)
@@ -47,6 +50,8 @@ struct DevDomain {
   float *pkey;         // N x Ne : |z| offset of the element centre from the target (pair order)
   int plist_n;         // entries of plist, 0 until the first assembly (rules and list are cached)
   long long n_gh, n_split;  // GH64 pairs, split ones among them
+  long long n_smooth;       // pairs whose lattice remainder takes a 4-, 6- or 8-point rule
+  long long n_free_pts;     // free-space kernel points summed by the split pairs
   double *P, *dP;      // (Ne+1) x 4 complex
   double *pv, *dpv;    // (Ne+1) x 4 x 3 complex
   bool assembled;
@@ -164,6 +169,7 @@ struct bqd2_problem {
   // the solution changed (sol_epoch counts the changes)
   FarGeom far;
   int far_enabled;           // bqd2_set_field_far (1 by default)
+  int smooth_mode;           // bqd2_set_remainder_rules (1 by default)
   std::atomic<unsigned long long> sol_epoch;
   std::mutex far_mu;
   double *far_amp[4];        // 2 sides x G orders x 8 doubles

@@ -98,6 +98,34 @@ static int large_layout_ok(const bqd2_model *m) {
 
 static int create_impl(const bqd2_model *m, int large, bqd2_problem **out);
 
+// Gauss-Legendre nodes and weights by Newton's iteration on the three-term
+// recurrence of the Legendre polynomials (Abramowitz and Stegun 25.4.29), from
+// the starting values cos(pi (i + 3/4) / (n + 1/2))
+extern "C" int bqd2_gauss_legendre(int n, double *x, double *w) {
+  BQD2_CHECK(n >= 1 && n <= 64 && x && w, BQD2_E_ARG, "bqd2_gauss_legendre: bad arguments");
+  for (int i = 0; i < (n + 1) / 2; i++) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        const double pm = p1;
+        p1 = p0;
+        p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * pm) / j;
+      }
+      dp = n * (z * p0 - p1) / (z * z - 1.0);
+      const double dz = p0 / dp;
+      z -= dz;
+      if (fabs(dz) <= 1e-17) break;
+    }
+    if (2 * i + 1 == n) z = 0.0;
+    const double wi = 2.0 / ((1.0 - z * z) * dp * dp);
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = wi;
+  }
+  return BQD2_OK;
+}
+
 // Far-field geometry of the periodic domain.  Above z_top + h (below z_bot - h)
 // every source point lies at least h lower (higher), so the lattice sum's
 // spectral series converges for every (point, source) pair of the field and
@@ -191,6 +219,9 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
   memcpy(T.zt_49, m->zt_49, sizeof(T.zt_49)); memcpy(T.et_49, m->et_49, sizeof(T.et_49));
   memcpy(T.wt_49, m->wt_49, sizeof(T.wt_49)); memcpy(T.zt_37, m->zt_37, sizeof(T.zt_37));
   memcpy(T.et_37, m->et_37, sizeof(T.et_37)); memcpy(T.wt_37, m->wt_37, sizeof(T.wt_37));
+  TRY(bqd2_gauss_legendre(4, T.xs4, T.ws4));
+  TRY(bqd2_gauss_legendre(6, T.xs6, T.ws6));
+  TRY(bqd2_gauss_legendre(8, T.xs8, T.ws8));
   TRY(upload(p, &p->tab, &T, 1));
 
   for (int d = 0; d <= m->MN; d++) {
@@ -284,6 +315,7 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
     BQD2_HIP(hipMemsetAsync(D.dpv, 0, ((size_t)D.Ne + 1) * 12 * 16, p->stream));
   }
   p->far_enabled = 1;
+  p->smooth_mode = 1;
   if (m->ps == 1) far_setup(p, m);
   p->na = 2 * m->NN;
   if (large) {

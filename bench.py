@@ -192,7 +192,7 @@ def measured_traffic(workload):
 
 def pair_flops(st):
     kinds = ("spatial", "propagating", "evanescent", "pure_spectral")
-    return sum(st.pair_terms[i] * F_TERM[k] for i, k in enumerate(kinds)) + st.pairs_split * 4096 * F_FREE
+    return sum(st.pair_terms[i] * F_TERM[k] for i, k in enumerate(kinds)) + st.free_points * F_FREE
 
 
 def cpu_baseline(model, req, npairs, nself, seed=0):

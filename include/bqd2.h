@@ -111,9 +111,18 @@ typedef struct {
   long long pair_terms[4], pair_evals;
   long long self_terms[4], self_evals;
   /* GH64 pairs of the periodic domain evaluated as the free-space kernel on
-   * GH64 plus the smooth remainder of the lattice sum on GL12 */
+   * GH64 plus the smooth remainder of the lattice sum on a Gauss rule */
   long long pairs_split;
+  /* pairs of the periodic domain whose lattice remainder takes a 4-, 6- or
+   * 8-point Gauss rule (GL12 pairs and split GH64 pairs), and the free-space
+   * kernel points summed by all split pairs */
+  long long pairs_smooth, free_points;
 } bqd2_assembly_stats;
+
+/* The n-point Gauss-Legendre rule on [-1, 1] (nodes ascending), 1 <= n <= 64:
+ * the rules of the smooth lattice remainder.  Host only, no device needed.  No
+ * counterpart in the reference, whose rules are GL12 and GH64 only. */
+int bqd2_gauss_legendre(int n, double *x, double *w);
 
 int bqd2_create(const bqd2_model *m, bqd2_problem **out);
 int bqd2_destroy(bqd2_problem *p);
@@ -123,6 +132,18 @@ int bqd2_destroy(bqd2_problem *p);
  *   G, H [N x N], dG, dH [2N x N] (interior and isolated domains only), dF [N x 3],
  * N = 4 Ne_d, row (t-1)*4+tn, column (s-1)*4+n. st may be NULL. */
 int bqd2_assemble(bqd2_problem *p, int did, bqd2_assembly_stats *st);
+
+/* No counterpart in the reference.  In the periodic domain the lattice sum of a
+ * pair is split into the free-space kernel, integrated on the reference's rule
+ * (GL12, or GH64 for near pairs), and the remainder, which is analytic over the
+ * element up to the other lattice images of the target and is integrated on a
+ * Gauss rule of 4, 6 or 8 points a side chosen per pair from the distance to
+ * those images (error bound below 1e-14 of the remainder); pairs with a point
+ * level with the target keep the reference's evaluation.  on = 0 keeps the
+ * lattice sum on every GL12 point (and GL12 for the remainder of split GH64
+ * pairs); on = 4, 6 or 8 takes that order wherever a remainder rule is allowed
+ * (convergence checks); the rules are decided again at the next assembly. */
+int bqd2_set_remainder_rules(bqd2_problem *p, int on);
 
 /* No counterpart in the reference.  bqd2_assemble of domain did for `count`
  * problems, with the off-diagonal pairs of their periodic domains computed in
