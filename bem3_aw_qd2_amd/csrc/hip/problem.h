@@ -99,6 +99,9 @@ struct FarGeom {
   double ztop, zbot, h;
 };
 enum { FAR_MAX_ORDERS = 8192, FAR_NCH = 8 };
+// the order grid of a periodic model (QPD2 layout qd8): G orders, (2 n1 + 1) x
+// (2 n2 + 1), and the height h; 0 when it would exceed FAR_MAX_ORDERS
+int far_grid(const double *qd8, int *n1, int *n2, double *h);
 
 struct bqd2_problem {
   DevMesh m;

@@ -134,6 +134,22 @@ extern "C" int bqd2_gauss_legendre(int n, double *x, double *w) {
 // beyond decay by exp(-FAR_DECAY) = 1e-20 or more over the height h.  h is a
 // quarter of the shorter lattice vector, which keeps the order grid near 4000
 // orders on lattices of comparable sides.
+int far_grid(const double *qd8, int *n1, int *n2, double *hp) {
+  const double k = qd8[0], vk = hypot(qd8[1], qd8[2]);
+  const double l1 = hypot(qd8[4], qd8[5]), l2 = hypot(qd8[6], qd8[7]);
+  const double FAR_DECAY = 46.0, h = 0.25 * std::min(l1, l2);
+  if (!(h > 0.0)) return 0;
+  const double R = sqrt(k * k + (FAR_DECAY / h) * (FAR_DECAY / h)) + vk;
+  // a_n + vk = n1 g1 + n2 g2 with g_i . d_j = 2 pi delta_ij: |n_i| <= R |d_i| / 2 pi
+  const double N1 = ceil(R * l1 / (2.0 * M_PI)), N2 = ceil(R * l2 / (2.0 * M_PI));
+  const double G = (2.0 * N1 + 1.0) * (2.0 * N2 + 1.0);
+  if (G > FAR_MAX_ORDERS) return 0;
+  if (n1) *n1 = (int)N1;
+  if (n2) *n2 = (int)N2;
+  if (hp) *hp = h;
+  return (int)G;
+}
+
 static void far_setup(bqd2_problem *p, const bqd2_model *m) {
   FarGeom &F = p->far;
   memset(&F, 0, sizeof(F));
@@ -150,14 +166,10 @@ static void far_setup(bqd2_problem *p, const bqd2_model *m) {
       zb = std::min(zb, z);
     }
   }
-  const GfParams &P = p->gf;
-  const double l1 = hypot(P.d1x, P.d1y), l2 = hypot(P.d2x, P.d2y);
-  const double FAR_DECAY = 46.0, h = 0.25 * std::min(l1, l2);
-  const double R = sqrt(P.k * P.k + (FAR_DECAY / h) * (FAR_DECAY / h)) + hypot(P.vkx, P.vky);
-  // a_n + vk = n1 g1 + n2 g2 with g_i . d_j = 2 pi delta_ij: |n_i| <= R |d_i| / 2 pi
-  const int N1 = (int)ceil(R * l1 / (2.0 * M_PI)), N2 = (int)ceil(R * l2 / (2.0 * M_PI));
-  const long long G = (long long)(2 * N1 + 1) * (2 * N2 + 1);
-  if (ne0 <= 0 || !(zt >= zb) || G > FAR_MAX_ORDERS) return;
+  int N1, N2;
+  double h;
+  const int G = far_grid(m->qd, &N1, &N2, &h);
+  if (ne0 <= 0 || !(zt >= zb) || G == 0) return;
   F.n1lo = -N1;
   F.n1n = 2 * N1 + 1;
   F.n2lo = -N2;
