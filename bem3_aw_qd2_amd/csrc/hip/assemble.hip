@@ -392,13 +392,20 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
 }
 
 // ------------------------------------------------------------------ free-space pairs
-__device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D, int s, int row) {
-  const int lane = threadIdx.x;
+// Four consecutive source elements s0 .. s0 + 3 of one row per wavefront, 16
+// lanes per pair (the pair's rule points strided over its lanes, sums over the
+// 16 lanes); a GH64 pair keeps its group's lanes longer than a GL12 one.
+__device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
+                                              int s0, int row) {
+  const int lane = threadIdx.x, sl = lane % LPP;
   int t, tn, td;
-  if (!row_target(M, D, row, t, tn, td) || s == t) return;
+  if (!row_target(M, D, row, t, tn, td)) return;
+  const int s = s0 + lane / LPP;
+  const bool on = s <= D.Ne && s != t;
+  const int sv = s <= D.Ne ? s : s0;  // a valid position for the idle groups' loads
   const Tab &T = *tab;
-  const int at = abs(td), sd = D.sid[s], as = abs(sd);
-  const bool gh = D.rule[(long long)row * D.Ne + (s - 1)] != 0;
+  const int at = abs(td), sd = D.sid[sv], as = abs(sd);
+  const bool gh = D.rule[(long long)row * D.Ne + (sv - 1)] != 0;
   const int n = gh ? 64 : 12;
   const double *x = gh ? T.xhi : T.xli;
   const double *w = gh ? T.whi : T.wli;
@@ -412,8 +419,8 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
 
   cd g[4] = {}, h[4] = {}, dg0[4] = {}, dg1[4] = {}, dh0[4] = {}, dh1[4] = {};
   double F = 0.0, F0 = 0.0, F1 = 0.0;
-  const int npts = rule_npts(tri, n);
-  for (int p = lane; p < npts; p += WAVE) {
+  const int npts = on ? rule_npts(tri, n) : 0;
+  for (int p = sl; p < npts; p += LPP) {
     double z, e, wt;
     rule_point(tri, n, x, w, p, z, e, wt);
     PtGeo q;
@@ -448,17 +455,17 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
   const double ifp = BQD2_IFP * (tri ? 1.0 / 24.0 : 1.0);
 #pragma unroll
   for (int ep = 0; ep < 4; ep++) {
-    g[ep] = cd{wave_sum(g[ep].x), wave_sum(g[ep].y)} * ifp;
-    h[ep] = cd{wave_sum(h[ep].x), wave_sum(h[ep].y)} * (ifp * sg);
-    dg0[ep] = cd{wave_sum(dg0[ep].x), wave_sum(dg0[ep].y)} * (-ifp);
-    dg1[ep] = cd{wave_sum(dg1[ep].x), wave_sum(dg1[ep].y)} * (-ifp);
-    dh0[ep] = cd{wave_sum(dh0[ep].x), wave_sum(dh0[ep].y)} * (ifp * sg);
-    dh1[ep] = cd{wave_sum(dh1[ep].x), wave_sum(dh1[ep].y)} * (ifp * sg);
+    g[ep] = cd{group_sum(g[ep].x), group_sum(g[ep].y)} * ifp;
+    h[ep] = cd{group_sum(h[ep].x), group_sum(h[ep].y)} * (ifp * sg);
+    dg0[ep] = cd{group_sum(dg0[ep].x), group_sum(dg0[ep].y)} * (-ifp);
+    dg1[ep] = cd{group_sum(dg1[ep].x), group_sum(dg1[ep].y)} * (-ifp);
+    dh0[ep] = cd{group_sum(dh0[ep].x), group_sum(dh0[ep].y)} * (ifp * sg);
+    dh1[ep] = cd{group_sum(dh1[ep].x), group_sum(dh1[ep].y)} * (ifp * sg);
   }
-  F = wave_sum(F) * ifp * sg;
-  F0 = wave_sum(F0) * ifp * sg;
-  F1 = wave_sum(F1) * ifp * sg;
-  if (lane == 0) {
+  F = group_sum(F) * ifp * sg;
+  F0 = group_sum(F0) * ifp * sg;
+  F1 = group_sum(F1) * ifp * sg;
+  if (on && sl == 0) {
     const size_t col = (size_t)(s - 1) * 4;
     const size_t base = dom_gh(D, row, col) * 2;
     const size_t b0 = dom_dgh(D, row, 0, col) * 2, b1 = dom_dgh(D, row, 1, col) * 2;
@@ -481,8 +488,11 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
   }
 }
 
+// element groups of four per row
+__host__ __device__ __forceinline__ int fs_groups(int ne) { return (ne + PPW - 1) / PPW; }
+
 __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  fs_pairs_body(M, tab, D, (int)blockIdx.x + 1, D.r0 + (int)blockIdx.y);
+  fs_pairs_body(M, tab, D, PPW * (int)blockIdx.x + 1, D.r0 + (int)blockIdx.y);
 }
 
 // ------------------------------------------------------------------ self elements
@@ -1127,11 +1137,12 @@ __device__ __forceinline__ void batch_task(const AsmBatch *__restrict__ inst, lo
   const Tab *tab = inst[i].tab;
   long long *counters = inst[i].counters;
   // row and element indices uniform too, as blockIdx is in the per-domain kernels
-  const int row = __builtin_amdgcn_readfirstlane((int)(KIND == BK_FS_PAIRS ? u / D.Ne : u));
+  const int row = __builtin_amdgcn_readfirstlane((int)(KIND == BK_FS_PAIRS ? u / fs_groups(D.Ne) : u));
   if (KIND == BK_Q0_PAIRS) q0_pairs_wave(M, tab, D, P, D.plist, (long long)D.plist_n, counters, u);
   if (KIND == BK_Q0_SELF) q0_self_body(M, tab, D, P, counters, row);
   if (KIND == BK_Q0_SELF_TRI) q0_self_tri_body(M, tab, D, P, counters, row);
-  if (KIND == BK_FS_PAIRS) fs_pairs_body(M, tab, D, __builtin_amdgcn_readfirstlane((int)(u % D.Ne)) + 1, row);
+  if (KIND == BK_FS_PAIRS)
+    fs_pairs_body(M, tab, D, PPW * __builtin_amdgcn_readfirstlane((int)(u % fs_groups(D.Ne))) + 1, row);
   if (KIND == BK_FS_SELF) fs_self_body(M, tab, D, row);
   if (KIND == BK_FS_SELF_TRI) fs_self_tri_body(M, tab, D, row);
 }
@@ -1258,7 +1269,7 @@ int bqd2_fs_rows(bqd2_problem *p, const DevDomain &D, hipStream_t st) {
   if (D.nr <= 0) return BQD2_OK;
   int rc = bqd2_dom_clear(D, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
+  hipLaunchKernelGGL(k_fs_pairs, dim3(fs_groups(D.Ne), D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
   hipLaunchKernelGGL(k_fs_self, dim3(D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
   if (D.has_tri) hipLaunchKernelGGL(k_fs_self_tri, dim3(D.nr), dim3(WAVE), 0, st, p->m, p->tab, D);
   hipLaunchKernelGGL(k_rows, dim3(bqd2_blocks(D.nr, 256)), dim3(256), 0, st, p->m, D);
@@ -1463,7 +1474,7 @@ int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
       }
     }
   } else {
-    hipLaunchKernelGGL(k_fs_pairs, dim3(D.Ne, D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
+    hipLaunchKernelGGL(k_fs_pairs, dim3(fs_groups(D.Ne), D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
   }
   BQD2_HIP(hipGetLastError());
   BQD2_HIP(hipEventRecord(ev.e2, st_));
@@ -1529,7 +1540,7 @@ extern "C" int bqd2_assemble_batch(bqd2_problem **ps, int count, int did, bqd2_a
       tasks[BK_Q0_SELF] += rows;
       tasks[BK_Q0_SELF_TRI] += rows_tri;
     } else {
-      tasks[BK_FS_PAIRS] += (long long)D.Ne * D.N;
+      tasks[BK_FS_PAIRS] += (long long)fs_groups(D.Ne) * D.N;
       tasks[BK_FS_SELF] += rows;
       tasks[BK_FS_SELF_TRI] += rows_tri;
     }
