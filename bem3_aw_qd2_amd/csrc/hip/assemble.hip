@@ -5,7 +5,7 @@
 // Green's function once per quadrature point and shape function.  Here:
 //   k_decide      one lane per (row, s): the GL12 dF/dv test that picks GL12 or
 //                 GH64 (d3qd2_elem.c:536-537, :851-852), bit-exact;
-//   k_q0_pairs    four (row, s) pairs of the periodic domain per wavefront,
+//   k_q0_pairs    eight (row, s) pairs of the periodic domain per wavefront,
 //                 16 lanes per pair, pairs grouped by rule on the host: the
 //                 chosen Gauss-Legendre rule spread over the pair's lanes, one
 //                 Green's function evaluation per point shared by the four
@@ -187,18 +187,26 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
 }
 
 // ------------------------------------------------------------------ periodic pairs
-// Four (row, s) pairs per wavefront, 16 lanes per pair.  The host sorts the
+// Eight (row, s) pairs per wavefront, 8 lanes per pair.  The host sorts the
 // pairs by rule and element kind, so the pairs of one wavefront share their
-// loop bounds; a GL12 rule (144 points) then fills every lane, where one pair
-// per wavefront leaves a quarter of the lanes idle in its last pass.
-#define PPW 4
-#define LPP 16
+// loop bounds; a pair's rule points are strided over its lanes (two lattice
+// evaluations per lane for a 4-point remainder rule, 18 free-space points for
+// GL12).  Measured against 4 pairs of 16 lanes and 16 pairs of 4 lanes on
+// sphere3456: 1027 ms against 1042 and 1033.
+#ifndef PPW
+#define PPW 8
+#endif
+#define LPP (WAVE / PPW)
+// the free-space pair kernel's layout: four elements of a row, 16 lanes each
+#define FS_PPW 4
+#define FS_LPP 16
 // single-wave workgroups per launch at most: the dispatch packet holds the grid
 // size in work-items as a 32-bit number (2^25 x 64 = 2^31)
 #define BQD2_MAX_WG (1LL << 25)
+template <int L = LPP>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
-  for (int off = LPP / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = L / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
@@ -397,10 +405,10 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
 // 16 lanes); a GH64 pair keeps its group's lanes longer than a GL12 one.
 __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
                                               int s0, int row) {
-  const int lane = threadIdx.x, sl = lane % LPP;
+  const int lane = threadIdx.x, sl = lane % FS_LPP;
   int t, tn, td;
   if (!row_target(M, D, row, t, tn, td)) return;
-  const int s = s0 + lane / LPP;
+  const int s = s0 + lane / FS_LPP;
   const bool on = s <= D.Ne && s != t;
   const int sv = s <= D.Ne ? s : s0;  // a valid position for the idle groups' loads
   const Tab &T = *tab;
@@ -420,7 +428,7 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
   cd g[4] = {}, h[4] = {}, dg0[4] = {}, dg1[4] = {}, dh0[4] = {}, dh1[4] = {};
   double F = 0.0, F0 = 0.0, F1 = 0.0;
   const int npts = on ? rule_npts(tri, n) : 0;
-  for (int p = sl; p < npts; p += LPP) {
+  for (int p = sl; p < npts; p += FS_LPP) {
     double z, e, wt;
     rule_point(tri, n, x, w, p, z, e, wt);
     PtGeo q;
@@ -455,16 +463,16 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
   const double ifp = BQD2_IFP * (tri ? 1.0 / 24.0 : 1.0);
 #pragma unroll
   for (int ep = 0; ep < 4; ep++) {
-    g[ep] = cd{group_sum(g[ep].x), group_sum(g[ep].y)} * ifp;
-    h[ep] = cd{group_sum(h[ep].x), group_sum(h[ep].y)} * (ifp * sg);
-    dg0[ep] = cd{group_sum(dg0[ep].x), group_sum(dg0[ep].y)} * (-ifp);
-    dg1[ep] = cd{group_sum(dg1[ep].x), group_sum(dg1[ep].y)} * (-ifp);
-    dh0[ep] = cd{group_sum(dh0[ep].x), group_sum(dh0[ep].y)} * (ifp * sg);
-    dh1[ep] = cd{group_sum(dh1[ep].x), group_sum(dh1[ep].y)} * (ifp * sg);
+    g[ep] = cd{group_sum<FS_LPP>(g[ep].x), group_sum<FS_LPP>(g[ep].y)} * ifp;
+    h[ep] = cd{group_sum<FS_LPP>(h[ep].x), group_sum<FS_LPP>(h[ep].y)} * (ifp * sg);
+    dg0[ep] = cd{group_sum<FS_LPP>(dg0[ep].x), group_sum<FS_LPP>(dg0[ep].y)} * (-ifp);
+    dg1[ep] = cd{group_sum<FS_LPP>(dg1[ep].x), group_sum<FS_LPP>(dg1[ep].y)} * (-ifp);
+    dh0[ep] = cd{group_sum<FS_LPP>(dh0[ep].x), group_sum<FS_LPP>(dh0[ep].y)} * (ifp * sg);
+    dh1[ep] = cd{group_sum<FS_LPP>(dh1[ep].x), group_sum<FS_LPP>(dh1[ep].y)} * (ifp * sg);
   }
-  F = group_sum(F) * ifp * sg;
-  F0 = group_sum(F0) * ifp * sg;
-  F1 = group_sum(F1) * ifp * sg;
+  F = group_sum<FS_LPP>(F) * ifp * sg;
+  F0 = group_sum<FS_LPP>(F0) * ifp * sg;
+  F1 = group_sum<FS_LPP>(F1) * ifp * sg;
   if (on && sl == 0) {
     const size_t col = (size_t)(s - 1) * 4;
     const size_t base = dom_gh(D, row, col) * 2;
@@ -489,10 +497,10 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
 }
 
 // element groups of four per row
-__host__ __device__ __forceinline__ int fs_groups(int ne) { return (ne + PPW - 1) / PPW; }
+__host__ __device__ __forceinline__ int fs_groups(int ne) { return (ne + FS_PPW - 1) / FS_PPW; }
 
 __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
-  fs_pairs_body(M, tab, D, PPW * (int)blockIdx.x + 1, D.r0 + (int)blockIdx.y);
+  fs_pairs_body(M, tab, D, FS_PPW * (int)blockIdx.x + 1, D.r0 + (int)blockIdx.y);
 }
 
 // ------------------------------------------------------------------ self elements
@@ -1142,7 +1150,7 @@ __device__ __forceinline__ void batch_task(const AsmBatch *__restrict__ inst, lo
   if (KIND == BK_Q0_SELF) q0_self_body(M, tab, D, P, counters, row);
   if (KIND == BK_Q0_SELF_TRI) q0_self_tri_body(M, tab, D, P, counters, row);
   if (KIND == BK_FS_PAIRS)
-    fs_pairs_body(M, tab, D, PPW * __builtin_amdgcn_readfirstlane((int)(u % fs_groups(D.Ne))) + 1, row);
+    fs_pairs_body(M, tab, D, FS_PPW * __builtin_amdgcn_readfirstlane((int)(u % fs_groups(D.Ne))) + 1, row);
   if (KIND == BK_FS_SELF) fs_self_body(M, tab, D, row);
   if (KIND == BK_FS_SELF_TRI) fs_self_tri_body(M, tab, D, row);
 }
@@ -1324,6 +1332,8 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
     std::vector<int> plist;
     if (periodic) {
       pair_list(D, rule, key, p->tripos[did], plist);
+      BQD2_CHECK(plist.size() <= (size_t)D.N * D.Ne + PLIST_PAD, BQD2_E_STATE,
+                 "bqd2_assemble: pair list of %zu entries exceeds its buffer", plist.size());
       if (!plist.empty()) {
         BQD2_HIP(hipMemcpyAsync(D.plist, plist.data(), plist.size() * sizeof(int), hipMemcpyHostToDevice, st_));
         BQD2_HIP(hipStreamSynchronize(st_));  // the host list is released on return

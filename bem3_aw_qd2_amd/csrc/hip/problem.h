@@ -46,7 +46,7 @@ struct DevDomain {
   double *dF;          // N x 3
   double *Fpair;       // nr x Ne x 3 : per pair F, dF/dtz, dF/dte
   unsigned char *rule; // N x Ne : 1 = GH64 rule
-  int *plist;          // N x Ne + 32 : (row, s) pairs sorted by rule and element kind
+  int *plist;          // N x Ne + PLIST_PAD : (row, s) pairs sorted by rule and element kind
   float *pkey;         // N x Ne : |z| offset of the element centre from the target (pair order)
   int plist_n;         // entries of plist, 0 until the first assembly (rules and list are cached)
   long long n_gh, n_split;  // GH64 pairs, split ones among them
@@ -99,6 +99,9 @@ struct FarGeom {
   double ztop, zbot, h;
 };
 enum { FAR_MAX_ORDERS = 8192, FAR_NCH = 8 };
+// entries of the pair list beyond N x Ne: each of the 18 classes (9 rules, two
+// element kinds) is padded to a multiple of the pairs per wavefront (up to 16)
+enum { PLIST_PAD = 512 };
 // the order grid of a periodic model (QPD2 layout qd8): G orders, (2 n1 + 1) x
 // (2 n2 + 1), and the height h; 0 when it would exceed FAR_MAX_ORDERS
 int far_grid(const double *qd8, int *n1, int *n2, double *h);

@@ -316,7 +316,7 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
     }
     TRY(bqd2_alloc(p, (void **)&D.dF, N * 3 * 8));
     TRY(bqd2_alloc(p, (void **)&D.rule, N * D.Ne));
-    if (!large || periodic0) TRY(bqd2_alloc(p, (void **)&D.plist, (N * D.Ne + 32) * sizeof(int)));
+    if (!large || periodic0) TRY(bqd2_alloc(p, (void **)&D.plist, (N * D.Ne + PLIST_PAD) * sizeof(int)));
     TRY(bqd2_alloc(p, (void **)&D.P, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.dP, ((size_t)D.Ne + 1) * 4 * 16));
     TRY(bqd2_alloc(p, (void **)&D.pv, ((size_t)D.Ne + 1) * 12 * 16));

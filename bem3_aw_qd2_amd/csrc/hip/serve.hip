@@ -151,7 +151,7 @@ extern "C" size_t bqd2_footprint(const bqd2_model *m, int solver, int npts, int 
     nmax = std::max(nmax, N);
     b += 2 * sq(N) * 16;                                   // G, H
     if (!(d == 0 && m->ps == 1)) b += 4 * sq(N) * 16;      // dG, dH
-    b += N * 3 * 8 + N * ne * (24 + 1 + 4 + 4) + 128;      // dF, Fpair, rule, pkey, plist
+    b += N * 3 * 8 + N * ne * (24 + 1 + 4 + 4) + 4 * PLIST_PAD + 128;  // dF, Fpair, rule, pkey, plist
     b += (ne + 1) * (2 * 64 + 2 * 192) + (ne + 1) * 4 + N * 4 + ne * 4;  // ..., sid, rowmap, fperm
   }
   const size_t na = 2 * (size_t)m->NN;

@@ -1046,7 +1046,12 @@ def main():
             "terms_per_launch": {"spatial": int(st0.pair_terms[0]), "propagating": int(st0.pair_terms[1]),
                                  "evanescent": int(st0.pair_terms[2]), "pure_spectral": int(st0.pair_terms[3])},
             "evals_per_launch": int(st0.pair_evals), "split_pairs": int(st0.pairs_split),
-            "gh_pairs": int(st0.pairs_gh), "flops_per_term": F_TERM, "flops_per_free_point": F_FREE}
+            "gh_pairs": int(st0.pairs_gh), "flops_per_term": F_TERM, "flops_per_free_point": F_FREE,
+            "remainder_rule_pairs": int(st0.pairs_smooth), "free_points_per_launch": int(st0.free_points),
+            # the lattice evaluations the reference's rules take for the same pairs
+            # (the whole sum on every GL12 / GH64 point, three parts per triangle)
+            "evals_per_launch_reference_rules": int((st0.pairs_gl * 144 + st0.pairs_gh * 4096) *
+                                                    (3 if args.triangles else 1))}
 
     # the field phase: Hessian evaluations of the field points (k_field_part, the
     # phase's dominant kernel; the phase time bounds its duration from above)
