@@ -97,7 +97,7 @@ __device__ __forceinline__ void rule_tab(const Tab &T, int n, const double *&x, 
 // none does.  The estimate is fitted to the measured convergence of the three
 // rules to GL12 on the 96- and 384-quad spheres of the 2 mm lattice (q 0.41 and
 // 0.21: 5e-10 and 5e-12 with 4 points, 1e-15 with 6 and 8).
-__device__ __forceinline__ int smooth_order(const ElemC &E, const double *rt, const GfParams &P) {
+__device__ __forceinline__ int smooth_order(const ElemC &E, const double *rt, const GfParams &P, bool tri) {
   double c[3], rad = 0.0;
   bil_rw_r(E, 0.0, 0.0, c);
 #pragma unroll
@@ -121,19 +121,26 @@ __device__ __forceinline__ int smooth_order(const ElemC &E, const double *rt, co
   }
   if (!(dmin > 2.0 * rad) || rad <= 0.0) return 0;
   const double q = 0.7071067811865476 * rad * fmax(fmax(P.k, 1.0 / P.E0), 1.0 / (dmin - rad));
-  if (q <= 0.088) return 4;
-  if (q <= 0.36) return 6;
-  if (q <= 0.73) return 8;
+  // the triangle's collapsed rule is exact to one degree less than the
+  // square's: its thresholds are those of the next order down
+  if (q <= (tri ? 0.05 : 0.088)) return 4;
+  if (q <= (tri ? 0.3 : 0.36)) return 6;
+  if (q <= (tri ? 0.6 : 0.73)) return 8;
   return 0;
 }
 
 // a point of rule n on the element with a z offset level with the target
-__device__ __forceinline__ bool any_degenerate(const ElemC &E, const Tab &T, const double *rt, bool tri, int n) {
+__device__ __forceinline__ bool any_degenerate(const ElemC &E, const Tab &T, const double *rt, bool tri, int n,
+                                               bool rem = false) {
   const double *x, *w;
   rule_tab(T, n, x, w);
   double z, e, wt;
-  for (int p = 0; p < rule_npts(tri, n); p++) {
-    rule_point(tri, n, x, w, p, z, e, wt);
+  const int np = rem ? rem_npts(n) : rule_npts(tri, n);
+  for (int p = 0; p < np; p++) {
+    if (rem)
+      rem_point(tri, n, x, w, p, z, e, wt);
+    else
+      rule_point(tri, n, x, w, p, z, e, wt);
     if (offset_degenerate(E, z, e, rt)) return true;
   }
   return false;
@@ -162,9 +169,9 @@ __global__ void __launch_bounds__(256) k_decide(DevMesh M, const Tab *__restrict
     rule = (tri ? needs_gh_tri(E, T, rt, vtz, vte) : needs_gh(E, T, rt, vtz, vte)) ? 1 : 0;
     if (rule && periodic && split_ok(E, T, rt, P, tri)) rule = 2;
     if (periodic && smooth && rule != 1) {
-      int n = smooth_order(E, rt, P);
+      int n = smooth_order(E, rt, P, tri);
       if (n && smooth > 1) n = smooth;
-      if (n && !any_degenerate(E, T, rt, tri, n) && (rule == 2 || !any_degenerate(E, T, rt, tri, 12)))
+      if (n && !any_degenerate(E, T, rt, tri, n, true) && (rule == 2 || !any_degenerate(E, T, rt, tri, 12)))
         rule = (unsigned char)((rule == 2 ? 3 : 6) + (n - 4) / 2);
     }
     key = (float)fabs(E.cr[2][0] - rt[2]);
@@ -307,7 +314,9 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
     const int n = rule_lat_n(rule);
     const double *x, *w;
     rule_tab(T, n, x, w);
-    const int npts = on ? rule_npts(tri, n) : 0;
+    // rules 3-8: the remainder on the whole-element rule; 0-2 the reference's
+    const bool rem = rule >= 3;
+    const int npts = on ? (rem ? rem_npts(n) : rule_npts(tri, n)) : 0;
     nev = (sl == 0) ? npts : 0;
     for (int p = sl; p < npts; p += LPP) {
       asm volatile("" ::: "memory");
@@ -321,7 +330,10 @@ __device__ __forceinline__ void q0_pairs_wave(const DevMesh &M, const Tab *__res
         for (int j = 0; j < 3; j++) Ep.cw[d][j] = G[12 + 3 * d + j];
       }
       double z, e, wt;
-      rule_point(tri, n, x, w, p, z, e, wt);
+      if (rem)
+        rem_point(tri, n, x, w, p, z, e, wt);
+      else
+        rule_point(tri, n, x, w, p, z, e, wt);
       PtGeo q;
       point_geo(Ep, z, e, G + 21, G + 24, G + 27, q);
 #pragma unroll

@@ -289,6 +289,28 @@ __device__ __forceinline__ void rule_point(bool tri, int n, const double *x, con
   }
 }
 
+// points of the n x n Gauss rule for a smooth integrand over the whole element:
+// the square's tensor rule, or on the triangle the collapsed (Duffy) rule,
+// (z, e) = V0 + s (V1 - V0) + s t (V2 - V1) with s = (1 + u)/2, t = (1 + v)/2
+// over the reference corners V0 = (-1/2, -sqrt3/2), V1 = (1, 0), V2 = (-1/2,
+// sqrt3/2), weight 3 (1 + u) w_u w_v: the same scale as the three parts' rule
+// (its weights also sum to 12)
+__device__ __forceinline__ int rem_npts(int n) { return n * n; }
+__device__ __forceinline__ void rem_point(bool tri, int n, const double *x, const double *w, int p, double &z,
+                                          double &e, double &wt) {
+  const int i = p / n, j = p - i * n;
+  if (!tri) {
+    z = x[i];
+    e = x[j];
+    wt = w[i] * w[j];
+  } else {
+    const double s = 0.5 * (1.0 + x[i]), st = s * 0.5 * (1.0 + x[j]);
+    z = -0.5 + 1.5 * s - 1.5 * st;
+    e = 0.5 * BQD2_SQ3 * (-1.0 + s + st);
+    wt = 3.0 * (1.0 + x[i]) * w[i] * w[j];
+  }
+}
+
 // GL12 dF/dv0, dF/dv1 of a triangle pair with the reference's operation order
 // (q0_lit_dcoef_v2_cc.c:196-229, lit_dcoef_v2.c:218-262): three parts summed per
 // j, times the part weight and w_j, summed over j, times w_i, then the constant.
