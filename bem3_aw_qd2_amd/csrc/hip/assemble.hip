@@ -415,14 +415,15 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
 // Four consecutive source elements s0 .. s0 + 3 of one row per wavefront, 16
 // lanes per pair (the pair's rule points strided over its lanes, sums over the
 // 16 lanes); a GH64 pair keeps its group's lanes longer than a GL12 one.
-__device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
-                                              int s0, int row) {
+// One pair (row, s) per group of FS_LPP lanes (the pair's rule points strided
+// over its lanes, sums over them); a group with on = false adds nothing.  The
+// arithmetic of a pair does not depend on the other groups of its wavefront.
+__device__ __forceinline__ void fs_pair_group(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
+                                              int row, int s, bool on) {
   const int lane = threadIdx.x, sl = lane % FS_LPP;
   int t, tn, td;
-  if (!row_target(M, D, row, t, tn, td)) return;
-  const int s = s0 + lane / FS_LPP;
-  const bool on = s <= D.Ne && s != t;
-  const int sv = s <= D.Ne ? s : s0;  // a valid position for the idle groups' loads
+  on = row_target(M, D, row, t, tn, td) && on && s != t;
+  const int sv = (s >= 1 && s <= D.Ne) ? s : D.Ne;  // a valid position for an idle group's loads
   const Tab &T = *tab;
   const int at = abs(td), sd = D.sid[sv], as = abs(sd);
   const bool gh = D.rule[(long long)row * D.Ne + (sv - 1)] != 0;
@@ -511,8 +512,33 @@ __device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__res
 // element groups of four per row
 __host__ __device__ __forceinline__ int fs_groups(int ne) { return (ne + FS_PPW - 1) / FS_PPW; }
 
+// the elements s0 .. s0 + 3 of one row
+__device__ __forceinline__ void fs_pairs_body(const DevMesh &M, const Tab *__restrict__ tab, const DevDomain &D,
+                                              int s0, int row) {
+  int t, tn, td;
+  if (!row_target(M, D, row, t, tn, td)) return;
+  const int s = s0 + (int)threadIdx.x / FS_LPP;
+  fs_pair_group(M, tab, D, row, s, s <= D.Ne);
+}
+
 __global__ void __launch_bounds__(WAVE) k_fs_pairs(DevMesh M, const Tab *__restrict__ tab, DevDomain D) {
   fs_pairs_body(M, tab, D, FS_PPW * (int)blockIdx.x + 1, D.r0 + (int)blockIdx.y);
+}
+
+// the pairs FS_PPW wg .. FS_PPW wg + FS_PPW - 1 of the domain's rule-sorted
+// list (padding entries -1 add nothing), so that the GH64 pairs share their
+// wavefronts with each other
+__global__ void __launch_bounds__(WAVE) k_fs_pairs_list(DevMesh M, const Tab *__restrict__ tab, DevDomain D,
+                                                        long long nlist, long long wg0) {
+  const long long wg = wg0 + blockIdx.x;
+  // classes are padded to multiples of PPW: a group of FS_PPW entries that
+  // starts with padding holds padding only
+  const int first = D.plist[wg * FS_PPW];
+  if (first < 0) return;
+  const long long li = wg * FS_PPW + threadIdx.x / FS_LPP;
+  const int pair = li < nlist ? D.plist[li] : -1;
+  const int gid = pair >= 0 ? pair : first;
+  fs_pair_group(M, tab, D, gid / D.Ne, gid % D.Ne + 1, pair >= 0);
 }
 
 // ------------------------------------------------------------------ self elements
@@ -1324,8 +1350,11 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
   if (first) {
     std::vector<unsigned char> rule((size_t)npairs);
     BQD2_HIP(hipMemcpyAsync(rule.data(), D.rule, (size_t)npairs, hipMemcpyDeviceToHost, st_));
+    // the pair list: both domains in the default layout (the periodic domain
+    // in the system layout, whose interior domain goes by row windows)
+    const bool listed = D.plist != nullptr && D.pkey != nullptr;
     std::vector<float> key;
-    if (periodic) {
+    if (listed) {
       key.resize((size_t)npairs);
       BQD2_HIP(hipMemcpyAsync(key.data(), D.pkey, (size_t)npairs * sizeof(float), hipMemcpyDeviceToHost, st_));
     }
@@ -1342,7 +1371,7 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
       }
     }
     std::vector<int> plist;
-    if (periodic) {
+    if (listed) {
       pair_list(D, rule, key, p->tripos[did], plist);
       BQD2_CHECK(plist.size() <= (size_t)D.N * D.Ne + PLIST_PAD, BQD2_E_STATE,
                  "bqd2_assemble: pair list of %zu entries exceeds its buffer", plist.size());
@@ -1351,7 +1380,7 @@ static int asm_begin(bqd2_problem *p, int did, AsmPhase &ev) {
         BQD2_HIP(hipStreamSynchronize(st_));  // the host list is released on return
       }
     }
-    D.plist_n = periodic ? (int)plist.size() : 1;
+    D.plist_n = listed ? (int)plist.size() : 1;
   }
   BQD2_HIP(hipEventRecord(ev.e1b, st_));
   return BQD2_OK;
@@ -1494,6 +1523,14 @@ int bqd2_assemble_full(bqd2_problem *p, int did, bqd2_assembly_stats *st) {
         BQD2_HIP(hipEventRecord(e, st_));
         parts.push_back(e);
       }
+    }
+  } else if (D.plist_n > 1) {
+    // the rule-sorted list, in launches of at most BQD2_MAX_WG wavefronts
+    const long long nwg = (D.plist_n + FS_PPW - 1) / FS_PPW;
+    for (long long a = 0; a < nwg; a += BQD2_MAX_WG) {
+      const long long b = std::min(nwg, a + BQD2_MAX_WG);
+      hipLaunchKernelGGL(k_fs_pairs_list, dim3((unsigned)(b - a)), dim3(WAVE), 0, st_, p->m, p->tab, D,
+                         (long long)D.plist_n, a);
     }
   } else {
     hipLaunchKernelGGL(k_fs_pairs, dim3(fs_groups(D.Ne), D.N), dim3(WAVE), 0, st_, p->m, p->tab, D);
