@@ -623,32 +623,29 @@ __global__ void __launch_bounds__(256) k_far_fin(GfParams P, FarGeom F, const do
   }
 }
 
-// far points: the sums over the orders, added to the point's first chunk of
-// k_field_part's partial sums (which hold F and grad F only for these points)
+// far points: the sums over the orders, one wavefront per point with the
+// orders strided over its lanes and summed across them in a fixed order, added
+// to the point's first chunk of k_field_part's partial sums (which hold F and
+// grad F only for these points)
 __global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom F, int n, int nchunk,
                                                  const double *__restrict__ r0s, const int *__restrict__ did_io,
                                                  const double *__restrict__ amp, double *__restrict__ part,
                                                  long long *__restrict__ counters) {
-  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  int side = -1;
-  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-  if (ip < n) {
-    x0 = r0s[3 * ip];
-    x1 = r0s[3 * ip + 1];
-    x2 = r0s[3 * ip + 2];
-    const int did = did_io[ip];
-    if (did >= 0) side = far_side(X, did, x2);
-  }
-  const unsigned long long nfar = __popcll(__ballot(side >= 0));
-  if (threadIdx.x == 0 && nfar) {
-    atomicAdd((unsigned long long *)&counters[CNT_FAR], nfar);
-    atomicAdd((unsigned long long *)&counters[CNT_FAR + 1], nfar * (unsigned long long)F.G);
-  }
+  const int ip = blockIdx.x, lane = threadIdx.x;
+  if (ip >= n) return;
+  const int did = did_io[ip];
+  if (did < 0) return;
+  const double x0 = r0s[3 * ip], x1 = r0s[3 * ip + 1], x2 = r0s[3 * ip + 2];
+  const int side = far_side(X, did, x2);
   if (side < 0) return;
+  if (lane == 0) {
+    atomicAdd((unsigned long long *)&counters[CNT_FAR], 1ull);
+    atomicAdd((unsigned long long *)&counters[CNT_FAR + 1], (unsigned long long)F.G);
+  }
   const double d = side == 0 ? x2 - F.ztop : F.zbot - x2;
   const double *am = amp + (size_t)side * F.G * 8;
   cd Pa = {}, dPa[3] = {}, PVa[3] = {};
-  for (int j = 0; j < F.G; j++) {
+  for (int j = lane; j < F.G; j += 64) {
     double ax, ay;
     cd bn;
     far_order(P, F, j, ax, ay, bn);
@@ -662,6 +659,12 @@ __global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom
     dPa[2] += mul_i(XA * (side == 0 ? bn : -bn));
     for (int c = 0; c < 3; c++) PVa[c] += Xn * cd{a[2 + 2 * c], a[3 + 2 * c]};
   }
+  Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
+  for (int c = 0; c < 3; c++) {
+    dPa[c] = cd{wave_sum(dPa[c].x), wave_sum(dPa[c].y)};
+    PVa[c] = cd{wave_sum(PVa[c].x), wave_sum(PVa[c].y)};
+  }
+  if (lane != 0) return;
   double *dst = part + (size_t)ip * nchunk * FP_N;
   dst[0] += Pa.x;
   dst[1] += Pa.y;
@@ -1327,8 +1330,8 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
   if (any_far)
-    hipLaunchKernelGGL(k_far_eval, dim3(bqd2_blocks(n, 64)), dim3(64), 0, st, X, p->gf, p->far, n, nchunk, d_r0, d_did,
-                       amp, d_part, ctr);
+    hipLaunchKernelGGL(k_far_eval, dim3(n), dim3(64), 0, st, X, p->gf, p->far, n, nchunk, d_r0, d_did, amp, d_part,
+                       ctr);
   hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, X, total, n, nchunk, d_part, d_l, d_pts,
                      d_did, d_out);
   BQD2_HIP(hipGetLastError());
@@ -1418,8 +1421,8 @@ extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, i
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
   if (X.far_on)
-    hipLaunchKernelGGL(k_far_eval, dim3(bqd2_blocks(n, 64)), dim3(64), 0, stream, X, p->gf, p->far, n, nchunk, d_r0,
-                       d_did, d_famp, d_part, ctr);
+    hipLaunchKernelGGL(k_far_eval, dim3(n), dim3(64), 0, stream, X, p->gf, p->far, n, nchunk, d_r0, d_did, d_famp,
+                       d_part, ctr);
   hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, stream, X, total, n, nchunk, d_part, d_l,
                      d_pts, d_did, d_out);
   if (d_errors) hipLaunchKernelGGL(k_copy_errors, dim3(1), dim3(64), 0, stream, ctr, d_errors);
