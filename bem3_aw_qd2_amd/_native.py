@@ -107,6 +107,8 @@ def hip_lib():
                                    C.c_void_p]
         lib.bqd2_set_remainder_rules.argtypes = [C.c_void_p, C.c_int]
         lib.bqd2_gauss_legendre.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        lib.bqd2_set_field_timing.argtypes = [C.c_void_p, C.c_int]
+        lib.bqd2_field_kernel_ms.argtypes = [C.c_void_p, C.c_void_p]
         lib.bqd2_set_field_far.argtypes = [C.c_void_p, C.c_int]
         lib.bqd2_field_far_info.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         lib.bqd2_field_bd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,

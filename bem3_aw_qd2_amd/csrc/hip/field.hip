@@ -1251,6 +1251,18 @@ extern "C" int bqd2_set_field_far(bqd2_problem *p, int on) {
   return BQD2_OK;
 }
 
+extern "C" int bqd2_set_field_timing(bqd2_problem *p, int on) {
+  BQD2_CHECK(p && (on == 0 || on == 1), BQD2_E_ARG, "bqd2_set_field_timing: bad arguments");
+  p->field_timing = on;
+  return BQD2_OK;
+}
+
+extern "C" int bqd2_field_kernel_ms(bqd2_problem *p, double *ms) {
+  BQD2_CHECK(p && ms, BQD2_E_ARG, "bqd2_field_kernel_ms: bad arguments");
+  *ms = p->field_ms_part;
+  return BQD2_OK;
+}
+
 extern "C" int bqd2_field_far_info(bqd2_problem *p, int *on, int *orders, double *zbot, double *ztop, double *h) {
   BQD2_CHECK(p, BQD2_E_ARG, "bqd2_field_far_info: null problem");
   if (on) *on = (p->m.ps == 1 && p->far.on && p->far_enabled) ? 1 : 0;
@@ -1327,8 +1339,15 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   if (any_far && (rc = far_amplitudes(p, cx, X, type, d_fpart, ctr, &amp))) return rc;
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (p->field_timing) {
+    BQD2_HIP(hipEventCreate(&e0));
+    BQD2_HIP(hipEventCreate(&e1));
+    BQD2_HIP(hipEventRecord(e0, st));
+  }
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
+  if (e1) BQD2_HIP(hipEventRecord(e1, st));
   if (any_far)
     hipLaunchKernelGGL(k_far_eval, dim3(n), dim3(64), 0, st, X, p->gf, p->far, n, nchunk, d_r0, d_did, amp, d_part,
                        ctr);
@@ -1340,6 +1359,13 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   long long cnt[CNT_N];
   BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
   BQD2_HIP(hipStreamSynchronize(st));
+  if (e1) {
+    float ms = 0.0f;
+    BQD2_HIP(hipEventElapsedTime(&ms, e0, e1));
+    p->field_ms_part = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   if (gf_evals) *gf_evals = cnt[CNT_FIELD + 4];
   BQD2_CHECK(cnt[CNT_ERR] == 0, BQD2_E_GF, "bqd2_field: Green's function error at %lld points", cnt[CNT_ERR]);
   return BQD2_OK;

@@ -181,6 +181,9 @@ struct bqd2_problem {
   double *far_amp[4];        // 2 sides x G orders x 8 doubles
   unsigned long long far_amp_epoch[4];  // sol_epoch + 1 they were made for (0: none)
   hipEvent_t far_ev[4];      // recorded after they were made
+  // bqd2_set_field_timing: HIP events around the field's point kernel
+  int field_timing;
+  double field_ms_part;      // k_field_part of the last bqd2_field call (ms)
 };
 enum { LG_EMPTY = 0, LG_G1H1 = 1, LG_AF = 2 };
 

@@ -143,6 +143,9 @@ class Request:
         check(hip.bqd2_field(p, self.n, self.pts.ctypes.data, 0, 1, self.out.ctypes.data, self.did.ctypes.data,
                              C.byref(ev)), "bqd2_field")
         self.field_counts = self._counters()
+        fms = C.c_double(0.0)
+        check(hip.bqd2_field_kernel_ms(p, C.byref(fms)), "bqd2_field_kernel_ms")
+        self.field_kernel_ms = fms.value
         t4 = time.perf_counter()
         rc = (C.c_double * 3)(0.0, 0.0, 0.0)
         self.F = np.zeros((2, 3))
@@ -177,15 +180,15 @@ class Request:
         return n + int(self.field_counts[15]) + int(self.force_counts[15])
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch of k_q0_pairs from the committed rocprofv3 summary of
-    this workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known
-    size), or None when there is none."""
+def measured_traffic(workload, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of k_q0_pairs (or of the kernel the key names) from
+    the committed rocprofv3 summary of this workload (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy kernel of known size), or None when there is none."""
     path = os.path.join(ROOT, "profiles", "k_q0_pairs_%s.md" % workload)
     if not os.path.exists(path):
         return None, None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1]), "profiles/k_q0_pairs_%s.md" % workload
     return None, None
 
@@ -992,6 +995,7 @@ def main():
     prob = host.aqd2_problem(model.buf, None)
     lam = cfg["host"][1] / cfg["f"]
     check(hip.bqd2_set_field_far(prob, args.field_far), "bqd2_set_field_far")
+    check(hip.bqd2_set_field_timing(prob, 1), "bqd2_set_field_timing")
     req = Request(hip, prob, model.mn(), grid_points(args.grid, 3.0 * lam, args.planes), args.solver)
     log("bench: workload %s, %d elements, %d unknowns" % (args.workload, model.ne(), 2 * model.nn()))
 
@@ -1009,7 +1013,7 @@ def main():
     F_ref = req.F.copy() if args.warmup else None
 
     torch.cuda.synchronize()
-    lat, phases, flops, ms_pairs, evals = [], [], [], [], 0
+    lat, phases, flops, ms_pairs, field_ms, evals = [], [], [], [], [], 0
     t_start = time.perf_counter()
     for i in range(args.steps):
         t = req.run()
@@ -1018,6 +1022,7 @@ def main():
         st0 = req.stats[0]
         flops.append(pair_flops(st0))
         ms_pairs.append(st0.ms_pairs)
+        field_ms.append(req.field_kernel_ms)
         evals += req.gf_evals()
         log("step %d: %.3f s (assemble %.3f, solve %.3f, velocity %.3f, field %.3f, force %.3f)"
             % (i, t["total"], t["assemble"], t["solve"], t["velocity"], t["field"], t["force"]))
@@ -1058,13 +1063,20 @@ def main():
     fc = req.field_counts
     kinds = ("spatial", "propagating", "evanescent", "pure_spectral")
     far_flops = float(int(fc[18]) * F_FAR["amplitude_term"] + int(fc[17]) * F_FAR["point_order_term"])
-    field_flops = float(sum(int(fc[11 + i]) * F_TERM_V2[k] for i, k in enumerate(kinds))) + far_flops
+    field_flops = float(sum(int(fc[11 + i]) * F_TERM_V2[k] for i, k in enumerate(kinds)))
     field_s = float(np.mean([p["field"] for p in phases]))
+    part_s = float(np.mean(field_ms)) * 1e-3
     fon, forders = C.c_int(), C.c_int()
     check(hip.bqd2_field_far_info(prob, C.byref(fon), C.byref(forders), None, None, None), "bqd2_field_far_info")
-    roof_field = {"bound": "valu", "kernel": "k_field_part + k_far_* (whole field phase timed)",
-                  "achieved": field_flops / field_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                  "frac": field_flops / field_s / 1e12 / PEAK_FP64_TFLOPS, "ms_per_launch": field_s * 1e3,
+    ftraffic, ftraffic_src = measured_traffic(args.workload, "k_field_part_traffic_bytes_per_launch")
+    # the Hessian lattice sums of the points near the boundary, k_field_part's
+    # own time (HIP events on the call's stream); the far points' plane-wave
+    # sums (k_far_*) and the rest of the phase are listed beside it
+    roof_field = {"bound": "valu", "kernel": "k_field_part",
+                  "achieved": field_flops / part_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                  "frac": field_flops / part_s / 1e12 / PEAK_FP64_TFLOPS, "ms_per_launch": part_s * 1e3,
+                  "traffic": ftraffic, "traffic_source": ftraffic_src,
+                  "field_phase_ms": field_s * 1e3, "far_flops_per_phase": far_flops,
                   "flops_per_launch": field_flops,
                   "terms_per_launch": {k: int(fc[11 + i]) for i, k in enumerate(kinds)},
                   "evals_per_launch": int(fc[15]), "flops_per_term": F_TERM_V2,

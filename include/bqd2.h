@@ -227,6 +227,11 @@ int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, d
  * the amplitudes, the number of orders, the boundary's z extent and h. */
 int bqd2_set_field_far(bqd2_problem *p, int on);
 int bqd2_field_far_info(bqd2_problem *p, int *on, int *orders, double *zbot, double *ztop, double *h);
+/* No counterpart in the reference (measurement).  With timing on, bqd2_field
+ * records HIP events around its point kernel on the call's stream;
+ * bqd2_field_kernel_ms gives that kernel's time in the last timed call. */
+int bqd2_set_field_timing(bqd2_problem *p, int on);
+int bqd2_field_kernel_ms(bqd2_problem *p, double *ms);
 
 /* bqd2_field for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
