@@ -37,9 +37,10 @@ struct FieldCtx {
   DomView dom[16];
   int MN, ps;
   int any_tri;  // the mesh has linear triangles: rules are sized for both kinds
-  int far_on;   // points of the periodic domain beyond far_h of the boundary's z
-                // extent [far_zbot, far_ztop] take the plane-wave amplitudes
+  int far_on;   // points of the periodic domain take the element bands at least
+                // far_h below or above them from plane-wave amplitudes
   double far_ztop, far_zbot, far_h;
+  const FarBand *far_bands;  // FAR_NB bands of domain 0's field order
   double pw_p[2], pw_rho0, pw_c0, pw_k0, pw_tv[3], pw_R[9];
   double d1x, d1y, d2x, d2y, idd, vkx, vky;
 };
@@ -213,13 +214,21 @@ __host__ __device__ __forceinline__ int rule_points(int type, int any_tri) {
 
 __device__ __forceinline__ cd ld(const double *a, size_t i) { return cd{a[2 * i], a[2 * i + 1]}; }
 
-// side of a field point of domain did at height z: 0 above the periodic
-// boundary by far_h or more, 1 below it by far_h or more, -1 otherwise
-__device__ __forceinline__ int far_side(const FieldCtx &X, int did, double z) {
-  if (!X.far_on || did != 0) return -1;
-  if (z >= X.far_ztop + X.far_h) return 0;
-  if (z <= X.far_zbot - X.far_h) return 1;
-  return -1;
+// The element bands of domain 0 that a point at height z takes from the
+// plane-wave amplitudes: bands 0..b0 lie at least far_h below it (b0 = -1: none),
+// bands b1..FAR_NB-1 at least far_h above it (b1 = FAR_NB: none); the lattice
+// sums cover the elements at field-order positions [es, ee)
+__device__ __forceinline__ void far_split(const FieldCtx &X, int did, double z, int &b0, int &b1, int &es, int &ee) {
+  b0 = -1;
+  b1 = FAR_NB;
+  es = 0;
+  ee = X.dom[did].Ne;
+  if (!X.far_on || did != 0) return;
+  const FarBand *B = X.far_bands;
+  while (b0 + 1 < FAR_NB && B[b0 + 1].zpre <= z - X.far_h) b0++;
+  while (b1 - 1 > b0 && B[b1 - 1].zsuf >= z + X.far_h) b1--;
+  es = b0 >= 0 ? B[b0].ee : 0;
+  ee = b1 < FAR_NB ? B[b1].es : X.dom[did].Ne;
 }
 
 // ------------------------------------------------------------ off-boundary field
@@ -254,8 +263,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
   const DomView &V = X.dom[did];
   const bool periodic = (X.ps == 1 && did == 0);
   const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
-  // far points: F and grad F here, the lattice sums from k_far_eval
-  const bool far = far_side(X, did, r0[2]) >= 0;
+  // elements of bands far below or above the point: F and grad F here, their
+  // lattice sums from k_far_eval
+  int fb0, fb1, fes, fee;
+  far_split(X, did, r0[2], fb0, fb1, fes, fee);
   const int npt = rule_points(type, X.any_tri);
   const double k = V.k;
   // Accumulators and the per-item coefficients live in LDS (per lane), so that
@@ -288,7 +299,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
     const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
     F += sg * wt * rdW * iR * iR;
     for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
-    if (far) continue;
+    const long long fpos = it / npt;
+    if (fpos < fes || fpos >= fee) continue;
     cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
     for (int m = 0; m < 4; m++) {
       if (Nv[m] == 0.0) continue;
@@ -464,26 +476,30 @@ __global__ void __launch_bounds__(256) k_field_fin(FieldCtx X, int total, int n,
 }
 
 // ------------------------------------------------------------ far field
-// A field point x of the periodic domain with x_z >= z_top + h (side 0) or
-// x_z <= z_bot - h (side 1), z_bot .. z_top the boundary's extent, has every
-// source point y at least h away in z.  There the lattice sum is the pure
-// spectral series (d3_qpgf_d2.c:484-533, :887-950, what fd_cell sums per pair)
+// The source points y of the periodic boundary that lie at least h below a
+// field point x (or at least h above it) have r_z = y_z - x_z of one sign.  For
+// them the lattice sum is the pure spectral series (d3_qpgf_d2.c:484-533,
+// :887-950, what fd_cell sums per pair)
 //   G(r) = (i/2A) sum_n exp(i a_n . r_t + i b_n |r_z|) / b_n,   r = y - x,
 //   a_n = -k_t + n1 g1 + n2 g2,  b_n = sqrt(k^2 - |a_n|^2), Im b_n >= 0,
 // whose gradient takes i kappa_n and Hessian -kappa_n kappa_n^T per term,
-// kappa_n = (a_n, b_n sgn r_z).  With r_z of one sign every term is a target
-// factor times a source factor, so the representation formula's sums over the
-// boundary items (k_field_part's alpha, beta, gamma_c, delta_c, weight vector w)
-// become per order four amplitudes, summed once for all points,
+// kappa_n = (a_n, b_n sgn r_z), and every term is a target factor times a source
+// factor.  So the representation formula's sums over those items (k_field_part's
+// alpha, beta, gamma_c, delta_c, weight vector w) become per order four
+// amplitudes, summed once for all points,
 //   A_n   = (i/2A)/b_n sum_items Y_n (alpha   + i (kappa_n . w) beta),
 //   B_n,c = (i/2A)/b_n sum_items Y_n (gamma_c + i (kappa_n . w) delta_c),
 // and per point a sum over the orders
 //   Pa = sum_n X_n A_n,  dPa_c = sum_n X_n (-i kappa_n,c) A_n,
 //   PVa_c = sum_n X_n B_n,c.
-// Side 0: Y_n = exp(i a_n . y_t + i b_n (z_top - y_z)),
-//         X_n = exp(-i a_n . x_t + i b_n (x_z - z_top)), kappa_z = -b_n;
-// side 1: Y_n = exp(i a_n . y_t + i b_n (y_z - z_bot)),
-//         X_n = exp(-i a_n . x_t + i b_n (z_bot - x_z)), kappa_z = b_n:
+// The items are the element bands of the field order (FarBand): the amplitudes
+// of bands 0..b (side 0, sources below, reference height zpre_b) and of bands
+// b..FAR_NB-1 (side 1, sources above, reference height zsuf_b) are kept for
+// every b, and a point takes the largest set below and above it that clears h.
+// Side 0: Y_n = exp(i a_n . y_t + i b_n (zpre_b - y_z)),
+//         X_n = exp(-i a_n . x_t + i b_n (x_z - zpre_b)), kappa_z = -b_n;
+// side 1: Y_n = exp(i a_n . y_t + i b_n (y_z - zsuf_b)),
+//         X_n = exp(-i a_n . x_t + i b_n (zsuf_b - x_z)), kappa_z = b_n:
 // no factor exceeds 1 in size.  The orders are the grid of FarGeom, which keeps
 // every order that has not decayed by 1e-20 over the height h.
 
@@ -503,24 +519,24 @@ __device__ __forceinline__ cd far_phase(double ph, const cd &bn, double d) {
   return cis(ph) * exp(-bn.y * d);
 }
 
-// partial amplitudes: blockIdx.x blocks of 64 orders, blockIdx.y the side,
-// blockIdx.z one of FAR_NCH item chunks of domain 0; the items of a chunk pass
-// through LDS 64 at a time.  part: (chunk, side, order) x 8 doubles
+// band amplitudes: blockIdx.x blocks of 64 orders, blockIdx.y the side,
+// blockIdx.z the band; the band's items pass through LDS 64 at a time.  The
+// source factors of side 0 refer to the band's zpre, those of side 1 to its
+// zsuf, so none exceeds 1 in size.  amp: (side, band, order) x 8 doubles.
 __global__ void __launch_bounds__(64) k_far_amp(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, GfParams P,
-                                                FarGeom F, int type, double *__restrict__ part) {
-  const int lane = threadIdx.x, j = blockIdx.x * 64 + lane, side = blockIdx.y, ch = blockIdx.z;
+                                                FarGeom F, int type, double *__restrict__ amp) {
+  const int lane = threadIdx.x, j = blockIdx.x * 64 + lane, side = blockIdx.y, band = blockIdx.z;
   const Tab &T = *tab;
   const DomView &V = X.dom[0];
+  const FarBand Bd = X.far_bands[band];
   const int npt = rule_points(type, X.any_tri);
-  const long long items = (long long)V.Ne * npt;
-  const long long per = (items + FAR_NCH - 1) / FAR_NCH;
-  const long long i0 = (long long)ch * per, i1 = min(items, i0 + per);
+  const long long i0 = (long long)Bd.es * npt, i1 = (long long)Bd.ee * npt;
   double ax = 0.0, ay = 0.0;
   cd bn = {0.0, 0.0};
   const bool ok = j < F.G;
   if (ok) far_order(P, F, j, ax, ay, bn);
   const cd kz = side == 0 ? -bn : bn;
-  const double zref = side == 0 ? F.ztop : F.zbot;
+  const double zref = side == 0 ? Bd.zpre : Bd.zsuf;
   // per item: y (3), w (3), alpha, beta, gamma[3], delta[3] (re, im)
   __shared__ double sit[22][64];
   cd A = {0.0, 0.0}, B[3] = {};
@@ -529,7 +545,7 @@ __global__ void __launch_bounds__(64) k_far_amp(DevMesh M, const Tab *__restrict
     double y[3] = {0.0, 0.0, zref}, w[3] = {0.0, 0.0, 0.0};
     cd al = {}, be = {}, gm[3] = {}, dl[3] = {};
     if (it < i1) {
-      const int s = (int)(it / npt) + 1, q = (int)(it % npt);
+      const int s = V.fperm[it / npt], q = (int)(it % npt);
       const int as = abs(V.sid[s]);
       ElemC E;
       if (type != 0) load_elem(M, as, E);
@@ -583,7 +599,7 @@ __global__ void __launch_bounds__(64) k_far_amp(DevMesh M, const Tab *__restrict
     __syncthreads();
   }
   if (!ok) return;
-  double *dst = part + (((size_t)ch * 2 + side) * F.G + j) * 8;
+  double *dst = amp + (((size_t)side * FAR_NB + band) * F.G + j) * 8;
   dst[0] = A.x;
   dst[1] = A.y;
   for (int c = 0; c < 3; c++) {
@@ -592,9 +608,11 @@ __global__ void __launch_bounds__(64) k_far_amp(DevMesh M, const Tab *__restrict
   }
 }
 
-// amplitudes: the chunks summed in order, times (i/2A)/b_n.  amp: (side, order)
-// x 8 doubles.  A Wood anomaly (b_n = 0) counts as an error, as in fd_cell.
-__global__ void __launch_bounds__(256) k_far_fin(GfParams P, FarGeom F, const double *__restrict__ part,
+// the bands' amplitudes summed over the bands below (side 0, in rising band
+// order) or above (side 1, falling), each sum moved to the next band's reference
+// height on the way, in place, then times (i/2A)/b_n.  A Wood anomaly (b_n = 0)
+// counts as an error, as in fd_cell.
+__global__ void __launch_bounds__(256) k_far_fin(GfParams P, FarGeom F, const FarBand *__restrict__ bands,
                                                  double *__restrict__ amp, long long amp_terms,
                                                  long long *__restrict__ counters) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -603,30 +621,32 @@ __global__ void __launch_bounds__(256) k_far_fin(GfParams P, FarGeom F, const do
   const int side = idx / F.G, j = idx % F.G;
   double ax, ay;
   cd bn;
-  double *dst = amp + (size_t)idx * 8;
-  if (!far_order(P, F, j, ax, ay, bn)) {
-    for (int i = 0; i < 8; i++) dst[i] = 0.0;
-    atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
-    return;
-  }
-  double acc[8];
-  for (int i = 0; i < 8; i++) acc[i] = 0.0;
-  for (int ch = 0; ch < FAR_NCH; ch++) {
-    const double *q = part + (((size_t)ch * 2 + side) * F.G + j) * 8;
-    for (int i = 0; i < 8; i++) acc[i] += q[i];
-  }
-  const cd cf = cd{0.0, 0.5 * P.aidd} * crecip(bn);
-  for (int i = 0; i < 4; i++) {
-    const cd v = cd{acc[2 * i], acc[2 * i + 1]} * cf;
-    dst[2 * i] = v.x;
-    dst[2 * i + 1] = v.y;
+  const bool good = far_order(P, F, j, ax, ay, bn);
+  if (!good) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
+  const cd cf = good ? cd{0.0, 0.5 * P.aidd} * crecip(bn) : cd{0.0, 0.0};
+  cd C[4] = {};
+  for (int k = 0; k < FAR_NB; k++) {
+    const int b = side == 0 ? k : FAR_NB - 1 - k;
+    double *a = amp + (((size_t)side * FAR_NB + b) * F.G + j) * 8;
+    if (k) {
+      const int bp = side == 0 ? b - 1 : b + 1;
+      const double shift = side == 0 ? bands[b].zpre - bands[bp].zpre : bands[bp].zsuf - bands[b].zsuf;
+      const cd f = far_phase(0.0, bn, shift);
+      for (int i = 0; i < 4; i++) C[i] = C[i] * f;
+    }
+    for (int i = 0; i < 4; i++) {
+      C[i] += cd{a[2 * i], a[2 * i + 1]};
+      const cd v = C[i] * cf;
+      a[2 * i] = v.x;
+      a[2 * i + 1] = v.y;
+    }
   }
 }
 
-// far points: the sums over the orders, one wavefront per point with the
-// orders strided over its lanes and summed across them in a fixed order, added
-// to the point's first chunk of k_field_part's partial sums (which hold F and
-// grad F only for these points)
+// the amplitude parts of the points' sums, one wavefront per point with the
+// orders strided over its lanes and summed across them in a fixed order (bands
+// below, then above), added to the point's first chunk of k_field_part's partial
+// sums (which hold the lattice sums of the other bands and F, grad F)
 __global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom F, int n, int nchunk,
                                                  const double *__restrict__ r0s, const int *__restrict__ did_io,
                                                  const double *__restrict__ amp, double *__restrict__ part,
@@ -636,28 +656,34 @@ __global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom
   const int did = did_io[ip];
   if (did < 0) return;
   const double x0 = r0s[3 * ip], x1 = r0s[3 * ip + 1], x2 = r0s[3 * ip + 2];
-  const int side = far_side(X, did, x2);
-  if (side < 0) return;
+  int b0, b1, es, ee;
+  far_split(X, did, x2, b0, b1, es, ee);
+  const int nsides = (b0 >= 0) + (b1 < FAR_NB);
+  if (!nsides) return;
   if (lane == 0) {
     atomicAdd((unsigned long long *)&counters[CNT_FAR], 1ull);
-    atomicAdd((unsigned long long *)&counters[CNT_FAR + 1], (unsigned long long)F.G);
+    atomicAdd((unsigned long long *)&counters[CNT_FAR + 1], (unsigned long long)F.G * nsides);
   }
-  const double d = side == 0 ? x2 - F.ztop : F.zbot - x2;
-  const double *am = amp + (size_t)side * F.G * 8;
   cd Pa = {}, dPa[3] = {}, PVa[3] = {};
-  for (int j = lane; j < F.G; j += 64) {
-    double ax, ay;
-    cd bn;
-    far_order(P, F, j, ax, ay, bn);
-    const cd Xn = far_phase(-(ax * x0 + ay * x1), bn, d);
-    const double *a = am + (size_t)j * 8;
-    const cd XA = Xn * cd{a[0], a[1]};
-    Pa += XA;
-    // -i kappa_c X_n A_n
-    dPa[0] += mul_i(XA) * (-ax);
-    dPa[1] += mul_i(XA) * (-ay);
-    dPa[2] += mul_i(XA * (side == 0 ? bn : -bn));
-    for (int c = 0; c < 3; c++) PVa[c] += Xn * cd{a[2 + 2 * c], a[3 + 2 * c]};
+  for (int side = 0; side < 2; side++) {
+    const int b = side == 0 ? b0 : b1;
+    if (b < 0 || b >= FAR_NB) continue;
+    const double d = side == 0 ? x2 - X.far_bands[b].zpre : X.far_bands[b].zsuf - x2;
+    const double *am = amp + ((size_t)side * FAR_NB + b) * F.G * 8;
+    for (int j = lane; j < F.G; j += 64) {
+      double ax, ay;
+      cd bn;
+      far_order(P, F, j, ax, ay, bn);
+      const cd Xn = far_phase(-(ax * x0 + ay * x1), bn, d);
+      const double *a = am + (size_t)j * 8;
+      const cd XA = Xn * cd{a[0], a[1]};
+      Pa += XA;
+      // -i kappa_c X_n A_n
+      dPa[0] += mul_i(XA) * (-ax);
+      dPa[1] += mul_i(XA) * (-ay);
+      dPa[2] += mul_i(XA * (side == 0 ? bn : -bn));
+      for (int c = 0; c < 3; c++) PVa[c] += Xn * cd{a[2 + 2 * c], a[3 + 2 * c]};
+    }
   }
   Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
   for (int c = 0; c < 3; c++) {
@@ -1201,22 +1227,20 @@ static FieldCtx make_ctx(const bqd2_problem *p) {
   X.far_ztop = p->far.ztop;
   X.far_zbot = p->far.zbot;
   X.far_h = p->far.h;
+  X.far_bands = p->far_bands;
   return X;
 }
 
-// device bytes of the far field's partial amplitudes and of its amplitudes
-static size_t far_part_bytes(const bqd2_problem *p) { return (size_t)FAR_NCH * 2 * p->far.G * 8 * sizeof(double); }
-static size_t far_amp_bytes(const bqd2_problem *p) { return (size_t)2 * p->far.G * 8 * sizeof(double); }
+// device bytes of the far field's amplitudes (2 sides x FAR_NB bands x G orders)
+static size_t far_amp_bytes(const bqd2_problem *p) { return (size_t)2 * FAR_NB * p->far.G * 8 * sizeof(double); }
 
-// the far field's amplitudes of rule type `type` into amp, through the partial
-// buffer part, on stream st
-static int far_launch(bqd2_problem *p, const FieldCtx &X, int type, double *part, double *amp, long long *ctr,
-                      hipStream_t st) {
+// the far field's amplitudes of rule type `type` into amp, on stream st
+static int far_launch(bqd2_problem *p, const FieldCtx &X, int type, double *amp, long long *ctr, hipStream_t st) {
   const FarGeom &F = p->far;
   const long long items = (long long)p->dom[0].Ne * rule_points(type, X.any_tri);
-  hipLaunchKernelGGL(k_far_amp, dim3((F.G + 63) / 64, 2, FAR_NCH), dim3(64), 0, st, p->m, p->tab, X, p->gf, F, type,
-                     part);
-  hipLaunchKernelGGL(k_far_fin, dim3(bqd2_blocks(2 * F.G, 256)), dim3(256), 0, st, p->gf, F, part, amp,
+  hipLaunchKernelGGL(k_far_amp, dim3((F.G + 63) / 64, 2, FAR_NB), dim3(64), 0, st, p->m, p->tab, X, p->gf, F, type,
+                     amp);
+  hipLaunchKernelGGL(k_far_fin, dim3(bqd2_blocks(2 * F.G, 256)), dim3(256), 0, st, p->gf, F, p->far_bands, amp,
                      2 * (long long)F.G * items, ctr);
   BQD2_HIP(hipGetLastError());
   return BQD2_OK;
@@ -1226,7 +1250,7 @@ static int far_launch(bqd2_problem *p, const FieldCtx &X, int type, double *part
 // caller's stream when the solution changed since they were last made (under
 // the problem's far-field lock), else the caller's stream waits for the call
 // that made them
-static int far_amplitudes(bqd2_problem *p, CallCtx *cx, const FieldCtx &X, int type, double *part, long long *ctr,
+static int far_amplitudes(bqd2_problem *p, CallCtx *cx, const FieldCtx &X, int type, long long *ctr,
                           const double **amp) {
   std::lock_guard<std::mutex> g(p->far_mu);
   const unsigned long long want = p->sol_epoch.load() + 1;
@@ -1238,7 +1262,7 @@ static int far_amplitudes(bqd2_problem *p, CallCtx *cx, const FieldCtx &X, int t
   int rc;
   if (!p->far_amp[type] && (rc = bqd2_alloc(p, (void **)&p->far_amp[type], far_amp_bytes(p)))) return rc;
   if (!p->far_ev[type]) BQD2_HIP(hipEventCreateWithFlags(&p->far_ev[type], hipEventDisableTiming));
-  if ((rc = far_launch(p, X, type, part, p->far_amp[type], ctr, cx->st))) return rc;
+  if ((rc = far_launch(p, X, type, p->far_amp[type], ctr, cx->st))) return rc;
   BQD2_HIP(hipEventRecord(p->far_ev[type], cx->st));
   p->far_amp_epoch[type] = want;
   *amp = p->far_amp[type];
@@ -1317,26 +1341,18 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
     nchunk = std::max(nchunk, (int)((items + chunk - 1) / chunk));
   }
   FieldCtx X = make_ctx(p);
-  // far points in the call (a point's height is not changed by the reduction
-  // to the lattice cell)
-  bool any_far = false;
-  if (X.far_on)
-    for (int i = 0; i < n && !any_far; i++)
-      any_far = pts[3 * i + 2] >= X.far_ztop + X.far_h || pts[3 * i + 2] <= X.far_zbot - X.far_h;
-  if (!any_far) X.far_on = 0;
-  double *d_fpart = nullptr;
+  const bool any_far = X.far_on != 0;  // the amplitudes serve every point of domain 0
   if ((rc = S.init(cx, pad256(nn * 24) * 2 + pad256(nn * 64) + pad256(nn * 8) + pad256(nn * 4) +
-                          pad256(nn * nchunk * FP_N * 8) + (any_far ? pad256(far_part_bytes(p)) : 0))))
+                          pad256(nn * nchunk * FP_N * 8))))
     return rc;
   if ((rc = S.get((void **)&d_pts, (size_t)n * 24)) || (rc = S.get((void **)&d_r0, (size_t)n * 24)) ||
       (rc = S.get((void **)&d_out, (size_t)n * 64)) || (rc = S.get((void **)&d_l, (size_t)n * 8)) ||
       (rc = S.get((void **)&d_did, (size_t)n * 4)) || (rc = S.get((void **)&d_part, nn * nchunk * FP_N * 8)))
     return rc;
-  if (any_far && (rc = S.get((void **)&d_fpart, far_part_bytes(p)))) return rc;
   BQD2_HIP(hipMemcpyAsync(d_pts, pts, (size_t)n * 24, hipMemcpyHostToDevice, st));
   BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
   const double *amp = nullptr;
-  if (any_far && (rc = far_amplitudes(p, cx, X, type, d_fpart, ctr, &amp))) return rc;
+  if (any_far && (rc = far_amplitudes(p, cx, X, type, ctr, &amp))) return rc;
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, st, p->m, X, p->rn, p->ed, n, d_pts, d_did,
                      d_r0, d_l);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1420,28 +1436,25 @@ extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, i
   }
   FieldCtx X = make_ctx(p);
   // scratch layout: counters, r0 (n x 3), lattice cell (n x 2 int), chunk
-  // partials, then the far field's partial amplitudes and amplitudes (made in
-  // every call: the same kernels on the same data as bqd2_field's, so the same
-  // bits)
+  // partials, then the far field's amplitudes (made in every call: the same
+  // kernels on the same data as bqd2_field's, so the same bits)
   const size_t o_r0 = 256, o_l = o_r0 + pad256(nn * 24), o_part = o_l + pad256(nn * 8);
-  const size_t o_fpart = o_part + pad256(nn * nchunk * FP_N * 8);
-  const size_t o_famp = o_fpart + (X.far_on ? pad256(far_part_bytes(p)) : 0);
+  const size_t o_famp = o_part + pad256(nn * nchunk * FP_N * 8);
   const size_t need = o_famp + (X.far_on ? far_amp_bytes(p) : 0);
   BQD2_CHECK(need <= (size_t)BQD2_NOALLOC_SCRATCH_BYTES, BQD2_E_ARG,
              "bqd2_field_noalloc: %d points of type %d need %zu bytes of scratch, more than its %zu bytes "
              "(at most %zu points per call on this problem)",
              n, type, need, (size_t)BQD2_NOALLOC_SCRATCH_BYTES,
-             (size_t)((BQD2_NOALLOC_SCRATCH_BYTES - 256 - 768 - (o_famp - o_fpart) - (need - o_famp)) /
-                      (24 + 8 + (size_t)nchunk * FP_N * 8)));
+             (size_t)((BQD2_NOALLOC_SCRATCH_BYTES - 256 - 768 - (need - o_famp)) / (24 + 8 + (size_t)nchunk * FP_N * 8)));
   char *base = nullptr;
   int rc;
   if ((rc = noalloc_scratch(&base))) return rc;
   long long *ctr = (long long *)base;
   double *d_r0 = (double *)(base + o_r0), *d_part = (double *)(base + o_part);
   int *d_l = (int *)(base + o_l);
-  double *d_fpart = (double *)(base + o_fpart), *d_famp = (double *)(base + o_famp);
+  double *d_famp = (double *)(base + o_famp);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, ctr);
-  if (X.far_on && (rc = far_launch(p, X, type, d_fpart, d_famp, ctr, stream))) return rc;
+  if (X.far_on && (rc = far_launch(p, X, type, d_famp, ctr, stream))) return rc;
   hipLaunchKernelGGL(k_domain_id, dim3(bqd2_blocks(n, 4)), dim3(256), 0, stream, p->m, X, p->rn, p->ed, n, d_pts,
                      d_did, d_r0, d_l);
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,

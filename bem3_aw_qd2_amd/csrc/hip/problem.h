@@ -98,7 +98,18 @@ struct FarGeom {
   int n1lo, n1n, n2lo, n2n, G;
   double ztop, zbot, h;
 };
-enum { FAR_MAX_ORDERS = 8192, FAR_NCH = 8 };
+// The field kernels visit the periodic domain's elements in FAR_NB bands of the
+// element centre's z (DevDomain.fperm).  Band b holds the elements at fperm
+// positions [es, ee); zpre is the highest node of bands 0..b, zsuf the lowest
+// node of bands b..FAR_NB-1.  A field point at height z takes the bands with
+// zpre <= z - h (below it) and those with zsuf >= z + h (above it) from
+// plane-wave amplitudes, the rest by the lattice sums.
+enum { FAR_NB = 32 };
+struct FarBand {
+  int es, ee;
+  double zpre, zsuf;
+};
+enum { FAR_MAX_ORDERS = 4096 };
 // entries of the pair list beyond N x Ne: each of the 18 classes (9 rules, two
 // element kinds) is padded to a multiple of the pairs per wavefront (up to 16)
 enum { PLIST_PAD = 512 };
@@ -178,7 +189,10 @@ struct bqd2_problem {
   int smooth_mode;           // bqd2_set_remainder_rules (1 by default)
   std::atomic<unsigned long long> sol_epoch;
   std::mutex far_mu;
-  double *far_amp[4];        // 2 sides x G orders x 8 doubles
+  FarBand *far_bands;        // FAR_NB bands (device)
+  std::vector<int> fband0;   // host: band of each fperm position of domain 0
+  std::vector<int> fperm0;   // host: domain 0's fperm
+  double *far_amp[4];        // 2 sides x FAR_NB bands x G orders x 8 doubles
   unsigned long long far_amp_epoch[4];  // sol_epoch + 1 they were made for (0: none)
   hipEvent_t far_ev[4];      // recorded after they were made
   // bqd2_set_field_timing: HIP events around the field's point kernel

@@ -170,6 +170,35 @@ static void far_setup(bqd2_problem *p, const bqd2_model *m) {
   double h;
   const int G = far_grid(m->qd, &N1, &N2, &h);
   if (ne0 <= 0 || !(zt >= zb) || G == 0) return;
+  // the bands of the field order: element ranges, and the extreme node heights
+  // of the bands below and above each
+  FarBand B[FAR_NB];
+  double bmax[FAR_NB], bmin[FAR_NB];
+  for (int b = 0; b < FAR_NB; b++) {
+    bmax[b] = -1e300;
+    bmin[b] = 1e300;
+    B[b].es = B[b].ee = 0;
+  }
+  int pos = 0;
+  for (int b = 0; b < FAR_NB; b++) {
+    B[b].es = pos;
+    while (pos < ne0 && p->fband0[pos] == b) {
+      const int s = abs(sid[p->fperm0[pos]]);
+      for (int j = 0; j < 4; j++) {
+        const int nd = m->ed[4 * (size_t)s + j];
+        if (j == 3 && nd == 0) continue;
+        const double z = m->rn[3 * (size_t)nd + 2];
+        bmax[b] = std::max(bmax[b], z);
+        bmin[b] = std::min(bmin[b], z);
+      }
+      pos++;
+    }
+    B[b].ee = pos;
+  }
+  if (pos != ne0) return;  // the band order does not cover the domain
+  for (int b = 0; b < FAR_NB; b++) B[b].zpre = std::max(bmax[b], b ? B[b - 1].zpre : -1e300);
+  for (int b = FAR_NB - 1; b >= 0; b--) B[b].zsuf = std::min(bmin[b], b + 1 < FAR_NB ? B[b + 1].zsuf : 1e300);
+  if (upload(p, &p->far_bands, B, FAR_NB)) return;
   F.n1lo = -N1;
   F.n1n = 2 * N1 + 1;
   F.n2lo = -N2;
@@ -298,6 +327,11 @@ static int create_impl(const bqd2_model *m, int large, bqd2_problem **out) {
       std::stable_sort(key.begin(), key.end());
       for (int i = 0; i < D.Ne; i++) ord[i] = key[i].second;
       TRY(upload(p, &D.fperm, ord.data(), ord.size()));
+      if (d == 0) {
+        p->fband0.assign((size_t)D.Ne, 0);
+        for (int i = 0; i < D.Ne; i++) p->fband0[i] = key[i].first.first;
+        p->fperm0.assign(ord.begin(), ord.begin() + D.Ne);
+      }
     }
     const size_t N = D.N;
     const bool periodic0 = (d == 0 && m->ps == 1);

@@ -173,10 +173,10 @@ extern "C" size_t bqd2_footprint(const bqd2_model *m, int solver, int npts, int 
       const size_t items = (size_t)m->dom_ne[d] * 4, per = (items + 4095) / 4096, chunk = 64 * std::max<size_t>(per, 1);
       nchunk = std::max(nchunk, (items + chunk - 1) / chunk);
     }
-    // the far field's partial amplitudes in the workspace, its amplitudes kept
-    const size_t far = m->ps == 1 ? (size_t)far_grid(m->qd, nullptr, nullptr, nullptr) * 2 * 64 : 0;
-    ws = std::max(ws, (size_t)npts * (48 + 64 + 12 + nchunk * 18 * 8) + 7 * 256 + FAR_NCH * far);
-    b += far;
+    // the far field's amplitudes, kept (2 sides x FAR_NB bands per order) and the band table
+    const size_t far = m->ps == 1 ? (size_t)far_grid(m->qd, nullptr, nullptr, nullptr) * 2 * FAR_NB * 64 : 0;
+    ws = std::max(ws, (size_t)npts * (48 + 64 + 12 + nchunk * 18 * 8) + 7 * 256);
+    b += far + FAR_NB * sizeof(FarBand);
   }
   if (force_type >= 0) {
     const size_t nbd = ne1 * 9, nch = ((size_t)m->Ne * 9 + 1023) / 1024 + 2;

@@ -216,15 +216,17 @@ int bqd2_upload_solution(bqd2_problem *p, int did, const double *P, const double
 int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, double *out, int *did,
                long long *gf_evals);
 
-/* Field points of the periodic domain at least h above or below the boundary's
- * z extent (h a quarter of the shorter lattice vector): the lattice sum there is
- * the pure spectral series of d3_qpgf_d2.c:484-533, whose terms factor into a
- * source and a target part, so bqd2_field sums the boundary once per order into
- * plane-wave amplitudes (kept on the device until the solution changes) and
- * each such point over the orders.  No counterpart in the reference, which sums
- * the series per (point, source) pair.  on = 0 takes the per-pair sums at every
- * point (for comparisons); bqd2_field_far_info reports whether the problem uses
- * the amplitudes, the number of orders, the boundary's z extent and h. */
+/* Field points of the periodic domain and the boundary elements at least h
+ * below or above them (h a quarter of the shorter lattice vector): between them
+ * the lattice sum is the pure spectral series of d3_qpgf_d2.c:484-533, whose
+ * terms factor into a source and a target part, so bqd2_field sums the boundary
+ * once per order into plane-wave amplitudes of 32 bands of elements by height
+ * (kept on the device until the solution changes) and each point over the
+ * orders for the bands that clear h, the other bands by the lattice sums.  No
+ * counterpart in the reference, which sums every (point, source) pair.  on = 0
+ * takes the per-pair sums at every point (for comparisons);
+ * bqd2_field_far_info reports whether the problem uses the amplitudes, the
+ * number of orders, the boundary's z extent and h. */
 int bqd2_set_field_far(bqd2_problem *p, int on);
 int bqd2_field_far_info(bqd2_problem *p, int *on, int *orders, double *zbot, double *ztop, double *h);
 /* No counterpart in the reference (measurement).  With timing on, bqd2_field
@@ -243,10 +245,10 @@ int bqd2_field_kernel_ms(bqd2_problem *p, double *ms);
  * buffers: d_pts (n x 3), d_out (n x 8 as in bqd2_field), d_did (n), d_errors
  * (may be NULL: receives the number of points whose Green's function
  * evaluation failed).  Results are the bits bqd2_field returns.  A call needs
- * at most 1024 + n (32 + 144 c) + 1152 G bytes, c <= 64 the element chunks per
- * point of the problem and rule, G <= 8192 the far field's orders (0 where it is
+ * at most 1024 + n (32 + 144 c) + 4096 G bytes, c <= 64 the element chunks per
+ * point of the problem and rule, G <= 4096 the far field's orders (0 where it is
  * not used; bqd2_field_far_info): the far field's amplitudes are made in every
- * call (32 MiB: about 3100 points at c = 64 and G = 4000); a larger n is
+ * call (32 MiB: about 1900 points at c = 64 and G = 3721); a larger n is
  * rejected with BQD2_E_ARG before anything runs.  Every
  * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
  * streams, two threads, two problems) race and must be ordered by the caller. */
