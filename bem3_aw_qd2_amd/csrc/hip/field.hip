@@ -238,6 +238,7 @@ __device__ __forceinline__ void far_split(const FieldCtx &X, int did, double z, 
 // point's chunks in order.  A single point is then spread over up to 64
 // wavefronts and gives the same bits alone as inside a batch.
 enum { FP_N = 18 };  // per chunk: Pa (2), dPa (6), PVa (6), F, dF (3)
+enum { BD_CHUNK = 1024, BP_N = 9 };  // k_field_bd's chunks: Pa (2), PVa (6), F
 
 __host__ __device__ __forceinline__ long long field_chunk_items(long long items) {
   const long long per = (items + 64 * 64 - 1) / (64 * 64);
@@ -645,17 +646,36 @@ __global__ void __launch_bounds__(256) k_far_fin(GfParams P, FarGeom F, const Fa
 
 // the amplitude parts of the points' sums, one wavefront per point with the
 // orders strided over its lanes and summed across them in a fixed order (bands
-// below, then above), added to the point's first chunk of k_field_part's partial
-// sums (which hold the lattice sums of the other bands and F, grad F)
-__global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom F, int n, int nchunk,
+// below, then above), added to the point's first chunk of partial sums (which
+// hold the lattice sums of the other bands and F, grad F).
+//   BD = 0: field points (r0s, did_io), k_field_part's chunks of FP_N doubles;
+//   BD = 1: boundary points of domain bd_did (element position tpos, local
+//           coordinates ze), k_field_bd's chunks of BP_N doubles (no gradient)
+template <int BD>
+__global__ void __launch_bounds__(64) k_far_eval(DevMesh M, FieldCtx X, GfParams P, FarGeom F, int n, int nchunk,
                                                  const double *__restrict__ r0s, const int *__restrict__ did_io,
-                                                 const double *__restrict__ amp, double *__restrict__ part,
-                                                 long long *__restrict__ counters) {
+                                                 const int *__restrict__ tpos, const double *__restrict__ ze,
+                                                 int bd_did, const double *__restrict__ amp,
+                                                 double *__restrict__ part, long long *__restrict__ counters) {
   const int ip = blockIdx.x, lane = threadIdx.x;
   if (ip >= n) return;
-  const int did = did_io[ip];
+  const int did = BD ? bd_did : did_io[ip];
   if (did < 0) return;
-  const double x0 = r0s[3 * ip], x1 = r0s[3 * ip + 1], x2 = r0s[3 * ip + 2];
+  double x0, x1, x2;
+  if (BD) {
+    // the boundary point as k_field_bd places it
+    ElemC Et;
+    load_elem(M, abs(X.dom[did].sid[tpos[ip]]), Et);
+    double rt[3], wt_[3];
+    bil_rw(Et, ze[2 * ip], ze[2 * ip + 1], rt, wt_);
+    x0 = rt[0];
+    x1 = rt[1];
+    x2 = rt[2];
+  } else {
+    x0 = r0s[3 * ip];
+    x1 = r0s[3 * ip + 1];
+    x2 = r0s[3 * ip + 2];
+  }
   int b0, b1, es, ee;
   far_split(X, did, x2, b0, b1, es, ee);
   const int nsides = (b0 >= 0) + (b1 < FAR_NB);
@@ -678,19 +698,31 @@ __global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom
       const double *a = am + (size_t)j * 8;
       const cd XA = Xn * cd{a[0], a[1]};
       Pa += XA;
-      // -i kappa_c X_n A_n
-      dPa[0] += mul_i(XA) * (-ax);
-      dPa[1] += mul_i(XA) * (-ay);
-      dPa[2] += mul_i(XA * (side == 0 ? bn : -bn));
+      if (!BD) {
+        // -i kappa_c X_n A_n
+        dPa[0] += mul_i(XA) * (-ax);
+        dPa[1] += mul_i(XA) * (-ay);
+        dPa[2] += mul_i(XA * (side == 0 ? bn : -bn));
+      }
       for (int c = 0; c < 3; c++) PVa[c] += Xn * cd{a[2 + 2 * c], a[3 + 2 * c]};
     }
   }
   Pa = cd{wave_sum(Pa.x), wave_sum(Pa.y)};
   for (int c = 0; c < 3; c++) {
-    dPa[c] = cd{wave_sum(dPa[c].x), wave_sum(dPa[c].y)};
+    if (!BD) dPa[c] = cd{wave_sum(dPa[c].x), wave_sum(dPa[c].y)};
     PVa[c] = cd{wave_sum(PVa[c].x), wave_sum(PVa[c].y)};
   }
   if (lane != 0) return;
+  if (BD) {
+    double *dst = part + (size_t)ip * nchunk * BP_N;
+    dst[0] += Pa.x;
+    dst[1] += Pa.y;
+    for (int c = 0; c < 3; c++) {
+      dst[2 + 2 * c] += PVa[c].x;
+      dst[3 + 2 * c] += PVa[c].y;
+    }
+    return;
+  }
   double *dst = part + (size_t)ip * nchunk * FP_N;
   dst[0] += Pa.x;
   dst[1] += Pa.y;
@@ -708,8 +740,6 @@ __global__ void __launch_bounds__(64) k_far_eval(FieldCtx X, GfParams P, FarGeom
 // wavefront per (point, chunk), and one more wavefront per point takes the
 // point's own element on its polar rule; k_field_bd_fin sums the chunks in
 // order.  The chunk count depends only on the mesh and the rule.
-enum { BD_CHUNK = 1024, BP_N = 9 };  // per chunk: Pa (2), PVa (6), F
-
 __host__ __device__ __forceinline__ int bd_item_chunks(long long items) {
   return (int)((items + BD_CHUNK - 1) / BD_CHUNK);
 }
@@ -758,6 +788,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const int nci = bd_item_chunks(items);
   const int ic = SELF ? nci : (int)blockIdx.y;
   const long long i0 = (long long)ic * BD_CHUNK, i1 = SELF ? 0 : min(items, i0 + BD_CHUNK);
+  // elements of bands far below or above the point: F here, their lattice sums
+  // from k_far_eval
+  int fb0, fb1, fes, fee;
+  far_split(X, did, rt[2], fb0, fb1, fes, fee);
   for (long long it = i0 + lane; it < i1; it += 64) {
     const int s = V.fperm[it / npt], q = (int)(it % npt);
     if (s == t) continue;
@@ -769,6 +803,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     double vR[3] = {r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]};
     double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
     double tD = dot3(vR, w) * iR * iR * iR;
+    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+    F += sg * wt * tD;
+    const long long fpos = it / npt;
+    if (fpos < fes || fpos >= fee) continue;
     cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
     for (int m = 0; m < 4; m++) {
       if (Nv[m] == 0.0) continue;
@@ -779,8 +817,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         dpvi[c] += ld(V.dpv, 12 * s + 3 * m + c) * Nv[m];
       }
     }
-    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
-    F += sg * wt * tD;
     // Pa += g alpha + (dg.W) beta, PVa_c += g gamma_c + (dg.W) delta_c with the
     // kernel value g and its normal derivative dg.W (free space or lattice sum)
     {
@@ -1099,15 +1135,20 @@ __global__ void __launch_bounds__(256) k_field_bd_fin(DevMesh M, FieldCtx X, int
 }
 
 // boundary field at n points already on the device: partial sums in part
-// (n x (chunks + 1) x BP_N doubles), results in out (n x 8)
+// (n x (chunks + 1) x BP_N doubles), results in out (n x 8); amp: the far
+// field's amplitudes of rule type `type` (domain 0 with X.far_on), else null
 static int launch_field_bd(bqd2_problem *p, hipStream_t st, FieldCtx &X, int type, int total, int did, int n,
-                           const int *d_t, const double *d_ze, double *d_part, double *d_out, long long *ctr) {
+                           const int *d_t, const double *d_ze, double *d_part, double *d_out, long long *ctr,
+                           const double *amp) {
   const int nch = bd_item_chunks((long long)p->dom[did].Ne * rule_points(type, p->any_tri)) + 1;
   if (nch > 1)
     hipLaunchKernelGGL(k_field_bd<0>, dim3(n, nch - 1), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t,
                        d_ze, d_part, ctr);
   hipLaunchKernelGGL(k_field_bd<1>, dim3(n, 1), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t, d_ze,
                      d_part, ctr);
+  if (amp)
+    hipLaunchKernelGGL(k_far_eval<1>, dim3(n), dim3(64), 0, st, p->m, X, p->gf, p->far, n, nch,
+                       (const double *)nullptr, (const int *)nullptr, d_t, d_ze, did, amp, d_part, ctr);
   hipLaunchKernelGGL(k_field_bd_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, p->m, X, total, did, n, nch, d_t,
                      d_ze, d_part, d_out);
   BQD2_HIP(hipGetLastError());
@@ -1365,7 +1406,8 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
                      d_did, d_part, ctr);
   if (e1) BQD2_HIP(hipEventRecord(e1, st));
   if (any_far)
-    hipLaunchKernelGGL(k_far_eval, dim3(n), dim3(64), 0, st, X, p->gf, p->far, n, nchunk, d_r0, d_did, amp, d_part,
+    hipLaunchKernelGGL(k_far_eval<0>, dim3(n), dim3(64), 0, st, p->m, X, p->gf, p->far, n, nchunk, d_r0, d_did,
+                       (const int *)nullptr, (const double *)nullptr, 0, amp, d_part,
                        ctr);
   hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, st, X, total, n, nchunk, d_part, d_l, d_pts,
                      d_did, d_out);
@@ -1460,8 +1502,8 @@ extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, i
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
   if (X.far_on)
-    hipLaunchKernelGGL(k_far_eval, dim3(n), dim3(64), 0, stream, X, p->gf, p->far, n, nchunk, d_r0, d_did, d_famp,
-                       d_part, ctr);
+    hipLaunchKernelGGL(k_far_eval<0>, dim3(n), dim3(64), 0, stream, p->m, X, p->gf, p->far, n, nchunk, d_r0, d_did,
+                       (const int *)nullptr, (const double *)nullptr, 0, d_famp, d_part, ctr);
   hipLaunchKernelGGL(k_field_fin, dim3(bqd2_blocks(n, 256)), dim3(256), 0, stream, X, total, n, nchunk, d_part, d_l,
                      d_pts, d_did, d_out);
   if (d_errors) hipLaunchKernelGGL(k_copy_errors, dim3(1), dim3(64), 0, stream, ctr, d_errors);
@@ -1494,7 +1536,9 @@ extern "C" int bqd2_field_bd(bqd2_problem *p, int did, int n, const int *t, cons
   BQD2_HIP(hipMemcpyAsync(d_ze, ze, (size_t)n * 16, hipMemcpyHostToDevice, st));
   FieldCtx X = make_ctx(p);
   BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
-  if ((rc = launch_field_bd(p, st, X, type, total, did, n, d_t, d_ze, d_part, d_out, ctr))) return rc;
+  const double *amp = nullptr;
+  if (X.far_on && did == 0 && (rc = far_amplitudes(p, cx, X, type, ctr, &amp))) return rc;
+  if ((rc = launch_field_bd(p, st, X, type, total, did, n, d_t, d_ze, d_part, d_out, ctr, amp))) return rc;
   BQD2_HIP(hipMemcpyAsync(out, d_out, (size_t)n * 64, hipMemcpyDeviceToHost, st));
   long long cnt[CNT_N];
   BQD2_HIP(hipMemcpyAsync(cnt, ctr, sizeof(cnt), hipMemcpyDeviceToHost, st));
@@ -1557,7 +1601,9 @@ extern "C" int bqd2_force(bqd2_problem *p, const double *rc, int type, double *F
     BQD2_HIP(hipMemsetAsync(ctr, 0, CNT_N * sizeof(long long), st));
     double *d_bdpart;
     if ((rc_ = S.get((void **)&d_bdpart, bd_part_bytes(p, type, 0, (int)nbd)))) return rc_;
-    if ((rc_ = launch_field_bd(p, st, X, type, 1, 0, (int)nbd, d_t, d_ze, d_bdpart, d_bd, ctr))) return rc_;
+    const double *amp = nullptr;
+    if (X.far_on && (rc_ = far_amplitudes(p, cx, X, type, ctr, &amp))) return rc_;
+    if ((rc_ = launch_field_bd(p, st, X, type, 1, 0, (int)nbd, d_t, d_ze, d_bdpart, d_bd, ctr, amp))) return rc_;
     BQD2_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_force_elem, dim3(bqd2_blocks(ne, 256)), dim3(256), 0, st, p->m, p->tab, X, type, d_bd, d_off,

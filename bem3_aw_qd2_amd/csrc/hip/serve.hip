@@ -167,21 +167,26 @@ extern "C" size_t bqd2_footprint(const bqd2_model *m, int solver, int npts, int 
   // GMRES: basis of restart + 1 vectors for the system, 3 right-hand sides for dpv
   ws = std::max(ws, 101 * na * 16 + 8 * na * 16);
   ws = std::max(ws, 3 * 101 * nmax * 16 + 8 * 3 * nmax * 16);
+  // the far field's amplitudes (2 sides x FAR_NB bands per order), one set per
+  // rule type in use, and the band table
+  const size_t far = m->ps == 1 ? (size_t)far_grid(m->qd, nullptr, nullptr, nullptr) * 2 * FAR_NB * 64 : 0;
+  int far_sets = 0;
+  if (m->ps == 1) b += FAR_NB * sizeof(FarBand);
   if (npts > 0) {
     size_t nchunk = 1;
     for (int d = 0; d <= m->MN; d++) {
       const size_t items = (size_t)m->dom_ne[d] * 4, per = (items + 4095) / 4096, chunk = 64 * std::max<size_t>(per, 1);
       nchunk = std::max(nchunk, (items + chunk - 1) / chunk);
     }
-    // the far field's amplitudes, kept (2 sides x FAR_NB bands per order) and the band table
-    const size_t far = m->ps == 1 ? (size_t)far_grid(m->qd, nullptr, nullptr, nullptr) * 2 * FAR_NB * 64 : 0;
     ws = std::max(ws, (size_t)npts * (48 + 64 + 12 + nchunk * 18 * 8) + 7 * 256);
-    b += far + FAR_NB * sizeof(FarBand);
+    far_sets++;  // rule type 0
   }
   if (force_type >= 0) {
     const size_t nbd = ne1 * 9, nch = ((size_t)m->Ne * 9 + 1023) / 1024 + 2;
     ws = std::max(ws, ne1 * 48 + nbd * (64 + 4 + 16 + 4) + nbd * nch * 9 * 8 + 8 * 256);
+    if (force_type != 0 || npts <= 0) far_sets++;  // the boundary field's rule type
   }
+  b += far * far_sets;
   return b + ws + ws / 2 + 2 * CNT_N * 8;
 }
 

@@ -15,7 +15,10 @@ summed once per order over each band's items), and the other bands per
 * the pressure against the oracle's (orc_field_s, rule type 2): 1e-10;
 * the amplitudes follow the solution: after a new incident amplitude and a new
   solve, far points equal the per-pair sums again and scale with the amplitude;
-* a far point alone gives the bits it gives inside a batch.
+* a far point alone gives the bits it gives inside a batch;
+* boundary points (bqd2_field_bd, the boundary field behind the type-1 force)
+  take the bands far below or above them from the same amplitudes: their field
+  and the forces of both rules against the per-pair sums.
 """
 import ctypes as C
 import math
@@ -178,5 +181,44 @@ def test_far_point_alone_equals_batch():
         for i in range(0, len(pts), 3):
             oi, _ = field(prob, pts[i:i + 1], 1, total=1)
             assert np.array_equal(oi[0], ob[i])
+    finally:
+        m.finalize()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_boundary_points_match_per_pair_sums(name):
+    m, prob = solved(name)
+    hip = hip_lib()
+    tri = CASES[name][1]
+    try:
+        ne = m.ne()  # one sphere: domain 0 holds every element
+        t = np.arange(1, ne + 1, dtype=np.int32)
+        ze = np.ascontiguousarray(np.random.default_rng(7).uniform(-0.9, 0.9, (ne, 2)))
+        res = {}
+        for far in (1, 0):
+            assert hip.bqd2_set_field_far(prob, far) == 0
+            out = np.zeros((ne, 8))
+            if not tri:  # quadrilateral local coordinates
+                assert hip.bqd2_field_bd(prob, 0, ne, t.ctypes.data, ze.ctypes.data, 1, 0, out.ctypes.data) == 0
+            FN = {}
+            for type_ in (0, 1):
+                F, N = (C.c_double * 3)(), (C.c_double * 3)()
+                assert hip.bqd2_force(prob, (C.c_double * 3)(0.0, 0.0, 0.0), type_, F, N) == 0
+                FN[type_] = (np.array(F[:]), np.array(N[:]))
+            res[far] = (out, FN)
+        assert hip.bqd2_set_field_far(prob, 1) == 0
+        if not tri:
+            p1, v1 = cplx(res[1][0])
+            p0, v0 = cplx(res[0][0])
+            print(name, "boundary points: max rel err p %.2e v %.2e" % (
+                np.max(np.abs(p1 - p0) / np.abs(p0)),
+                np.max(np.linalg.norm(v1 - v0, axis=1) / np.linalg.norm(v0, axis=1))), flush=True)
+            compare(p1, v1, p0, v0, 1e-10)
+        for type_ in (0, 1):
+            (F1, N1), (F0, N0) = res[1][1][type_], res[0][1][type_]
+            eF = np.linalg.norm(F1 - F0) / np.linalg.norm(F0)
+            eN = np.linalg.norm(N1 - N0) / (np.linalg.norm(N0) + 1e-3 * np.linalg.norm(F0))
+            print(name, "force type", type_, "rel err F %.2e N %.2e" % (eF, eN), flush=True)
+            assert eF < 1e-10 and eN < 1e-10
     finally:
         m.finalize()
