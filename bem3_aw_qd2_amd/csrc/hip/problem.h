@@ -109,7 +109,7 @@ struct FarBand {
   int es, ee;
   double zpre, zsuf;
 };
-enum { FAR_MAX_ORDERS = 16384 };
+enum { FAR_MAX_ORDERS = 32768 };
 // entries of the pair list beyond N x Ne: each of the 18 classes (9 rules, two
 // element kinds) is padded to a multiple of the pairs per wavefront (up to 16)
 enum { PLIST_PAD = 512 };

@@ -132,8 +132,9 @@ extern "C" int bqd2_gauss_legendre(int n, double *x, double *w) {
 // its order-n term is a product of a target factor and a source factor; the
 // orders kept are those with |a_n| <= sqrt(k^2 + (FAR_DECAY / h)^2): the ones
 // beyond decay by exp(-FAR_DECAY) = 1e-20 or more over the height h.  h is the
-// smallest of 1/8, 1/6 and 1/4 of the shorter lattice vector whose grid stays
-// within FAR_MAX_ORDERS (1/8: about 14 600 orders on lattices of equal sides).
+// smallest of 1/12, 1/8, 1/6 and 1/4 of the shorter lattice vector whose grid
+// stays within FAR_MAX_ORDERS (1/12: about 32 000 orders on lattices of equal
+// sides).
 // A smaller h lets more of the element bands of points level with the boundary
 // take the amplitudes; each point then sums more orders, which costs far less
 // than the lattice sums it replaces.
@@ -141,7 +142,7 @@ int far_grid(const double *qd8, int *n1, int *n2, double *hp) {
   const double k = qd8[0], vk = hypot(qd8[1], qd8[2]);
   const double l1 = hypot(qd8[4], qd8[5]), l2 = hypot(qd8[6], qd8[7]);
   const double FAR_DECAY = 46.0;
-  for (const double f : {0.125, 1.0 / 6.0, 0.25}) {
+  for (const double f : {1.0 / 12.0, 0.125, 1.0 / 6.0, 0.25}) {
     const double h = f * std::min(l1, l2);
     if (!(h > 0.0)) return 0;
     const double R = sqrt(k * k + (FAR_DECAY / h) * (FAR_DECAY / h)) + vk;

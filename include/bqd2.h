@@ -217,7 +217,7 @@ int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, int total, d
                long long *gf_evals);
 
 /* Field points of the periodic domain and the boundary elements at least h
- * below or above them (h 1/8 to 1/4 of the shorter lattice vector): between them
+ * below or above them (h 1/12 to 1/4 of the shorter lattice vector): between them
  * the lattice sum is the pure spectral series of d3_qpgf_d2.c:484-533, whose
  * terms factor into a source and a target part, so bqd2_field sums the boundary
  * once per order into plane-wave amplitudes of 32 bands of elements by height
@@ -246,13 +246,13 @@ int bqd2_field_kernel_ms(bqd2_problem *p, double *ms);
  * (may be NULL: receives the number of points whose Green's function
  * evaluation failed).  Results are the bits bqd2_field returns.  A call needs
  * at most 1024 + n (32 + 144 c) + 4096 G bytes, c <= 64 the element chunks per
- * point of the problem and rule, G <= 16384 the far field's orders (0 where it
+ * point of the problem and rule, G <= 32768 the far field's orders (0 where it
  * is not used; bqd2_field_far_info): the far field's amplitudes are made in
- * every call (128 MiB: about 8000 points at c = 64 and G = 14641); a larger n is
+ * every call (256 MiB: about 14 800 points at c = 64 and G = 32041); a larger n is
  * rejected with BQD2_E_ARG before anything runs.  Every
  * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
  * streams, two threads, two problems) race and must be ordered by the caller. */
-#define BQD2_NOALLOC_SCRATCH_BYTES (128u << 20)
+#define BQD2_NOALLOC_SCRATCH_BYTES (256u << 20)
 int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, int type, int total, double *d_out,
                        int *d_did, long long *d_errors, void *stream);
 

@@ -108,7 +108,7 @@ def test_far_points_match_per_pair_sums(name):
     try:
         on, orders, zb, zt, h = far_info(prob)
         print(name, "far field: on", on, "orders", orders, "z extent", zb, zt, "h", h, flush=True)
-        assert on == 1 and 0 < orders <= 16384 and h > 0 and zt > zb
+        assert on == 1 and 0 < orders <= 32768 and h > 0 and zt > zb
         far, near = points(zb, zt, h, np.random.default_rng(11))
         for type_ in (0, 1, 2, 3):
             print("type", type_, flush=True)
