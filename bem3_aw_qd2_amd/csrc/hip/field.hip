@@ -4,6 +4,10 @@
 //   k_domain_id   point classification by signed solid angles (domain_id,
 //                 bem3_aw_qd2.c:1148-1257), one wavefront per point, the
 //                 reference's domain order and early exit
+//   k_field_F     F and grad F of every chunk of a field point's items (the
+//                 chunks of k_field_part that lie wholly in element bands taken
+//                 from the far field's amplitudes end at once); k_field_bd_F
+//                 likewise for boundary points
 //   k_field_part  partial sums of the representation formula and its gradient
 //                 (pv_s_dbieq_aqd2, bem3_aw_qd2_field.c:227-297 with
 //                 q0_bil_dcoef_grad_* / bil_dcoef_grad_*) over chunks of a
@@ -257,18 +261,26 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
   const int did = did_io[ip];
   double *dst = part + ((size_t)ip * nchunk + ic) * FP_N;
   if (did < 0) {
-    if (lane < FP_N) dst[lane] = 0.0;
+    if (lane < 14) dst[lane] = 0.0;
     return;
   }
   const Tab &T = *tab;
   const DomView &V = X.dom[did];
   const bool periodic = (X.ps == 1 && did == 0);
   const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
-  // elements of bands far below or above the point: F and grad F here, their
-  // lattice sums from k_far_eval
+  // elements of bands far below or above the point: their lattice sums from
+  // k_far_eval (F and grad F of every item from k_field_F)
   int fb0, fb1, fes, fee;
   far_split(X, did, r0[2], fb0, fb1, fes, fee);
   const int npt = rule_points(type, X.any_tri);
+  const long long items = (long long)V.Ne * npt;
+  const long long chunk = field_chunk_items(items);
+  const long long i0 = (long long)ic * chunk, i1 = min(items, i0 + chunk);
+  if (i1 <= (long long)fes * npt || i0 >= (long long)fee * npt) {
+    // nothing of this chunk is summed here
+    if (lane < 14) dst[lane] = 0.0;
+    return;
+  }
   const double k = V.k;
   // Accumulators and the per-item coefficients live in LDS (per lane), so that
   // only the offset stays in registers across the Hessian lattice sum; the empty
@@ -280,15 +292,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
   __shared__ double scf[22][64];
 #pragma unroll
   for (int i = 0; i < 14; i++) sacc[i][lane] = 0.0;
-  double F = 0.0, dF[3] = {0, 0, 0};
   GfTally tl{0, 0, 0, 0};
   long long evals = 0;
   int err = 0;
-  const long long items = (long long)V.Ne * npt;
-  const long long chunk = field_chunk_items(items);
-  const long long i0 = (long long)ic * chunk, i1 = min(items, i0 + chunk);
   for (long long it = i0 + lane; it < i1; it += 64) {
-    const int s = V.fperm[it / npt], q = (int)(it % npt);
+    const long long fpos = it / npt;
+    if (fpos < fes || fpos >= fee) continue;
+    const int s = V.fperm[fpos], q = (int)(it % npt);
     const int sd = V.sid[s], as = abs(sd);
     ElemC E;
     if (type != 0) load_elem(M, as, E);
@@ -298,10 +308,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
     double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
     double rdW = dot3(vR, w) * iR;
     const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
-    F += sg * wt * rdW * iR * iR;
-    for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
-    const long long fpos = it / npt;
-    if (fpos < fes || fpos >= fee) continue;
     cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
     for (int m = 0; m < 4; m++) {
       if (Nv[m] == 0.0) continue;
@@ -393,14 +399,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
   for (int c2 = 0; c2 < 3; c2++) {
     dPa[c2] = cd{wave_sum(dPa[c2].x), wave_sum(dPa[c2].y)};
     PVa[c2] = cd{wave_sum(PVa[c2].x), wave_sum(PVa[c2].y)};
-    dF[c2] = wave_sum(dF[c2]);
   }
-  F = wave_sum(F);
   long long tsp = wave_sum_ll(tl.sp), tpr = wave_sum_ll(tl.pr), tev = wave_sum_ll(tl.ev), tfd = wave_sum_ll(tl.fd);
   evals = wave_sum_ll(evals);
   int anyerr = __any(err != 0);
   if (lane != 0) return;
-  tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
+  if (evals) tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
   if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   dst[0] = Pa.x;
   dst[1] = Pa.y;
@@ -409,9 +413,59 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BQD2_FI
     dst[3 + 2 * c2] = dPa[c2].y;
     dst[8 + 2 * c2] = PVa[c2].x;
     dst[9 + 2 * c2] = PVa[c2].y;
-    dst[15 + c2] = dF[c2];
   }
-  dst[14] = F;
+}
+
+// F and grad F (the boundary's solid angle seen from the point, which picks
+// the form of the representation formula, and its gradient) of every chunk of
+// a field point's items, one wavefront per point, into slots 14..17 of
+// k_field_part's chunk records.  Apart from the lattice sums this is the only
+// work on the elements far below or above a point, so it runs apart from
+// k_field_part's register-heavy wavefronts.
+__global__ void __launch_bounds__(64) k_field_F(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, int type, int n,
+                                                int nchunk, const double *__restrict__ r0s,
+                                                const int *__restrict__ did_io, double *__restrict__ part) {
+  const int ip = blockIdx.x, lane = threadIdx.x;
+  if (ip >= n) return;
+  const int did = did_io[ip];
+  double *rec = part + (size_t)ip * nchunk * FP_N;
+  if (did < 0) {
+    for (int c = lane; c < nchunk; c += 64)
+      for (int i = 14; i < FP_N; i++) rec[(size_t)c * FP_N + i] = 0.0;
+    return;
+  }
+  const Tab &T = *tab;
+  const DomView &V = X.dom[did];
+  const bool periodic = (X.ps == 1 && did == 0);
+  const double r0[3] = {r0s[3 * ip], r0s[3 * ip + 1], r0s[3 * ip + 2]};
+  const int npt = rule_points(type, X.any_tri);
+  const long long items = (long long)V.Ne * npt;
+  const long long chunk = field_chunk_items(items);
+  for (int ic = 0; ic < nchunk; ic++) {
+    const long long i0 = (long long)ic * chunk, i1 = min(items, i0 + chunk);
+    double F = 0.0, dF[3] = {0, 0, 0};
+    for (long long it = i0 + lane; it < i1; it += 64) {
+      const int s = V.fperm[it / npt], q = (int)(it % npt);
+      const int sd = V.sid[s], as = abs(sd);
+      ElemC E;
+      if (type != 0) load_elem(M, as, E);
+      double r[3], w[3], wt, Nv[4];
+      if (!source_point(M, T, E, as, type, q, r, w, wt, Nv)) continue;
+      double vR[3] = {r[0] - r0[0], r[1] - r0[1], r[2] - r0[2]};
+      double aR = abs3(vR), iR = 1.0 / aR;
+      double rdW = dot3(vR, w) * iR;
+      const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+      F += sg * wt * rdW * iR * iR;
+      for (int c2 = 0; c2 < 3; c2++) dF[c2] += sg * wt * iR * iR * iR * (3.0 * vR[c2] * iR * rdW - w[c2]);
+    }
+    F = wave_sum(F);
+    for (int c2 = 0; c2 < 3; c2++) dF[c2] = wave_sum(dF[c2]);
+    if (lane == 0) {
+      double *dst = rec + (size_t)ic * FP_N;
+      dst[14] = F;
+      for (int c2 = 0; c2 < 3; c2++) dst[15 + c2] = dF[c2];
+    }
+  }
 }
 
 // one thread per field point: chunk sums in order, then the representation
@@ -788,12 +842,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   const int nci = bd_item_chunks(items);
   const int ic = SELF ? nci : (int)blockIdx.y;
   const long long i0 = (long long)ic * BD_CHUNK, i1 = SELF ? 0 : min(items, i0 + BD_CHUNK);
-  // elements of bands far below or above the point: F here, their lattice sums
-  // from k_far_eval
+  // elements of bands far below or above the point: their lattice sums from
+  // k_far_eval; F of every item of the chunks from k_field_bd_F
   int fb0, fb1, fes, fee;
   far_split(X, did, rt[2], fb0, fb1, fes, fee);
+  if (!SELF && (i1 <= (long long)fes * npt || i0 >= (long long)fee * npt)) {
+    // nothing of this chunk is summed here
+    if (lane < 8) part[((size_t)ip * (nci + 1) + ic) * BP_N + lane] = 0.0;
+    return;
+  }
   for (long long it = i0 + lane; it < i1; it += 64) {
-    const int s = V.fperm[it / npt], q = (int)(it % npt);
+    const long long fpos = it / npt;
+    if (fpos < fes || fpos >= fee) continue;
+    const int s = V.fperm[fpos], q = (int)(it % npt);
     if (s == t) continue;
     const int sd = V.sid[s], as = abs(sd);
     ElemC E;
@@ -804,9 +865,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     double aR = abs3(vR), iR = 1.0 / aR, aW = abs3(w);
     double tD = dot3(vR, w) * iR * iR * iR;
     const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
-    F += sg * wt * tD;
-    const long long fpos = it / npt;
-    if (fpos < fes || fpos >= fee) continue;
     cd Pi = {}, dPi = {}, pvi[3] = {}, dpvi[3] = {};
     for (int m = 0; m < 4; m++) {
       if (Nv[m] == 0.0) continue;
@@ -1084,7 +1142,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   evals = wave_sum_ll(evals);
   int anyerr = __any(err != 0);
   if (lane != 0) return;
-  tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
+  if (evals) tally_add(counters + CNT_FIELD, tsp, tpr, tev, tfd, evals);
   if (anyerr) atomicAdd((unsigned long long *)&counters[CNT_ERR], 1ull);
   double *dst = part + ((size_t)ip * (nci + 1) + ic) * BP_N;
   dst[0] = Pa.x;
@@ -1093,7 +1151,54 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     dst[2 + 2 * c2] = PVa[c2].x;
     dst[3 + 2 * c2] = PVa[c2].y;
   }
-  dst[8] = F;
+  if (SELF) dst[8] = F;  // the item chunks' F comes from k_field_bd_F
+}
+
+// F (the boundary's solid angle seen from the boundary point, without its own
+// element) of every item chunk of k_field_bd, one wavefront per point, into
+// slot 8 of the chunk records: the work on the elements far below or above the
+// point apart from the lattice sums, kept out of k_field_bd's register-heavy
+// wavefronts
+__global__ void __launch_bounds__(64) k_field_bd_F(DevMesh M, const Tab *__restrict__ tab, FieldCtx X, int type,
+                                                   int did, int n, const int *__restrict__ tpos,
+                                                   const double *__restrict__ ze, double *__restrict__ part) {
+  const int ip = blockIdx.x, lane = threadIdx.x;
+  if (ip >= n) return;
+  const Tab &T = *tab;
+  const DomView &V = X.dom[did];
+  const bool periodic = (X.ps == 1 && did == 0);
+  const int t = tpos[ip];
+  double rt[3];
+  {
+    ElemC Et;
+    load_elem(M, abs(V.sid[t]), Et);
+    double wt_[3];
+    bil_rw(Et, ze[2 * ip], ze[2 * ip + 1], rt, wt_);
+  }
+  const int npt = rule_points(type, X.any_tri);
+  const long long items = (long long)V.Ne * npt;
+  const int nci = bd_item_chunks(items);
+  for (int ic = 0; ic < nci; ic++) {
+    const long long i0 = (long long)ic * BD_CHUNK, i1 = min(items, i0 + BD_CHUNK);
+    double F = 0.0;
+    for (long long it = i0 + lane; it < i1; it += 64) {
+      const int s = V.fperm[it / npt], q = (int)(it % npt);
+      if (s == t) continue;
+      const int sd = V.sid[s], as = abs(sd);
+      ElemC E;
+      if (type != 0) load_elem(M, as, E);
+      double r[3], w[3], wt, Nv[4];
+      if (!source_point(M, T, E, as, type, q, r, w, wt, Nv)) continue;
+      double vR[3] = {r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]};
+      double aR = abs3(vR), iR = 1.0 / aR;
+      double tD = dot3(vR, w) * iR * iR * iR;
+      const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+      F += sg * wt * tD;
+    }
+    F *= BQD2_IFP;
+    F = wave_sum(F);
+    if (lane == 0) part[((size_t)ip * (nci + 1) + ic) * BP_N + 8] = F;
+  }
 }
 
 // one thread per boundary point: the chunk sums in order, the normalisation
@@ -1141,9 +1246,11 @@ static int launch_field_bd(bqd2_problem *p, hipStream_t st, FieldCtx &X, int typ
                            const int *d_t, const double *d_ze, double *d_part, double *d_out, long long *ctr,
                            const double *amp) {
   const int nch = bd_item_chunks((long long)p->dom[did].Ne * rule_points(type, p->any_tri)) + 1;
-  if (nch > 1)
+  if (nch > 1) {
     hipLaunchKernelGGL(k_field_bd<0>, dim3(n, nch - 1), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t,
                        d_ze, d_part, ctr);
+    hipLaunchKernelGGL(k_field_bd_F, dim3(n), dim3(64), 0, st, p->m, p->tab, X, type, did, n, d_t, d_ze, d_part);
+  }
   hipLaunchKernelGGL(k_field_bd<1>, dim3(n, 1), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, did, n, d_t, d_ze,
                      d_part, ctr);
   if (amp)
@@ -1405,6 +1512,7 @@ extern "C" int bqd2_field(bqd2_problem *p, int n, const double *pts, int type, i
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, st, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
   if (e1) BQD2_HIP(hipEventRecord(e1, st));
+  hipLaunchKernelGGL(k_field_F, dim3(n), dim3(64), 0, st, p->m, p->tab, X, type, n, nchunk, d_r0, d_did, d_part);
   if (any_far)
     hipLaunchKernelGGL(k_far_eval<0>, dim3(n), dim3(64), 0, st, p->m, X, p->gf, p->far, n, nchunk, d_r0, d_did,
                        (const int *)nullptr, (const double *)nullptr, 0, amp, d_part,
@@ -1501,6 +1609,7 @@ extern "C" int bqd2_field_noalloc(bqd2_problem *p, int n, const double *d_pts, i
                      d_did, d_r0, d_l);
   hipLaunchKernelGGL(k_field_part, dim3(n, nchunk), dim3(64), 0, stream, p->m, p->tab, X, p->gf, type, n, nchunk, d_r0,
                      d_did, d_part, ctr);
+  hipLaunchKernelGGL(k_field_F, dim3(n), dim3(64), 0, stream, p->m, p->tab, X, type, n, nchunk, d_r0, d_did, d_part);
   if (X.far_on)
     hipLaunchKernelGGL(k_far_eval<0>, dim3(n), dim3(64), 0, stream, p->m, X, p->gf, p->far, n, nchunk, d_r0, d_did,
                        (const int *)nullptr, (const double *)nullptr, 0, d_famp, d_part, ctr);
